@@ -1,0 +1,377 @@
+"""Scene descriptions and the seeded workload generators.
+
+A :class:`SceneSpec` is an in-memory CRSD scene (the reference's text scene
+format, parsed by ``SdParser::parseSd``, source/file-io/scene-description/
+sdParser.cpp:19-35). ``to_crsd`` writes it as CRSD text; large triangle meshes
+go to raw float32 side-car files referenced by ``positions-file`` /
+``normals-file`` / ``uvws-file`` keys of a ``triangle-mesh`` primitive, the
+binary form this library's CRSD reader accepts beside the reference's
+``vector3r-array`` text (primitiveInstantiator.cpp:64-90).
+
+Generators (all seeded):
+  * :func:`cornell_box`            36-triangle Cornell box, 0.25 x 0.25 light of radiance 17
+  * :func:`procedural_spheres`     64 Perlin-displaced level-5 icospheres (1.31M triangles)
+  * :func:`furnace_box`            closed Lambertian box whose walls all emit (closed form)
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PRIM_TRIANGLE = 0
+PRIM_RECTANGLE = 1
+
+
+def _f(x: float) -> str:
+    # 9 significant digits round-trip a binary32 through std::stold + cast
+    return "%.9g" % float(np.float32(x))
+
+
+def _v(v) -> str:
+    return " ".join(_f(x) for x in v)
+
+
+@dataclass
+class Material:
+    name: str
+    type: str = "matte-lambertian"        # or "conductor-microfacet"
+    albedo: tuple = (0.5, 0.5, 0.5)       # lambert albedo (constant)
+    albedo_texture: str | None = None     # name of a texture-spectrum block
+    f0: tuple = (1.0, 1.0, 1.0)           # schlick f0 (conductor)
+    roughness: float = 0.5
+    roughness_texture: str | None = None  # name of a texture-real block
+
+
+@dataclass
+class Texture:
+    name: str
+    data: np.ndarray                      # H x W x C float32, row 0 = v in [0, 1/H)
+    kind: str = "spectrum"                # "spectrum" (3 channels) or "real" (1 channel)
+    wrap: str = "repeat"
+
+
+@dataclass
+class Mesh:
+    """A group of primitives sharing a name and a material."""
+    name: str
+    material: str
+    kind: int = PRIM_TRIANGLE             # PRIM_TRIANGLE or PRIM_RECTANGLE
+    v: np.ndarray = None                  # n x 9 float32 (rectangle: v1 v2 v3)
+    normals: np.ndarray | None = None     # n x 9 float32 or None
+    uvws: np.ndarray | None = None        # n x 9 float32 or None
+    as_mesh: bool = False                 # write as one triangle-mesh block
+
+
+@dataclass
+class Light:
+    primitive: str                        # name of the emitting primitive
+    color: tuple = (1.0, 1.0, 1.0)
+    watt: float = 1.0
+    back_face: bool = False
+
+
+@dataclass
+class SceneSpec:
+    width: int = 256
+    height: int = 256
+    filter: str = "box"
+    filter_width: float | None = None
+    filter_height: float | None = None
+    tile: int = 16
+    cam_pos: tuple = (0.0, 0.0, 5.0)
+    cam_dir: tuple = (0.0, 0.0, -1.0)
+    cam_up: tuple = (0.0, 1.0, 0.0)
+    fov: float = 40.0
+    sampler: str = "stratified"
+    spp: int = 16
+    max_depth: int = 8
+    split_mode: str = "sah"
+    cluster: str = "uniform"
+    output: str = "out.png"
+    materials: list = field(default_factory=list)
+    textures: list = field(default_factory=list)
+    meshes: list = field(default_factory=list)
+    lights: list = field(default_factory=list)
+
+    # -- derived views ------------------------------------------------------
+    def num_prims(self) -> int:
+        return int(sum(len(m.v) for m in self.meshes))
+
+    def prim_arrays(self):
+        """Creation-order primitive arrays as the CRSD reader produces them."""
+        types, vs, ns, us, mats = [], [], [], [], []
+        mat_index = {m.name: i for i, m in enumerate(self.materials)}
+        for m in self.meshes:
+            n = len(m.v)
+            types.append(np.full(n, m.kind, np.int32))
+            vs.append(np.asarray(m.v, np.float32).reshape(n, 9))
+            ns.append(np.zeros((n, 9), np.float32) if m.normals is None else np.asarray(m.normals, np.float32))
+            if m.uvws is None:
+                u = np.zeros((n, 9), np.float32)
+                u[:, 0] = np.nan
+            else:
+                u = np.asarray(m.uvws, np.float32)
+            us.append(u)
+            mats.append(np.full(n, mat_index[m.material], np.int32))
+        cat = lambda a, shape: np.concatenate(a) if a else np.zeros(shape, np.float32)
+        return (cat(types, (0,)).astype(np.int32), cat(vs, (0, 9)), cat(ns, (0, 9)), cat(us, (0, 9)),
+                cat(mats, (0,)).astype(np.int32))
+
+    def light_prims(self):
+        """Primitive index bound to each light: the first primitive carrying the
+        name (std::map::insert keeps the first, primitiveInstantiator.cpp:119-125)."""
+        first, base = {}, 0
+        for m in self.meshes:
+            first.setdefault(m.name, base)
+            base += len(m.v)
+        return [first[l.primitive] for l in self.lights]
+
+    # -- CRSD ---------------------------------------------------------------
+    def to_crsd(self, sidecar_dir: str | None = None, sidecar_threshold: int = 64) -> str:
+        out = []
+        flt = [f'[filter string "{self.filter}"]']
+        if self.filter_width is not None:
+            flt.append(f'[filter-width real "{_f(self.filter_width)}"]')
+        if self.filter_height is not None:
+            flt.append(f'[filter-height real "{_f(self.filter_height)}"]')
+        out.append("film {\n"
+                   f'    [image-width int32 "{self.width}"]\n    [image-height int32 "{self.height}"]\n'
+                   f'    [tile-size-x int32 "{self.tile}"]\n    [tile-size-y int32 "{self.tile}"]\n'
+                   f'    [output-filename string "{self.output}"]\n    ' + "\n    ".join(flt) + "\n}")
+        out.append("camera {\n    [type string \"perspective-pinhole\"]\n"
+                   f'    [position vector3r "{_v(self.cam_pos)}"]\n    [direction vector3r "{_v(self.cam_dir)}"]\n'
+                   f'    [up vector3r "{_v(self.cam_up)}"]\n    [fov real "{_f(self.fov)}"]\n}}')
+        out.append("renderer {\n    [type string \"sampling\"]\n"
+                   f'    [sampler string "{self.sampler}"]\n    [sample-number int32 "{self.spp}"]\n'
+                   f'    [estimator string "pt"]\n    [max-depth int32 "{self.max_depth}"]\n}}')
+        out.append(f'accelerator {{\n    [type string "bvh"]\n    [split-mode string "{self.split_mode}"]\n}}')
+        out.append(f'light-cluster {{\n    [type string "{self.cluster}"]\n}}')
+        for t in self.textures:
+            arr = np.ascontiguousarray(t.data, np.float32)
+            if sidecar_dir is None:
+                raise ValueError("image textures need a side-car directory")
+            path = os.path.join(sidecar_dir, f"tex_{t.name}.f32")
+            arr.tofile(path)
+            blk = "texture-spectrum" if t.kind == "spectrum" else "texture-real"
+            out.append(f'{blk} {{\n    [type string "image-raw"]\n    [name string "{t.name}"]\n'
+                       f'    [filename string "{path}"]\n    [width int32 "{arr.shape[1]}"]\n'
+                       f'    [height int32 "{arr.shape[0]}"]\n    [wrap-mode string "{t.wrap}"]\n}}')
+        for m in self.materials:
+            if m.type == "matte-lambertian":
+                alb = (f'[albedo texture "{m.albedo_texture}"]' if m.albedo_texture
+                       else f'[albedo rgb "{_v(m.albedo)}"]')
+                out.append(f'material {{\n    [type string "matte-lambertian"]\n    [name string "{m.name}"]\n'
+                           f'    {alb}\n}}')
+            else:
+                rough = (f'[roughness texture "{m.roughness_texture}"]' if m.roughness_texture
+                         else f'[roughness real "{_f(m.roughness)}"]')
+                out.append(f'material {{\n    [type string "conductor-microfacet"]\n    [name string "{m.name}"]\n'
+                           f'    [microfacet-type string "ggx"]\n    [fresnel-type string "schlick"]\n'
+                           f'    [f0 rgb "{_v(m.f0)}"]\n    {rough}\n}}')
+        for mi, m in enumerate(self.meshes):
+            v = np.asarray(m.v, np.float32).reshape(-1, 9)
+            if m.kind == PRIM_RECTANGLE or not m.as_mesh:
+                kind = "rectangle" if m.kind == PRIM_RECTANGLE else "triangle"
+                for r in v:
+                    out.append(f'primitive {{\n    [type string "{kind}"]\n    [name string "{m.name}"]\n'
+                               f'    [bsdf material "{m.material}"]\n    [v1 vector3r "{_v(r[0:3])}"]\n'
+                               f'    [v2 vector3r "{_v(r[3:6])}"]\n    [v3 vector3r "{_v(r[6:9])}"]\n}}')
+                continue
+            lines = [f'primitive {{\n    [type string "triangle-mesh"]\n    [name string "{m.name}"]\n'
+                     f'    [bsdf material "{m.material}"]']
+            arrays = [("positions", v), ("normals", m.normals), ("uvws", m.uvws)]
+            for key, arr in arrays:
+                if arr is None:
+                    continue
+                arr = np.ascontiguousarray(np.asarray(arr, np.float32).reshape(-1, 9))
+                if sidecar_dir is not None and len(arr) > sidecar_threshold:
+                    path = os.path.join(sidecar_dir, f"mesh{mi}_{key}.f32")
+                    arr.tofile(path)
+                    lines.append(f'    [{key}-file string "{path}"]')
+                else:
+                    triples = ", ".join(_v(t) for t in arr.reshape(-1, 3))
+                    lines.append(f'    [{key} vector3r-array "{triples}"]')
+            out.append("\n".join(lines) + "\n}")
+        for l in self.lights:
+            out.append(f'light {{\n    [type string "area"]\n    [primitive primitive "{l.primitive}"]\n'
+                       f'    [color rgb "{_v(l.color)}"]\n    [watt real "{_f(l.watt)}"]\n'
+                       f'    [is-back-face-emit bool "{"true" if l.back_face else "false"}"]\n}}')
+        return "# generated by cadise_amd.scenes\n" + "\n".join(out) + "\n"
+
+
+# -- geometry helpers -----------------------------------------------------------
+
+def quad_tris(a, b, c, d):
+    """Two triangles (a, b, c), (a, c, d) of a planar quad a-b-c-d."""
+    a, b, c, d = (np.asarray(x, np.float32) for x in (a, b, c, d))
+    return np.stack([np.concatenate([a, b, c]), np.concatenate([a, c, d])]).astype(np.float32)
+
+
+def box_tris(lo, hi):
+    """12 triangles of an axis-aligned box, outward winding."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    p = lambda x, y, z: (x, y, z)
+    faces = [
+        (p(x0, y0, z0), p(x0, y0, z1), p(x0, y1, z1), p(x0, y1, z0)),  # -x
+        (p(x1, y0, z0), p(x1, y1, z0), p(x1, y1, z1), p(x1, y0, z1)),  # +x
+        (p(x0, y0, z0), p(x1, y0, z0), p(x1, y0, z1), p(x0, y0, z1)),  # -y
+        (p(x0, y1, z0), p(x0, y1, z1), p(x1, y1, z1), p(x1, y1, z0)),  # +y
+        (p(x0, y0, z0), p(x0, y1, z0), p(x1, y1, z0), p(x1, y0, z0)),  # -z
+        (p(x0, y0, z1), p(x1, y0, z1), p(x1, y1, z1), p(x0, y1, z1)),  # +z
+    ]
+    return np.concatenate([quad_tris(*f) for f in faces])
+
+
+def cornell_box(width=256, height=256, spp=16, max_depth=8, filter="box") -> SceneSpec:
+    """Cornell box of 36 triangles: 5 walls, 2 axis-aligned boxes, and a
+    0.25 x 0.25 ceiling light as two triangles, each its own area light of
+    radiance 17 (watt = 3 * 17 * pi * area, singleAreaLight.cpp:20-37)."""
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter=filter,
+                  cam_pos=(0.5, 0.5, 1.9), cam_dir=(0.0, 0.0, -1.0), cam_up=(0.0, 1.0, 0.0), fov=40.0)
+    s.materials = [Material("white", albedo=(0.63, 0.63, 0.63)), Material("red", albedo=(0.63, 0.065, 0.05)),
+                   Material("green", albedo=(0.14, 0.45, 0.091))]
+    s.meshes = [
+        Mesh("floor", "white", v=quad_tris((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0))),
+        Mesh("ceiling", "white", v=quad_tris((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1))),
+        Mesh("back", "white", v=quad_tris((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0))),
+        Mesh("left", "red", v=quad_tris((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1))),
+        Mesh("right", "green", v=quad_tris((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0))),
+        Mesh("shortbox", "white", v=box_tris((0.13, 0.0, 0.55), (0.43, 0.3, 0.85))),
+        Mesh("tallbox", "white", v=box_tris((0.55, 0.0, 0.2), (0.85, 0.6, 0.5))),
+    ]
+    # light quad just below the ceiling, wound so that its normal points down
+    y = 0.999
+    lt = quad_tris((0.375, y, 0.375), (0.625, y, 0.375), (0.625, y, 0.625), (0.375, y, 0.625))
+    s.meshes.append(Mesh("light0", "white", v=lt[0:1]))
+    s.meshes.append(Mesh("light1", "white", v=lt[1:2]))
+    area = 0.5 * 0.25 * 0.25
+    s.lights = [Light("light0", (1, 1, 1), 3 * 17 * math.pi * area), Light("light1", (1, 1, 1), 3 * 17 * math.pi * area)]
+    return s
+
+
+def furnace_box(albedo=0.5, radiance=1.0, width=32, height=32, spp=4, max_depth=4) -> SceneSpec:
+    """Closed cube, Lambertian albedo a on every wall, every wall triangle an
+    area light of radiance Le emitting inwards. E[L] = Le * sum_{k=0}^{D} a^k
+    for max depth D (every vertex adds one power of a)."""
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="box",
+                  cam_pos=(0.0, 0.0, 0.0), cam_dir=(0.0, 0.0, -1.0), cam_up=(0.0, 1.0, 0.0), fov=60.0)
+    s.materials = [Material("wall", albedo=(albedo, albedo, albedo))]
+    tris = box_tris((-1, -1, -1), (1, 1, 1))
+    # box_tris winds outwards; emission must face inwards -> back_face emit
+    for i, t in enumerate(tris):
+        s.meshes.append(Mesh(f"w{i}", "wall", v=t[None, :]))
+        a = np.asarray(t, np.float64).reshape(3, 3)
+        area = 0.5 * np.linalg.norm(np.cross(a[1] - a[0], a[2] - a[0]))
+        s.lights.append(Light(f"w{i}", (1, 1, 1), 3 * radiance * math.pi * area, back_face=True))
+    return s
+
+
+# -- procedural spheres ------------------------------------------------------------
+
+def icosphere(level: int):
+    """Unit icosphere: (vertices V x 3 float64, faces F x 3 int)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = np.asarray(verts, np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.asarray(faces, np.int64)
+    for _ in range(level):
+        edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        edges.sort(axis=1)
+        uniq, inv = np.unique(edges, axis=0, return_inverse=True)
+        mids = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mids /= np.linalg.norm(mids, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mids])
+        inv = inv.reshape(3, -1) + base
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = inv[0], inv[1], inv[2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f
+
+
+def _perlin3(p: np.ndarray, perm: np.ndarray) -> np.ndarray:
+    """Perlin's improved noise (K. Perlin, "Improving Noise", SIGGRAPH 2002), vectorized."""
+    pi = np.floor(p).astype(np.int64)
+    pf = p - pi
+    pi &= 255
+    fade = lambda t: t * t * t * (t * (t * 6 - 15) + 10)
+    u, v, w = fade(pf[:, 0]), fade(pf[:, 1]), fade(pf[:, 2])
+
+    def grad(h, x, y, z):
+        h = h & 15
+        a = np.where(h < 8, x, y)
+        b = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -a, a) + np.where(h & 2, -b, b)
+
+    X, Y, Z = pi[:, 0], pi[:, 1], pi[:, 2]
+    x, y, z = pf[:, 0], pf[:, 1], pf[:, 2]
+    P = perm
+    A = P[X] + Y; AA = P[A] + Z; AB = P[A + 1] + Z
+    B = P[X + 1] + Y; BA = P[B] + Z; BB = P[B + 1] + Z
+    lerp = lambda t, a, b: a + t * (b - a)
+    return lerp(w,
+                lerp(v, lerp(u, grad(P[AA], x, y, z), grad(P[BA], x - 1, y, z)),
+                     lerp(u, grad(P[AB], x, y - 1, z), grad(P[BB], x - 1, y - 1, z))),
+                lerp(v, lerp(u, grad(P[AA + 1], x, y, z - 1), grad(P[BA + 1], x - 1, y, z - 1)),
+                     lerp(u, grad(P[AB + 1], x, y - 1, z - 1), grad(P[BB + 1], x - 1, y - 1, z - 1))))
+
+
+def procedural_spheres(seed=0x20261203, n_spheres=64, level=5, width=1920, height=1080, spp=64, max_depth=8,
+                       with_normals=True) -> SceneSpec:
+    """64 subdivided icospheres (20 * 4^level triangles each) with vertices
+    displaced along the normal by Perlin noise of amplitude 0.1 r; radii
+    log-uniform in [0.2, 2]; centres rejection-sampled without overlap above a
+    two-triangle ground plane; GGX conductors with roughness U[0.05, 1] and
+    f0 U[0.1, 0.9]^3; four quad (rectangle) lights."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm, perm]).astype(np.int64)
+    uv, uf = icosphere(level)
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="box", fov=60.0,
+                  cam_pos=(0.0, 9.0, 30.0), cam_dir=(0.0, -0.28, -1.0), cam_up=(0.0, 1.0, 0.0))
+    s.materials.append(Material("ground", albedo=(0.5, 0.5, 0.5)))
+    s.materials.append(Material("lightmat", albedo=(0.8, 0.8, 0.8)))
+    extent = 16.0
+    centres, radii = [], []
+    while len(centres) < n_spheres:
+        r = float(np.exp(rng.uniform(math.log(0.2), math.log(2.0))))
+        c = np.array([rng.uniform(-extent, extent), r * 1.1 + rng.uniform(0.0, 3.0), rng.uniform(-extent, extent)])
+        if all(np.linalg.norm(c - c2) > (r + r2) * 1.15 for c2, r2 in zip(centres, radii)):
+            centres.append(c)
+            radii.append(r)
+    for i, (c, r) in enumerate(zip(centres, radii)):
+        n = _perlin3(uv * 2.5 + rng.uniform(0, 100, 3), perm)
+        disp = uv * (r * (1.0 + 0.1 * n))[:, None]
+        pos = disp + c
+        tri = pos[uf].reshape(-1, 9).astype(np.float32)
+        nrm = None
+        if with_normals:
+            fn = np.cross(pos[uf[:, 1]] - pos[uf[:, 0]], pos[uf[:, 2]] - pos[uf[:, 0]])
+            vn = np.zeros_like(pos)
+            for k in range(3):
+                np.add.at(vn, uf[:, k], fn)
+            vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+            nrm = vn[uf].reshape(-1, 9).astype(np.float32)
+        mname = f"ggx{i}"
+        s.materials.append(Material(mname, type="conductor-microfacet", f0=tuple(rng.uniform(0.1, 0.9, 3)),
+                                    roughness=float(rng.uniform(0.05, 1.0))))
+        s.meshes.append(Mesh(f"sphere{i}", mname, v=tri, normals=nrm, as_mesh=True))
+    g = 60.0
+    s.meshes.append(Mesh("ground", "ground", v=quad_tris((-g, 0, -g), (-g, 0, g), (g, 0, g), (g, 0, -g))))
+    lights = [((-10, 14, -10), 3.0), ((10, 14, -10), 3.0), ((-10, 14, 10), 3.0), ((10, 14, 10), 3.0)]
+    for i, ((x, y, z), h) in enumerate(lights):
+        # rectangle v1 v2 v3 -> vA vB vC; normal cross(vA - vB, vC - vB) points down
+        rect = np.array([[x - h, y, z - h, x - h, y, z + h, x + h, y, z + h]], np.float32)
+        s.meshes.append(Mesh(f"lamp{i}", "lightmat", kind=PRIM_RECTANGLE, v=rect))
+        s.lights.append(Light(f"lamp{i}", (1.0, 0.95, 0.85), 4000.0))
+    return s

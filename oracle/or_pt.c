@@ -1,0 +1,1092 @@
+/*
+ * Oracle (test infrastructure): scene assembly, surface details, BSDFs,
+ * area lights, camera, sampler, film, and the PtEstimator bounce loop.
+ * Each function cites the reference function it restates.
+ */
+#include "or_internal.h"
+#include "seqrng.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ---- random numbers --------------------------------------------------------- */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+ * as 1, 2, 3", SC 2011). */
+void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline float u32_to_unit(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+/* Slots of the Philox layout within one bounce b: counter (pixel, sample, 2b + slot/4, 0). */
+enum { SL_PICK = 0, SL_LPOS0 = 1, SL_LPOS1 = 2, SL_RR = 3, SL_MIS0 = 4, SL_MIS1 = 5, SL_CONT0 = 6, SL_CONT1 = 7 };
+
+typedef struct {
+    int      mode;
+    uint64_t state;
+    uint32_t key[2];
+    uint32_t pixel, sample;
+    int64_t  draws;
+} or_rng;
+
+static float rng_u(or_rng* r, int bounce, int slot) {
+    r->draws++;
+    if (r->mode == OR_RNG_SEQ) return seqrng_next(&r->state);
+    const uint32_t ctr[4] = {r->pixel, r->sample, (uint32_t)(2 * bounce + slot / 4), 0u};
+    uint32_t out[4];
+    or_philox(ctr, r->key, out);
+    return u32_to_unit(out[slot & 3]);
+}
+
+/* Vector2R(Random::nextReal(), Random::nextReal()): g++ evaluates the two
+ * constructor arguments right to left, so the first draw lands in y. */
+static void rng_u2(or_rng* r, int bounce, int slot_x, int slot_y, float* x, float* y) {
+    if (r->mode == OR_RNG_SEQ) {
+        *y = rng_u(r, bounce, slot_y);
+        *x = rng_u(r, bounce, slot_x);
+    } else {
+        *x = rng_u(r, bounce, slot_x);
+        *y = rng_u(r, bounce, slot_y);
+    }
+}
+
+/* ---- scene assembly ---------------------------------------------------------- */
+
+static float prim_area(const or_prim* p) {
+    if (p->type == OR_PRIM_RECTANGLE) return v3_len(p->e1) * v3_len(p->e2); /* rectangle.cpp:179-181 */
+    return 0.5f * v3_len(v3_cross(p->e1, p->e2));                            /* triangle.cpp:199-201 */
+}
+
+/* 4x4 float product as TMatrix4::operator*= (tMatrix4.ipp:102-116) */
+static void mat_mul(float* a, const float* b) {
+    float tmp[16];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc += a[4 * r + k] * b[4 * k + c];
+            tmp[4 * r + c] = acc;
+        }
+    memcpy(a, tmp, sizeof(tmp));
+}
+
+/* PerspectivePinholeCamera ctor + updateTransform (perspectivePinholeCamera.cpp:14-50),
+ * TMatrix4::lookAt / translate / scale (tMatrix4.ipp:25-60), Camera::_getAspectRatio
+ * (camera.cpp:16-18) */
+static void camera_setup(or_scene* s) {
+    const or_scene_desc* d = &s->desc;
+    const v3 pos = v3_load(d->cam_pos), dir = v3_load(d->cam_dir), up = v3_load(d->cam_up);
+    const v3 nz = v3_normalize(v3_neg(dir));
+    const v3 nx = v3_normalize(v3_cross(up, nz));
+    const v3 ny = v3_cross(nz, nx);
+    const float c2w[16] = {nx.x, ny.x, nz.x, pos.x, nx.y, ny.y, nz.y, pos.y,
+                           nx.z, ny.z, nz.z, pos.z, 0.0f, 0.0f, 0.0f, 1.0f};
+    memcpy(s->camera_to_world, c2w, sizeof(c2w));
+    s->cam_pos = pos;
+
+    const double aspect  = (double)(size_t)d->width / (double)(size_t)d->height;
+    const double sensorW = (double)d->sensor_width_mm;
+    const double sensorH = sensorW / aspect;
+    const float  resX = (float)(size_t)d->width, resY = (float)(size_t)d->height;
+    const double fov = (double)d->fov_deg;
+    const float  halfRad = (float)(fov * 0.5) * OR_DEG2RAD;
+    const double offset = (sensorW * 0.5) / (double)tanf(halfRad);
+    float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const float tr[16] = {1, 0, 0, (float)(-(sensorW * 0.5)), 0, 1, 0, (float)(sensorH * 0.5),
+                          0, 0, 1, (float)(-offset), 0, 0, 0, 1};
+    const float sc[16] = {(float)(sensorW / (double)resX), 0, 0, 0, 0, (float)(-sensorH / (double)resY), 0, 0,
+                          0, 0, 1.0f, 0, 0, 0, 0, 1};
+    mat_mul(m, tr);
+    mat_mul(m, sc);
+    memcpy(s->film_to_camera, m, sizeof(m));
+}
+
+or_scene* or_scene_create(const or_scene_desc* desc) {
+    or_scene* s = (or_scene*)calloc(1, sizeof(or_scene));
+    s->desc = *desc;
+    s->nprims = desc->nprims;
+    s->prims = (or_prim*)calloc((size_t)(desc->nprims > 0 ? desc->nprims : 1), sizeof(or_prim));
+    for (int64_t i = 0; i < desc->nprims; ++i) {
+        or_prim* p = &s->prims[i];
+        const float* v = desc->prim_v + 9 * i;
+        p->type = desc->prim_type[i];
+        p->vA = v3_load(v); p->vB = v3_load(v + 3); p->vC = v3_load(v + 6);
+        p->mat = desc->prim_mat[i];
+        p->light = -1;
+        if (p->type == OR_PRIM_RECTANGLE) {
+            /* Rectangle ctor (rectangle.cpp:14-37) */
+            p->e1 = v3_sub(p->vA, p->vB);
+            p->e2 = v3_sub(p->vC, p->vB);
+            p->vD = v3_add(v3_add(p->vB, p->e1), p->e2);
+            p->uvwA = v3_make(1, 0, 0); p->uvwB = v3_make(0, 0, 0);
+            p->uvwC = v3_make(0, 1, 0); p->uvwD = v3_make(1, 1, 0);
+        } else {
+            /* Triangle ctor (triangle.cpp:15-42) + TriangleMesh setters (triangleMesh.cpp:20-40) */
+            p->e1 = v3_sub(p->vB, p->vA);
+            p->e2 = v3_sub(p->vC, p->vA);
+            v3 N = v3_cross(p->e1, p->e2);
+            N = v3_is_zero(N) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(N);
+            p->nA = p->nB = p->nC = N;
+            p->uvwA = v3_make(0, 0, 0); p->uvwB = v3_make(1, 0, 0); p->uvwC = v3_make(0, 1, 0);
+            if (desc->prim_n) {
+                const float* n = desc->prim_n + 9 * i;
+                const v3 a = v3_load(n), b = v3_load(n + 3), c = v3_load(n + 6);
+                if (!v3_is_zero(a)) p->nA = a;
+                if (!v3_is_zero(b)) p->nB = b;
+                if (!v3_is_zero(c)) p->nC = c;
+            }
+            if (desc->prim_uvw && !isnan(desc->prim_uvw[9 * i])) {
+                const float* u = desc->prim_uvw + 9 * i;
+                p->uvwA = v3_load(u); p->uvwB = v3_load(u + 3); p->uvwC = v3_load(u + 6);
+            }
+        }
+    }
+    s->nmats = desc->nmats;
+    s->mats = (or_material*)calloc((size_t)(desc->nmats > 0 ? desc->nmats : 1), sizeof(or_material));
+    memcpy(s->mats, desc->mats, sizeof(or_material) * (size_t)desc->nmats);
+    s->nlights = desc->nlights;
+    s->lights = (or_light_state*)calloc((size_t)(desc->nlights > 0 ? desc->nlights : 1), sizeof(or_light_state));
+    for (int32_t i = 0; i < desc->nlights; ++i) {
+        /* SingleAreaLight ctor (singleAreaLight.cpp:20-37) */
+        const or_light* L = &desc->lights[i];
+        const v3 color = v3_load(L->color);
+        const v3 unit = v3_divs(color, v3_sum(color));
+        const v3 total = v3_scale(unit, L->watt);
+        const float area = prim_area(&s->prims[L->prim]);
+        s->lights[i].prim = L->prim;
+        s->lights[i].radiance = v3_scale(v3_divs(total, area), OR_INV_PI);
+        s->lights[i].back_face = L->back_face;
+        s->prims[L->prim].light = i; /* Primitive::setAreaLight: the last binding wins */
+    }
+    /* PowerLightCluster + Distribution1D (powerLightCluster.cpp:9-25, distribution1D.cpp:10-40);
+     * approximateFlux (singleAreaLight.cpp:168-184), luminance (rgbSpectrum.cpp:33-37) */
+    s->power_cdf = (float*)calloc((size_t)desc->nlights + 1, sizeof(float));
+    if (desc->nlights > 0) {
+        const float delta = 1.0f / (float)(size_t)desc->nlights;
+        s->power_cdf[0] = 0.0f;
+        for (int32_t i = 1; i <= desc->nlights; ++i) {
+            const v3 le = s->lights[i - 1].radiance;
+            const float lum = 0.0126f * le.x + 0.7152f * le.y + 0.0722f * le.z;
+            float flux = lum * prim_area(&s->prims[s->lights[i - 1].prim]) * OR_PI;
+            if (flux <= 0.0f) flux = 1.0f;
+            s->power_cdf[i] = s->power_cdf[i - 1] + flux * delta;
+        }
+        const float sum = s->power_cdf[desc->nlights];
+        if (sum > 0.0f) {
+            const float inv = 1.0f / sum;
+            for (int32_t i = 1; i <= desc->nlights; ++i) s->power_cdf[i] *= inv;
+        } else {
+            for (int32_t i = 1; i <= desc->nlights; ++i) s->power_cdf[i] = (float)(size_t)i * delta;
+        }
+        s->power_cdf[desc->nlights] = 1.0f;
+    }
+    camera_setup(s);
+    /* StratifiedSampler ctor (stratifiedSampler.cpp:14-20), math::nearest_lower_square_number (math.cpp:18-23) */
+    {
+        const size_t lower = (size_t)floorf(sqrtf((float)(size_t)desc->spp));
+        s->strata_base = (int)(size_t)sqrt((double)(lower * lower));
+    }
+    or_build_bvh(s);
+    return s;
+}
+
+void or_scene_destroy(or_scene* s) {
+    if (!s) return;
+    free(s->prims); free(s->mats); free(s->lights); free(s->power_cdf);
+    free(s->nodes); free(s->ordered);
+    free(s);
+}
+
+/* ---- surface interaction ------------------------------------------------------- */
+
+typedef struct {
+    int32_t prim;
+    v3 P, Ng, Ns, uvw, wi, wo;
+} or_si;
+
+/* Triangle::_positionToBarycentric (triangle.cpp:233-262) */
+static v3 barycentric(const or_prim* p, v3 P) {
+    const v3 v0 = v3_sub(p->vB, p->vA), v1 = v3_sub(p->vC, p->vA), v2 = v3_sub(P, p->vA);
+    const float d00 = v3_dot(v0, v0), d01 = v3_dot(v0, v1), d11 = v3_dot(v1, v1);
+    const float d20 = v3_dot(v2, v0), d21 = v3_dot(v2, v1);
+    const float den = d00 * d11 - d01 * d01;
+    if (den == 0.0f) return v3_make(0, 0, 0);
+    const float inv = 1.0f / den;
+    const float u = (d11 * d20 - d01 * d21) * inv;
+    const float v = (d00 * d21 - d01 * d20) * inv;
+    return v3_make(1.0f - u - v, u, v);
+}
+
+static v3 bary_mix(v3 b, v3 a, v3 bb, v3 c) {
+    return v3_add(v3_add(v3_scale(a, b.x), v3_scale(bb, b.y)), v3_scale(c, b.z));
+}
+
+/* Scene::isIntersecting detail (scene.cpp:30-48), Triangle::evaluateSurfaceDetail
+ * (triangle.cpp:125-160), Rectangle::evaluateSurfaceDetail (rectangle.cpp:102-132) */
+static void surface_detail(const or_scene* s, int32_t prim, v3 o, v3 d, float t, or_si* si) {
+    const or_prim* p = &s->prims[prim];
+    si->prim = prim;
+    si->wi = v3_neg(d);
+    si->P = v3_add(o, v3_scale(d, t));
+    if (p->type == OR_PRIM_RECTANGLE) {
+        v3 N = v3_cross(p->e1, p->e2);
+        N = v3_is_zero(N) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(N);
+        si->Ng = si->Ns = N;
+        const v3 onPlane = v3_sub(si->P, p->vB);
+        const float p1 = v3_dot(onPlane, v3_normalize(p->e1)) / v3_len(p->e1);
+        const float p2 = v3_dot(onPlane, v3_normalize(p->e2)) / v3_len(p->e1);
+        const v3 l1 = v3_lerp(p->uvwB, p->uvwC, p2);
+        const v3 l2 = v3_lerp(p->uvwA, p->uvwD, p2);
+        si->uvw = v3_lerp(l1, l2, p1);
+    } else {
+        v3 Ng = v3_cross(p->e1, p->e2);
+        Ng = v3_is_zero(Ng) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(Ng);
+        const v3 b = barycentric(p, si->P);
+        si->uvw = bary_mix(b, p->uvwA, p->uvwB, p->uvwC);
+        v3 Ns = bary_mix(b, p->nA, p->nB, p->nC);
+        Ns = v3_is_zero(Ns) ? Ng : v3_normalize(Ns);
+        si->Ns = Ns;
+        si->Ng = Ng;
+    }
+}
+
+static int scene_intersect(const or_scene* s, v3 o, v3 d, float minT, float maxT, or_si* si, float* tout) {
+    int32_t prim = -1;
+    float t = maxT;
+    if (!or_closest(s, o, d, minT, &t, &prim)) return 0;
+    surface_detail(s, prim, o, d, t, si);
+    if (tout) *tout = t;
+    return 1;
+}
+
+void or_intersect_detail(const or_scene* s, const float* rays, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        float* o = out + 14 * i;
+        or_si si;
+        float t;
+        if (scene_intersect(s, v3_load(r), v3_load(r + 4), r[3], r[7], &si, &t)) {
+            o[0] = (float)si.prim; o[1] = t;
+            v3_store(o + 2, si.P); v3_store(o + 5, si.Ng); v3_store(o + 8, si.Ns); v3_store(o + 11, si.uvw);
+        } else {
+            memset(o, 0, 14 * sizeof(float));
+            o[0] = -1.0f;
+        }
+    }
+}
+
+/* ---- textures (tConstantTexture.ipp:14-19, tNearestPixelSampler.ipp:13-31,
+ *      repeatUvwWrapper.cpp:9-15, clampUvwWrapper.cpp:8-12) ------------------- */
+
+static v3 tex_eval(const or_texture* t, v3 uvw) {
+    if (!t->data) return v3_make(t->value[0], t->value[1], t->value[2]);
+    float u, v;
+    if (t->wrap_clamp) { u = m_clamp(uvw.x, 0.0f, 1.0f); v = m_clamp(uvw.y, 0.0f, 1.0f); }
+    else { u = uvw.x - floorf(uvw.x); v = uvw.y - floorf(uvw.y); }
+    int32_t x = (int32_t)(u * (float)t->width);
+    int32_t y = (int32_t)(v * (float)t->height);
+    x = (x < t->width) ? x : x - 1;
+    y = (y < t->height) ? y : y - 1;
+    const float* px = t->data + ((size_t)y * (size_t)t->width + (size_t)x) * (size_t)t->channels;
+    if (t->channels == 1) return v3_make(px[0], px[0], px[0]);
+    return v3_make(px[0], px[1], px[2]);
+}
+
+/* ---- BSDFs ---------------------------------------------------------------------- */
+
+typedef struct { v3 f; v3 L; float pdf; int valid; } or_bsdf_sample;
+
+/* math::build_coordinate_system (math.cpp:36-58) */
+static void frame(v3 y, v3* z, v3* x) {
+    if (fabsf(y.x) > fabsf(y.y)) {
+        v3 zz = v3_make(-y.z, 0.0f, y.x);
+        *z = v3_divs(zz, sqrtf(y.x * y.x + y.z * y.z));
+    } else {
+        v3 zz = v3_make(0.0f, y.z, -y.y);
+        *z = v3_divs(zz, sqrtf(y.y * y.y + y.z * y.z));
+    }
+    *x = v3_cross(y, *z);
+}
+static v3 to_world(v3 x, v3 y, v3 z, v3 l) {
+    return v3_add(v3_add(v3_scale(x, l.x), v3_scale(y, l.y)), v3_scale(z, l.z));
+}
+
+/* TRoughnessMapper<SQUARE>::map (tRoughnessMapper.h:18-40) */
+static float rough_to_alpha(float r) {
+    const float safe = m_max(r, 0.001f);
+    return m_max(safe * safe, 0.001f);
+}
+/* IsotropicGgx::distributionD (isotropicGgx.cpp:11-34) */
+static float ggx_D(float a, v3 N, v3 H) {
+    const float NdotH = v3_dot(N, H);
+    if (NdotH <= 0.0f) return 0.0f;
+    const float a2 = a * a, c2 = NdotH * NdotH, c4 = c2 * c2;
+    const float tan2 = (1.0f - c2) / c2;
+    const float den = OR_PI * c4 * (a2 + tan2) * (a2 + tan2);
+    return a2 / den;
+}
+/* IsotropicGgx::shadowingMaskingG (isotropicGgx.cpp:36-69), Microfacet::_isShadowingMaskingValid (microfacet.cpp:11-23) */
+static float ggx_G(float a, v3 V, v3 L, v3 N, v3 H) {
+    const float VdotH = v3_dot(V, H), VdotN0 = v3_dot(V, N), LdotH = v3_dot(L, H), LdotN0 = v3_dot(L, N);
+    if (!((VdotH * VdotN0 > 0.0f) && (LdotH * LdotN0 > 0.0f))) return 0.0f;
+    const float a2 = a * a;
+    const float VdotN = v3_dot(V, N), VdotN2 = VdotN * VdotN;
+    const float tanV2 = (1.0f - VdotN2) / VdotN2;
+    const float G1V = 2.0f / (1.0f + sqrtf(1.0f + a2 * tanV2));
+    const float LdotN = v3_dot(L, N), LdotN2 = LdotN * LdotN;
+    const float tanL2 = (1.0f - LdotN2) / LdotN2;
+    const float G1L = 2.0f / (1.0f + sqrtf(1.0f + a2 * tanL2));
+    return G1V * G1L;
+}
+/* SchlickConductorFresnel::evaluateReflectance (schlickConductorFresnel.cpp:28-41) */
+static v3 schlick(v3 f0, float cosI) {
+    const float c = 1.0f - fabsf(cosI);
+    const v3 f0c = v3_complement(f0);
+    return v3_add(f0, v3_scale(v3_scale(v3_scale(f0c, c), c * c), c * c));
+}
+
+static float mat_alpha(const or_material* m, v3 uvw) { return rough_to_alpha(tex_eval(&m->roughness, uvw).x); }
+
+/* LambertianDiffuse::evaluate (lambertianDiffuse.cpp:27-44), ConductorMicrofacet::evaluate
+ * (conductorMicrofacet.cpp:32-67) */
+static v3 bsdf_eval(const or_material* m, const or_si* si) {
+    const v3 Ns = si->Ns, V = si->wi, L = si->wo;
+    if (m->type == OR_MAT_LAMBERT) {
+        if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return v3_make(0, 0, 0);
+        return v3_scale(tex_eval(&m->albedo, si->uvw), OR_INV_PI);
+    }
+    const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns);
+    if (VdotN * LdotN <= 0.0f) return v3_make(0, 0, 0);
+    v3 H = v3_scale(v3_add(V, L), m_sign(VdotN));
+    if (v3_is_zero(H)) return v3_make(0, 0, 0);
+    H = v3_normalize(H);
+    const float a = mat_alpha(m, si->uvw);
+    const float LdotH = v3_dot(L, H);
+    const v3 F = schlick(v3_load(m->f0), LdotH);
+    const float D = ggx_D(a, Ns, H);
+    const float G = ggx_G(a, V, L, Ns, H);
+    return v3_divs(v3_scale(v3_scale(F, G), D), 4.0f * fabsf(VdotN * LdotN));
+}
+
+/* LambertianDiffuse::evaluatePdfW (lambertianDiffuse.cpp:83-96), ConductorMicrofacet::evaluatePdfW
+ * (conductorMicrofacet.cpp:123-156) */
+static float bsdf_pdf(const or_material* m, const or_si* si) {
+    const v3 Ns = si->Ns, V = si->wi, L = si->wo;
+    if (m->type == OR_MAT_LAMBERT) {
+        if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return 0.0f;
+        return v3_absdot(L, Ns) * OR_INV_PI;
+    }
+    const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns);
+    if (VdotN * LdotN <= 0.0f) return 0.0f;
+    v3 H = v3_scale(v3_add(V, L), m_sign(VdotN));
+    if (v3_is_zero(H)) return 0.0f;
+    H = v3_normalize(H);
+    const float a = mat_alpha(m, si->uvw);
+    const float D = ggx_D(a, Ns, H);
+    const float NdotH = v3_dot(Ns, H), LdotH = v3_dot(L, H);
+    const float pdfH = fabsf(D * NdotH);
+    const float pdfL = fabsf(pdfH / (4.0f * LdotH));
+    if (!isfinite(pdfL)) return 0.0f;
+    return pdfL;
+}
+
+/* LambertianDiffuse::evaluateSample (lambertianDiffuse.cpp:46-81) with
+ * Hemisphere::cosineWeightedSampling (hemisphere.cpp:30-48); ConductorMicrofacet::evaluateSample
+ * (conductorMicrofacet.cpp:69-121) with IsotropicGgx::sampleHalfVectorH (isotropicGgx.cpp:71-100).
+ * u0, u1 = sample.x, sample.y. */
+static or_bsdf_sample bsdf_sample(const or_material* m, const or_si* si, float u0, float u1) {
+    or_bsdf_sample out;
+    memset(&out, 0, sizeof(out));
+    const v3 Ns = si->Ns, V = si->wi;
+    v3 z, x;
+    frame(Ns, &z, &x);
+    if (m->type == OR_MAT_LAMBERT) {
+        const float phi = OR_TWO_PI * u0;
+        const float theta = acosf(1.0f - 2.0f * u1) * 0.5f;
+        const float cosT = cosf(theta);
+        const float sinT = sqrtf(1.0f - cosT * cosT);
+        const v3 l = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
+        const float pdf = cosT * OR_INV_PI;
+        v3 L = v3_normalize(to_world(x, Ns, z, l));
+        if (v3_dot(V, Ns) <= 0.0f) L = v3_neg(L);
+        out.f = v3_scale(tex_eval(&m->albedo, si->uvw), OR_INV_PI);
+        out.L = L;
+        out.pdf = pdf;
+    } else {
+        const float Nf = (v3_dot(V, Ns) > 0.0f) ? 1.0f : -1.0f;
+        const float a = mat_alpha(m, si->uvw);
+        const float s0 = m_clamp(u0, 0.0f, 0.9999f), s1 = m_clamp(u1, 0.0f, 0.9999f);
+        const float phi = OR_TWO_PI * s0;
+        const float theta = atanf(a * sqrtf(s1 / (1.0f - s1)));
+        v3 h;
+        if (!isfinite(theta)) h = v3_make(0.0f, 1.0f, 0.0f);
+        else {
+            const float cosT = cosf(theta);
+            const float sinT = sqrtf(1.0f - cosT * cosT);
+            h = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
+        }
+        const v3 H = v3_normalize(to_world(x, Ns, z, h));
+        const v3 L = v3_reflect(V, v3_scale(H, Nf));
+        const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns), LdotH = v3_dot(L, H), NdotH = v3_dot(Ns, H);
+        if (VdotN * LdotN <= 0.0f) return out;
+        const v3 F = schlick(v3_load(m->f0), LdotH);
+        const float D = ggx_D(a, Ns, H);
+        const float G = ggx_G(a, V, L, Ns, H);
+        const float pdfH = fabsf(D * NdotH);
+        const float pdfL = fabsf(pdfH / (4.0f * LdotH));
+        if (!isfinite(pdfL)) return out;
+        out.f = v3_divs(v3_scale(v3_scale(F, G), D), 4.0f * fabsf(VdotN * LdotN));
+        out.L = L;
+        out.pdf = pdfL;
+    }
+    /* BsdfSample::isValid (bsdfSample.cpp:39-41) */
+    out.valid = !v3_is_zero(out.f) && out.pdf > 0.0f;
+    return out;
+}
+
+void or_bsdf_query(const or_scene* s, int mi, const float* in, int64_t n, float* out) {
+    const or_material* m = &s->mats[mi];
+    for (int64_t i = 0; i < n; ++i) {
+        const float* q = in + 14 * i;
+        float* o = out + 12 * i;
+        or_si si;
+        memset(&si, 0, sizeof(si));
+        si.Ns = si.Ng = v3_load(q);
+        si.wi = v3_load(q + 3);
+        si.wo = v3_load(q + 6);
+        si.uvw = v3_load(q + 9);
+        v3_store(o, bsdf_eval(m, &si));
+        o[3] = bsdf_pdf(m, &si);
+        /* q[12] is drawn first and lands in sample.y (right-to-left evaluation) */
+        const or_bsdf_sample b = bsdf_sample(m, &si, q[13], q[12]);
+        v3_store(o + 4, b.f);
+        v3_store(o + 7, b.L);
+        o[10] = b.pdf;
+        o[11] = (float)b.valid;
+    }
+}
+
+/* ---- area lights ---------------------------------------------------------------- */
+
+/* AreaLight::_canEmit (areaLight.cpp:24-28) */
+static int can_emit(const or_light_state* L, v3 dir, v3 N) { return (v3_dot(dir, N) >= 0.0f) != (L->back_face != 0); }
+
+typedef struct { v3 pos, Ns, radiance; float pdfW; int valid; } or_light_smp;
+
+/* SingleAreaLight::evaluateDirectSample (singleAreaLight.cpp:54-85) with
+ * Triangle::evaluatePositionSample (triangle.cpp:162-193) + SampleWarp::uniformTriangleUv
+ * (sampleWarp.cpp:10-19), or Rectangle::evaluatePositionSample (rectangle.cpp:134-169).
+ * The rectangle's rejection loop draws (s, t) pairs until t <= short/long; in the
+ * Philox layout the accepted t is drawn directly as u * short/long (the same
+ * distribution, uniform on [0, short/long)). */
+static or_light_smp light_sample(const or_scene* s, int li, v3 target, or_rng* rng, int bounce) {
+    or_light_smp out;
+    memset(&out, 0, sizeof(out));
+    const or_light_state* L = &s->lights[li];
+    const or_prim* p = &s->prims[L->prim];
+    v3 P, Ns;
+    float pdfA;
+    if (p->type == OR_PRIM_RECTANGLE) {
+        v3 eA = p->e1, eB = p->e2;
+        if (v3_len(eA) < v3_len(eB)) { const v3 t = eA; eA = eB; eB = t; }
+        const float lw = v3_len(eA), sw = v3_len(eB);
+        float su, tu;
+        if (rng->mode == OR_RNG_SEQ) {
+            do {
+                su = rng_u(rng, bounce, SL_LPOS0);
+                tu = rng_u(rng, bounce, SL_LPOS1);
+            } while (tu > sw / lw);
+        } else {
+            su = rng_u(rng, bounce, SL_LPOS0);
+            tu = rng_u(rng, bounce, SL_LPOS1) * (sw / lw);
+        }
+        P = v3_add(v3_add(p->vB, v3_scale(eA, su)), v3_scale(v3_normalize(eB), tu * v3_len(eA)));
+        v3 N = v3_cross(p->e1, p->e2);
+        N = v3_is_zero(N) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(N);
+        Ns = N;
+        pdfA = 1.0f / prim_area(p);
+    } else {
+        float u0, u1;
+        rng_u2(rng, bounce, SL_LPOS0, SL_LPOS1, &u0, &u1);
+        const float sqrtA = sqrtf(u0);
+        const float st0 = 1.0f - sqrtA, st1 = sqrtA * u1;
+        P = v3_add(v3_add(p->vA, v3_scale(p->e1, st0)), v3_scale(p->e2, st1));
+        const v3 b = barycentric(p, P);
+        v3 Ng = v3_cross(p->e1, p->e2);
+        Ng = v3_is_zero(Ng) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(Ng);
+        v3 n = bary_mix(b, p->nA, p->nB, p->nC);
+        Ns = v3_is_zero(n) ? Ng : n; /* not renormalized (triangle.cpp:182-183) */
+        pdfA = 1.0f / prim_area(p);
+    }
+    if (!(pdfA > 0.0f)) return out;
+    const v3 emitVec = v3_sub(target, P);
+    if (!can_emit(L, emitVec, Ns)) return out;
+    const float d2 = v3_len2(emitVec);
+    const float cosL = v3_absdot(v3_divs(emitVec, sqrtf(d2)), Ns);
+    if (cosL <= 0.0f) return out;
+    out.pos = P;
+    out.Ns = Ns;
+    out.radiance = L->radiance;
+    out.pdfW = pdfA * d2 / cosL;
+    /* DirectLightSample::isValid (directLightSample.cpp:57-59) */
+    out.valid = !v3_is_zero(out.radiance) && out.pdfW > 0.0f;
+    return out;
+}
+
+/* SingleAreaLight::emittance (singleAreaLight.cpp:39-52) */
+static v3 light_emittance(const or_scene* s, int li, const or_si* si) {
+    const or_light_state* L = &s->lights[li];
+    if (!can_emit(L, si->wi, si->Ns)) return v3_make(0, 0, 0);
+    return L->radiance;
+}
+/* SingleAreaLight::evaluateDirectPdfW (singleAreaLight.cpp:87-109) */
+static float light_pdfW(const or_scene* s, int li, const or_si* si, v3 target) {
+    const or_light_state* L = &s->lights[li];
+    const v3 emitVec = v3_sub(target, si->P);
+    if (!can_emit(L, si->wi, si->Ns)) return 0.0f;
+    const float c = v3_absdot(si->wi, si->Ns);
+    if (c <= 0.0f) return 0.0f;
+    const float pdfA = 1.0f / prim_area(&s->prims[L->prim]);
+    return pdfA * v3_len2(emitVec) / c;
+}
+
+void or_light_radiance(const or_scene* s, float* out) {
+    for (int32_t i = 0; i < s->nlights; ++i) v3_store(out + 3 * i, s->lights[i].radiance);
+}
+
+void or_light_sample(const or_scene* s, int li, const float* in, int64_t n, int nu, float* out) {
+    /* Explicit uniforms in the reference's draw order: a triangle draws
+     * (y, x) (right-to-left argument evaluation), a rectangle draws (s, t)
+     * pairs until t <= short/long. */
+    const or_light_state* L = &s->lights[li];
+    const or_prim* p = &s->prims[L->prim];
+    for (int64_t i = 0; i < n; ++i) {
+        const float* q = in + (3 + nu) * i;
+        const float* vals = q + 3;
+        float* o = out + 11 * i;
+        or_light_smp r;
+        memset(&r, 0, sizeof(r));
+        v3 P, Ns;
+        if (p->type == OR_PRIM_RECTANGLE) {
+            v3 eA = p->e1, eB = p->e2;
+            if (v3_len(eA) < v3_len(eB)) { const v3 t = eA; eA = eB; eB = t; }
+            const float lw = v3_len(eA), sw = v3_len(eB);
+            float su, tu;
+            int k = 0;
+            do {
+                su = vals[k];
+                tu = vals[k + 1];
+                k += 2;
+            } while (tu > sw / lw && k + 1 < nu);
+            P = v3_add(v3_add(p->vB, v3_scale(eA, su)), v3_scale(v3_normalize(eB), tu * v3_len(eA)));
+            v3 N = v3_cross(p->e1, p->e2);
+            Ns = v3_is_zero(N) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(N);
+        } else {
+            const float u1 = vals[0], u0 = vals[1];
+            const float sqrtA = sqrtf(u0);
+            P = v3_add(v3_add(p->vA, v3_scale(p->e1, 1.0f - sqrtA)), v3_scale(p->e2, sqrtA * u1));
+            const v3 b = barycentric(p, P);
+            v3 Ng = v3_cross(p->e1, p->e2);
+            Ng = v3_is_zero(Ng) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(Ng);
+            const v3 nn = bary_mix(b, p->nA, p->nB, p->nC);
+            Ns = v3_is_zero(nn) ? Ng : nn;
+        }
+        const float pdfA = 1.0f / prim_area(p);
+        const v3 emitVec = v3_sub(v3_load(q), P);
+        if (pdfA > 0.0f && can_emit(L, emitVec, Ns)) {
+            const float d2 = v3_len2(emitVec);
+            const float cosL = v3_absdot(v3_divs(emitVec, sqrtf(d2)), Ns);
+            if (cosL > 0.0f) {
+                r.pos = P; r.Ns = Ns; r.radiance = L->radiance; r.pdfW = pdfA * d2 / cosL;
+                r.valid = !v3_is_zero(r.radiance) && r.pdfW > 0.0f;
+            }
+        }
+        v3_store(o, r.pos);
+        v3_store(o + 3, r.Ns);
+        v3_store(o + 6, r.radiance);
+        o[9] = r.pdfW;
+        o[10] = (float)r.valid;
+    }
+}
+
+void or_light_hit(const or_scene* s, const float* rays, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        float* o = out + 5 * i;
+        o[0] = -1.0f; o[1] = o[2] = o[3] = o[4] = 0.0f;
+        or_si si;
+        if (!scene_intersect(s, v3_load(r), v3_load(r + 4), r[3], r[7], &si, NULL)) continue;
+        const int li = s->prims[si.prim].light;
+        if (li < 0) continue;
+        o[0] = (float)li;
+        v3_store(o + 1, light_emittance(s, li, &si));
+        o[4] = light_pdfW(s, li, &si, v3_load(r));
+    }
+}
+
+/* ---- camera ------------------------------------------------------------------------ */
+
+/* PerspectivePinholeCamera::spawnPrimaryRay (perspectivePinholeCamera.cpp:52-73),
+ * TMatrix4::transformPoint / transformVector (tMatrix4.ipp:180-195), Ray::setDirection (ray.cpp:63-67) */
+static v3 camera_dir(const or_scene* s, float fx, float fy) {
+    const float* m = s->film_to_camera;
+    const float x = fx, y = fy, z = 0.0f;
+    const v3 pc = v3_make(m[0] * x + m[1] * y + m[2] * z + m[3], m[4] * x + m[5] * y + m[6] * z + m[7],
+                          m[8] * x + m[9] * y + m[10] * z + m[11]);
+    const float* c = s->camera_to_world;
+    const v3 d = v3_make(c[0] * pc.x + c[1] * pc.y + c[2] * pc.z, c[4] * pc.x + c[5] * pc.y + c[6] * pc.z,
+                         c[8] * pc.x + c[9] * pc.y + c[10] * pc.z);
+    return v3_normalize(d);
+}
+
+void or_camera_matrices(const or_scene* s, float* f2c, float* c2w) {
+    memcpy(f2c, s->film_to_camera, sizeof(s->film_to_camera));
+    memcpy(c2w, s->camera_to_world, sizeof(s->camera_to_world));
+}
+
+void or_camera_rays(const or_scene* s, const float* film_xy, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        v3_store(out + 6 * i, s->cam_pos);
+        v3_store(out + 6 * i + 3, camera_dir(s, film_xy[2 * i], film_xy[2 * i + 1]));
+    }
+}
+
+void or_normalize(const float* v, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) v3_store(out + 3 * i, v3_normalize(v3_load(v + 3 * i)));
+}
+
+/* ---- estimator --------------------------------------------------------------------- */
+
+/* TMis<POWER>::weight (tMis.h:21-47) */
+static float mis_power(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 / (a2 + b2);
+}
+
+/* UniformLightCluster::sampleOneLight (uniformLightCluster.cpp:15-24) with Random::nextIndex
+ * (random.cpp:21-26); PowerLightCluster::sampleOneLight (powerLightCluster.cpp:27-34) with
+ * Distribution1D::sampleDiscrete / continuousToDiscrete (distribution1D.cpp:89-125) */
+static int pick_light(const or_scene* s, or_rng* rng, int bounce, float* pdf) {
+    const float u = rng_u(rng, bounce, SL_PICK);
+    const size_t n = (size_t)s->nlights;
+    if (s->desc.cluster == OR_CLUSTER_POWER) {
+        size_t idx = 0;
+        for (size_t i = n - 1 + 1; i-- > 0;) {
+            if (s->power_cdf[i] <= u) { idx = i; break; }
+        }
+        *pdf = s->power_cdf[idx + 1] - s->power_cdf[idx];
+        return (int)idx;
+    }
+    size_t idx = (size_t)(0.0f + u * (float)n);
+    idx = idx < n - 1 ? idx : n - 1;
+    *pdf = 1.0f / (float)n;
+    return (int)idx;
+}
+
+typedef struct {
+    int64_t closest, anyhit;
+} or_counts;
+
+/* DirectLightEvaluator::evaluate (directLightEvaluator.cpp:18-109) */
+static v3 direct_light(const or_scene* s, const or_si* si0, const or_material* m, int li, or_rng* rng, int bounce,
+                       or_counts* cnt) {
+    or_si si = *si0;
+    v3 Ld = v3_make(0, 0, 0);
+    const v3 P = si.P, Ns = si.Ns;
+    {
+        const or_light_smp ls = light_sample(s, li, P, rng, bounce);
+        if (ls.valid) {
+            const v3 LV = v3_sub(ls.pos, P);
+            const float dist = v3_len(LV);
+            const v3 L = v3_divs(LV, dist);
+            si.wo = L;
+            const v3 sd = v3_normalize(L); /* Ray(P, L) normalizes again (ray.cpp:19-27) */
+            cnt->anyhit++;
+            if (!or_occluded(s, P, sd, OR_RAY_EPS, dist - OR_RAY_EPS)) {
+                const v3 f = bsdf_eval(m, &si);
+                const v3 factor = v3_scale(f, v3_absdot(L, Ns));
+                const float bpdf = bsdf_pdf(m, &si);
+                const float w = mis_power(ls.pdfW, bpdf);
+                Ld = v3_add(Ld, v3_divs(v3_mul(v3_scale(ls.radiance, w), factor), ls.pdfW));
+            }
+        }
+    }
+    {
+        float u0, u1;
+        rng_u2(rng, bounce, SL_MIS0, SL_MIS1, &u0, &u1);
+        const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+        if (bs.valid) {
+            const v3 d = v3_normalize(bs.L);
+            or_si hit;
+            cnt->closest++;
+            if (scene_intersect(s, P, d, OR_RAY_EPS, FLT_MAX, &hit, NULL)) {
+                if (s->prims[hit.prim].light == li) {
+                    const v3 factor = v3_scale(bs.f, v3_absdot(bs.L, Ns));
+                    const v3 rad = light_emittance(s, li, &hit);
+                    const float lpdf = light_pdfW(s, li, &hit, P);
+                    const float w = mis_power(bs.pdf, lpdf);
+                    Ld = v3_add(Ld, v3_divs(v3_mul(v3_scale(rad, w), factor), bs.pdf));
+                }
+            }
+        }
+    }
+    return Ld;
+}
+
+/* PtEstimator::estimate (ptEstimator.cpp:25-123) with RussianRoulette::isSurvivedOnNextRound
+ * (russianRoulette.cpp:9-28). Both materials here are non-specular, so every vertex does
+ * next-event estimation and emission is counted only at bounce 0. */
+static v3 estimate(const or_scene* s, v3 o, v3 d, or_rng* rng, or_counts* cnt) {
+    v3 L = v3_make(0, 0, 0), beta = v3_make(1, 1, 1);
+    int countEmit = 1;
+    v3 ro = o, rd = d;
+    for (int bounce = 0; bounce < s->desc.max_depth; ++bounce) {
+        or_si si;
+        cnt->closest++;
+        if (!scene_intersect(s, ro, rd, OR_RAY_EPS, FLT_MAX, &si, NULL)) break;
+        const or_prim* p = &s->prims[si.prim];
+        const or_material* m = &s->mats[p->mat];
+        if (p->light >= 0 && countEmit) L = v3_add(L, v3_mul(beta, light_emittance(s, p->light, &si)));
+        countEmit = 0;
+        float pickPdf;
+        const int li = pick_light(s, rng, bounce, &pickPdf);
+        const v3 Ld = v3_divs(direct_light(s, &si, m, li, rng, bounce, cnt), pickPdf);
+        L = v3_add(L, v3_mul(beta, Ld));
+        float u0, u1;
+        rng_u2(rng, bounce, SL_CONT0, SL_CONT1, &u0, &u1);
+        const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+        if (!bs.valid) break;
+        const float LdotN = v3_absdot(bs.L, si.Ns);
+        beta = v3_mul(beta, v3_divs(v3_scale(bs.f, LdotN), bs.pdf));
+        if (bounce > 2) {
+            const float q = m_clamp(1.0f - v3_max_comp(beta), 0.05f, 1.0f);
+            const float u = rng_u(rng, bounce, SL_RR);
+            if (u > q) beta = v3_scale(beta, 1.0f / (1.0f - q));
+            else break;
+        }
+        if (v3_is_zero(beta)) break;
+        ro = si.P;
+        rd = v3_normalize(bs.L);
+    }
+    return L;
+}
+
+static void rng_init(or_rng* r, int mode, uint64_t seed, uint64_t key, uint32_t pixel, uint32_t sample) {
+    memset(r, 0, sizeof(*r));
+    r->mode = mode;
+    r->state = key;
+    r->key[0] = (uint32_t)seed;
+    r->key[1] = (uint32_t)(seed >> 32);
+    r->pixel = pixel;
+    r->sample = sample;
+}
+
+void or_estimate(const or_scene* s, int rng_mode, uint64_t seed, const float* film_xy, const uint64_t* keys,
+                 int64_t n, float* out_rgb, int64_t* out_draws) {
+    for (int64_t i = 0; i < n; ++i) {
+        or_rng rng;
+        rng_init(&rng, rng_mode, seed, keys[i], (uint32_t)(keys[i] >> 32), (uint32_t)keys[i]);
+        or_counts cnt = {0, 0};
+        const v3 d = camera_dir(s, film_xy[2 * i], film_xy[2 * i + 1]);
+        v3_store(out_rgb + 3 * i, estimate(s, s->cam_pos, d, &rng, &cnt));
+        if (out_draws) out_draws[i] = rng.draws;
+    }
+}
+
+/* ---- film ------------------------------------------------------------------------------ */
+
+/* filter evaluate: BoxFilter (boxFilter.cpp:9-11), GaussianFilter (gaussianFilter.cpp:9-33),
+ * ConeFilter (coneFilter.cpp:13-16), MitchellFilter (mitchellFilter.cpp:16-40) */
+static float mitchell1d(float b, float c, float d) {
+    const float ad = fabsf(d);
+    const float den = 1.0f / 6.0f;
+    if (ad < 1.0f)
+        return ((12.0f - 9.0f * b - 6.0f * c) * ad * ad * ad + (-18.0f + 12.0f * b + 6.0f * c) * ad * ad +
+                (6.0f - 2.0f * b)) * den;
+    if (ad < 2.0f)
+        return ((-b - 6.0f * c) * ad * ad * ad + (6.0f * b + 30.0f * c) * ad * ad + (-12.0f * b - 48.0f * c) * ad +
+                (8.0f * b + 24.0f * c)) * den;
+    return 0.0f;
+}
+static float filter_eval(const or_scene_desc* d, float x, float y) {
+    const float hx = d->filter_w / 2.0f, hy = d->filter_h / 2.0f;
+    switch (d->filter) {
+        case OR_FILTER_GAUSSIAN: {
+            const float ex = -1.0f / (2.0f * d->sigma_x * d->sigma_x);
+            const float ey = -1.0f / (2.0f * d->sigma_y * d->sigma_y);
+            const float edge = d->amplitude * expf(ex * hx * hx + ey * hy * hy);
+            const float v = d->amplitude * expf(ex * x * x + ey * y * y);
+            return m_max(0.0f, v - edge);
+        }
+        case OR_FILTER_CONE: return m_max(hx - fabsf(x), 0.0f) * m_max(hy - fabsf(y), 0.0f);
+        case OR_FILTER_MITCHELL: return mitchell1d(d->mitchell_b, d->mitchell_c, x) * mitchell1d(d->mitchell_b, d->mitchell_c, y);
+        default: return 1.0f;
+    }
+}
+
+/* FilmTile::addSample (filmTile.cpp:21-66): NaN/inf samples dropped, footprint clamped to the tile */
+static void film_add(const or_scene_desc* d, float* film, int tx0, int ty0, int tx1, int ty1, float fx, float fy,
+                     v3 rgb) {
+    if (v3_has_nan(rgb) || v3_has_inf(rgb)) return;
+    const float hx = d->filter_w / 2.0f, hy = d->filter_h / 2.0f;
+    float mnx = fx - hx, mny = fy - hy, mxx = fx + hx, mxy = fy + hy;
+    mnx = std_max(mnx, (float)tx0); mny = std_max(mny, (float)ty0);
+    mxx = std_min(mxx, (float)tx1); mxy = std_min(mxy, (float)ty1);
+    const int x0 = (int)ceilf(mnx - 0.5f), y0 = (int)ceilf(mny - 0.5f);
+    const int x1 = (int)(floorf(mxx - 0.5f) + 1), y1 = (int)(floorf(mxy - 0.5f) + 1);
+    for (int iy = y0; iy < y1; ++iy)
+        for (int ix = x0; ix < x1; ++ix) {
+            const float x = (float)ix - (fx - 0.5f);
+            const float y = (float)iy - (fy - 0.5f);
+            const float w = filter_eval(d, x, y);
+            float* px = film + 4 * ((size_t)iy * (size_t)d->width + (size_t)ix);
+            px[0] += rgb.x * w; px[1] += rgb.y * w; px[2] += rgb.z * w; px[3] += w;
+        }
+}
+
+/* Kensler's hashed permutation of [0, l) (A. Kensler, "Correlated Multi-Jittered
+ * Sampling", Pixar Technical Memo 13-01, 2013): the GPU tracer's per-pixel order of
+ * strata, standing in for SampleRecord2D::shuffle's Fisher-Yates draw. */
+static uint32_t kensler_permute(uint32_t i, uint32_t l, uint32_t p) {
+    uint32_t w = l - 1;
+    w |= w >> 1; w |= w >> 2; w |= w >> 4; w |= w >> 8; w |= w >> 16;
+    do {
+        i ^= p; i *= 0xe170893du;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8; i *= 0x0929eb3fu;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1; i *= 1u | p >> 27;
+        i *= 0x6935fa69u;
+        i ^= (i & w) >> 11; i *= 0x74dcb303u;
+        i ^= (i & w) >> 2; i *= 0x9e501cc3u;
+        i ^= (i & w) >> 2; i *= 0xc860a3dfu;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+
+/* Film position of sample k of pixel (ix, iy) in the Philox layout. */
+static void philox_film_pos(const or_scene* s, uint64_t seed, int ix, int iy, int k, float* fx, float* fy) {
+    const uint32_t pixel = (uint32_t)((size_t)iy * (size_t)s->desc.width + (size_t)ix);
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t c1[4] = {pixel, (uint32_t)k, 0xFFFFFFFFu, 0u};
+    uint32_t r1[4];
+    or_philox(c1, key, r1);
+    const float u0 = u32_to_unit(r1[0]), u1 = u32_to_unit(r1[1]);
+    float jx = u0, jy = u1;
+    if (s->desc.sampler == OR_SAMPLER_STRATIFIED) {
+        const uint32_t c2[4] = {pixel, 0u, 0xFFFFFFFEu, 0u};
+        uint32_t r2[4];
+        or_philox(c2, key, r2);
+        const uint32_t base = (uint32_t)s->strata_base;
+        const uint32_t j = kensler_permute((uint32_t)k, (uint32_t)s->desc.spp, r2[0]);
+        if (j < base * base) {
+            const float dx = 1.0f / (float)base;
+            jx = ((float)(j % base) + u0) * dx;
+            jy = ((float)(j / base) + u1) * dx;
+        }
+    }
+    *fx = (float)ix + jx;
+    *fy = (float)iy + jy;
+}
+
+typedef struct {
+    const or_scene* s;
+    int      mode;
+    uint64_t seed;
+    int      spp_begin, spp_end;
+    float*   film;
+    int      ntiles;
+    volatile int next;
+    pthread_mutex_t lock;
+} render_job;
+
+/* One tile: EstimatorTileWork::work (estimatorTileWork.cpp:32-61) with the stratified
+ * sampler (stratifiedSampler.cpp:38-73, sampleRecord2D.cpp:16-36) or the random one
+ * (randomSampler.cpp:27-38). */
+static void render_tile(render_job* J, int t) {
+    const or_scene* s = J->s;
+    const or_scene_desc* d = &s->desc;
+    const int ntx = (d->width + d->tile_x - 1) / d->tile_x;
+    const int x0 = (t % ntx) * d->tile_x, y0 = (t / ntx) * d->tile_y;
+    const int x1 = x0 + d->tile_x < d->width ? x0 + d->tile_x : d->width;
+    const int y1 = y0 + d->tile_y < d->height ? y0 + d->tile_y : d->height;
+    const int spp = d->spp;
+    float* jit = (float*)malloc(sizeof(float) * 2 * (size_t)(spp > 0 ? spp : 1));
+    for (int iy = y0; iy < y1; ++iy)
+        for (int ix = x0; ix < x1; ++ix) {
+            const uint64_t pixel = (uint64_t)iy * (uint64_t)d->width + (uint64_t)ix;
+            if (J->mode == OR_RNG_SEQ) {
+                uint64_t st = seqrng_key(J->seed, pixel, 1ull << 40);
+                if (d->sampler == OR_SAMPLER_STRATIFIED) {
+                    const int base = s->strata_base;
+                    const float dx = 1.0f / (float)base;
+                    int idx = 0;
+                    for (int sy = 0; sy < base; ++sy)
+                        for (int sx = 0; sx < base; ++sx) {
+                            const float jx = seqrng_next(&st), jy = seqrng_next(&st);
+                            jit[2 * idx] = ((float)(size_t)sx + jx) * dx;
+                            jit[2 * idx + 1] = ((float)(size_t)sy + jy) * dx;
+                            idx++;
+                        }
+                    for (; idx < spp; ++idx) {
+                        jit[2 * idx] = seqrng_next(&st);
+                        jit[2 * idx + 1] = seqrng_next(&st);
+                    }
+                    for (int i = 0; i < spp; ++i) {
+                        /* Random::nextIndex(i, spp) */
+                        const size_t interval = (size_t)(spp - i);
+                        size_t r = (size_t)((float)(size_t)i + seqrng_next(&st) * (float)interval);
+                        r = r < (size_t)i ? (size_t)i : (r > (size_t)spp - 1 ? (size_t)spp - 1 : r);
+                        const float a0 = jit[2 * i], a1 = jit[2 * i + 1];
+                        jit[2 * i] = jit[2 * r]; jit[2 * i + 1] = jit[2 * r + 1];
+                        jit[2 * r] = a0; jit[2 * r + 1] = a1;
+                    }
+                } else {
+                    for (int i = 0; i < spp; ++i) {
+                        jit[2 * i] = seqrng_next(&st);
+                        jit[2 * i + 1] = seqrng_next(&st);
+                    }
+                }
+            }
+            for (int k = J->spp_begin; k < J->spp_end; ++k) {
+                float fx, fy;
+                if (J->mode == OR_RNG_SEQ) {
+                    fx = (float)ix + jit[2 * k];
+                    fy = (float)iy + jit[2 * k + 1];
+                } else {
+                    philox_film_pos(s, J->seed, ix, iy, k, &fx, &fy);
+                }
+                or_rng rng;
+                rng_init(&rng, J->mode, J->seed, seqrng_key(J->seed, pixel, (uint64_t)k), (uint32_t)pixel,
+                         (uint32_t)k);
+                or_counts cnt = {0, 0};
+                const v3 rgb = estimate(s, s->cam_pos, camera_dir(s, fx, fy), &rng, &cnt);
+                film_add(d, J->film, x0, y0, x1, y1, fx, fy, rgb);
+            }
+        }
+    free(jit);
+}
+
+static void* render_worker(void* arg) {
+    render_job* J = (render_job*)arg;
+    while (1) {
+        pthread_mutex_lock(&J->lock);
+        const int t = J->next++;
+        pthread_mutex_unlock(&J->lock);
+        if (t >= J->ntiles) break;
+        render_tile(J, t);
+    }
+    return NULL;
+}
+
+void or_render_linear(const or_scene* s, int rng_mode, uint64_t seed, int spp_begin, int spp_end, int nthreads,
+                      float* film) {
+    const or_scene_desc* d = &s->desc;
+    render_job J;
+    memset(&J, 0, sizeof(J));
+    J.s = s; J.mode = rng_mode; J.seed = seed; J.film = film;
+    J.spp_begin = spp_begin; J.spp_end = spp_end;
+    J.ntiles = ((d->width + d->tile_x - 1) / d->tile_x) * ((d->height + d->tile_y - 1) / d->tile_y);
+    pthread_mutex_init(&J.lock, NULL);
+    memset(film, 0, sizeof(float) * 4 * (size_t)d->width * (size_t)d->height);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, render_worker, &J);
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&J.lock);
+}
+
+/* ---- CPU baseline timing -------------------------------------------------------------- */
+
+typedef struct {
+    const or_scene* s;
+    int64_t begin, end;
+    int     spp;
+    uint64_t seed;
+    int64_t rays[2], samples;
+} time_job;
+
+static void* time_worker(void* arg) {
+    time_job* T = (time_job*)arg;
+    const or_scene* s = T->s;
+    for (int64_t p = T->begin; p < T->end; ++p) {
+        const int ix = (int)(p % s->desc.width), iy = (int)(p / s->desc.width);
+        for (int k = 0; k < T->spp; ++k) {
+            float fx, fy;
+            philox_film_pos(s, T->seed, ix, iy, k, &fx, &fy);
+            or_rng rng;
+            rng_init(&rng, OR_RNG_PHILOX, T->seed, 0, (uint32_t)p, (uint32_t)k);
+            or_counts cnt = {0, 0};
+            (void)estimate(s, s->cam_pos, camera_dir(s, fx, fy), &rng, &cnt);
+            T->rays[0] += cnt.closest;
+            T->rays[1] += cnt.anyhit;
+            T->samples += 1;
+        }
+    }
+    return NULL;
+}
+
+double or_time_render(const or_scene* s, int64_t begin, int64_t end, int spp, int nthreads, uint64_t seed,
+                      int64_t* out_rays, int64_t* out_samples) {
+    if (nthreads < 1) nthreads = 1;
+    time_job* jobs = (time_job*)calloc((size_t)nthreads, sizeof(time_job));
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    const int64_t chunk = (end - begin + nthreads - 1) / nthreads;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < nthreads; ++i) {
+        jobs[i].s = s; jobs[i].spp = spp; jobs[i].seed = seed;
+        jobs[i].begin = begin + i * chunk;
+        jobs[i].end = jobs[i].begin + chunk < end ? jobs[i].begin + chunk : end;
+        pthread_create(&th[i], NULL, time_worker, &jobs[i]);
+    }
+    out_rays[0] = out_rays[1] = 0;
+    *out_samples = 0;
+    for (int i = 0; i < nthreads; ++i) {
+        pthread_join(th[i], NULL);
+        out_rays[0] += jobs[i].rays[0];
+        out_rays[1] += jobs[i].rays[1];
+        *out_samples += jobs[i].samples;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(jobs); free(th);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* Film::save (film.cpp:159-196) with math::forward_gamma_correction (math.cpp:92-99)
+ * and ImageUtils::hdrToLdr (imageUtils.cpp:23-36). */
+static float gamma_fwd(float v) {
+    if (v <= 0.0031308f) return 12.92f * v;
+    return 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+}
+void or_film_to_srgb8(const float* film, int64_t npix, uint8_t* out) {
+    for (int64_t i = 0; i < npix; ++i) {
+        const float* p = film + 4 * i;
+        const float invW = (p[3] == 0.0f) ? 0.0f : 1.0f / p[3];
+        const float splat = 0.0f;
+        for (int c = 0; c < 3; ++c) {
+            const float lin = p[c] * invW + splat;
+            const float srgb = gamma_fwd(lin);
+            out[3 * i + c] = (uint8_t)m_clamp(srgb * 255.0f + 0.5f, 0.0f, 255.0f);
+        }
+    }
+}
