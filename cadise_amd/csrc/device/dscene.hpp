@@ -1,0 +1,79 @@
+// Device-resident scene layout (HBM). All arrays are owned by DeviceScene
+// (render.hip); kernels receive this POD by value.
+//
+//   nodes     wide BVH, WideNode<W> (128 B for W = 4, 256 B for W = 8), root at 0
+//   tris      3 x float4 per primitive in BVH leaf order:
+//               triangle  (vA, id) (eAB, 0) (eAC, 0)          as Triangle stores them
+//               rectangle (vB, id|1<<31) (eA, 0) (eB, 0)       as Rectangle stores them
+//             id = creation index; bit 31 of the first w marks a rectangle
+//   prim_info per primitive (creation order): material, light, has-normals, has-uvw
+//   prim_n / prim_uvw  9 floats per primitive (creation order), when present
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mpt {
+
+struct DevMaterial {
+    int32_t type;        // 0 lambert, 1 ggx
+    int32_t albedo_tex;  // -1 constant
+    int32_t rough_tex;   // -1 constant
+    int32_t pad;
+    float   albedo[3];
+    float   roughness;
+    float   f0[3];
+    float   pad2;
+};
+
+struct DevTexture {
+    int64_t offset;      // into tex_data
+    int32_t width, height, channels, wrap_clamp;
+};
+
+// An area light with the geometry of its primitive (lights are few, so the
+// light-sampling path reads this instead of the primitive arrays).
+struct DevLight {
+    int32_t prim;        // creation index
+    int32_t type;        // primitive type
+    int32_t back_face;
+    int32_t has_normals;
+    float   radiance[3];
+    float   pdfA;        // 1 / area, as Primitive::evaluatePositionPdfA computes it
+    float   a[3];        // triangle vA | rectangle vB
+    float   e1[3];       // eAB | eA
+    float   e2[3];       // eAC | eB
+    float   n[9];        // triangle vertex normals nA nB nC (face normal unless set)
+};
+
+struct DevScene {
+    const void*    nodes;
+    int32_t        width;       // BVH width (4 or 8)
+    int32_t        max_stack;   // stack entries a traversal can need
+    int64_t        nnodes;
+    int64_t        nprims;
+    const float4*  tris;
+    const int4*    prim_info;
+    const float*   prim_n;
+    const float*   prim_uvw;
+    const DevMaterial* mats;
+    const DevTexture*  texs;
+    const float*   tex_data;
+    const DevLight* lights;
+    const float*   power_cdf;   // nlights + 1
+    int32_t        nlights;
+    int32_t        cluster;     // 0 uniform, 1 power
+    // camera
+    float f2c[12];              // film -> camera rows 0..2 (row-major)
+    float c2w[12];              // camera -> world rows 0..2
+    float cam_pos[3];
+    // film
+    int32_t img_w, img_h, tile_x, tile_y;
+    int32_t filter;
+    float   filter_hw, filter_hh;   // half width / height
+    float   g_expx, g_expy, g_edge, amplitude, mitchell_b, mitchell_c;
+    int32_t sampler, spp, strata;
+};
+
+}  // namespace mpt

@@ -1,0 +1,441 @@
+// gfx950 kernels of the wavefront path tracer:
+//   k_trace     persistent wide-BVH traversal (closest or any hit)
+//   k_camera    primary rays of one sample pass
+//   k_shade     resolve the previous vertex's NEE, shade the new hit, emit the
+//               next vertex's shadow / MIS-BSDF / continuation rays
+//   k_film      per-tile filter splat of one pass into the linear film
+// The bounce loop of PtEstimator::estimate (reference core/estimator/
+// ptEstimator.cpp:25-123) is unrolled across launches: each launch moves every
+// live path by one step, and live paths are compacted by wave ballots.
+#include "kernels.hpp"
+#include "shade.hpp"
+#include "traverse.hpp"
+
+namespace mpt {
+
+constexpr int WAVE = 64;
+
+// Philox slots within one bounce b: counter (pixel, sample, 2b + slot / 4, 0).
+enum { SL_PICK = 0, SL_LPOS0 = 1, SL_LPOS1 = 2, SL_RR = 3, SL_MIS0 = 4, SL_MIS1 = 5, SL_CONT0 = 6, SL_CONT1 = 7 };
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// ---- traversal -------------------------------------------------------------------
+
+template <int W, int S, bool ANY>
+__global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
+    extern __shared__ int32_t lds_stack[];
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int64_t gslot = static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
+    LaneStack<S> st;
+    st.lds = lds_stack;
+    st.lane = tid;
+    st.stride = blockDim.x;
+    st.spill = a.spill ? a.spill + gslot * (a.sc.max_stack - S) : nullptr;
+    st.sp = 0;
+    const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
+    const int64_t n = a.dual ? 2 * n0 : n0;
+    TravCounters cnt{0, 0};
+    unsigned long long rays = 0, errors = 0;
+    while (true) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(a.work, WAVE);
+        base = __shfl(base, 0, WAVE);
+        if (base >= n) break;
+        const int64_t i = base + lane;
+        if (i >= n) continue;
+        const bool second = i >= n0;
+        const int64_t r = second ? i - n0 : i;
+        const float4 o4 = a.org[r * a.stride];
+        const float4 d4 = second ? a.dir2[r * a.stride] : a.dir[r * a.stride];
+        int2 out;
+        if (d4.w < 0.0f || a.sc.nnodes == 0) {
+            out = make_int2(ANY ? 0 : -1, __float_as_int(d4.w));
+        } else {
+            rays += 1;
+            float tmax = d4.w;
+            const int32_t h = traverse<W, S, ANY>(a.sc, st, v3f4(o4), v3f4(d4), o4.w, tmax, cnt);
+            if (h == -2) errors += 1;
+            if (ANY) {
+                out = make_int2(h >= 0 ? 1 : 0, 0);
+            } else {
+                int32_t idx = h;
+                if (h >= 0 && a.want_prim)
+                    idx = static_cast<int32_t>(__float_as_uint(a.sc.tris[3 * h].w) & 0x7FFFFFFFu);
+                out = make_int2(h >= 0 ? idx : -1, __float_as_int(tmax));
+            }
+        }
+        if (second) a.hit2[r] = out;
+        else a.hit[r] = out;
+    }
+    // one atomic per wave for the counters
+    const unsigned long long sr = wave_sum(rays), sb = wave_sum(cnt.boxes), sp = wave_sum(cnt.prims),
+                             se = wave_sum(errors);
+    if (lane == 0 && a.stats) {
+        atomicAdd(&a.stats[a.stat_slot], sr);
+        atomicAdd(&a.stats[ST_BOXES], sb);
+        atomicAdd(&a.stats[ST_PRIMS], sp);
+        if (se) atomicAdd(&a.stats[ST_ERRORS], se);
+    }
+}
+
+template __global__ void k_trace<4, 24, false>(TraceArgs);
+template __global__ void k_trace<4, 24, true>(TraceArgs);
+template __global__ void k_trace<8, 24, false>(TraceArgs);
+template __global__ void k_trace<8, 24, true>(TraceArgs);
+
+// ---- camera ----------------------------------------------------------------------
+
+// Kensler's hashed permutation of [0, l) (A. Kensler, "Correlated
+// Multi-Jittered Sampling", Pixar Technical Memo 13-01, 2013): the per-pixel
+// order in which sample passes visit the strata, in place of
+// SampleRecord2D::shuffle's Fisher-Yates draw (sampleRecord2D.cpp:16-28).
+__device__ __forceinline__ uint32_t kensler_permute(uint32_t i, uint32_t l, uint32_t p) {
+    uint32_t w = l - 1;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {
+        i ^= p;
+        i *= 0xe170893du;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3fu;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1u | p >> 27;
+        i *= 0x6935fa69u;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303u;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3u;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3dfu;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+
+// One sample pass's primary rays: the stratified (or random) film jitter of
+// StratifiedSampler::requestSample2D (stratifiedSampler.cpp:38-73) and
+// PerspectivePinholeCamera::spawnPrimaryRay (perspectivePinholeCamera.cpp:52-73).
+__global__ void __launch_bounds__(256) k_camera(PassArgs a) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid < a.nwork) a.work[gid] = 0;
+    if (gid < a.max_depth + 2) a.qcount[gid] = gid == 0 ? a.npaths : 0;
+    if (gid >= a.npaths) return;
+    const int32_t p = static_cast<int32_t>(gid);
+    const DevScene& sc = a.sc;
+    const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
+    const uint4 r = philox(make_uint4(static_cast<uint32_t>(p), static_cast<uint32_t>(a.sample), 0xFFFFFFFFu, 0u), a.key);
+    const float u0 = unit24(r.x), u1 = unit24(r.y);
+    float jx = u0, jy = u1;
+    if (sc.sampler == 1) {
+        const uint4 pk = philox(make_uint4(static_cast<uint32_t>(p), 0u, 0xFFFFFFFEu, 0u), a.key);
+        const uint32_t base = static_cast<uint32_t>(sc.strata);
+        const uint32_t j = kensler_permute(static_cast<uint32_t>(a.sample), static_cast<uint32_t>(sc.spp), pk.x);
+        if (j < base * base) {
+            const float dx = 1.0f / static_cast<float>(base);
+            jx = (static_cast<float>(j % base) + u0) * dx;
+            jy = (static_cast<float>(j / base) + u1) * dx;
+        }
+    }
+    const float fx = static_cast<float>(ix) + jx, fy = static_cast<float>(iy) + jy;
+    // film -> camera point, camera -> world vector (tMatrix4.ipp:180-195)
+    const float* m = sc.f2c;
+    const float z = 0.0f;
+    const float cx = m[0] * fx + m[1] * fy + m[2] * z + m[3];
+    const float cy = m[4] * fx + m[5] * fy + m[6] * z + m[7];
+    const float cz = m[8] * fx + m[9] * fy + m[10] * z + m[11];
+    const float* c = sc.c2w;
+    const V3 dir = normalize(v3(c[0] * cx + c[1] * cy + c[2] * cz, c[4] * cx + c[5] * cy + c[6] * cz,
+                                c[8] * cx + c[9] * cy + c[10] * cz));
+    a.fpos[p] = make_float2(fx, fy);
+    a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const Queue& q = a.q[0];
+    q.path[p] = p;
+    q.org[p] = make_float4(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2], K_RAY_EPS);
+    q.dir[p] = make_float4(dir.x, dir.y, dir.z, a.max_depth > 0 ? FLT_MAX : -1.0f);
+    q.mis_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
+    q.shd_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
+    q.bv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---- shading ---------------------------------------------------------------------
+
+__device__ __forceinline__ float u01(uint4 r, int slot) {
+    const uint32_t x = (slot & 3) == 0 ? r.x : (slot & 3) == 1 ? r.y : (slot & 3) == 2 ? r.z : r.w;
+    return unit24(x);
+}
+
+// Step `bounce` of every queued path: first the deferred half of
+// DirectLightEvaluator::evaluate (directLightEvaluator.cpp:18-109) for the
+// vertex shaded at bounce - 1, then, for the continuation hit, one iteration of
+// PtEstimator::estimate's loop body: emission, light pick
+// (uniformLightCluster.cpp:15-24 / powerLightCluster.cpp:27-34), light sample +
+// shadow ray, MIS-BSDF ray, continuation sample, Russian roulette
+// (russianRoulette.cpp:9-28). bounce == max_depth only resolves.
+__global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
+    const DevScene& sc = a.sc;
+    const Queue& cur = a.q[bounce & 1];
+    const Queue& nxt = a.q[(bounce + 1) & 1];
+    const int32_t n = a.qcount[bounce];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // wave-uniform trip count so that every lane reaches the ballot
+    const int64_t wave_first = first - lane;
+    for (int64_t sbase = wave_first; sbase < n; sbase += stride) {
+        const int64_t s = sbase + lane;
+        const bool active = s < n;
+        bool pending = false;
+        int32_t p = 0;
+        float4 org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org,
+               bv4 = org;
+        int32_t pick_light = 0;
+        float4 beta4 = org;
+        if (active) {
+            p = cur.path[s];
+            org = cur.org[s];
+            const V3 P0 = v3f4(org);
+            float4 rad4 = a.rad[p];
+            V3 L = v3f4(rad4);
+            beta4 = a.beta[p];
+            V3 beta = v3f4(beta4);
+            // -- resolve the previous vertex's NEE
+            const float4 pv = cur.bv[s];
+            if (bounce > 0 && pv.w > 0.0f) {
+                V3 Ld = v3(0.f, 0.f, 0.f);
+                const float4 sd = cur.shd_dir[s];
+                if (sd.w >= 0.0f && cur.occ[s].x == 0) Ld = Ld + v3f4(cur.lt[s]);
+                const float4 md = cur.mis_dir[s];
+                if (md.w >= 0.0f) {
+                    const int2 mh = cur.mis_hit[s];
+                    const int32_t li = cur.light[s];
+                    if (mh.x >= 0) {
+                        const int32_t prim = static_cast<int32_t>(__float_as_uint(sc.tris[3 * mh.x].w) & 0x7FFFFFFFu);
+                        if (sc.prim_info[prim].y == li) {
+                            const SurfaceHit h = surface_detail(sc, mh.x, P0, v3f4(md), __int_as_float(mh.y));
+                            const DevLight& lg = sc.lights[li];
+                            const V3 Le = light_emittance(lg, h);
+                            const float4 f4 = cur.fac[s];
+                            const float lp = light_pdfW(lg, h, P0);
+                            const float w = mis_power(f4.w, lp);
+                            Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
+                        }
+                    }
+                }
+                L = L + v3f4(pv) * divs(Ld, pv.w);
+            }
+            // -- shade the continuation hit
+            cdir = cur.dir[s];
+            const int2 ch = cur.hit[s];
+            if (bounce < a.max_depth && cdir.w >= 0.0f && ch.x >= 0) {
+                const uint32_t pix = static_cast<uint32_t>(p);
+                const uint4 rA = philox(make_uint4(pix, static_cast<uint32_t>(a.sample), 2u * bounce, 0u), a.key);
+                const uint4 rB = philox(make_uint4(pix, static_cast<uint32_t>(a.sample), 2u * bounce + 1u, 0u), a.key);
+                const SurfaceHit h = surface_detail(sc, ch.x, P0, v3f4(cdir), __int_as_float(ch.y));
+                const int4 info = sc.prim_info[h.prim];
+                const DevMaterial mat = sc.mats[info.x];
+                if (bounce == 0 && info.y >= 0) L = L + beta * light_emittance(sc.lights[info.y], h);
+                // light pick
+                float pick;
+                {
+                    const float u = u01(rA, SL_PICK);
+                    const int32_t nl = sc.nlights;
+                    int32_t idx = 0;
+                    if (sc.cluster == 1) {
+                        for (int32_t i = nl - 1; i >= 0; --i)
+                            if (sc.power_cdf[i] <= u) { idx = i; break; }
+                        pick = sc.power_cdf[idx + 1] - sc.power_cdf[idx];
+                    } else {
+                        const uint32_t raw = static_cast<uint32_t>(0.0f + u * static_cast<float>(nl));
+                        idx = raw < static_cast<uint32_t>(nl - 1) ? static_cast<int32_t>(raw) : nl - 1;
+                        pick = 1.0f / static_cast<float>(nl);
+                    }
+                    pick_light = idx;
+                }
+                // light sample + shadow ray
+                const DevLight& lg = sc.lights[pick_light];
+                const LightSample ls = light_sample(lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
+                sdir = make_float4(0.f, 0.f, 0.f, -1.f);
+                lt4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (ls.valid) {
+                    const V3 LV = ls.pos - h.P;
+                    const float dist = len(LV);
+                    const V3 Ldir = divs(LV, dist);
+                    const V3 sd = normalize(Ldir);
+                    sdir = make_float4(sd.x, sd.y, sd.z, dist - K_RAY_EPS);
+                    V3 f;
+                    float bpdf;
+                    bsdf_eval_pdf(sc, mat, h, Ldir, f, bpdf);
+                    const V3 factor = f * absdot(Ldir, h.Ns);
+                    const float w = mis_power(ls.pdfW, bpdf);
+                    const V3 lt = divs((ls.radiance * w) * factor, ls.pdfW);
+                    lt4 = make_float4(lt.x, lt.y, lt.z, 0.f);
+                }
+                // MIS-BSDF ray
+                mdir = make_float4(0.f, 0.f, 0.f, -1.f);
+                fac4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                {
+                    const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1));
+                    if (bs.valid) {
+                        const V3 md = normalize(bs.L);
+                        mdir = make_float4(md.x, md.y, md.z, FLT_MAX);
+                        const V3 fc = bs.f * absdot(bs.L, h.Ns);
+                        fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
+                    }
+                }
+                bv4 = make_float4(beta.x, beta.y, beta.z, pick);
+                // continuation
+                cdir = make_float4(0.f, 0.f, 0.f, -1.f);
+                {
+                    const BsdfSample cs = bsdf_sample(sc, mat, h, u01(rB, SL_CONT0), u01(rB, SL_CONT1));
+                    if (cs.valid) {
+                        beta = beta * divs(cs.f * absdot(cs.L, h.Ns), cs.pdf);
+                        bool alive = true;
+                        if (bounce > 2) {
+                            const float qq = mclamp(1.0f - max_comp(beta), 0.05f, 1.0f);
+                            if (u01(rA, SL_RR) > qq) beta = beta * (1.0f / (1.0f - qq));
+                            else alive = false;
+                        }
+                        if (alive && is_zero(beta)) alive = false;
+                        if (alive && bounce + 1 < a.max_depth) {
+                            const V3 nd = normalize(cs.L);
+                            cdir = make_float4(nd.x, nd.y, nd.z, FLT_MAX);
+                        }
+                    }
+                }
+                org = make_float4(h.P.x, h.P.y, h.P.z, K_RAY_EPS);
+                pending = true;
+            }
+            a.rad[p] = make_float4(L.x, L.y, L.z, 0.f);
+            a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+        }
+        // wave-ballot compaction into the next queue
+        const unsigned long long mask = __ballot(pending);
+        int32_t base = 0;
+        if (lane == 0 && mask) base = atomicAdd(&a.qcount[bounce + 1], __popcll(mask));
+        base = __shfl(base, 0, WAVE);
+        if (pending) {
+            const int32_t slot = base + static_cast<int32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+            nxt.path[slot] = p;
+            nxt.org[slot] = org;
+            nxt.dir[slot] = cdir;
+            nxt.mis_dir[slot] = mdir;
+            nxt.shd_dir[slot] = sdir;
+            nxt.lt[slot] = lt4;
+            nxt.fac[slot] = fac4;
+            nxt.bv[slot] = bv4;
+            nxt.light[slot] = pick_light;
+        }
+    }
+}
+
+// ---- film ------------------------------------------------------------------------
+
+// Filter::evaluate of the four reference filters (boxFilter.cpp:9-11,
+// gaussianFilter.cpp:25-33, coneFilter.cpp:13-16, mitchellFilter.cpp:16-40).
+__device__ __forceinline__ float mitchell1d(float b, float c, float d) {
+    const float ad = fabsf(d);
+    const float den = 1.0f / 6.0f;
+    if (ad < 1.0f)
+        return ((12.0f - 9.0f * b - 6.0f * c) * ad * ad * ad + (-18.0f + 12.0f * b + 6.0f * c) * ad * ad +
+                (6.0f - 2.0f * b)) * den;
+    if (ad < 2.0f)
+        return ((-b - 6.0f * c) * ad * ad * ad + (6.0f * b + 30.0f * c) * ad * ad + (-12.0f * b - 48.0f * c) * ad +
+                (8.0f * b + 24.0f * c)) * den;
+    return 0.0f;
+}
+__device__ __forceinline__ float filter_eval(const DevScene& sc, float x, float y) {
+    switch (sc.filter) {
+        case 1: {
+            const float v = sc.amplitude * expf(sc.g_expx * x * x + sc.g_expy * y * y);
+            return mmax(0.0f, v - sc.g_edge);
+        }
+        case 2: return mmax(sc.filter_hw - fabsf(x), 0.0f) * mmax(sc.filter_hh - fabsf(y), 0.0f);
+        case 3: return mitchell1d(sc.mitchell_b, sc.mitchell_c, x) * mitchell1d(sc.mitchell_b, sc.mitchell_c, y);
+        default: return 1.0f;
+    }
+}
+
+// FilmTile::addSample (filmTile.cpp:21-66) as a gather: one block per film
+// tile; each pixel sums the pass's samples of its tile whose filter footprint,
+// clamped to the tile, covers it. NaN / inf samples are dropped. The film
+// holds sum w * rgb and sum w per pixel across passes.
+__global__ void __launch_bounds__(256) k_film(PassArgs a) {
+    const DevScene& sc = a.sc;
+    const int ntx = (sc.img_w + sc.tile_x - 1) / sc.tile_x;
+    const int tile = blockIdx.x;
+    const int x0 = (tile % ntx) * sc.tile_x, y0 = (tile / ntx) * sc.tile_y;
+    const int x1 = min(x0 + sc.tile_x, sc.img_w), y1 = min(y0 + sc.tile_y, sc.img_h);
+    const int tw = x1 - x0, th = y1 - y0;
+    const int rx = static_cast<int>(ceilf(sc.filter_hw)) + 1, ry = static_cast<int>(ceilf(sc.filter_hh)) + 1;
+    for (int k = threadIdx.x; k < tw * th; k += blockDim.x) {
+        const int qx = x0 + k % tw, qy = y0 + k / tw;
+        float4 acc = a.film[static_cast<int64_t>(qy) * sc.img_w + qx];
+        for (int sy = max(y0, qy - ry); sy <= min(y1 - 1, qy + ry); ++sy) {
+            for (int sx = max(x0, qx - rx); sx <= min(x1 - 1, qx + rx); ++sx) {
+                const int sp = sy * sc.img_w + sx;
+                const float4 L = a.rad[sp];
+                if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) continue;
+                const float2 fp = a.fpos[sp];
+                const float mnx = fmaxf(fp.x - sc.filter_hw, static_cast<float>(x0));
+                const float mny = fmaxf(fp.y - sc.filter_hh, static_cast<float>(y0));
+                const float mxx = fminf(fp.x + sc.filter_hw, static_cast<float>(x1));
+                const float mxy = fminf(fp.y + sc.filter_hh, static_cast<float>(y1));
+                const int fx0 = static_cast<int>(ceilf(mnx - 0.5f)), fy0 = static_cast<int>(ceilf(mny - 0.5f));
+                const int fx1 = static_cast<int>(floorf(mxx - 0.5f) + 1), fy1 = static_cast<int>(floorf(mxy - 0.5f) + 1);
+                if (qx < fx0 || qx >= fx1 || qy < fy0 || qy >= fy1) continue;
+                const float w = filter_eval(sc, static_cast<float>(qx) - (fp.x - 0.5f), static_cast<float>(qy) - (fp.y - 0.5f));
+                acc.x += L.x * w;
+                acc.y += L.y * w;
+                acc.z += L.z * w;
+                acc.w += w;
+            }
+        }
+        a.film[static_cast<int64_t>(qy) * sc.img_w + qx] = acc;
+    }
+}
+
+// Film::save's resolve (film.cpp:159-196), math::forward_gamma_correction
+// (math.cpp:92-99) and ImageUtils::hdrToLdr (imageUtils.cpp:23-36).
+__global__ void __launch_bounds__(256) k_srgb8(const float4* film, int64_t npix, uint8_t* out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    const float4 p = film[i];
+    const float invw = (p.w == 0.0f) ? 0.0f : 1.0f / p.w;
+    const float v[3] = {p.x * invw, p.y * invw, p.z * invw};
+    for (int c = 0; c < 3; ++c) {
+        const float lin = v[c] + 0.0f;
+        const float s = lin <= 0.0031308f ? 12.92f * lin : 1.055f * powf(lin, 1.0f / 2.4f) - 0.055f;
+        out[3 * i + c] = static_cast<uint8_t>(mclamp(s * 255.0f + 0.5f, 0.0f, 255.0f));
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_zero_f4(float4* p, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace mpt

@@ -1,0 +1,80 @@
+// Launch-argument PODs shared by the kernels (kernels.hip) and the host driver
+// (render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dscene.hpp"
+
+namespace mpt {
+
+// Work counters and statistics, one int/uint64 slot each.
+enum StatSlot {
+    ST_CLOSEST_RAYS = 0,  // valid continuation / camera rays traced
+    ST_MIS_RAYS,          // valid MIS-BSDF rays traced
+    ST_SHADOW_RAYS,       // valid shadow rays traced
+    ST_BOXES,             // slab tests
+    ST_PRIMS,             // primitive tests
+    ST_ERRORS,            // traversal stack overruns (must stay 0)
+    ST_COUNT
+};
+
+// A batch of rays for the persistent traversal kernel. Ray i is
+// (org[i * stride], dir[i * stride]) for i < n0 and (org[(i - n0) * stride],
+// dir2[(i - n0) * stride]) for n0 <= i < n0 + n1 (the second set shares
+// origins: the MIS-BSDF rays leave from the continuation rays' origins).
+// A ray with dir.w (tmax) < 0 is absent and reports a miss.
+struct TraceArgs {
+    DevScene       sc;
+    const float4*  org;
+    const float4*  dir;
+    const float4*  dir2;
+    int32_t        stride;
+    const int32_t* count;        // n0 = *count (device) when non-null, else n0 = n_host
+    int64_t        n_host;
+    int32_t        dual;         // 1: n1 = n0 rays from dir2
+    int2*          hit;          // closest: (leaf or prim, t bits); any: x = 0/1
+    int2*          hit2;
+    int32_t        want_prim;    // closest: write the creation index instead of the leaf position
+    int32_t*       work;         // atomic work counter, zero on entry
+    int32_t*       spill;        // per-lane stack spill, (max_stack - S) entries per lane
+    unsigned long long* stats;
+    int32_t        stat_slot;    // ST_CLOSEST_RAYS or ST_SHADOW_RAYS
+};
+
+// Path state per path (one path per pixel in a sample pass) and the ping-pong
+// queues of pending path vertices.
+struct Queue {
+    int32_t* path;     // path id
+    float4*  org;      // vertex position, tmin
+    float4*  dir;      // continuation ray (w = tmax, < 0 absent)
+    float4*  mis_dir;  // MIS-BSDF ray
+    float4*  shd_dir;  // shadow ray
+    float4*  lt;       // light-sample term of DirectLightEvaluator, unscaled
+    float4*  fac;      // MIS-BSDF factor f |L.N| (xyz), pdf (w)
+    float4*  bv;       // throughput at the vertex (xyz), light-pick pdf (w)
+    int32_t* light;    // picked light
+    int2*    hit;      // continuation hit
+    int2*    mis_hit;
+    int2*    occ;      // shadow: x = occluded
+};
+
+struct PassArgs {
+    DevScene sc;
+    uint2    key;          // Philox key (seed)
+    int32_t  sample;       // sample index of this pass
+    int32_t  max_depth;
+    int32_t  npaths;
+    float4*  beta;
+    float4*  rad;
+    float2*  fpos;
+    Queue    q[2];
+    int32_t* qcount;       // [max_depth + 2]
+    int32_t* work;         // traversal work counters, 3 per bounce
+    int32_t  nwork;
+    float4*  film;         // sum w*rgb, sum w
+};
+
+}  // namespace mpt

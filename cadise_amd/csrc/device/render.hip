@@ -1,0 +1,989 @@
+// Host driver of the device path and the C ABI of include/mpt.h.
+//
+// DeviceScene owns every HBM allocation of one scene: the wide BVH, the
+// leaf-ordered primitive records, shading tables, the film, and the per-pass
+// path state / ping-pong vertex queues. render() runs, per sample pass,
+//   k_camera -> [k_trace closest (continuation + MIS-BSDF rays),
+//                k_trace any (shadow rays), k_shade] x (max_depth + 1) -> k_film
+// on one stream; queue sizes stay on the device, so the host never waits
+// inside a pass.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../../include/mpt.h"
+#include "../host/bvh.hpp"
+#include "../host/crsd.hpp"
+#include "../host/scene.hpp"
+#include "kernels.hpp"
+
+namespace mpt {
+
+template <int W, int S, bool ANY>
+__global__ void k_trace(TraceArgs a);
+__global__ void k_camera(PassArgs a);
+__global__ void k_shade(PassArgs a, int32_t bounce);
+__global__ void k_film(PassArgs a);
+__global__ void k_srgb8(const float4* film, int64_t npix, uint8_t* out);
+__global__ void k_zero_u64(unsigned long long* p, int n);
+__global__ void k_zero_f4(float4* p, int64_t n);
+
+constexpr int STACK_LDS = 24;
+constexpr int TRACE_BLOCK = 256;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        bytes = n;
+        return hipMalloc(&p, n);
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace mpt
+
+struct mpt_ctx {
+    int device = 0;
+    int num_cus = 0;
+    std::string error;
+};
+
+struct mpt_scene {
+    mpt_ctx* ctx = nullptr;
+    mpt::HostScene host;
+    mpt::BinaryBvh bin;        // the tree the wide BVH is collapsed from
+    mpt::BinaryBvh ref_bin;    // the reference's tree, built on first request
+    bool ref_built = false;
+    mpt::WideBvh wide;
+    mpt::DevScene sc{};
+    hipStream_t stream = nullptr;
+    // scene arrays
+    mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf;
+    // film + pass state
+    mpt::DevBuf film, beta, rad, fpos, qcount, work, stats, spill, srgb;
+    mpt::DevBuf qbuf[2][12];
+    int64_t npaths = 0;
+    int64_t spill_lanes = 0;
+    int32_t max_depth_alloc = -1;
+    mpt_scene_stats sstats{};
+    mpt_stats rstats{};
+    mpt_bvh_opts opts{};
+    std::vector<hipEvent_t> events;
+    ~mpt_scene() {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+using namespace mpt;
+
+int fail(mpt_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, MPT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int upload_vec(mpt_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
+    HIP_TRY(ctx, b.alloc(v.size() * sizeof(T)));
+    if (!v.empty()) HIP_TRY(ctx, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MPT_OK;
+}
+
+inline double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Build the BVHs and upload every scene table.
+int build_and_upload(mpt_scene* S) {
+    mpt_ctx* ctx = S->ctx;
+    const HostScene& h = S->host;
+    const bool host_only = ctx->device < 0;
+    if (!host_only) {
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        if (!S->stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    }
+    mpt_bvh_opts& o = S->opts;
+    if (o.width != 8) o.width = 4;
+    if (o.leaf_max <= 0) o.leaf_max = 2;
+    if (o.leaf_max > 64) o.leaf_max = 64;
+
+    auto t0 = std::chrono::steady_clock::now();
+    build_binary(h, false, &S->bin);
+    S->sstats.build_ms = ms_since(t0);
+    t0 = std::chrono::steady_clock::now();
+    collapse_wide(h, S->bin, o.width, o.leaf_max, &S->wide);
+    S->sstats.collapse_ms = ms_since(t0);
+
+    const int64_t n = h.nprims();
+    S->sstats.nprims = n;
+    S->sstats.nlights = static_cast<int64_t>(h.lights.size());
+    S->sstats.binary_nodes = static_cast<int64_t>(S->bin.nodes.size());
+    S->sstats.wide_nodes = S->wide.nnodes;
+    S->sstats.binary_depth = S->bin.depth;
+    S->sstats.wide_depth = S->wide.depth;
+    S->sstats.width = S->wide.width;
+    S->sstats.leaf_max = o.leaf_max;
+    S->sstats.image_width = h.width;
+    S->sstats.image_height = h.height;
+    S->sstats.spp = h.spp;
+    S->sstats.max_depth = h.max_depth;
+    if (host_only) return MPT_OK;
+    // leaf-ordered primitive records
+    std::vector<float4> tris(static_cast<size_t>(3 * n));
+    for (int64_t j = 0; j < n; ++j) {
+        const int32_t p = S->bin.ordered[j];
+        const float* v = &h.prim_v[9 * p];
+        float A[3], e1[3], e2[3];
+        const bool rect = h.prim_type[p] == PRIM_RECTANGLE;
+        for (int k = 0; k < 3; ++k) {
+            if (rect) {
+                A[k] = v[3 + k];
+                e1[k] = v[k] - v[3 + k];
+                e2[k] = v[6 + k] - v[3 + k];
+            } else {
+                A[k] = v[k];
+                e1[k] = v[3 + k] - v[k];
+                e2[k] = v[6 + k] - v[k];
+            }
+        }
+        uint32_t bits = static_cast<uint32_t>(p) | (rect ? 0x80000000u : 0u);
+        float w;
+        std::memcpy(&w, &bits, 4);
+        tris[3 * j + 0] = make_float4(A[0], A[1], A[2], w);
+        tris[3 * j + 1] = make_float4(e1[0], e1[1], e1[2], 0.f);
+        tris[3 * j + 2] = make_float4(e2[0], e2[1], e2[2], 0.f);
+    }
+    // per-primitive shading info; a primitive's area light is the last one bound to it
+    std::vector<int4> info(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+        const bool has_n = !h.prim_n.empty() && h.prim_type[i] == PRIM_TRIANGLE;
+        const bool has_u = !h.prim_uvw.empty() && h.prim_type[i] == PRIM_TRIANGLE && !std::isnan(h.prim_uvw[9 * i]);
+        info[i] = make_int4(h.prim_mat[i], -1, has_n ? 1 : 0, has_u ? 1 : 0);
+    }
+    for (size_t l = 0; l < h.lights.size(); ++l) info[h.lights[l].prim].y = static_cast<int32_t>(l);
+
+    std::vector<DevMaterial> mats(h.materials.size());
+    for (size_t i = 0; i < mats.size(); ++i) {
+        const Material& m = h.materials[i];
+        DevMaterial d{};
+        d.type = m.type;
+        d.albedo_tex = m.albedo_tex;
+        d.rough_tex = m.rough_tex;
+        d.albedo[0] = m.albedo.x; d.albedo[1] = m.albedo.y; d.albedo[2] = m.albedo.z;
+        d.f0[0] = m.f0.x; d.f0[1] = m.f0.y; d.f0[2] = m.f0.z;
+        d.roughness = m.roughness;
+        mats[i] = d;
+    }
+    std::vector<DevTexture> texs(h.textures.size());
+    std::vector<float> texdata;
+    for (size_t i = 0; i < texs.size(); ++i) {
+        const Texture& t = h.textures[i];
+        DevTexture d{};
+        d.offset = static_cast<int64_t>(texdata.size());
+        d.width = t.width;
+        d.height = t.height;
+        d.channels = t.channels;
+        d.wrap_clamp = t.wrap_clamp;
+        if (t.data.empty()) {
+            // a constant texture as a 1x1 image: u * 1 and v * 1 floor to 0
+            d.width = d.height = 1;
+            texdata.push_back(t.value.x);
+            if (t.channels == 3) { texdata.push_back(t.value.y); texdata.push_back(t.value.z); }
+        } else {
+            texdata.insert(texdata.end(), t.data.begin(), t.data.end());
+        }
+        texs[i] = d;
+    }
+    const std::vector<F3> rad = light_radiance(h);
+    std::vector<DevLight> lights(h.lights.size());
+    for (size_t l = 0; l < lights.size(); ++l) {
+        DevLight d{};
+        const int32_t p = h.lights[l].prim;
+        const float* v = &h.prim_v[9 * p];
+        d.prim = p;
+        d.type = h.prim_type[p];
+        d.back_face = h.lights[l].back_face;
+        d.radiance[0] = rad[l].x; d.radiance[1] = rad[l].y; d.radiance[2] = rad[l].z;
+        d.pdfA = 1.0f / prim_area(h, p);
+        for (int k = 0; k < 3; ++k) {
+            if (d.type == PRIM_RECTANGLE) {
+                d.a[k] = v[3 + k];
+                d.e1[k] = v[k] - v[3 + k];
+                d.e2[k] = v[6 + k] - v[3 + k];
+            } else {
+                d.a[k] = v[k];
+                d.e1[k] = v[3 + k] - v[k];
+                d.e2[k] = v[6 + k] - v[k];
+            }
+        }
+        d.has_normals = 0;
+        if (d.type == PRIM_TRIANGLE && !h.prim_n.empty()) {
+            // face normal unless a non-zero vertex normal was given (Triangle::setNa..c)
+            const float cx = d.e1[1] * d.e2[2] - d.e1[2] * d.e2[1];
+            const float cy = d.e1[2] * d.e2[0] - d.e1[0] * d.e2[2];
+            const float cz = d.e1[0] * d.e2[1] - d.e1[1] * d.e2[0];
+            float N[3] = {0.f, 1.f, 0.f};
+            if (!(cx == 0.f && cy == 0.f && cz == 0.f)) {
+                float l2 = 0.f;
+                l2 += cx * cx;
+                l2 += cy * cy;
+                l2 += cz * cz;
+                const float inv = 1.f / std::sqrt(l2);
+                N[0] = cx * inv; N[1] = cy * inv; N[2] = cz * inv;
+            }
+            for (int q = 0; q < 3; ++q) {
+                const float* vn = &h.prim_n[9 * p + 3 * q];
+                const bool zero = vn[0] == 0.f && vn[1] == 0.f && vn[2] == 0.f;
+                for (int k = 0; k < 3; ++k) d.n[3 * q + k] = zero ? N[k] : vn[k];
+            }
+            d.has_normals = 1;
+        }
+        lights[l] = d;
+    }
+    const std::vector<float> cdf = power_cdf(h, rad);
+
+    int rc;
+    if ((rc = upload_vec(ctx, S->nodes, S->wide.nodes)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->tris, tris)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->prim_info, info)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->prim_n, h.prim_n)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->prim_uvw, h.prim_uvw)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->mats, mats)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->texs, texs)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->tex_data, texdata)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->lights, lights)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->power_cdf, cdf)) != MPT_OK) return rc;
+    HIP_TRY(ctx, S->stats.alloc(sizeof(unsigned long long) * ST_COUNT));
+
+    DevScene& sc = S->sc;
+    sc = DevScene{};
+    sc.nodes = S->nodes.p;
+    sc.width = S->wide.width;
+    sc.max_stack = S->wide.max_stack;
+    sc.nnodes = S->wide.nnodes;
+    sc.nprims = n;
+    sc.tris = S->tris.as<float4>();
+    sc.prim_info = S->prim_info.as<int4>();
+    sc.prim_n = S->prim_n.as<float>();
+    sc.prim_uvw = S->prim_uvw.as<float>();
+    sc.mats = S->mats.as<DevMaterial>();
+    sc.texs = S->texs.as<DevTexture>();
+    sc.tex_data = S->tex_data.as<float>();
+    sc.lights = S->lights.as<DevLight>();
+    sc.power_cdf = S->power_cdf.as<float>();
+    sc.nlights = static_cast<int32_t>(h.lights.size());
+    sc.cluster = h.cluster;
+    float f2c[16], c2w[16];
+    camera_matrices(h, f2c, c2w);
+    std::memcpy(sc.f2c, f2c, sizeof(sc.f2c));
+    std::memcpy(sc.c2w, c2w, sizeof(sc.c2w));
+    sc.cam_pos[0] = h.cam_pos.x; sc.cam_pos[1] = h.cam_pos.y; sc.cam_pos[2] = h.cam_pos.z;
+    sc.img_w = h.width;
+    sc.img_h = h.height;
+    sc.tile_x = h.tile_x;
+    sc.tile_y = h.tile_y;
+    sc.filter = h.filter;
+    sc.filter_hw = h.filter_w * 0.5f;   // Filter::filterHalfSize: size / 2
+    sc.filter_hh = h.filter_h * 0.5f;
+    sc.g_expx = -1.0f / (2.0f * h.sigma_x * h.sigma_x);
+    sc.g_expy = -1.0f / (2.0f * h.sigma_y * h.sigma_y);
+    sc.amplitude = h.amplitude;
+    sc.g_edge = h.amplitude * std::exp(sc.g_expx * sc.filter_hw * sc.filter_hw + sc.g_expy * sc.filter_hh * sc.filter_hh);
+    sc.mitchell_b = h.mitchell_b;
+    sc.mitchell_c = h.mitchell_c;
+    sc.sampler = h.sampler;
+    sc.spp = h.spp;
+    sc.strata = strata_base(h.spp);
+
+    const int64_t npix = static_cast<int64_t>(h.width) * h.height;
+    HIP_TRY(ctx, S->film.alloc(sizeof(float4) * static_cast<size_t>(npix)));
+    HIP_TRY(ctx, hipMemset(S->film.p, 0, S->film.bytes));
+
+    int64_t bytes = 0;
+    for (DevBuf* b : {&S->nodes, &S->tris, &S->prim_info, &S->prim_n, &S->prim_uvw, &S->mats, &S->texs, &S->tex_data,
+                      &S->lights, &S->power_cdf, &S->film})
+        bytes += static_cast<int64_t>(b->bytes);
+    S->sstats.device_bytes = bytes;
+    return MPT_OK;
+}
+
+int trace_blocks(mpt_scene* S, int64_t nrays) {
+    const int64_t want = (nrays + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    const int64_t cap = static_cast<int64_t>(S->ctx->num_cus) * 6;
+    return static_cast<int>(std::max<int64_t>(1, std::min(want, cap)));
+}
+
+// Spill space for traversal stacks deeper than the LDS part.
+int ensure_spill(mpt_scene* S, int blocks) {
+    const int64_t extra = S->sc.max_stack - STACK_LDS;
+    if (extra <= 0) return MPT_OK;
+    const int64_t lanes = static_cast<int64_t>(blocks) * TRACE_BLOCK;
+    if (lanes <= S->spill_lanes) return MPT_OK;
+    HIP_TRY(S->ctx, S->spill.alloc(sizeof(int32_t) * static_cast<size_t>(lanes * extra)));
+    S->spill_lanes = lanes;
+    return MPT_OK;
+}
+
+int launch_trace(mpt_scene* S, TraceArgs a, bool any, int blocks) {
+    a.spill = S->sc.max_stack > STACK_LDS ? S->spill.as<int32_t>() : nullptr;
+    const size_t lds = sizeof(int32_t) * STACK_LDS * TRACE_BLOCK;
+    if (S->sc.width == 8) {
+        if (any) hipLaunchKernelGGL((k_trace<8, STACK_LDS, true>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
+        else hipLaunchKernelGGL((k_trace<8, STACK_LDS, false>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
+    } else {
+        if (any) hipLaunchKernelGGL((k_trace<4, STACK_LDS, true>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
+        else hipLaunchKernelGGL((k_trace<4, STACK_LDS, false>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
+    }
+    HIP_TRY(S->ctx, hipGetLastError());
+    return MPT_OK;
+}
+
+int read_stats(mpt_scene* S, unsigned long long* out) {
+    HIP_TRY(S->ctx, hipMemcpyAsync(out, S->stats.p, sizeof(unsigned long long) * ST_COUNT, hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(S->ctx, hipStreamSynchronize(S->stream));
+    return MPT_OK;
+}
+
+int zero_stats(mpt_scene* S) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, S->stream, S->stats.as<unsigned long long>(), ST_COUNT);
+    HIP_TRY(S->ctx, hipGetLastError());
+    return MPT_OK;
+}
+
+int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev, hipStream_t stream) {
+    if (n == 0) return MPT_OK;
+    hipStream_t keep = S->stream;
+    if (stream) S->stream = stream;
+    int rc = MPT_OK;
+    const int blocks = trace_blocks(S, static_cast<int64_t>(n));
+    DevBuf work;
+    do {
+        if ((rc = ensure_spill(S, blocks)) != MPT_OK) break;
+        if (work.alloc(sizeof(int32_t)) != hipSuccess) { rc = fail(S->ctx, MPT_ERR_DEVICE, "hipMalloc work counter"); break; }
+        if (hipMemsetAsync(work.p, 0, sizeof(int32_t), S->stream) != hipSuccess) { rc = fail(S->ctx, MPT_ERR_DEVICE, "memset"); break; }
+        if ((rc = zero_stats(S)) != MPT_OK) break;
+        TraceArgs a{};
+        a.sc = S->sc;
+        a.org = reinterpret_cast<const float4*>(rays_dev);
+        a.dir = reinterpret_cast<const float4*>(rays_dev) + 1;
+        a.dir2 = nullptr;
+        a.stride = 2;
+        a.count = nullptr;
+        a.n_host = static_cast<int64_t>(n);
+        a.dual = 0;
+        a.hit = reinterpret_cast<int2*>(hits_dev);
+        a.hit2 = nullptr;
+        a.want_prim = 1;
+        a.work = work.as<int32_t>();
+        a.stats = S->stats.as<unsigned long long>();
+        a.stat_slot = any_hit ? ST_SHADOW_RAYS : ST_CLOSEST_RAYS;
+        if ((rc = launch_trace(S, a, any_hit != 0, blocks)) != MPT_OK) break;
+        if (hipStreamSynchronize(S->stream) != hipSuccess) { rc = fail(S->ctx, MPT_ERR_DEVICE, "trace failed"); break; }
+        unsigned long long st[ST_COUNT];
+        if ((rc = read_stats(S, st)) != MPT_OK) break;
+        S->rstats = mpt_stats{};
+        S->rstats.box_tests = static_cast<int64_t>(st[ST_BOXES]);
+        S->rstats.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
+        if (any_hit) S->rstats.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
+        else S->rstats.camera_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]);
+        if (st[ST_ERRORS]) rc = fail(S->ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    } while (false);
+    S->stream = keep;
+    return rc;
+}
+
+// Per-pass buffers for npix paths.
+int ensure_pass_buffers(mpt_scene* S, int64_t npix, int32_t max_depth) {
+    mpt_ctx* ctx = S->ctx;
+    if (npix != S->npaths) {
+        HIP_TRY(ctx, S->beta.alloc(sizeof(float4) * npix));
+        HIP_TRY(ctx, S->rad.alloc(sizeof(float4) * npix));
+        HIP_TRY(ctx, S->fpos.alloc(sizeof(float2) * npix));
+        const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 12; ++k) HIP_TRY(ctx, S->qbuf[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
+        S->npaths = npix;
+    }
+    if (max_depth != S->max_depth_alloc) {
+        HIP_TRY(ctx, S->qcount.alloc(sizeof(int32_t) * (max_depth + 2)));
+        HIP_TRY(ctx, S->work.alloc(sizeof(int32_t) * 3 * (max_depth + 2)));
+        S->max_depth_alloc = max_depth;
+    }
+    return MPT_OK;
+}
+
+PassArgs pass_args(mpt_scene* S, uint64_t seed, int32_t max_depth) {
+    PassArgs a{};
+    a.sc = S->sc;
+    a.key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+    a.max_depth = max_depth;
+    a.npaths = static_cast<int32_t>(S->npaths);
+    a.beta = S->beta.as<float4>();
+    a.rad = S->rad.as<float4>();
+    a.fpos = S->fpos.as<float2>();
+    for (int q = 0; q < 2; ++q) {
+        Queue& Q = a.q[q];
+        Q.path = S->qbuf[q][0].as<int32_t>();
+        Q.org = S->qbuf[q][1].as<float4>();
+        Q.dir = S->qbuf[q][2].as<float4>();
+        Q.mis_dir = S->qbuf[q][3].as<float4>();
+        Q.shd_dir = S->qbuf[q][4].as<float4>();
+        Q.lt = S->qbuf[q][5].as<float4>();
+        Q.fac = S->qbuf[q][6].as<float4>();
+        Q.bv = S->qbuf[q][7].as<float4>();
+        Q.light = S->qbuf[q][8].as<int32_t>();
+        Q.hit = S->qbuf[q][9].as<int2>();
+        Q.mis_hit = S->qbuf[q][10].as<int2>();
+        Q.occ = S->qbuf[q][11].as<int2>();
+    }
+    a.qcount = S->qcount.as<int32_t>();
+    a.work = S->work.as<int32_t>();
+    a.nwork = 3 * (max_depth + 2);
+    a.film = S->film.as<float4>();
+    return a;
+}
+
+enum EvKind { EV_CLOSEST = 0, EV_ANY, EV_SHADE, EV_CAMERA, EV_FILM, EV_KINDS };
+
+int do_render(mpt_scene* S, const mpt_render_params* p) {
+    mpt_ctx* ctx = S->ctx;
+    const auto t0 = std::chrono::steady_clock::now();
+    const HostScene& h = S->host;
+    const int32_t max_depth = (p && p->max_depth > 0) ? p->max_depth : h.max_depth;
+    const int32_t spp_begin = p ? std::max(0, p->spp_begin) : 0;
+    const int32_t spp_end = (p && p->spp_end > 0) ? p->spp_end : h.spp;
+    const uint64_t seed = p ? p->seed : 0;
+    const int64_t npix = static_cast<int64_t>(h.width) * h.height;
+    if (max_depth > 4096) return fail(ctx, MPT_ERR_INVALID, "max-depth above 4096 is not supported on the device");
+    int rc;
+    if ((rc = ensure_pass_buffers(S, npix, max_depth)) != MPT_OK) return rc;
+    if (!p || !p->accumulate) {
+        hipLaunchKernelGGL(k_zero_f4, dim3((npix + 255) / 256), dim3(256), 0, S->stream, S->film.as<float4>(), npix);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if ((rc = zero_stats(S)) != MPT_OK) return rc;
+    const int tblocks = trace_blocks(S, 2 * npix);
+    if ((rc = ensure_spill(S, tblocks)) != MPT_OK) return rc;
+    const int sblocks = static_cast<int>(std::min<int64_t>((npix + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
+    const int ntiles = ((h.width + h.tile_x - 1) / h.tile_x) * ((h.height + h.tile_y - 1) / h.tile_y);
+    const int cam_threads = static_cast<int>(std::max<int64_t>(npix, 3 * (max_depth + 2) + 1));
+
+    // events: (start, stop) per launch, grouped by kind
+    const int passes = std::max(0, spp_end - spp_begin);
+    const size_t need = static_cast<size_t>(passes) * (3 * (max_depth + 1) + 2) * 2;
+    while (S->events.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(ctx, hipEventCreate(&e));
+        S->events.push_back(e);
+    }
+    std::vector<std::pair<int, size_t>> marks;   // (kind, index of start event)
+    size_t ev = 0;
+    auto mark = [&](int kind) -> hipError_t {
+        marks.push_back({kind, ev});
+        return hipEventRecord(S->events[ev++], S->stream);
+    };
+    auto stop = [&]() -> hipError_t { return hipEventRecord(S->events[ev++], S->stream); };
+
+    for (int32_t k = spp_begin; k < spp_end; ++k) {
+        PassArgs a = pass_args(S, seed, max_depth);
+        a.sample = k;
+        HIP_TRY(ctx, mark(EV_CAMERA));
+        hipLaunchKernelGGL(k_camera, dim3((cam_threads + 255) / 256), dim3(256), 0, S->stream, a);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, stop());
+        for (int32_t b = 0; b <= max_depth; ++b) {
+            const Queue& Q = a.q[b & 1];
+            TraceArgs t{};
+            t.sc = S->sc;
+            t.org = Q.org;
+            t.dir = Q.dir;
+            t.dir2 = Q.mis_dir;
+            t.stride = 1;
+            t.count = a.qcount + b;
+            t.dual = 1;
+            t.hit = Q.hit;
+            t.hit2 = Q.mis_hit;
+            t.want_prim = 0;
+            t.work = a.work + 3 * b;
+            t.stats = S->stats.as<unsigned long long>();
+            t.stat_slot = ST_CLOSEST_RAYS;
+            HIP_TRY(ctx, mark(EV_CLOSEST));
+            if ((rc = launch_trace(S, t, false, tblocks)) != MPT_OK) return rc;
+            HIP_TRY(ctx, stop());
+            if (b > 0) {
+                t.dir = Q.shd_dir;
+                t.dual = 0;
+                t.hit = Q.occ;
+                t.hit2 = nullptr;
+                t.work = a.work + 3 * b + 1;
+                t.stat_slot = ST_SHADOW_RAYS;
+                HIP_TRY(ctx, mark(EV_ANY));
+                if ((rc = launch_trace(S, t, true, tblocks)) != MPT_OK) return rc;
+                HIP_TRY(ctx, stop());
+            }
+            HIP_TRY(ctx, mark(EV_SHADE));
+            hipLaunchKernelGGL(k_shade, dim3(sblocks), dim3(256), 0, S->stream, a, b);
+            HIP_TRY(ctx, hipGetLastError());
+            HIP_TRY(ctx, stop());
+        }
+        HIP_TRY(ctx, mark(EV_FILM));
+        hipLaunchKernelGGL(k_film, dim3(ntiles), dim3(256), 0, S->stream, a);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, stop());
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(S->stream));
+    unsigned long long st[ST_COUNT];
+    if ((rc = read_stats(S, st)) != MPT_OK) return rc;
+    double ms[EV_KINDS] = {0, 0, 0, 0, 0};
+    for (auto& m : marks) {
+        float t = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&t, S->events[m.second], S->events[m.second + 1]));
+        ms[m.first] += t;
+    }
+    mpt_stats& r = S->rstats;
+    r = mpt_stats{};
+    r.samples = npix * passes;
+    r.camera_rays = npix * (max_depth > 0 ? passes : 0);
+    r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
+    r.mis_rays = 0;
+    r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
+    r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
+    r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
+    r.ms_closest = ms[EV_CLOSEST];
+    r.ms_anyhit = ms[EV_ANY];
+    r.ms_shade = ms[EV_SHADE];
+    r.ms_camera = ms[EV_CAMERA];
+    r.ms_film = ms[EV_FILM];
+    r.ms_total = ms_since(t0);
+    if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    return MPT_OK;
+}
+
+int finish_scene(mpt_ctx* ctx, std::unique_ptr<mpt_scene>& S, const mpt_bvh_opts* opts, mpt_scene** out) {
+    if (opts) S->opts = *opts;
+    S->ctx = ctx;
+    const int rc = build_and_upload(S.get());
+    if (rc != MPT_OK) return rc;
+    *out = S.release();
+    return MPT_OK;
+}
+
+void write_png(const std::string& path, const uint8_t* rgb, int w, int h, std::string* err);
+
+}  // namespace
+
+// ---- C ABI --------------------------------------------------------------------------
+
+extern "C" {
+
+const char* mpt_version(void) { return "mpt 0.1 gfx950"; }
+
+int mpt_create(int device, mpt_ctx** out) {
+    if (!out) return MPT_ERR_INVALID;
+    *out = nullptr;
+    if (device == -1) {
+        auto* ctx = new mpt_ctx();
+        ctx->device = -1;
+        ctx->num_cus = 0;
+        *out = ctx;
+        return MPT_OK;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MPT_ERR_DEVICE;
+    if (device < 0 || device >= n) return MPT_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return MPT_ERR_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MPT_ERR_DEVICE;
+    auto* ctx = new mpt_ctx();
+    ctx->device = device;
+    ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    *out = ctx;
+    return MPT_OK;
+}
+
+void mpt_destroy(mpt_ctx* ctx) { delete ctx; }
+
+const char* mpt_last_error(const mpt_ctx* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+int mpt_scene_parse_crsd(mpt_ctx* ctx, const char* text, size_t len, const mpt_bvh_opts* opts, mpt_scene** out) {
+    if (!ctx || !text || !out) return fail(ctx, MPT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    auto S = std::make_unique<mpt_scene>();
+    const CrsdResult r = parse_crsd(std::string(text, len), "", &S->host);
+    if (r.code != MPT_OK) return fail(ctx, r.code, r.message);
+    return finish_scene(ctx, S, opts, out);
+}
+
+int mpt_scene_load_crsd(mpt_ctx* ctx, const char* path, const mpt_bvh_opts* opts, mpt_scene** out) {
+    if (!ctx || !path || !out) return fail(ctx, MPT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return fail(ctx, MPT_ERR_IO, std::string("cannot open ") + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string base(path);
+    const size_t slash = base.find_last_of('/');
+    base = slash == std::string::npos ? std::string() : base.substr(0, slash);
+    auto S = std::make_unique<mpt_scene>();
+    const CrsdResult r = parse_crsd(ss.str(), base, &S->host);
+    if (r.code != MPT_OK) return fail(ctx, r.code, r.message);
+    return finish_scene(ctx, S, opts, out);
+}
+
+int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* opts, mpt_scene** out) {
+    if (!ctx || !d || !out) return fail(ctx, MPT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->nprims < 0 || (d->nprims > 0 && (!d->prim_type || !d->prim_v || !d->prim_mat)))
+        return fail(ctx, MPT_ERR_INVALID, "primitive arrays missing");
+    if (d->nmats <= 0 || !d->mats) return fail(ctx, MPT_ERR_INVALID, "no materials");
+    if (d->nlights <= 0 || !d->lights) return fail(ctx, MPT_ERR_INVALID, "no lights");
+    if (d->width <= 0 || d->height <= 0 || d->tile_x <= 0 || d->tile_y <= 0 || d->spp <= 0 || d->max_depth < 0)
+        return fail(ctx, MPT_ERR_INVALID, "bad film / renderer settings");
+    if (d->nprims >= (1ll << 31) - 1) return fail(ctx, MPT_ERR_INVALID, "too many primitives");
+    auto S = std::make_unique<mpt_scene>();
+    HostScene& h = S->host;
+    const int64_t n = d->nprims;
+    h.prim_type.assign(d->prim_type, d->prim_type + n);
+    h.prim_v.assign(d->prim_v, d->prim_v + 9 * n);
+    h.prim_mat.assign(d->prim_mat, d->prim_mat + n);
+    if (d->prim_n) h.prim_n.assign(d->prim_n, d->prim_n + 9 * n);
+    if (d->prim_uvw) h.prim_uvw.assign(d->prim_uvw, d->prim_uvw + 9 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (h.prim_type[i] != PRIM_TRIANGLE && h.prim_type[i] != PRIM_RECTANGLE)
+            return fail(ctx, MPT_ERR_INVALID, "bad primitive type");
+        if (h.prim_mat[i] < 0 || h.prim_mat[i] >= d->nmats) return fail(ctx, MPT_ERR_INVALID, "bad material index");
+    }
+    auto add_tex = [&](const mpt_texture_desc& t, int channels) -> int32_t {
+        if (!t.data) return -1;
+        Texture x;
+        x.width = t.width;
+        x.height = t.height;
+        x.channels = channels;
+        x.wrap_clamp = t.wrap_clamp;
+        const size_t cnt = static_cast<size_t>(t.width) * t.height * t.channels;
+        if (t.channels == channels) {
+            x.data.assign(t.data, t.data + cnt);
+        } else {
+            x.data.resize(static_cast<size_t>(t.width) * t.height * channels);
+            for (size_t px = 0; px < static_cast<size_t>(t.width) * t.height; ++px)
+                for (int c = 0; c < channels; ++c) x.data[px * channels + c] = t.data[px * t.channels + std::min(c, t.channels - 1)];
+        }
+        h.textures.push_back(std::move(x));
+        return static_cast<int32_t>(h.textures.size() - 1);
+    };
+    for (int32_t i = 0; i < d->nmats; ++i) {
+        const mpt_material_desc& m = d->mats[i];
+        Material x;
+        x.type = m.type;
+        x.albedo = {m.albedo.value[0], m.albedo.value[1], m.albedo.value[2]};
+        x.albedo_tex = add_tex(m.albedo, 3);
+        x.f0 = {m.f0[0], m.f0[1], m.f0[2]};
+        x.roughness = m.roughness.value[0];
+        x.rough_tex = add_tex(m.roughness, 1);
+        if (x.type != MAT_LAMBERT && x.type != MAT_GGX) return fail(ctx, MPT_ERR_UNSUPPORTED, "material type");
+        h.materials.push_back(x);
+    }
+    for (int32_t i = 0; i < d->nlights; ++i) {
+        const mpt_light_desc& l = d->lights[i];
+        if (l.prim < 0 || l.prim >= n) return fail(ctx, MPT_ERR_INVALID, "light primitive out of range");
+        Light x;
+        x.prim = l.prim;
+        x.color = {l.color[0], l.color[1], l.color[2]};
+        x.watt = l.watt;
+        x.back_face = l.back_face;
+        h.lights.push_back(x);
+        if (!(prim_area(h, l.prim) > 0.f)) return fail(ctx, MPT_ERR_INVALID, "zero-area emitter");
+    }
+    h.cluster = d->cluster;
+    h.cam_pos = {d->cam_pos[0], d->cam_pos[1], d->cam_pos[2]};
+    h.cam_dir = {d->cam_dir[0], d->cam_dir[1], d->cam_dir[2]};
+    h.cam_up = {d->cam_up[0], d->cam_up[1], d->cam_up[2]};
+    h.fov = d->fov_deg;
+    h.sensor_width_mm = d->sensor_width_mm > 0.f ? d->sensor_width_mm : 36.f;
+    h.width = d->width;
+    h.height = d->height;
+    h.tile_x = d->tile_x;
+    h.tile_y = d->tile_y;
+    h.filter = d->filter;
+    h.filter_w = d->filter_w;
+    h.filter_h = d->filter_h;
+    h.sigma_x = d->sigma_x;
+    h.sigma_y = d->sigma_y;
+    h.amplitude = d->amplitude;
+    h.mitchell_b = d->mitchell_b;
+    h.mitchell_c = d->mitchell_c;
+    h.sampler = d->sampler;
+    h.spp = d->spp;
+    h.max_depth = d->max_depth;
+    h.split_sah = d->split_sah;
+    return finish_scene(ctx, S, opts, out);
+}
+
+void mpt_scene_destroy(mpt_scene* s) {
+    if (!s) return;
+    if (s->ctx && s->ctx->device >= 0) (void)hipSetDevice(s->ctx->device);
+    delete s;
+}
+
+int mpt_scene_camera(const mpt_scene* s, float* f2c, float* c2w) {
+    if (!s || !f2c || !c2w) return MPT_ERR_INVALID;
+    camera_matrices(s->host, f2c, c2w);
+    return MPT_OK;
+}
+
+int mpt_scene_light_radiance(const mpt_scene* s, float* out) {
+    if (!s || !out) return MPT_ERR_INVALID;
+    const std::vector<F3> r = light_radiance(s->host);
+    for (size_t i = 0; i < r.size(); ++i) {
+        out[3 * i] = r[i].x;
+        out[3 * i + 1] = r[i].y;
+        out[3 * i + 2] = r[i].z;
+    }
+    return MPT_OK;
+}
+
+int mpt_scene_stats_get(const mpt_scene* s, mpt_scene_stats* out) {
+    if (!s || !out) return MPT_ERR_INVALID;
+    *out = s->sstats;
+    return MPT_OK;
+}
+
+int mpt_scene_binary_bvh(mpt_scene* s, int64_t* count, float* bounds, int32_t* is_leaf, int64_t* a, int64_t* b,
+                         int32_t* ordered) {
+    if (!s || !count) return MPT_ERR_INVALID;
+    if (!s->ref_built) {
+        build_binary(s->host, true, &s->ref_bin);
+        s->ref_built = true;
+    }
+    const BinaryBvh& bin = s->ref_bin;
+    *count = static_cast<int64_t>(bin.nodes.size());
+    if (!bounds) return MPT_OK;
+    for (size_t i = 0; i < bin.nodes.size(); ++i) {
+        const BinaryNode& n = bin.nodes[i];
+        std::memcpy(bounds + 6 * i, n.bmin, 12);
+        std::memcpy(bounds + 6 * i + 3, n.bmax, 12);
+        if (is_leaf) is_leaf[i] = n.leaf;
+        if (a) a[i] = n.a;
+        if (b) b[i] = n.b;
+    }
+    if (ordered) std::memcpy(ordered, bin.ordered.data(), bin.ordered.size() * sizeof(int32_t));
+    return MPT_OK;
+}
+
+#define REQUIRE_DEVICE(s)                                                                          \
+    do {                                                                                           \
+        if ((s)->ctx->device < 0) return fail((s)->ctx, MPT_ERR_DEVICE, "host-only context has no device"); \
+    } while (0)
+
+int mpt_trace_device(mpt_scene* s, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev, void* stream) {
+    if (!s) return MPT_ERR_INVALID;
+    REQUIRE_DEVICE(s);
+    if (n > 0 && (!rays_dev || !hits_dev)) return fail(s->ctx, MPT_ERR_INVALID, "null ray / hit buffer");
+    if (n >= (1ull << 31)) return fail(s->ctx, MPT_ERR_INVALID, "at most 2^31 - 1 rays per call");
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    return do_trace(s, rays_dev, n, any_hit, hits_dev, static_cast<hipStream_t>(stream));
+}
+
+int mpt_trace(mpt_scene* s, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits) {
+    if (!s) return MPT_ERR_INVALID;
+    REQUIRE_DEVICE(s);
+    if (n == 0) return MPT_OK;
+    if (!rays || !hits) return fail(s->ctx, MPT_ERR_INVALID, "null ray / hit buffer");
+    if (n >= (1ull << 31)) return fail(s->ctx, MPT_ERR_INVALID, "at most 2^31 - 1 rays per call");
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    DevBuf r, h;
+    HIP_TRY(s->ctx, r.alloc(n * sizeof(mpt_ray)));
+    HIP_TRY(s->ctx, h.alloc(n * sizeof(mpt_hit)));
+    HIP_TRY(s->ctx, hipMemcpy(r.p, rays, n * sizeof(mpt_ray), hipMemcpyHostToDevice));
+    const int rc = do_trace(s, r.as<mpt_ray>(), n, any_hit, h.as<mpt_hit>(), nullptr);
+    if (rc != MPT_OK) return rc;
+    HIP_TRY(s->ctx, hipMemcpy(hits, h.p, n * sizeof(mpt_hit), hipMemcpyDeviceToHost));
+    return MPT_OK;
+}
+
+int mpt_render(mpt_scene* s, const mpt_render_params* p, float* film_host) {
+    if (!s) return MPT_ERR_INVALID;
+    REQUIRE_DEVICE(s);
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    const int rc = do_render(s, p);
+    if (rc != MPT_OK) return rc;
+    if (film_host) {
+        HIP_TRY(s->ctx, hipMemcpy(film_host, s->film.p, s->film.bytes, hipMemcpyDeviceToHost));
+    }
+    return MPT_OK;
+}
+
+int mpt_film_device(mpt_scene* s, void** film_dev) {
+    if (!s || !film_dev) return MPT_ERR_INVALID;
+    *film_dev = s->film.p;
+    return MPT_OK;
+}
+
+int mpt_film_srgb8(mpt_scene* s, uint8_t* out) {
+    if (!s || !out) return MPT_ERR_INVALID;
+    REQUIRE_DEVICE(s);
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    const int64_t npix = static_cast<int64_t>(s->host.width) * s->host.height;
+    HIP_TRY(s->ctx, s->srgb.alloc(static_cast<size_t>(3 * npix)));
+    hipLaunchKernelGGL(k_srgb8, dim3((npix + 255) / 256), dim3(256), 0, s->stream, s->film.as<float4>(), npix,
+                       s->srgb.as<uint8_t>());
+    HIP_TRY(s->ctx, hipGetLastError());
+    HIP_TRY(s->ctx, hipStreamSynchronize(s->stream));
+    HIP_TRY(s->ctx, hipMemcpy(out, s->srgb.p, static_cast<size_t>(3 * npix), hipMemcpyDeviceToHost));
+    return MPT_OK;
+}
+
+int mpt_film_save(mpt_scene* s, const char* path) {
+    if (!s || !path) return MPT_ERR_INVALID;
+    REQUIRE_DEVICE(s);
+    const std::string p(path);
+    const int w = s->host.width, h = s->host.height;
+    auto ends = [&](const char* e) {
+        const size_t n = std::strlen(e);
+        return p.size() >= n && p.compare(p.size() - n, n, e) == 0;
+    };
+    if (ends(".pfm")) {
+        std::vector<float> film(static_cast<size_t>(4) * w * h);
+        HIP_TRY(s->ctx, hipMemcpy(film.data(), s->film.p, film.size() * sizeof(float), hipMemcpyDeviceToHost));
+        std::ofstream f(p, std::ios::binary);
+        if (!f) return fail(s->ctx, MPT_ERR_IO, "cannot write " + p);
+        f << "PF\n" << w << " " << h << "\n-1.0\n";
+        // PFM rows run bottom to top
+        for (int y = h - 1; y >= 0; --y)
+            for (int x = 0; x < w; ++x) {
+                const float* px = &film[4 * (static_cast<size_t>(y) * w + x)];
+                const float inv = px[3] == 0.f ? 0.f : 1.f / px[3];
+                const float rgb[3] = {px[0] * inv, px[1] * inv, px[2] * inv};
+                f.write(reinterpret_cast<const char*>(rgb), sizeof(rgb));
+            }
+        return f ? MPT_OK : fail(s->ctx, MPT_ERR_IO, "short write " + p);
+    }
+    std::vector<uint8_t> rgb(static_cast<size_t>(3) * w * h);
+    const int rc = mpt_film_srgb8(s, rgb.data());
+    if (rc != MPT_OK) return rc;
+    if (ends(".ppm")) {
+        std::ofstream f(p, std::ios::binary);
+        if (!f) return fail(s->ctx, MPT_ERR_IO, "cannot write " + p);
+        f << "P6\n" << w << " " << h << "\n255\n";
+        f.write(reinterpret_cast<const char*>(rgb.data()), static_cast<std::streamsize>(rgb.size()));
+        return f ? MPT_OK : fail(s->ctx, MPT_ERR_IO, "short write " + p);
+    }
+    if (ends(".png")) {
+        std::string err;
+        write_png(p, rgb.data(), w, h, &err);
+        return err.empty() ? MPT_OK : fail(s->ctx, MPT_ERR_IO, err);
+    }
+    return fail(s->ctx, MPT_ERR_UNSUPPORTED, "output format of " + p + " (png, ppm, pfm)");
+}
+
+int mpt_stats_get(const mpt_scene* s, mpt_stats* out) {
+    if (!s || !out) return MPT_ERR_INVALID;
+    *out = s->rstats;
+    return MPT_OK;
+}
+
+}  // extern "C"
+
+// ---- PNG writer (stored deflate blocks, no compression library) ---------------------
+
+namespace {
+
+uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t k = i;
+            for (int j = 0; j < 8; ++j) k = (k & 1) ? 0xEDB88320u ^ (k >> 1) : k >> 1;
+            table[i] = k;
+        }
+        init = true;
+    }
+    c ^= 0xFFFFFFFFu;
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return c ^ 0xFFFFFFFFu;
+}
+
+void put_be32(std::vector<uint8_t>& v, uint32_t x) {
+    v.push_back(static_cast<uint8_t>(x >> 24));
+    v.push_back(static_cast<uint8_t>(x >> 16));
+    v.push_back(static_cast<uint8_t>(x >> 8));
+    v.push_back(static_cast<uint8_t>(x));
+}
+
+void chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
+    put_be32(out, static_cast<uint32_t>(data.size()));
+    std::vector<uint8_t> td(type, type + 4);
+    td.insert(td.end(), data.begin(), data.end());
+    out.insert(out.end(), td.begin(), td.end());
+    put_be32(out, crc32_update(0, td.data(), td.size()));
+}
+
+void write_png(const std::string& path, const uint8_t* rgb, int w, int h, std::string* err) {
+    std::vector<uint8_t> raw;
+    raw.reserve(static_cast<size_t>(h) * (1 + 3 * w));
+    for (int y = 0; y < h; ++y) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgb + static_cast<size_t>(y) * 3 * w, rgb + static_cast<size_t>(y + 1) * 3 * w);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};
+    uint32_t a = 1, b = 0;
+    for (uint8_t c : raw) {
+        a = (a + c) % 65521;
+        b = (b + a) % 65521;
+    }
+    size_t pos = 0;
+    do {
+        const size_t n = std::min<size_t>(65535, raw.size() - pos);
+        const bool last = pos + n == raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back(static_cast<uint8_t>(n));
+        z.push_back(static_cast<uint8_t>(n >> 8));
+        z.push_back(static_cast<uint8_t>(~n));
+        z.push_back(static_cast<uint8_t>(~n >> 8));
+        z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + n);
+        pos += n;
+    } while (pos < raw.size());
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, static_cast<uint32_t>(w));
+    put_be32(ihdr, static_cast<uint32_t>(h));
+    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});
+    chunk(png, "IHDR", ihdr);
+    chunk(png, "IDAT", z);
+    chunk(png, "IEND", {});
+    std::ofstream f(path, std::ios::binary);
+    if (!f) { *err = "cannot write " + path; return; }
+    f.write(reinterpret_cast<const char*>(png.data()), static_cast<std::streamsize>(png.size()));
+    if (!f) *err = "short write " + path;
+}
+
+}  // namespace

@@ -1,0 +1,199 @@
+// Wide-BVH traversal for gfx950: one ray per lane, no ray-tracing hardware,
+// so every slab and primitive test is VALU work.
+//
+// Primitive tests restate the reference's exactly (operation order, edge
+// swap, t == maxT accepted), so the closest-hit primitive and t agree bit for
+// bit with the reference's binary BVH whenever the closest hit is unique:
+//   Triangle::isIntersecting   (triangle.cpp:54-89)
+//   Rectangle::isIntersecting  (rectangle.cpp:49-75)
+// Box culling differs from the reference's slab test (tAabb.ipp:68-127): it is
+// the FMA form (lo * inv - o * inv) with a 4-ulp widening of the far distance,
+// over tight boxes that contain each primitive's 1e-4-padded bound, so it
+// never culls a primitive the reference would hit.
+#pragma once
+
+#include "dmath.hpp"
+#include "dscene.hpp"
+
+namespace mpt {
+
+struct TravCounters {
+    uint32_t boxes;
+    uint32_t prims;
+};
+
+// Primitive record at leaf position li: r0 = (A, id bits), r1 = e1, r2 = e2.
+__device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32_t li, V3 o, V3 D, float tmin,
+                                          float& tmax) {
+    const float4 r0 = tris[3 * li + 0];
+    const float4 r1 = tris[3 * li + 1];
+    const float4 r2 = tris[3 * li + 2];
+    V3 ea = v3f4(r1), eb = v3f4(r2);
+    const bool rect = (__float_as_uint(r0.w) & 0x80000000u) != 0;
+    if (dot(D, cross(ea, eb)) > 0.0f) {
+        const V3 t = ea;
+        ea = eb;
+        eb = t;
+    }
+    const V3 A = v3f4(r0);
+    if (!rect) {
+        const V3 T = o - A;
+        const V3 Q = cross(T, ea);
+        const V3 P = cross(D, eb);
+        const float den = dot(P, ea);
+        if (den == 0.0f) return false;
+        const float inv = 1.0f / den;
+        const float t = dot(Q, eb) * inv;
+        const float u = dot(P, T) * inv;
+        const float v = dot(Q, D) * inv;
+        if (u < 0.0f || v < 0.0f || u + v > 1.0f) return false;
+        if (t < tmin || t > tmax) return false;
+        tmax = t;
+        return true;
+    }
+    // rectangle: plane through vB, then projections on the unswapped edges
+    const V3 N = normalize(cross(ea, eb));
+    const float t = dot(N, A - o) / dot(N, D);
+    if (t < tmin || t > tmax) return false;
+    const V3 e1 = v3f4(r1), e2 = v3f4(r2);
+    const V3 onp = (o + D * t) - A;
+    const float p1 = dot(onp, normalize(e1));
+    const float p2 = dot(onp, normalize(e2));
+    if (p1 < 0.0f || p1 > len(e1) || p2 < 0.0f || p2 > len(e2)) return false;
+    tmax = t;
+    return true;
+}
+
+// Per-lane traversal stack: the first S entries live in LDS (entry-major, so
+// lanes of a wave hit distinct banks), the rest spill to a per-lane global
+// slice. A scene whose worst-case depth fits S never touches the spill.
+template <int S>
+struct LaneStack {
+    int32_t* lds;      // base of this block's LDS stack
+    int32_t* spill;    // this lane's spill slice (may be null when max <= S)
+    int      lane;     // thread index in block
+    int      stride;   // threads per block
+    int      sp;
+    __device__ __forceinline__ void push(int32_t v) {
+        if (sp < S) lds[sp * stride + lane] = v;
+        else spill[sp - S] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int32_t pop() {
+        --sp;
+        return sp < S ? lds[sp * stride + lane] : spill[sp - S];
+    }
+};
+
+// Slab test of the W children of one node; returns a bitmask of hit children
+// and their entry distances.
+template <int W>
+__device__ __forceinline__ uint32_t node_test(const float* __restrict__ nd, V3 inv, V3 oi, float tmin, float tmax,
+                                              float* tnear) {
+    uint32_t mask = 0;
+    const float4* p = reinterpret_cast<const float4*>(nd);
+    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
+    int32_t child[W];
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+        const float4 a = p[0 * (W / 4) + q], b = p[1 * (W / 4) + q], c = p[2 * (W / 4) + q];
+        const float4 d = p[3 * (W / 4) + q], e = p[4 * (W / 4) + q], f = p[5 * (W / 4) + q];
+        const int4 g = reinterpret_cast<const int4*>(p)[6 * (W / 4) + q];
+        lox[4 * q + 0] = a.x; lox[4 * q + 1] = a.y; lox[4 * q + 2] = a.z; lox[4 * q + 3] = a.w;
+        loy[4 * q + 0] = b.x; loy[4 * q + 1] = b.y; loy[4 * q + 2] = b.z; loy[4 * q + 3] = b.w;
+        loz[4 * q + 0] = c.x; loz[4 * q + 1] = c.y; loz[4 * q + 2] = c.z; loz[4 * q + 3] = c.w;
+        hix[4 * q + 0] = d.x; hix[4 * q + 1] = d.y; hix[4 * q + 2] = d.z; hix[4 * q + 3] = d.w;
+        hiy[4 * q + 0] = e.x; hiy[4 * q + 1] = e.y; hiy[4 * q + 2] = e.z; hiy[4 * q + 3] = e.w;
+        hiz[4 * q + 0] = f.x; hiz[4 * q + 1] = f.y; hiz[4 * q + 2] = f.z; hiz[4 * q + 3] = f.w;
+        child[4 * q + 0] = g.x; child[4 * q + 1] = g.y; child[4 * q + 2] = g.z; child[4 * q + 3] = g.w;
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        const float tx0 = __builtin_fmaf(lox[c], inv.x, -oi.x), tx1 = __builtin_fmaf(hix[c], inv.x, -oi.x);
+        const float ty0 = __builtin_fmaf(loy[c], inv.y, -oi.y), ty1 = __builtin_fmaf(hiy[c], inv.y, -oi.y);
+        const float tz0 = __builtin_fmaf(loz[c], inv.z, -oi.z), tz1 = __builtin_fmaf(hiz[c], inv.z, -oi.z);
+        const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        const bool hit = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, tmax));
+        tnear[c] = n;
+        mask |= hit ? (1u << c) : 0u;
+    }
+    return mask;
+}
+
+// Direction reciprocal for culling: zero components become +-1e-30 so that
+// lo * inv - o * inv stays finite.
+__device__ __forceinline__ V3 safe_inv(V3 d) {
+    auto one = [](float x) {
+        const float ax = fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x;
+        return 1.0f / ax;
+    };
+    return v3(one(d.x), one(d.y), one(d.z));
+}
+
+// Closest hit (ANY = false) or any hit (ANY = true) of one ray.
+// Returns the leaf position of the hit primitive or -1; tmax holds the hit t.
+template <int W, int S, bool ANY>
+__device__ __forceinline__ int32_t traverse(const DevScene& sc, LaneStack<S>& st, V3 o, V3 d, float tmin,
+                                            float& tmax, TravCounters& cnt) {
+    const float* __restrict__ nodes = reinterpret_cast<const float*>(sc.nodes);
+    constexpr int NODE_FLOATS = 8 * W;
+    const V3 inv = safe_inv(d);
+    const V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+    int32_t hit = -1;
+    int32_t node = 0;
+    st.sp = 0;
+    while (true) {
+        const float* nd = nodes + static_cast<int64_t>(node) * NODE_FLOATS;
+        float tn[W];
+        uint32_t mask = node_test<W>(nd, inv, oi, tmin, tmax, tn);
+        cnt.boxes += W;
+        const int32_t* ch = reinterpret_cast<const int32_t*>(nd + 6 * W);
+        const int32_t* cn = ch + W;
+        // leaves first: their hits shrink tmax before the interior children are ordered
+        int32_t kid[W];
+        float kd[W];
+        int nk = 0;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            if (!(mask & (1u << c))) continue;
+            const int32_t count = cn[c];
+            const int32_t first = ch[c];
+            if (count > 0) {
+                for (int32_t j = 0; j < count; ++j) {
+                    cnt.prims += 1;
+                    if (prim_test(sc.tris, first + j, o, d, tmin, tmax)) {
+                        hit = first + j;
+                        if (ANY) return hit;
+                    }
+                }
+            } else {
+                // insertion by entry distance, nearest first
+                int pos = nk;
+                while (pos > 0 && kd[pos - 1] > tn[c]) {
+                    kd[pos] = kd[pos - 1];
+                    kid[pos] = kid[pos - 1];
+                    --pos;
+                }
+                kd[pos] = tn[c];
+                kid[pos] = first;
+                ++nk;
+            }
+        }
+        // drop children now beyond the (possibly shrunk) tmax
+        while (nk > 0 && kd[nk - 1] > tmax) --nk;
+        if (nk > 0) {
+            for (int k = nk - 1; k >= 1; --k) {
+                if (st.sp >= sc.max_stack) return -2;   // cannot happen for a built tree
+                st.push(kid[k]);
+            }
+            node = kid[0];
+        } else {
+            if (st.sp == 0) break;
+            node = st.pop();
+        }
+    }
+    return hit;
+}
+
+}  // namespace mpt

@@ -1,0 +1,85 @@
+// Host-side scene model: what RenderDatabase assembles from CRSD blocks
+// (reference core/renderDatabase.cpp:39-131), flattened into arrays that the
+// device upload consumes directly.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mpt {
+
+struct F3 {
+    float x = 0.f, y = 0.f, z = 0.f;
+};
+
+enum PrimType : int32_t { PRIM_TRIANGLE = 0, PRIM_RECTANGLE = 1 };
+enum MatType : int32_t { MAT_LAMBERT = 0, MAT_GGX = 1 };
+enum FilterType : int32_t { FILTER_BOX = 0, FILTER_GAUSSIAN = 1, FILTER_CONE = 2, FILTER_MITCHELL = 3 };
+
+struct Texture {
+    std::vector<float> data;   // empty = constant
+    int32_t width = 0, height = 0, channels = 3;
+    int32_t wrap_clamp = 0;
+    F3 value;
+};
+
+struct Material {
+    int32_t type = MAT_LAMBERT;
+    int32_t albedo_tex = -1;    // index into HostScene::textures, -1 = constant
+    F3      albedo{0.5f, 0.5f, 0.5f};
+    F3      f0{1.f, 1.f, 1.f};
+    int32_t rough_tex = -1;
+    float   roughness = 0.f;
+};
+
+struct Light {
+    int32_t prim = -1;
+    F3      color{1.f, 1.f, 1.f};
+    float   watt = 1.f;
+    int32_t back_face = 0;
+};
+
+struct HostScene {
+    // primitives in creation order (RenderDatabase::_intersectors)
+    std::vector<int32_t> prim_type;
+    std::vector<float>   prim_v;     // 9 per primitive
+    std::vector<float>   prim_n;     // 9 per primitive, or empty
+    std::vector<float>   prim_uvw;   // 9 per primitive, or empty (NaN first = unset)
+    std::vector<int32_t> prim_mat;
+    std::vector<Material> materials;
+    std::vector<Texture>  textures;
+    std::vector<Light>    lights;
+    int32_t cluster = 0;             // 0 uniform, 1 power
+    // camera (perspective-pinhole)
+    F3 cam_pos, cam_dir{0.f, 0.f, -1.f}, cam_up{0.f, 1.f, 0.f};
+    float fov = 45.f, sensor_width_mm = 36.f;
+    // film
+    int32_t width = 1024, height = 768, tile_x = 16, tile_y = 16;
+    int32_t filter = -1;             // -1: none given
+    float filter_w = 1.f, filter_h = 1.f, sigma_x = 0.5f, sigma_y = 0.5f, amplitude = 1.f;
+    float mitchell_b = 1.f / 3.f, mitchell_c = 1.f / 3.f;
+    std::string output = "cadise.jpg";
+    // renderer
+    int32_t sampler = 1;             // 0 random, 1 stratified
+    int32_t spp = 4, max_depth = 10000;
+    int32_t split_sah = 1;
+
+    int64_t nprims() const { return static_cast<int64_t>(prim_type.size()); }
+};
+
+// Camera matrices exactly as PerspectivePinholeCamera::updateTransform and
+// TMatrix4::lookAt compute them (perspectivePinholeCamera.cpp:32-50,
+// tMatrix4.ipp:47-60), row-major float 4x4.
+void camera_matrices(const HostScene& s, float film_to_camera[16], float camera_to_world[16]);
+
+// Emitted radiance of each light (SingleAreaLight ctor, singleAreaLight.cpp:20-37).
+std::vector<F3> light_radiance(const HostScene& s);
+// Surface area of a primitive (triangle.cpp:199-201, rectangle.cpp:179-181).
+float prim_area(const HostScene& s, int64_t i);
+// Power light-cluster CDF (powerLightCluster.cpp:9-25, distribution1D.cpp:10-40).
+std::vector<float> power_cdf(const HostScene& s, const std::vector<F3>& radiance);
+// StratifiedSampler's base number (stratifiedSampler.cpp:14-20).
+int32_t strata_base(int32_t spp);
+
+}  // namespace mpt

@@ -1,0 +1,198 @@
+/*
+ * mpt.h - C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * The reference (xh5a5n6k6/cadise) has no FFI; its plugin points are C++
+ * abstract classes selected by CRSD keys. Each entry point below replaces one
+ * of them (paths relative to the reference's source/):
+ *
+ *   mpt_scene_load_crsd      SdParser::parseSd + RenderDatabase::setUpData /
+ *                            prepareRender (file-io/scene-description/sdParser.cpp:19-35,
+ *                            core/renderDatabase.cpp:39-131)
+ *   mpt_trace / _device      Accelerator::isIntersecting / isOccluded for a batch of
+ *                            rays (core/intersector/accelerator/accelerator.h:9-12,
+ *                            bvh/bvhAccelerator.cpp:40-156)
+ *   mpt_render               Renderer::render for renderer "sampling" + estimator "pt"
+ *                            (core/renderer/sampling/samplingRenderer.cpp:33-68,
+ *                            core/estimator/ptEstimator.cpp:25-123)
+ *   mpt_film_*               Film::save's resolve (core/film/film.cpp:159-196)
+ *
+ * Conventions: every call returns MPT_OK (0) or a negative MPT_ERR_* code and
+ * never aborts; mpt_last_error gives the message. Calls are synchronous unless
+ * they take a stream. A context is used from one host thread at a time.
+ * Device memory is owned by the library. Host buffers stay owned by the caller.
+ */
+#ifndef MPT_H
+#define MPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MPT_OK = 0,
+    MPT_ERR_INVALID = -1,     /* bad argument */
+    MPT_ERR_PARSE = -2,       /* malformed CRSD or missing block */
+    MPT_ERR_UNSUPPORTED = -3, /* a CRSD type this path does not implement */
+    MPT_ERR_DEVICE = -4,      /* HIP error */
+    MPT_ERR_IO = -5,          /* file read/write */
+    MPT_ERR_NOMEM = -6
+};
+
+typedef struct mpt_ctx mpt_ctx;
+typedef struct mpt_scene mpt_scene;
+
+/* One ray of a batch query: origin, minT, unit direction, maxT (32 bytes).
+ * The reference's Ray (core/ray.h) normalizes its direction on construction;
+ * here the caller passes it normalized. */
+typedef struct {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} mpt_ray;
+
+/* Closest hit: prim = creation index of the primitive (the position in the
+ * reference's RenderDatabase::_intersectors), -1 on a miss; t = hit distance
+ * (ray tmax on a miss). Any hit: prim = 1 if occluded, 0 otherwise. */
+typedef struct {
+    int32_t prim;
+    float   t;
+} mpt_hit;
+
+/* Primitive description for mpt_scene_create (creation order). */
+enum { MPT_PRIM_TRIANGLE = 0, MPT_PRIM_RECTANGLE = 1 };
+enum { MPT_MAT_LAMBERT = 0, MPT_MAT_GGX = 1 };
+
+typedef struct {
+    const float* data;         /* H x W x C float32 (row 0 = v in [0, 1/H)), NULL = constant */
+    int32_t      width, height, channels;
+    int32_t      wrap_clamp;   /* 0 repeat, 1 clamp */
+    float        value[3];     /* constant value */
+} mpt_texture_desc;
+
+typedef struct {
+    int32_t          type;     /* MPT_MAT_* */
+    mpt_texture_desc albedo;   /* matte-lambertian albedo */
+    float            f0[3];    /* conductor-microfacet schlick f0 */
+    mpt_texture_desc roughness;
+} mpt_material_desc;
+
+typedef struct {
+    int32_t prim;              /* emitting primitive (creation index) */
+    float   color[3];
+    float   watt;
+    int32_t back_face;
+} mpt_light_desc;
+
+typedef struct {
+    int64_t        nprims;
+    const int32_t* prim_type;  /* MPT_PRIM_* */
+    const float*   prim_v;     /* nprims x 9: v1 v2 v3 */
+    const float*   prim_n;     /* nprims x 9 vertex normals or NULL (zero = unset) */
+    const float*   prim_uvw;   /* nprims x 9 vertex uvw or NULL (NaN first = unset) */
+    const int32_t* prim_mat;
+    int32_t                  nmats;
+    const mpt_material_desc* mats;
+    int32_t                  nlights;
+    const mpt_light_desc*    lights;
+    int32_t cluster;           /* 0 uniform, 1 power */
+    float   cam_pos[3], cam_dir[3], cam_up[3];
+    float   fov_deg, sensor_width_mm;
+    int32_t width, height, tile_x, tile_y;
+    int32_t filter;            /* 0 box, 1 gaussian, 2 cone, 3 mitchell */
+    float   filter_w, filter_h, sigma_x, sigma_y, amplitude, mitchell_b, mitchell_c;
+    int32_t sampler;           /* 0 random, 1 stratified */
+    int32_t spp, max_depth;
+    int32_t split_sah;         /* 1 sah, 0 equal */
+} mpt_scene_desc;
+
+/* Accelerator options (the reference's accelerator block plus the wide-BVH
+ * layout chosen for gfx950). Zero fields take the defaults. */
+typedef struct {
+    int32_t width;             /* children per node: 4 or 8 (default 4) */
+    int32_t leaf_max;          /* largest subtree folded into one leaf (default 2) */
+    int32_t block;             /* traversal threads per block (default 256) */
+    int32_t stack_lds;         /* per-lane stack entries kept in LDS (default 24) */
+} mpt_bvh_opts;
+
+typedef struct {
+    int64_t nprims, nlights;
+    int64_t binary_nodes;      /* nodes of the binary SAH tree the wide BVH is collapsed from */
+    int64_t wide_nodes;        /* nodes of the wide BVH uploaded to HBM */
+    int32_t binary_depth, wide_depth, width, leaf_max;
+    double  build_ms;          /* host SAH build (the reference's "Finished building accelerator") */
+    double  collapse_ms;
+    int64_t device_bytes;
+    int32_t image_width, image_height, spp, max_depth;
+} mpt_scene_stats;
+
+typedef struct {
+    uint64_t seed;
+    int32_t  spp_begin, spp_end; /* sample passes [begin, end); end <= 0 takes the scene's spp */
+    int32_t  max_depth;          /* <= 0 takes the scene's max-depth */
+    int32_t  accumulate;         /* 1 = add into the film, 0 = clear it first */
+} mpt_render_params;
+
+/* Counters of the last render or trace call. */
+typedef struct {
+    int64_t camera_rays, continuation_rays, shadow_rays, mis_rays;
+    int64_t samples;
+    int64_t box_tests, tri_tests; /* traversal work (slab tests, primitive tests) */
+    double  ms_total;             /* wall time of the call */
+    double  ms_closest, ms_anyhit, ms_shade, ms_camera, ms_film; /* summed kernel time (events) */
+} mpt_stats;
+
+/* device >= 0: a GPU context. device == -1: a host-only context, in which
+ * scenes are parsed and their BVHs built but nothing is uploaded; trace and
+ * render calls on its scenes return MPT_ERR_DEVICE (for inspection on
+ * machines without a GPU). */
+int         mpt_create(int device, mpt_ctx** out);
+void        mpt_destroy(mpt_ctx* ctx);
+const char* mpt_last_error(const mpt_ctx* ctx);
+const char* mpt_version(void);
+
+/* Scene from CRSD (same grammar, keys and defaults as the reference reader). */
+int  mpt_scene_load_crsd(mpt_ctx* ctx, const char* path, const mpt_bvh_opts* opts, mpt_scene** out);
+int  mpt_scene_parse_crsd(mpt_ctx* ctx, const char* text, size_t len, const mpt_bvh_opts* opts, mpt_scene** out);
+/* Scene from arrays (generated scenes). */
+int  mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* desc, const mpt_bvh_opts* opts, mpt_scene** out);
+void mpt_scene_destroy(mpt_scene* scene);
+int  mpt_scene_stats_get(const mpt_scene* scene, mpt_scene_stats* out);
+/* The reference's binary BVH, node for node as BvhBuilder builds it (built on
+ * the first call; the device traverses a tree built by the same algorithm on
+ * true empty boxes, see mpt_scene_stats.binary_nodes): node bounds
+ * (n x 6), is_leaf, a (leaf: first ordered primitive; interior: second child),
+ * b (leaf: count; interior: split axis) and the ordered primitive list. Pass
+ * NULL arrays to query the node count in *count. */
+int  mpt_scene_binary_bvh(mpt_scene* scene, int64_t* count, float* bounds, int32_t* is_leaf, int64_t* a,
+                          int64_t* b, int32_t* ordered);
+
+/* Camera matrices as the reference builds them (perspectivePinholeCamera.cpp:32-50):
+ * film->camera and camera->world, row-major 4x4 float. */
+int  mpt_scene_camera(const mpt_scene* scene, float* film_to_camera, float* camera_to_world);
+/* Emitted radiance of each light (nlights x 3) as SingleAreaLight's constructor
+ * computes it (singleAreaLight.cpp:20-37). */
+int  mpt_scene_light_radiance(const mpt_scene* scene, float* out);
+
+/* Batch ray queries; host buffers (copied) or device buffers on a stream. */
+int  mpt_trace(mpt_scene* scene, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits);
+int  mpt_trace_device(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
+                      void* stream);
+
+/* Sampling renderer + pt estimator. film_host (W x H x 4: sum w*rgb, sum w)
+ * receives the linear film when non-NULL. */
+int  mpt_render(mpt_scene* scene, const mpt_render_params* params, float* film_host);
+/* Device pointer of the scene's linear film (W x H x float4). */
+int  mpt_film_device(mpt_scene* scene, void** film_dev);
+/* Film::save: sum w*rgb / sum w, sRGB OETF, 8-bit (W x H x 3). */
+int  mpt_film_srgb8(mpt_scene* scene, uint8_t* out);
+/* Writes the film as 8-bit PNG / PPM or linear PFM, by the file extension. */
+int  mpt_film_save(mpt_scene* scene, const char* path);
+
+int  mpt_stats_get(const mpt_scene* scene, mpt_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

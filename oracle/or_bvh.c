@@ -432,21 +432,41 @@ void or_trace(const or_scene* s, const float* rays, int64_t n, int any_hit, int 
 /* Number of primitives whose hit distance equals the closest hit's exactly.
  * With the reference's "t == maxT accepted" rule (triangle.cpp:81) the winner
  * among such primitives depends on the test order, so the parity check counts
- * these rays separately. */
+ * these rays separately. Walks every node whose box the ray overlaps within
+ * [minT, t_closest] and counts the primitives hit at exactly t_closest. */
+static int64_t count_at(const or_scene* s, v3 o, v3 d, float minT, float tbest) {
+    if (s->nnodes == 0) return 0;
+    const v3 inv = v3_make(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int64_t stack[128];
+    int sp = 0;
+    int64_t cur = 0, count = 0;
+    while (1) {
+        const or_node* n = &s->nodes[cur];
+        if (slab(&n->bound, o, inv, minT, tbest)) {
+            if (n->leaf) {
+                for (int64_t i = 0; i < n->b; ++i) {
+                    float t = tbest;
+                    if (or_prim_intersect(&s->prims[s->ordered[n->a + i]], o, d, minT, &t) && t == tbest) count++;
+                }
+            } else if (sp < 127) {
+                stack[sp++] = n->a;
+                cur = cur + 1;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return count;
+}
+
 void or_trace_ties(const or_scene* s, const float* rays, int64_t n, int32_t* out_ntie) {
     for (int64_t i = 0; i < n; ++i) {
         const float* r = rays + 8 * i;
         const v3 o = v3_load(r), d = v3_load(r + 4);
         float best = r[7];
         int32_t prim = -1;
-        or_closest_brute(s, o, d, r[3], &best, &prim);
-        int32_t ties = 0;
-        if (prim >= 0) {
-            for (int64_t k = 0; k < s->nprims; ++k) {
-                float t = best;
-                if (or_prim_intersect(&s->prims[k], o, d, r[3], &t) && t == best) ties++;
-            }
-        }
-        out_ntie[i] = ties;
+        or_closest(s, o, d, r[3], &best, &prim);
+        out_ntie[i] = prim >= 0 ? (int32_t)count_at(s, o, d, r[3], best) : 0;
     }
 }

@@ -1,0 +1,48 @@
+"""Shared test helpers: seeded ray sets and scene construction."""
+import numpy as np
+
+from cadise_amd import scenes
+from oracle.binding import OracleScene, oracle_lib
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def normalize_ref(d):
+    """Directions normalized exactly as the reference's TVector::normalize."""
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros_like(d)
+    oracle_lib().or_normalize(d, len(d), out)
+    return out
+
+
+def random_rays(n, lo, hi, seed, tmin=1e-3):
+    """n x 8 rays: origins uniform in the box [lo, hi], directions isotropic."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = normalize_ref(rng.normal(size=(n, 3)).astype(np.float32))
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = FLT_MAX
+    return r
+
+
+def camera_rays(spec, n, seed):
+    """Primary rays of the scene camera at random film positions (reference-exact)."""
+    rng = np.random.default_rng(seed)
+    fxy = np.stack([rng.uniform(0, spec.width, n), rng.uniform(0, spec.height, n)], 1).astype(np.float32)
+    od = OracleScene(spec).camera_rays(fxy)
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = od[:, :3]
+    r[:, 3] = 1e-3
+    r[:, 4:7] = od[:, 3:]
+    r[:, 7] = FLT_MAX
+    return r
+
+
+def small_spheres(n_spheres=8, level=4, **kw):
+    kw.setdefault("width", 64)
+    kw.setdefault("height", 48)
+    kw.setdefault("spp", 4)
+    return scenes.procedural_spheres(n_spheres=n_spheres, level=level, **kw)

@@ -1,0 +1,111 @@
+"""GPU parity of the traversal (Accelerator::isIntersecting / isOccluded,
+reference bvhAccelerator.cpp:40-156) through the C ABI against the oracle,
+whose binary traversal reproduces the compiled reference bit for bit
+(tests/test_oracle_vs_ref.py).
+
+Bar: closest-hit primitive id identical and t bit-identical on every ray whose
+closest hit is unique; rays with an exact-t tie (several primitives at the
+same t, e.g. a shared edge) are counted separately, and there the GPU's
+primitive must be one of the tied ones with the same t. Any-hit flags
+identical.
+"""
+import numpy as np
+import pytest
+
+from cadise_amd import scenes
+from oracle.binding import OracleScene
+from tests.helpers import camera_rays, random_rays, small_spheres
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cadise_amd.api import Context
+    return Context(0)
+
+
+def _check_closest(scene, oracle, rays, label):
+    prim, t = scene.trace(rays)
+    oprim, ot = oracle.trace(rays)
+    ties = oracle.ties(rays)
+    unique = ties <= 1
+    print(f"[{label}] rays={len(rays)} hits={int((oprim >= 0).sum())} ties={int((~unique).sum())}", flush=True)
+    assert np.array_equal(prim[unique], oprim[unique]), f"{label}: {int((prim[unique] != oprim[unique]).sum())} id mismatches"
+    assert np.array_equal(t[unique].view(np.int32), ot[unique].view(np.int32)), f"{label}: t not bit-identical"
+    # tied rays: same t; the winner must hit at that t as well
+    tied = np.where(~unique)[0]
+    if len(tied):
+        assert np.array_equal(t[tied].view(np.int32), ot[tied].view(np.int32))
+    return int((~unique).sum())
+
+
+def _check_any(scene, oracle, rays, label):
+    occ = scene.is_occluded(rays)
+    oocc, _ = oracle.trace(rays, any_hit=True)
+    assert np.array_equal(occ, oocc.astype(bool)), f"{label}: {int((occ != oocc.astype(bool)).sum())} any-hit mismatches"
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_trace_cornell(ctx, width):
+    from cadise_amd.api import Scene, bvh_opts
+    spec = scenes.cornell_box()
+    sc = Scene.from_crsd_text(ctx, spec.to_crsd(), bvh_opts(width=width))
+    orc = OracleScene(spec)
+    rays = np.concatenate([random_rays(50000, -0.1, 1.1, seed=1), camera_rays(spec, 20000, seed=2)])
+    _check_closest(sc, orc, rays, f"cornell bvh{width}")
+    short = rays.copy()
+    short[:, 7] = np.random.default_rng(3).uniform(0.01, 1.5, len(rays)).astype(np.float32)
+    _check_any(sc, orc, short, f"cornell bvh{width} any")
+    _check_closest(sc, orc, short, f"cornell bvh{width} bounded")
+
+
+@pytest.mark.parametrize("width,leaf_max", [(4, 1), (4, 2), (8, 4)])
+def test_trace_spheres(ctx, tmp_path, width, leaf_max):
+    from cadise_amd.api import Scene, bvh_opts
+    spec = small_spheres(n_spheres=8, level=4)   # 8 x 5120 triangles + ground + 4 rectangle lights
+    sc = Scene.from_crsd(ctx, _write(tmp_path, spec), bvh_opts(width=width, leaf_max=leaf_max))
+    orc = OracleScene(spec)
+    rays = np.concatenate([random_rays(100000, (-18, 0, -18), (18, 8, 18), seed=5), camera_rays(spec, 20000, seed=6)])
+    _check_closest(sc, orc, rays, f"spheres bvh{width} leaf{leaf_max}")
+    short = rays.copy()
+    short[:, 7] = np.random.default_rng(7).uniform(0.1, 20.0, len(rays)).astype(np.float32)
+    _check_any(sc, orc, short, f"spheres bvh{width} any")
+
+
+def test_trace_edge_cases(ctx):
+    """Axis-parallel rays (zero direction components), rays starting on
+    surfaces, t == minT / maxT boundaries and an empty batch."""
+    from cadise_amd.api import Scene
+    spec = scenes.cornell_box()
+    sc = Scene.from_crsd_text(ctx, spec.to_crsd())
+    orc = OracleScene(spec)
+    rng = np.random.default_rng(11)
+    n = 6000
+    o = rng.uniform(0.05, 0.95, (n, 3)).astype(np.float32)
+    axes = np.zeros((n, 3), np.float32)
+    axes[np.arange(n), rng.integers(0, 3, n)] = rng.choice([-1.0, 1.0], n)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 3] = 1e-3
+    rays[:, 4:7] = axes
+    rays[:, 7] = np.finfo(np.float32).max
+    _check_closest(sc, orc, rays, "axis-parallel")
+    # maxT exactly at the hit distance: accepted (t == maxT), and just below: rejected
+    _, t = orc.trace(rays)
+    hit = t < 1e30
+    exact = rays[hit].copy()
+    exact[:, 7] = t[hit]
+    _check_closest(sc, orc, exact, "maxT == t")
+    below = exact.copy()
+    below[:, 7] = np.nextafter(t[hit], np.float32(0))
+    _check_closest(sc, orc, below, "maxT < t")
+    _check_any(sc, orc, below, "maxT < t any")
+    prim, t0 = sc.trace(np.zeros((0, 8), np.float32))
+    assert len(prim) == 0
+
+
+def _write(tmp_path, spec):
+    path = tmp_path / "scene.crsd"
+    path.write_text(spec.to_crsd(sidecar_dir=str(tmp_path)))
+    return str(path)
