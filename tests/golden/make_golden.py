@@ -1,0 +1,105 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Runs only where the reference harness is built (oracle/_ref/libcadise_ref.so,
+compiled by oracle/Makefile from the reference's own sources). Every fixture
+is inputs plus the reference's outputs for them; the tests compare the oracle
+(and, on the GPU, the product) against these files.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from cadise_amd import scenes  # noqa: E402
+from oracle.binding import RefScene, ref_available, seq_key  # noqa: E402
+from tests.helpers import normalize_ref, stable_dirs  # noqa: E402
+
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+
+
+def _rays(n, lo, hi, seed):
+    """Seeded rays whose unit directions the reference's Ray constructor keeps unchanged."""
+    rng = np.random.default_rng(seed)
+    d, keep = stable_dirs(rng.normal(size=(2 * n, 3)).astype(np.float32))
+    d = d[keep][:n]
+    r = np.zeros((len(d), 8), np.float32)
+    r[:, :3] = rng.uniform(lo, hi, (len(d), 3))
+    r[:, 3] = 1e-3
+    r[:, 4:7] = d
+    r[:, 7] = FLT_MAX
+    return r
+
+
+def scene_set():
+    """The golden scenes: (name, SceneSpec)."""
+    cornell = scenes.cornell_box(width=32, height=32, spp=4, max_depth=8)
+    cornell_g = scenes.cornell_box(width=24, height=20, spp=4, max_depth=6, filter="gaussian")
+    spheres = scenes.procedural_spheres(n_spheres=5, level=2, width=24, height=16, spp=2, max_depth=6, seed=99)
+    spheres.cluster = "power"
+    equal = scenes.procedural_spheres(n_spheres=4, level=2, width=8, height=8, spp=1, seed=7, with_normals=False)
+    equal.split_mode = "equal"
+    return [("cornell", cornell), ("cornell_gauss", cornell_g), ("spheres", spheres), ("spheres_equal", equal)]
+
+
+def main():
+    if not ref_available():
+        raise SystemExit("oracle/_ref/libcadise_ref.so is not built (make -C oracle ref)")
+    for name, spec in scene_set():
+        side = tempfile.mkdtemp()
+        R = RefScene(spec.to_crsd(sidecar_dir=side))
+        out = {}
+        bvh = R.bvh(sah=spec.split_mode != "equal")
+        for k, v in bvh.items():
+            out["bvh_" + k] = v
+        lo, hi = (-0.2, 1.2) if name.startswith("cornell") else ((-18, 0, -18), (18, 6, 18))
+        rays = _rays(4000, lo, hi, seed=len(name))
+        prim, t, d = R.trace(rays)
+        assert np.array_equal(d.view(np.int32), rays[:, 4:7].view(np.int32))
+        rays_n = rays
+        occ, _, _ = R.trace(np.concatenate([rays[:, :7], np.full((len(rays), 1), 2.0, np.float32)], 1), any_hit=True)
+        out.update(rays=rays_n, closest_prim=prim, closest_t=t, anyhit_maxT2=occ)
+        out["detail"] = R.intersect_detail(rays_n)
+        rng = np.random.default_rng(4)
+        fxy = np.stack([rng.uniform(0, spec.width, 1500), rng.uniform(0, spec.height, 1500)], 1).astype(np.float32)
+        out["film_xy"] = fxy
+        out["camera"] = R.camera_rays(fxy)
+        keys = np.array([seq_key(2024, i, 1) for i in range(len(fxy))], np.uint64)
+        rgb, draws = R.estimate(fxy, keys)
+        out.update(est_keys=keys, est_rgb=rgb, est_draws=draws)
+        out["film"] = R.render_linear(seed=31, threads=4)
+        out["light_hit"] = R.light_hit(rays_n)
+        # single-function vectors
+        if name == "cornell" or name == "spheres":
+            m = len(spec.materials)
+            q = np.zeros((m, 300, 14), np.float32)
+            for mi in range(m):
+                ns = normalize_ref(rng.normal(size=(300, 3)).astype(np.float32))
+                wi = normalize_ref(rng.normal(size=(300, 3)).astype(np.float32))
+                wo = normalize_ref(rng.normal(size=(300, 3)).astype(np.float32))
+                q[mi, :, 0:3], q[mi, :, 3:6], q[mi, :, 6:9] = ns, wi, wo
+                q[mi, :, 9:12] = rng.uniform(0, 1, (300, 3))
+                q[mi, :, 12:14] = rng.uniform(0, 1, (300, 2))
+            bsdf = np.stack([R.bsdf_query(spec.materials[mi].name, q[mi]) for mi in range(min(m, 6))])
+            out.update(bsdf_in=q[: len(bsdf)], bsdf_out=bsdf)
+            nl = len(spec.lights)
+            lq = np.zeros((nl, 200, 3 + 16), np.float32)
+            lq[:, :, 0:3] = rng.uniform(lo, hi, (nl, 200, 3)) if name == "cornell" else rng.uniform(-10, 10, (nl, 200, 3))
+            lq[:, :, 3:] = rng.uniform(0, 1, (nl, 200, 16))
+            out["light_in"] = lq
+            out["light_out"] = np.stack([R.light_sample(li, lq[li], 16) for li in range(nl)])
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **out)
+        print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

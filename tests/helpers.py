@@ -15,11 +15,26 @@ def normalize_ref(d):
     return out
 
 
+def stable_dirs(d):
+    """Directions that the reference's normalize maps to themselves bit for bit
+    (its Ray constructor normalizes every direction it is given), by
+    re-normalizing until a fixed point; rows that never settle are dropped."""
+    d = normalize_ref(d)
+    keep = np.ones(len(d), bool)
+    for _ in range(4):
+        nd = normalize_ref(d)
+        same = np.all(nd.view(np.int32) == d.view(np.int32), axis=1)
+        d = nd
+        keep = same
+    return d, keep
+
+
 def random_rays(n, lo, hi, seed, tmin=1e-3):
     """n x 8 rays: origins uniform in the box [lo, hi], directions isotropic."""
     rng = np.random.default_rng(seed)
     o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
-    d = normalize_ref(rng.normal(size=(n, 3)).astype(np.float32))
+    d, keep = stable_dirs(rng.normal(size=(n, 3)).astype(np.float32))
+    d[~keep] = normalize_ref(d[~keep])
     r = np.zeros((n, 8), np.float32)
     r[:, :3] = o
     r[:, 3] = tmin
