@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
     const int64_t n = a.dual ? 2 * n0 : n0;
     TravCounters cnt{0, 0};
-    unsigned long long rays = 0, errors = 0;
+    unsigned long long rays = 0, rays2 = 0, errors = 0;
     while (true) {
         int base = 0;
         if (lane == 0) base = atomicAdd(a.work, WAVE);
@@ -57,7 +57,8 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         if (d4.w < 0.0f || a.sc.nnodes == 0) {
             out = make_int2(ANY ? 0 : -1, __float_as_int(d4.w));
         } else {
-            rays += 1;
+            if (second) rays2 += 1;
+            else rays += 1;
             float tmax = d4.w;
             const int32_t h = traverse<W, S, ANY>(a.sc, st, v3f4(o4), v3f4(d4), o4.w, tmax, cnt);
             if (h == -2) errors += 1;
@@ -74,12 +75,13 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         else a.hit[r] = out;
     }
     // one atomic per wave for the counters
-    const unsigned long long sr = wave_sum(rays), sb = wave_sum(cnt.boxes), sp = wave_sum(cnt.prims),
-                             se = wave_sum(errors);
+    const unsigned long long sr = wave_sum(rays), sr2 = wave_sum(rays2), sb = wave_sum(cnt.boxes),
+                             sp = wave_sum(cnt.prims), se = wave_sum(errors);
     if (lane == 0 && a.stats) {
         atomicAdd(&a.stats[a.stat_slot], sr);
-        atomicAdd(&a.stats[ST_BOXES], sb);
-        atomicAdd(&a.stats[ST_PRIMS], sp);
+        if (sr2) atomicAdd(&a.stats[ST_MIS_RAYS], sr2);
+        atomicAdd(&a.stats[ANY ? ST_BOXES_ANY : ST_BOXES], sb);
+        atomicAdd(&a.stats[ANY ? ST_PRIMS_ANY : ST_PRIMS], sp);
         if (se) atomicAdd(&a.stats[ST_ERRORS], se);
     }
 }

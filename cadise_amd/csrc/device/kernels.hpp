@@ -15,8 +15,10 @@ enum StatSlot {
     ST_CLOSEST_RAYS = 0,  // valid continuation / camera rays traced
     ST_MIS_RAYS,          // valid MIS-BSDF rays traced
     ST_SHADOW_RAYS,       // valid shadow rays traced
-    ST_BOXES,             // slab tests
-    ST_PRIMS,             // primitive tests
+    ST_BOXES,             // slab tests of closest-hit queries
+    ST_PRIMS,             // primitive tests of closest-hit queries
+    ST_BOXES_ANY,         // slab tests of any-hit queries
+    ST_PRIMS_ANY,         // primitive tests of any-hit queries
     ST_ERRORS,            // traversal stack overruns (must stay 0)
     ST_COUNT
 };

@@ -409,8 +409,10 @@ int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_h
         unsigned long long st[ST_COUNT];
         if ((rc = read_stats(S, st)) != MPT_OK) break;
         S->rstats = mpt_stats{};
-        S->rstats.box_tests = static_cast<int64_t>(st[ST_BOXES]);
-        S->rstats.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
+        S->rstats.box_tests = static_cast<int64_t>(st[any_hit ? ST_BOXES_ANY : ST_BOXES]);
+        S->rstats.tri_tests = static_cast<int64_t>(st[any_hit ? ST_PRIMS_ANY : ST_PRIMS]);
+        S->rstats.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
+        S->rstats.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
         if (any_hit) S->rstats.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
         else S->rstats.camera_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]);
         if (st[ST_ERRORS]) rc = fail(S->ctx, MPT_ERR_DEVICE, "traversal stack overrun");
@@ -572,10 +574,14 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     r.samples = npix * passes;
     r.camera_rays = npix * (max_depth > 0 ? passes : 0);
     r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
-    r.mis_rays = 0;
+    r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
     r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
     r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
     r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
+    r.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
+    r.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
+    r.closest_launches = static_cast<int64_t>(passes) * (max_depth + 1);
+    r.anyhit_launches = static_cast<int64_t>(passes) * max_depth;
     r.ms_closest = ms[EV_CLOSEST];
     r.ms_anyhit = ms[EV_ANY];
     r.ms_shade = ms[EV_SHADE];

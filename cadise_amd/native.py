@@ -67,6 +67,8 @@ class RenderParams(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("camera_rays", C.c_int64), ("continuation_rays", C.c_int64), ("shadow_rays", C.c_int64),
                 ("mis_rays", C.c_int64), ("samples", C.c_int64), ("box_tests", C.c_int64), ("tri_tests", C.c_int64),
+                ("box_tests_any", C.c_int64), ("tri_tests_any", C.c_int64), ("closest_launches", C.c_int64),
+                ("anyhit_launches", C.c_int64),
                 ("ms_total", C.c_double), ("ms_closest", C.c_double), ("ms_anyhit", C.c_double),
                 ("ms_shade", C.c_double), ("ms_camera", C.c_double), ("ms_film", C.c_double)]
 

@@ -327,25 +327,32 @@ def _perlin3(p: np.ndarray, perm: np.ndarray) -> np.ndarray:
 
 
 def procedural_spheres(seed=0x20261203, n_spheres=64, level=5, width=1920, height=1080, spp=64, max_depth=8,
-                       with_normals=True) -> SceneSpec:
+                       with_normals=True, offset=(64.0, 0.0, 64.0)) -> SceneSpec:
     """64 subdivided icospheres (20 * 4^level triangles each) with vertices
     displaced along the normal by Perlin noise of amplitude 0.1 r; radii
     log-uniform in [0.2, 2]; centres rejection-sampled without overlap above a
     two-triangle ground plane; GGX conductors with roughness U[0.05, 1] and
-    f0 U[0.1, 0.9]^3; four quad (rectangle) lights."""
+    f0 U[0.1, 0.9]^3; four quad (rectangle) lights.
+
+    `offset` places the scene in the positive octant: the reference's empty
+    bounding box has max = FLT_MIN (tAabb.ipp:13-17), so any node holding
+    negative coordinates reaches the origin and its SAH stops splitting; with
+    positive coordinates its BVH is a normal SAH tree and its CPU timing fair."""
     rng = np.random.default_rng(seed)
     perm = rng.permutation(256)
     perm = np.concatenate([perm, perm, perm]).astype(np.int64)
     uv, uf = icosphere(level)
+    ox, oy, oz = offset
     s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="box", fov=60.0,
-                  cam_pos=(0.0, 9.0, 30.0), cam_dir=(0.0, -0.28, -1.0), cam_up=(0.0, 1.0, 0.0))
+                  cam_pos=(ox + 0.0, oy + 9.0, oz + 30.0), cam_dir=(0.0, -0.28, -1.0), cam_up=(0.0, 1.0, 0.0))
     s.materials.append(Material("ground", albedo=(0.5, 0.5, 0.5)))
     s.materials.append(Material("lightmat", albedo=(0.8, 0.8, 0.8)))
     extent = 16.0
     centres, radii = [], []
     while len(centres) < n_spheres:
         r = float(np.exp(rng.uniform(math.log(0.2), math.log(2.0))))
-        c = np.array([rng.uniform(-extent, extent), r * 1.1 + rng.uniform(0.0, 3.0), rng.uniform(-extent, extent)])
+        c = np.array([ox + rng.uniform(-extent, extent), oy + r * 1.1 + rng.uniform(0.0, 3.0),
+                      oz + rng.uniform(-extent, extent)])
         if all(np.linalg.norm(c - c2) > (r + r2) * 1.15 for c2, r2 in zip(centres, radii)):
             centres.append(c)
             radii.append(r)
@@ -367,9 +374,11 @@ def procedural_spheres(seed=0x20261203, n_spheres=64, level=5, width=1920, heigh
                                     roughness=float(rng.uniform(0.05, 1.0))))
         s.meshes.append(Mesh(f"sphere{i}", mname, v=tri, normals=nrm, as_mesh=True))
     g = 60.0
-    s.meshes.append(Mesh("ground", "ground", v=quad_tris((-g, 0, -g), (-g, 0, g), (g, 0, g), (g, 0, -g))))
+    s.meshes.append(Mesh("ground", "ground", v=quad_tris((ox - g, oy, oz - g), (ox - g, oy, oz + g),
+                                                         (ox + g, oy, oz + g), (ox + g, oy, oz - g))))
     lights = [((-10, 14, -10), 3.0), ((10, 14, -10), 3.0), ((-10, 14, 10), 3.0), ((10, 14, 10), 3.0)]
     for i, ((x, y, z), h) in enumerate(lights):
+        x, y, z = x + ox, y + oy, z + oz
         # rectangle v1 v2 v3 -> vA vB vC; normal cross(vA - vB, vC - vB) points down
         rect = np.array([[x - h, y, z - h, x - h, y, z + h, x + h, y, z + h]], np.float32)
         s.meshes.append(Mesh(f"lamp{i}", "lightmat", kind=PRIM_RECTANGLE, v=rect))

@@ -138,7 +138,9 @@ typedef struct {
 typedef struct {
     int64_t camera_rays, continuation_rays, shadow_rays, mis_rays;
     int64_t samples;
-    int64_t box_tests, tri_tests; /* traversal work (slab tests, primitive tests) */
+    int64_t box_tests, tri_tests; /* closest-hit traversal work (slab tests, primitive tests) */
+    int64_t box_tests_any, tri_tests_any; /* any-hit traversal work */
+    int64_t closest_launches, anyhit_launches;
     double  ms_total;             /* wall time of the call */
     double  ms_closest, ms_anyhit, ms_shade, ms_camera, ms_film; /* summed kernel time (events) */
 } mpt_stats;

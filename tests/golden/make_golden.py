@@ -43,9 +43,11 @@ def scene_set():
     """The golden scenes: (name, SceneSpec)."""
     cornell = scenes.cornell_box(width=32, height=32, spp=4, max_depth=8)
     cornell_g = scenes.cornell_box(width=24, height=20, spp=4, max_depth=6, filter="gaussian")
-    spheres = scenes.procedural_spheres(n_spheres=5, level=2, width=24, height=16, spp=2, max_depth=6, seed=99)
+    spheres = scenes.procedural_spheres(n_spheres=5, level=2, width=24, height=16, spp=2, max_depth=6, seed=99,
+                                        offset=(0.0, 0.0, 0.0))
     spheres.cluster = "power"
-    equal = scenes.procedural_spheres(n_spheres=4, level=2, width=8, height=8, spp=1, seed=7, with_normals=False)
+    equal = scenes.procedural_spheres(n_spheres=4, level=2, width=8, height=8, spp=1, seed=7, with_normals=False,
+                                      offset=(0.0, 0.0, 0.0))
     equal.split_mode = "equal"
     return [("cornell", cornell), ("cornell_gauss", cornell_g), ("spheres", spheres), ("spheres_equal", equal)]
 

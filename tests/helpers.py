@@ -60,4 +60,5 @@ def small_spheres(n_spheres=8, level=4, **kw):
     kw.setdefault("width", 64)
     kw.setdefault("height", 48)
     kw.setdefault("spp", 4)
+    kw.setdefault("offset", (0.0, 0.0, 0.0))
     return scenes.procedural_spheres(n_spheres=n_spheres, level=level, **kw)

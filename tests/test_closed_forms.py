@@ -44,7 +44,7 @@ def test_white_furnace(mode, albedo, depth):
 
 
 def test_bvh_equals_brute_force():
-    spec = scenes.procedural_spheres(n_spheres=6, level=2, width=8, height=8, spp=1, seed=5)
+    spec = scenes.procedural_spheres(n_spheres=6, level=2, width=8, height=8, spp=1, seed=5, offset=(0, 0, 0))
     O = OracleScene(spec)
     rays = random_rays(3000, (-18, 0, -18), (18, 6, 18), seed=9)
     pb, tb = O.trace(rays, use_bvh=True)
