@@ -26,8 +26,15 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // ---- traversal -------------------------------------------------------------------
 
+// Persistent traversal with per-lane refill: a lane whose ray has finished
+// takes a new one at the next refill point (wave ballot + one atomic per
+// wave) instead of idling until the whole wave's 64 rays are done, so lanes
+// stay busy despite the very uneven cost of incoherent rays. Between refills
+// every lane runs up to STEPS node visits.
 template <int W, int S, bool ANY>
 __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
+    constexpr int STEPS = 8;
+    constexpr int REFILL = 16;   // refill once this many lanes are idle
     extern __shared__ int32_t lds_stack[];
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -42,37 +49,76 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     const int64_t n = a.dual ? 2 * n0 : n0;
     TravCounters cnt{0, 0};
     unsigned long long rays = 0, rays2 = 0, errors = 0;
+    const bool no_tree = a.sc.nnodes == 0;
+    LaneRay r;
+    int64_t rid = -1;            // ray of this lane, -1 = idle
+    bool exhausted = false;      // wave-uniform: the work counter has run out
     while (true) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(a.work, WAVE);
-        base = __shfl(base, 0, WAVE);
-        if (base >= n) break;
-        const int64_t i = base + lane;
-        if (i >= n) continue;
-        const bool second = i >= n0;
-        const int64_t r = second ? i - n0 : i;
-        const float4 o4 = a.org[r * a.stride];
-        const float4 d4 = second ? a.dir2[r * a.stride] : a.dir[r * a.stride];
-        int2 out;
-        if (d4.w < 0.0f || a.sc.nnodes == 0) {
-            out = make_int2(ANY ? 0 : -1, __float_as_int(d4.w));
-        } else {
-            if (second) rays2 += 1;
-            else rays += 1;
-            float tmax = d4.w;
-            const int32_t h = traverse<W, S, ANY>(a.sc, st, v3f4(o4), v3f4(d4), o4.w, tmax, cnt);
-            if (h == -2) errors += 1;
-            if (ANY) {
-                out = make_int2(h >= 0 ? 1 : 0, 0);
-            } else {
-                int32_t idx = h;
-                if (h >= 0 && a.want_prim)
-                    idx = static_cast<int32_t>(__float_as_uint(a.sc.tris[3 * h].w) & 0x7FFFFFFFu);
-                out = make_int2(h >= 0 ? idx : -1, __float_as_int(tmax));
+        // -- refill idle lanes
+        const unsigned long long idle = __ballot(rid < 0);
+        const int nidle = __popcll(idle);
+        if (!exhausted && (nidle >= REFILL || nidle == WAVE)) {
+            const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(a.work, nidle);
+            base = __shfl(base, leader, WAVE);
+            if (base + nidle >= n) exhausted = true;
+            if (rid < 0) {
+                const int64_t i = base + __popcll(idle & ((1ull << lane) - 1ull));
+                if (i < n) {
+                    const bool second = i >= n0;
+                    const int64_t q = second ? i - n0 : i;
+                    const float4 o4 = a.org[q * a.stride];
+                    const float4 d4 = second ? a.dir2[q * a.stride] : a.dir[q * a.stride];
+                    if (d4.w < 0.0f || no_tree) {
+                        const int2 out = make_int2(ANY ? 0 : -1, __float_as_int(d4.w));
+                        if (second) a.hit2[q] = out;
+                        else a.hit[q] = out;
+                    } else {
+                        if (second) rays2 += 1;
+                        else rays += 1;
+                        rid = i;
+                        r.o = v3f4(o4);
+                        r.d = v3f4(d4);
+                        r.inv = safe_inv(r.d);
+                        r.oi = v3(r.o.x * r.inv.x, r.o.y * r.inv.y, r.o.z * r.inv.z);
+                        r.tmin = o4.w;
+                        r.tmax = d4.w;
+                        r.hit = -1;
+                        r.node = 0;
+                        st.sp = 0;
+                    }
+                }
             }
         }
-        if (second) a.hit2[r] = out;
-        else a.hit[r] = out;
+        if (__ballot(rid >= 0) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        // -- traversal steps
+        for (int s = 0; s < STEPS; ++s) {
+            if (rid >= 0 && visit<W, S, ANY>(a.sc, st, r, cnt)) {
+                if (r.hit == -2) {
+                    errors += 1;
+                    r.hit = -1;
+                }
+                const bool second = rid >= n0;
+                const int64_t q = second ? rid - n0 : rid;
+                int2 out;
+                if (ANY) {
+                    out = make_int2(r.hit >= 0 ? 1 : 0, 0);
+                } else {
+                    int32_t idx = r.hit;
+                    if (r.hit >= 0 && a.want_prim)
+                        idx = static_cast<int32_t>(__float_as_uint(a.sc.tris[3 * r.hit].w) & 0x7FFFFFFFu);
+                    out = make_int2(r.hit >= 0 ? idx : -1, __float_as_int(r.tmax));
+                }
+                if (second) a.hit2[q] = out;
+                else a.hit[q] = out;
+                rid = -1;
+            }
+            if (__ballot(rid >= 0) == 0ull) break;
+        }
     }
     // one atomic per wave for the counters
     const unsigned long long sr = wave_sum(rays), sr2 = wave_sum(rays2), sb = wave_sum(cnt.boxes),

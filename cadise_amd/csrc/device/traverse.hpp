@@ -85,42 +85,6 @@ struct LaneStack {
     }
 };
 
-// Slab test of the W children of one node; returns a bitmask of hit children
-// and their entry distances.
-template <int W>
-__device__ __forceinline__ uint32_t node_test(const float* __restrict__ nd, V3 inv, V3 oi, float tmin, float tmax,
-                                              float* tnear) {
-    uint32_t mask = 0;
-    const float4* p = reinterpret_cast<const float4*>(nd);
-    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
-    int32_t child[W];
-#pragma unroll
-    for (int q = 0; q < W / 4; ++q) {
-        const float4 a = p[0 * (W / 4) + q], b = p[1 * (W / 4) + q], c = p[2 * (W / 4) + q];
-        const float4 d = p[3 * (W / 4) + q], e = p[4 * (W / 4) + q], f = p[5 * (W / 4) + q];
-        const int4 g = reinterpret_cast<const int4*>(p)[6 * (W / 4) + q];
-        lox[4 * q + 0] = a.x; lox[4 * q + 1] = a.y; lox[4 * q + 2] = a.z; lox[4 * q + 3] = a.w;
-        loy[4 * q + 0] = b.x; loy[4 * q + 1] = b.y; loy[4 * q + 2] = b.z; loy[4 * q + 3] = b.w;
-        loz[4 * q + 0] = c.x; loz[4 * q + 1] = c.y; loz[4 * q + 2] = c.z; loz[4 * q + 3] = c.w;
-        hix[4 * q + 0] = d.x; hix[4 * q + 1] = d.y; hix[4 * q + 2] = d.z; hix[4 * q + 3] = d.w;
-        hiy[4 * q + 0] = e.x; hiy[4 * q + 1] = e.y; hiy[4 * q + 2] = e.z; hiy[4 * q + 3] = e.w;
-        hiz[4 * q + 0] = f.x; hiz[4 * q + 1] = f.y; hiz[4 * q + 2] = f.z; hiz[4 * q + 3] = f.w;
-        child[4 * q + 0] = g.x; child[4 * q + 1] = g.y; child[4 * q + 2] = g.z; child[4 * q + 3] = g.w;
-    }
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
-        const float tx0 = __builtin_fmaf(lox[c], inv.x, -oi.x), tx1 = __builtin_fmaf(hix[c], inv.x, -oi.x);
-        const float ty0 = __builtin_fmaf(loy[c], inv.y, -oi.y), ty1 = __builtin_fmaf(hiy[c], inv.y, -oi.y);
-        const float tz0 = __builtin_fmaf(loz[c], inv.z, -oi.z), tz1 = __builtin_fmaf(hiz[c], inv.z, -oi.z);
-        const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-        const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-        const bool hit = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, tmax));
-        tnear[c] = n;
-        mask |= hit ? (1u << c) : 0u;
-    }
-    return mask;
-}
-
 // Direction reciprocal for culling: zero components become +-1e-30 so that
 // lo * inv - o * inv stays finite.
 __device__ __forceinline__ V3 safe_inv(V3 d) {
@@ -131,69 +95,114 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
     return v3(one(d.x), one(d.y), one(d.z));
 }
 
-// Closest hit (ANY = false) or any hit (ANY = true) of one ray.
-// Returns the leaf position of the hit primitive or -1; tmax holds the hit t.
+// Traversal state of one lane: the ray, its running closest distance and
+// hit, the node to visit next (-1: pop the stack) and the stack depth.
+struct LaneRay {
+    V3      o, d, inv, oi;
+    float   tmin, tmax;
+    int32_t hit;
+    int32_t node;
+};
+
+// Compare-exchange of a bitonic sorting network on (distance, child) pairs.
+__device__ __forceinline__ void cmpx(float& da, int32_t& ia, float& db, int32_t& ib, bool asc) {
+    const bool sw = (da > db) == asc;
+    const float td = sw ? db : da;
+    const int32_t ti = sw ? ib : ia;
+    db = sw ? da : db;
+    ib = sw ? ia : ib;
+    da = td;
+    ia = ti;
+}
+
+// One node visit: slab-test the W children, test the primitives of hit
+// leaves (shrinking tmax), sort the hit interior children by entry distance,
+// continue with the nearest and push the rest. Returns true when the ray has
+// finished (any hit found, or the stack is empty). All register arrays are
+// indexed with compile-time indices.
 template <int W, int S, bool ANY>
-__device__ __forceinline__ int32_t traverse(const DevScene& sc, LaneStack<S>& st, V3 o, V3 d, float tmin,
-                                            float& tmax, TravCounters& cnt) {
-    const float* __restrict__ nodes = reinterpret_cast<const float*>(sc.nodes);
-    constexpr int NODE_FLOATS = 8 * W;
-    const V3 inv = safe_inv(d);
-    const V3 oi = v3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
-    int32_t hit = -1;
-    int32_t node = 0;
-    st.sp = 0;
-    while (true) {
-        const float* nd = nodes + static_cast<int64_t>(node) * NODE_FLOATS;
-        float tn[W];
-        uint32_t mask = node_test<W>(nd, inv, oi, tmin, tmax, tn);
-        cnt.boxes += W;
-        const int32_t* ch = reinterpret_cast<const int32_t*>(nd + 6 * W);
-        const int32_t* cn = ch + W;
-        // leaves first: their hits shrink tmax before the interior children are ordered
-        int32_t kid[W];
-        float kd[W];
-        int nk = 0;
+__device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, LaneRay& r, TravCounters& cnt) {
+    if (r.node < 0) {
+        if (st.sp == 0) return true;
+        r.node = st.pop();
+    }
+    const float4* p = reinterpret_cast<const float4*>(sc.nodes) + static_cast<int64_t>(r.node) * (2 * W);
+    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
+    int32_t child[W], count[W];
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-            if (!(mask & (1u << c))) continue;
-            const int32_t count = cn[c];
-            const int32_t first = ch[c];
-            if (count > 0) {
-                for (int32_t j = 0; j < count; ++j) {
-                    cnt.prims += 1;
-                    if (prim_test(sc.tris, first + j, o, d, tmin, tmax)) {
-                        hit = first + j;
-                        if (ANY) return hit;
-                    }
+    for (int q = 0; q < W / 4; ++q) {
+        const float4 a = p[0 * (W / 4) + q], b = p[1 * (W / 4) + q], c = p[2 * (W / 4) + q];
+        const float4 d = p[3 * (W / 4) + q], e = p[4 * (W / 4) + q], f = p[5 * (W / 4) + q];
+        const int4 g = reinterpret_cast<const int4*>(p)[6 * (W / 4) + q];
+        const int4 h = reinterpret_cast<const int4*>(p)[7 * (W / 4) + q];
+        lox[4 * q + 0] = a.x; lox[4 * q + 1] = a.y; lox[4 * q + 2] = a.z; lox[4 * q + 3] = a.w;
+        loy[4 * q + 0] = b.x; loy[4 * q + 1] = b.y; loy[4 * q + 2] = b.z; loy[4 * q + 3] = b.w;
+        loz[4 * q + 0] = c.x; loz[4 * q + 1] = c.y; loz[4 * q + 2] = c.z; loz[4 * q + 3] = c.w;
+        hix[4 * q + 0] = d.x; hix[4 * q + 1] = d.y; hix[4 * q + 2] = d.z; hix[4 * q + 3] = d.w;
+        hiy[4 * q + 0] = e.x; hiy[4 * q + 1] = e.y; hiy[4 * q + 2] = e.z; hiy[4 * q + 3] = e.w;
+        hiz[4 * q + 0] = f.x; hiz[4 * q + 1] = f.y; hiz[4 * q + 2] = f.z; hiz[4 * q + 3] = f.w;
+        child[4 * q + 0] = g.x; child[4 * q + 1] = g.y; child[4 * q + 2] = g.z; child[4 * q + 3] = g.w;
+        count[4 * q + 0] = h.x; count[4 * q + 1] = h.y; count[4 * q + 2] = h.z; count[4 * q + 3] = h.w;
+    }
+    cnt.boxes += W;
+    float tn[W];
+    bool hitc[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        const float tx0 = __builtin_fmaf(lox[c], r.inv.x, -r.oi.x), tx1 = __builtin_fmaf(hix[c], r.inv.x, -r.oi.x);
+        const float ty0 = __builtin_fmaf(loy[c], r.inv.y, -r.oi.y), ty1 = __builtin_fmaf(hiy[c], r.inv.y, -r.oi.y);
+        const float tz0 = __builtin_fmaf(loz[c], r.inv.z, -r.oi.z), tz1 = __builtin_fmaf(hiz[c], r.inv.z, -r.oi.z);
+        const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
+        const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+        hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
+        tn[c] = n;
+    }
+    // leaves first: their hits shrink tmax before the interior children are ordered
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        if (hitc[c] && count[c] > 0) {
+            for (int32_t j = 0; j < count[c]; ++j) {
+                cnt.prims += 1;
+                if (prim_test(sc.tris, child[c] + j, r.o, r.d, r.tmin, r.tmax)) {
+                    r.hit = child[c] + j;
+                    if (ANY) return true;
                 }
-            } else {
-                // insertion by entry distance, nearest first
-                int pos = nk;
-                while (pos > 0 && kd[pos - 1] > tn[c]) {
-                    kd[pos] = kd[pos - 1];
-                    kid[pos] = kid[pos - 1];
-                    --pos;
-                }
-                kd[pos] = tn[c];
-                kid[pos] = first;
-                ++nk;
             }
-        }
-        // drop children now beyond the (possibly shrunk) tmax
-        while (nk > 0 && kd[nk - 1] > tmax) --nk;
-        if (nk > 0) {
-            for (int k = nk - 1; k >= 1; --k) {
-                if (st.sp >= sc.max_stack) return -2;   // cannot happen for a built tree
-                st.push(kid[k]);
-            }
-            node = kid[0];
-        } else {
-            if (st.sp == 0) break;
-            node = st.pop();
         }
     }
-    return hit;
+    float key[W];
+    int32_t kid[W];
+    int nk = 0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        const bool inner = hitc[c] && count[c] == 0 && tn[c] <= r.tmax;
+        key[c] = inner ? tn[c] : INFINITY;
+        kid[c] = child[c];
+        nk += inner ? 1 : 0;
+    }
+    // bitonic network, ascending distance
+#pragma unroll
+    for (int k = 2; k <= W; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const int l = i ^ j;
+                if (l > i) cmpx(key[i], kid[i], key[l], kid[l], (i & k) == 0);
+            }
+        }
+    }
+    if (st.sp + nk - 1 > sc.max_stack) {   // bounded by the tree depth; never taken for a built tree
+        r.hit = -2;
+        return true;
+    }
+#pragma unroll
+    for (int k = W - 1; k >= 1; --k) {
+        if (k < nk) st.push(kid[k]);
+    }
+    r.node = nk > 0 ? kid[0] : -1;
+    if (nk == 0 && st.sp == 0) return true;
+    return false;
 }
 
 }  // namespace mpt
