@@ -44,15 +44,17 @@ def cpu_baseline(spec, seconds_target=15.0, seed=7):
     else:
         S = OracleScene(spec)
     # calibrate on a small window around the image centre, then size the sample
+    # (pixels around the centre, then whole frames of more samples) to ~seconds_target
     begin = npix // 2 - 2000
     secs, rays, smp = S.time_render(begin, begin + 2000, 1, threads, seed)
-    rate = max(rays.sum(), 1) / max(secs, 1e-6)
-    n = int(min(npix, max(4000, 2000 * seconds_target / max(secs, 1e-3))))
+    want = 2000 * seconds_target / max(secs, 1e-3)
+    n = int(min(npix, max(4000, want)))
+    spp = int(max(1, min(spec.spp, round(want / npix)))) if n == npix else 1
     begin = max(0, npix // 2 - n // 2)
-    secs, rays, smp = S.time_render(begin, begin + n, 1, threads, seed)
+    secs, rays, smp = S.time_render(begin, begin + n, spp, threads, seed)
     total = int(rays.sum())
     return {"value": total / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
-            "sample": f"{n} pixels around the frame centre x 1 spp, depth 8 ({total} ray queries, {secs:.1f} s)",
+            "sample": f"{n} pixels around the frame centre x {spp} spp, depth 8 ({total} ray queries, {secs:.1f} s)",
             "samples_per_s": smp / secs}
 
 
@@ -65,7 +67,19 @@ def main():
     ap.add_argument("--leaf-max", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device copies of 1 GiB (scale reference for rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("copy calibration: 3 x 1 GiB device copies (expect 3145728 KiB read, 3145728 KiB written)", flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design (pixel/sample sharding "
                                    "with one film reduction) is described in DESIGN.md and not launched"}))
@@ -139,6 +153,18 @@ def main():
                      "prim_tests_per_ray": round(st["tri_tests"] / max(closest, 1), 2)},
         "setup_s": round(setup_s, 2),
     }
+    # HBM bytes per launch of this kernel, measured with rocprofv3 (FETCH_SIZE,
+    # WRITE_SIZE, copy-calibrated) on this workload: profiles/traffic.json
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            meas = json.load(f).get(f"k_trace<{st_scene['width']},24,false>")
+        if meas and st_scene["leaf_max"] == 2:
+            out["roofline"]["traffic"] = meas["hbm_bytes_per_launch"]
+            out["roofline"]["traffic_source"] = meas["source"]
+            out["roofline"]["l2_hit_rate"] = meas["l2_hit_rate"]
+            out["roofline"]["valu_busy_pct"] = meas["valu_busy_pct"]
+    except (OSError, ValueError):
+        pass
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds)

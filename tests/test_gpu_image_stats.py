@@ -1,0 +1,95 @@
+"""Image-level parity of the GPU renderer with the reference's own renderer
+(the compiled reference, oracle/_ref, run on this host as the checker) at
+equal spp, with independent random streams.
+
+Per pixel and channel, z = (mean_gpu - mean_ref) / sqrt(var_gpu/R + var_ref/R)
+over R independent renders each; with unbiased estimators |z| > 3 on about
+0.3% of the (pixel, channel) pairs. Bars written here: at most 1.5% of pairs
+beyond 3 sigma, the whole-image means within 3 combined standard errors, and
+the relMSE of the GPU image against a high-spp GPU image reported.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cadise_amd import scenes
+from oracle.binding import RefScene, ref_available
+from tests.helpers import small_spheres
+
+pytestmark = pytest.mark.gpu
+
+R = 8
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from cadise_amd.api import Context
+    return Context(0)
+
+
+def _stack_gpu(sc, spp, seeds):
+    from cadise_amd.api import film_mean
+    return np.stack([film_mean(sc.render(seed=s, spp_begin=0, spp_end=spp)) for s in seeds]).astype(np.float64)
+
+
+def _stack_ref(ref, seeds):
+    out = []
+    for s in seeds:
+        f = ref.render_linear(seed=s, threads=8)
+        out.append(f[..., :3] / np.where(f[..., 3:4] > 0, f[..., 3:4], 1))
+    return np.stack(out).astype(np.float64)
+
+
+def _zstats(g, r, label):
+    mg, mr = g.mean(0), r.mean(0)
+    vg, vr = g.var(0, ddof=1), r.var(0, ddof=1)
+    se = np.sqrt(vg / len(g) + vr / len(r))
+    live = se > 0
+    z = np.zeros_like(mg)
+    z[live] = (mg[live] - mr[live]) / se[live]
+    frac = float((np.abs(z[live]) > 3).mean())
+    # the global mean: pixels are independent
+    G, Rm = g.mean(axis=(1, 2, 3)), r.mean(axis=(1, 2, 3))
+    gse = math.sqrt(G.var(ddof=1) / len(G) + Rm.var(ddof=1) / len(Rm))
+    print(f"[{label}] pairs beyond 3 sigma: {frac:.4f}; mean gpu={G.mean():.6f} ref={Rm.mean():.6f} "
+          f"(3 se = {3 * gse:.6f})", flush=True)
+    # exact zero-variance pixels (background) must agree exactly
+    assert np.allclose(mg[~live], mr[~live])
+    return frac, abs(G.mean() - Rm.mean()), gse
+
+
+@pytest.mark.skipif(not ref_available(), reason="compiled reference harness (oracle/_ref) not built")
+def test_cornell_vs_reference_renderer(ctx):
+    from cadise_amd.api import Scene
+    spec = scenes.cornell_box(width=64, height=64, spp=16, max_depth=8)
+    text = spec.to_crsd()
+    sc = Scene.from_crsd_text(ctx, text)
+    ref = RefScene(text)
+    g = _stack_gpu(sc, 16, range(100, 100 + R))
+    r = _stack_ref(ref, range(200, 200 + R))
+    frac, dmean, gse = _zstats(g, r, "cornell 64x64 16spp")
+    assert frac < 0.015
+    assert dmean < 3 * gse + 1e-9
+    # relMSE of one 16-spp GPU image against a 1024-spp GPU image (reported)
+    from cadise_amd.api import film_mean
+    hi = film_mean(sc.render(seed=9, spp_begin=0, spp_end=1024)).astype(np.float64)
+    rel = float(((g[0] - hi) ** 2 / (hi ** 2 + 0.01)).mean())
+    print(f"[cornell] relMSE(16 spp vs 1024 spp) = {rel:.4e}", flush=True)
+
+
+@pytest.mark.skipif(not ref_available(), reason="compiled reference harness (oracle/_ref) not built")
+def test_spheres_ggx_power_cluster_vs_reference_renderer(ctx, tmp_path):
+    from cadise_amd.api import Scene
+    spec = small_spheres(n_spheres=6, level=3, width=48, height=32, spp=8, max_depth=6)
+    spec.cluster = "power"
+    path = tmp_path / "s.crsd"
+    path.write_text(spec.to_crsd(sidecar_dir=str(tmp_path)))
+    text = path.read_text()
+    sc = Scene.from_crsd(ctx, str(path))
+    ref = RefScene(text)
+    g = _stack_gpu(sc, 8, range(300, 300 + R))
+    r = _stack_ref(ref, range(400, 400 + R))
+    frac, dmean, gse = _zstats(g, r, "spheres ggx power 48x32 8spp")
+    assert frac < 0.015
+    assert dmean < 3 * gse + 1e-9
