@@ -384,3 +384,105 @@ def procedural_spheres(seed=0x20261203, n_spheres=64, level=5, width=1920, heigh
         s.meshes.append(Mesh(f"lamp{i}", "lightmat", kind=PRIM_RECTANGLE, v=rect))
         s.lights.append(Light(f"lamp{i}", (1.0, 0.95, 0.85), 4000.0))
     return s
+
+
+# -- traversal stress ----------------------------------------------------------------
+
+def displaced_grid(nx: int, nz: int, size_x: float, size_z: float, amp: float, perm: np.ndarray,
+                   shift) -> np.ndarray:
+    """An nx x nz quad grid (2 nx nz triangles) in the xz plane centred on the
+    origin, heights = amp * two octaves of Perlin noise; returns float64
+    triangles (2 nx nz) x 3 x 3."""
+    xs = np.linspace(-size_x / 2, size_x / 2, nx + 1)
+    zs = np.linspace(-size_z / 2, size_z / 2, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    p = np.stack([X.ravel() * 0.35 + shift[0], Z.ravel() * 0.35 + shift[1], np.full(X.size, shift[2])], 1)
+    y = amp * (_perlin3(p, perm) + 0.5 * _perlin3(p * 2.0 + 17.0, perm))
+    verts = np.stack([X.ravel(), y, Z.ravel()], 1)
+    i, j = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    a = (i * (nz + 1) + j).ravel()
+    b, c = a + 1, a + (nz + 1)
+    d = c + 1
+    faces = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)])
+    return verts[faces]
+
+
+def _random_rotations(rng, n):
+    """n uniformly distributed rotation matrices (normalized Gaussian quaternions)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], 1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], 1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)], 1)
+
+
+def traversal_stress(seed=0x20261203 ^ 0x3, n_instances=400, n_bases=4, grid=(125, 200), size=(12.0, 19.2),
+                     amp=1.0, width=1024, height=1024, spp=4, max_depth=4) -> SceneSpec:
+    """Traversal stress scene: n_instances copies (flattened to world space) of
+    n_bases seeded displaced grids of 2 * 125 * 200 = 50,000 triangles each,
+    randomly rotated, scaled by U[0.6, 1.4] and translated into [15, 85]^3 of a
+    100^3 box (400 instances: 20,000,000 triangles). Every coordinate is
+    positive (the reference's empty-box quirk, tAabb.ipp:13-17, is then inert).
+    Lambertian grey, one rectangle light under the box's lid; the camera looks
+    into the box from z = -35."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm, perm]).astype(np.int64)
+    bases = [displaced_grid(grid[0], grid[1], size[0], size[1], amp, perm, rng.uniform(0, 50, 3))
+             for _ in range(n_bases)]
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="box", fov=60.0,
+                  cam_pos=(50.0, 50.0, -35.0), cam_dir=(0.0, 0.0, 1.0), cam_up=(0.0, 1.0, 0.0))
+    s.materials.append(Material("grey", albedo=(0.6, 0.6, 0.6)))
+    rots = _random_rotations(rng, n_instances)
+    scales = rng.uniform(0.6, 1.4, n_instances)
+    trans = rng.uniform(15.0, 85.0, (n_instances, 3))
+    which = rng.integers(0, n_bases, n_instances)
+    for k in range(n_instances):
+        tri = bases[which[k]] @ (rots[k].T * scales[k]) + trans[k]
+        s.meshes.append(Mesh(f"inst{k}", "grey", v=tri.reshape(-1, 9).astype(np.float32), as_mesh=True))
+    # one 30 x 30 rectangle light under the lid of the box, facing down
+    s.materials.append(Material("lightmat", albedo=(0.8, 0.8, 0.8)))
+    rect = np.array([[35.0, 99.5, 35.0, 35.0, 99.5, 65.0, 65.0, 99.5, 65.0]], np.float32)
+    s.meshes.append(Mesh("sun", "lightmat", kind=PRIM_RECTANGLE, v=rect))
+    s.lights.append(Light("sun", (1.0, 0.95, 0.85), 200000.0))
+    return s
+
+
+def surface_rays(spec: SceneSpec, n: int, seed: int) -> np.ndarray:
+    """n x 8 secondary rays: origins at uniform points of random triangles,
+    directions cosine-distributed about the geometric normal (random side),
+    tmin 1e-3, tmax FLT_MAX. Directions are unit length in float32 but not
+    normalized the reference's way (callers that need that re-normalize)."""
+    rng = np.random.default_rng(seed)
+    counts = np.array([len(m.v) for m in spec.meshes])
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    idx = rng.integers(0, starts[-1], n)
+    mesh = np.searchsorted(starts, idx, side="right") - 1
+    v = np.empty((n, 9), np.float64)
+    for mi in np.unique(mesh):
+        sel = mesh == mi
+        v[sel] = spec.meshes[mi].v[idx[sel] - starts[mi]]
+    a, b, c = v[:, 0:3], v[:, 3:6], v[:, 6:9]
+    u1, u2 = rng.random(n), rng.random(n)
+    su = np.sqrt(u1)
+    p = a * (1 - su)[:, None] + b * (su * (1 - u2))[:, None] + c * (su * u2)[:, None]
+    nrm = np.cross(b - a, c - a)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    nrm *= rng.choice([-1.0, 1.0], n)[:, None]
+    # cosine-weighted direction in the frame of nrm
+    r1, r2 = rng.random(n), rng.random(n)
+    phi, sr = 2 * np.pi * r1, np.sqrt(r2)
+    helper = np.where(np.abs(nrm[:, :1]) > 0.9, [[0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0]])
+    t1 = np.cross(helper, nrm)
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(nrm, t1)
+    d = t1 * (sr * np.cos(phi))[:, None] + t2 * (sr * np.sin(phi))[:, None] + nrm * np.sqrt(1 - r2)[:, None]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.empty((n, 8), np.float32)
+    rays[:, 0:3] = p
+    rays[:, 3] = 1e-3
+    rays[:, 4:7] = d
+    rays[:, 7] = np.finfo(np.float32).max
+    return rays

@@ -43,11 +43,11 @@ def random_rays(n, lo, hi, seed, tmin=1e-3):
     return r
 
 
-def camera_rays(spec, n, seed):
+def camera_rays(spec, n, seed, oracle=None):
     """Primary rays of the scene camera at random film positions (reference-exact)."""
     rng = np.random.default_rng(seed)
     fxy = np.stack([rng.uniform(0, spec.width, n), rng.uniform(0, spec.height, n)], 1).astype(np.float32)
-    od = OracleScene(spec).camera_rays(fxy)
+    od = (oracle or OracleScene(spec)).camera_rays(fxy)
     r = np.zeros((n, 8), np.float32)
     r[:, :3] = od[:, :3]
     r[:, 3] = 1e-3
@@ -62,3 +62,17 @@ def small_spheres(n_spheres=8, level=4, **kw):
     kw.setdefault("spp", 4)
     kw.setdefault("offset", (0.0, 0.0, 0.0))
     return scenes.procedural_spheres(n_spheres=n_spheres, level=level, **kw)
+
+
+def stress_rays(spec, oracle, n, seed):
+    """n seeded rays for the traversal stress check: half camera primaries,
+    half secondary rays leaving random surface points with cosine-distributed
+    directions (directions at the reference's normalize fixed point)."""
+    cam = camera_rays(spec, n // 2, seed, oracle)
+    need = n - n // 2
+    sec = scenes.surface_rays(spec, int(need * 1.25) + 1000, seed + 1)
+    d, keep = stable_dirs(sec[:, 4:7])
+    sec[:, 4:7] = d
+    sec = sec[keep][:need]
+    assert len(sec) == need
+    return np.concatenate([cam, sec])

@@ -123,3 +123,15 @@ def test_host_only_context_refuses_device_calls(ctx):
     assert e.value.code == -4
     with pytest.raises(MptError):
         sc.trace(np.zeros((4, 8), np.float32))
+
+
+def test_reference_tree_matches_oracle_on_stress_instances(ctx):
+    """The reference's SAH tree at 300k triangles of the traversal-stress
+    geometry (six rotated displaced grids): topology, bounds and primitive
+    order equal the oracle's (bvhBuilder.cpp:18-358)."""
+    spec = scenes.traversal_stress(n_instances=6)
+    b = Scene.from_spec(ctx, spec).binary_bvh()
+    o = OracleScene(spec).bvh()
+    assert np.array_equal(bits(b["bounds"]), bits(o["bounds"]))
+    for k in ("leaf", "a", "b", "ordered"):
+        assert np.array_equal(b[k], o[k]), k
