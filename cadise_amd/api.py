@@ -16,7 +16,8 @@ import ctypes as C
 import numpy as np
 
 from . import native
-from .native import BvhOpts, LightDesc, MaterialDesc, MptError, RenderParams, SceneDesc, SceneStats, Stats, Texture
+from .native import (MPT_RENDER_KEEP_SAMPLES, BvhOpts, LightDesc, MaterialDesc, MptError, RenderParams, SceneDesc,
+                     SceneStats, Stats, Texture)
 
 _FILTERS = {"box": 0, "gaussian": 1, "cone": 2, "mitchell": 3}
 _FILTER_W = {"box": 1.0, "gaussian": 4.0, "cone": 4.0, "mitchell": 4.0}
@@ -59,6 +60,7 @@ class Scene:
         ctx.check(self.L.mpt_scene_stats_get(self.h, C.byref(st)))
         self.width, self.height = st.image_width, st.image_height
         self.spp, self.max_depth = st.spp, st.max_depth
+        self.tex_index = {}
 
     # -- construction -----------------------------------------------------------
     @classmethod
@@ -80,6 +82,8 @@ class Scene:
         t, v, n, u, m = spec.prim_arrays()
         keep = [t, v, n, u, m]
         tex = {x.name: x for x in spec.textures}
+        arrays = {}        # one array per texture name: materials sharing it share one device texture
+        tex_index = {}     # texture name -> index in the scene's texture list (creation order)
 
         def texture(const, name, channels):
             T = Texture()
@@ -87,7 +91,10 @@ class Scene:
                 c = const if isinstance(const, (tuple, list)) else (const, const, const)
                 T.value[:] = [float(np.float32(x)) for x in c]
                 return T
-            arr = np.ascontiguousarray(tex[name].data, np.float32)
+            if name not in arrays:
+                arrays[name] = np.ascontiguousarray(tex[name].data, np.float32)
+                tex_index[name] = len(tex_index)
+            arr = arrays[name]
             keep.append(arr)
             T.data = arr.ctypes.data
             T.height, T.width = arr.shape[0], arr.shape[1]
@@ -130,7 +137,9 @@ class Scene:
         h = C.c_void_p()
         ctx.check(ctx.L.mpt_scene_create(ctx.h, C.byref(d), C.byref(opts) if opts else None, C.byref(h)))
         del keep
-        return cls(ctx, h)
+        sc = cls(ctx, h)
+        sc.tex_index = tex_index
+        return sc
 
     def close(self):
         if getattr(self, "h", None):
@@ -195,11 +204,50 @@ class Scene:
 
     # -- Renderer -----------------------------------------------------------------------
     def render(self, seed: int = 0, spp_begin: int = 0, spp_end: int = 0, max_depth: int = 0,
-               accumulate: bool = False, readback: bool = True):
-        p = RenderParams(seed, spp_begin, spp_end, max_depth, int(accumulate))
+               accumulate: bool = False, readback: bool = True, keep_samples: bool = False):
+        p = RenderParams(seed, spp_begin, spp_end, max_depth, int(accumulate),
+                         MPT_RENDER_KEEP_SAMPLES if keep_samples else 0)
         film = np.zeros((self.height, self.width, 4), np.float32) if readback else None
         self.ctx.check(self.L.mpt_render(self.h, C.byref(p), film.ctypes.data if readback else None))
         return film
+
+    # -- differentiable rendering ---------------------------------------------------------
+    def texture_info(self, index: int):
+        """(width, height, channels, offset) of texture `index` in the flat texture array."""
+        w, h, c, o = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        self.ctx.check(self.L.mpt_scene_texture_info(self.h, index, C.byref(w), C.byref(h), C.byref(c), C.byref(o)))
+        return w.value, h.value, c.value, o.value
+
+    def set_texture(self, index: int, data: np.ndarray):
+        data = np.ascontiguousarray(data, np.float32)
+        w, h, c, _ = self.texture_info(index)
+        assert data.size == w * h * c, (data.shape, (h, w, c))
+        self.ctx.check(self.L.mpt_scene_set_texture(self.h, index, data.ctypes.data))
+
+    def set_light_radiance(self, light: int, rgb):
+        self.ctx.check(self.L.mpt_scene_set_light_radiance(self.h, light,
+                                                           np.ascontiguousarray(rgb, np.float32).ctypes.data))
+
+    def adjoint(self, d_image: np.ndarray, seed: int = 0, spp_begin: int = 0, spp_end: int = 0, max_depth: int = 0):
+        """dLoss/d(texture elements) (flat, float32) and dLoss/dLe (nlights x 3) for
+        d_image = dLoss/dI (H x W x 3) of the render with the same arguments."""
+        st = self.stats()
+        d_image = np.ascontiguousarray(d_image, np.float32)
+        assert d_image.shape == (self.height, self.width, 3)
+        d_tex = np.zeros(max(1, st["tex_elements"]), np.float32)
+        d_light = np.zeros((max(1, st["nlights"]), 3), np.float32)
+        p = RenderParams(seed, spp_begin, spp_end, max_depth, 0, 0)
+        self.ctx.check(self.L.mpt_render_adjoint(self.h, C.byref(p), d_image.ctypes.data, d_tex.ctypes.data,
+                                                 d_light.ctypes.data))
+        return d_tex[: st["tex_elements"]], d_light[: st["nlights"]]
+
+    def texture_views(self, flat: np.ndarray):
+        """Split a flat texture-element array into H x W x C views, one per texture."""
+        out = []
+        for i in range(self.stats()["ntextures"]):
+            w, h, c, o = self.texture_info(i)
+            out.append(flat[o:o + w * h * c].reshape(h, w, c))
+        return out
 
     def film_device_ptr(self) -> int:
         p = C.c_void_p()

@@ -7,6 +7,7 @@
 // The bounce loop of PtEstimator::estimate (reference core/estimator/
 // ptEstimator.cpp:25-123) is unrolled across launches: each launch moves every
 // live path by one step, and live paths are compacted by wave ballots.
+#include "adjoint.hpp"
 #include "kernels.hpp"
 #include "shade.hpp"
 #include "traverse.hpp"
@@ -173,10 +174,52 @@ __device__ __forceinline__ uint32_t kensler_permute(uint32_t i, uint32_t l, uint
     return (i + p) % l;
 }
 
+// Filter::evaluate of the four reference filters (boxFilter.cpp:9-11,
+// gaussianFilter.cpp:25-33, coneFilter.cpp:13-16, mitchellFilter.cpp:16-40).
+__device__ __forceinline__ float mitchell1d(float b, float c, float d) {
+    const float ad = fabsf(d);
+    const float den = 1.0f / 6.0f;
+    if (ad < 1.0f)
+        return ((12.0f - 9.0f * b - 6.0f * c) * ad * ad * ad + (-18.0f + 12.0f * b + 6.0f * c) * ad * ad +
+                (6.0f - 2.0f * b)) * den;
+    if (ad < 2.0f)
+        return ((-b - 6.0f * c) * ad * ad * ad + (6.0f * b + 30.0f * c) * ad * ad + (-12.0f * b - 48.0f * c) * ad +
+                (8.0f * b + 24.0f * c)) * den;
+    return 0.0f;
+}
+__device__ __forceinline__ float filter_eval(const DevScene& sc, float x, float y) {
+    switch (sc.filter) {
+        case 1: {
+            const float v = sc.amplitude * expf(sc.g_expx * x * x + sc.g_expy * y * y);
+            return mmax(0.0f, v - sc.g_edge);
+        }
+        case 2: return mmax(sc.filter_hw - fabsf(x), 0.0f) * mmax(sc.filter_hh - fabsf(y), 0.0f);
+        case 3: return mitchell1d(sc.mitchell_b, sc.mitchell_c, x) * mitchell1d(sc.mitchell_b, sc.mitchell_c, y);
+        default: return 1.0f;
+    }
+}
+
+// Pixel range [x0, x1) x [y0, y1) a sample at film position (fx, fy) of the
+// tile [tx0, tx1) x [ty0, ty1) reaches: FilmTile::addSample's footprint,
+// clamped to the tile (filmTile.cpp:36-50).
+__device__ __forceinline__ int4 footprint(const DevScene& sc, float fx, float fy, int tx0, int ty0, int tx1, int ty1) {
+    const float mnx = fmaxf(fx - sc.filter_hw, static_cast<float>(tx0));
+    const float mny = fmaxf(fy - sc.filter_hh, static_cast<float>(ty0));
+    const float mxx = fminf(fx + sc.filter_hw, static_cast<float>(tx1));
+    const float mxy = fminf(fy + sc.filter_hh, static_cast<float>(ty1));
+    return make_int4(static_cast<int>(ceilf(mnx - 0.5f)), static_cast<int>(ceilf(mny - 0.5f)),
+                     static_cast<int>(floorf(mxx - 0.5f) + 1), static_cast<int>(floorf(mxy - 0.5f) + 1));
+}
+
+__device__ __forceinline__ bool finite3(float4 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
 // One sample pass's primary rays: the stratified (or random) film jitter of
 // StratifiedSampler::requestSample2D (stratifiedSampler.cpp:38-73) and
 // PerspectivePinholeCamera::spawnPrimaryRay (perspectivePinholeCamera.cpp:52-73).
-__global__ void __launch_bounds__(256) k_camera(PassArgs a) {
+// ADJ: also the path's adjoint weight A = sum_q dLoss/dI_q w(q, s) / W_q over the
+// sample's footprint (zero for samples the film drops: NaN / inf).
+template <bool ADJ>
+__global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gid < a.nwork) a.work[gid] = 0;
     if (gid < a.max_depth + 2) a.qcount[gid] = gid == 0 ? a.npaths : 0;
@@ -217,6 +260,25 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a) {
     q.mis_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
     q.shd_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
     q.bv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (ADJ) {
+        V3 A = v3(0.f, 0.f, 0.f);
+        if (finite3(g.samples[p])) {
+            const int tx0 = (ix / sc.tile_x) * sc.tile_x, ty0 = (iy / sc.tile_y) * sc.tile_y;
+            const int tx1 = min(tx0 + sc.tile_x, sc.img_w), ty1 = min(ty0 + sc.tile_y, sc.img_h);
+            const int4 fp = footprint(sc, fx, fy, tx0, ty0, tx1, ty1);
+            for (int qy = fp.y; qy < fp.w; ++qy)
+                for (int qx = fp.x; qx < fp.z; ++qx) {
+                    const int64_t qi = static_cast<int64_t>(qy) * sc.img_w + qx;
+                    const float W = g.film[qi].w;
+                    if (W == 0.0f) continue;
+                    const float w = filter_eval(sc, static_cast<float>(qx) - (fx - 0.5f), static_cast<float>(qy) - (fy - 0.5f)) / W;
+                    A = A + v3(g.dimg[3 * qi], g.dimg[3 * qi + 1], g.dimg[3 * qi + 2]) * w;
+                }
+        }
+        g.A[p] = make_float4(A.x, A.y, A.z, 0.f);
+        g.q[0].par[p] = make_int2(-1, 0);
+        g.q[0].sC[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
 // ---- shading ---------------------------------------------------------------------
@@ -226,6 +288,10 @@ __device__ __forceinline__ float u01(uint4 r, int slot) {
     return unit24(x);
 }
 
+constexpr int TEX_SLOTS = 128;
+constexpr int LIGHT_SLOTS = 64;
+constexpr int SHADE_WAVES = 256 / WAVE;
+
 // Step `bounce` of every queued path: first the deferred half of
 // DirectLightEvaluator::evaluate (directLightEvaluator.cpp:18-109) for the
 // vertex shaded at bounce - 1, then, for the continuation hit, one iteration of
@@ -233,18 +299,38 @@ __device__ __forceinline__ float u01(uint4 r, int slot) {
 // (uniformLightCluster.cpp:15-24 / powerLightCluster.cpp:27-34), light sample +
 // shadow ray, MIS-BSDF ray, continuation sample, Russian roulette
 // (russianRoulette.cpp:9-28). bounce == max_depth only resolves.
-__global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
+//
+// ADJ replays the same steps with the same numbers (the radiance sum L below is
+// bit-identical to the forward pass's) and adds the path-replay gradient: when
+// vertex b-1's NEE resolves, its light and MIS-BSDF terms give dLoss/dtexel and
+// dLoss/dLe directly; its continuation gives A * (L_sample - L) * d ln f/dtexel,
+// (L_sample - L) being all radiance the path gathers from vertex b on. Sampling
+// pdfs, MIS weights and the roulette factor are held fixed. Each lane then has
+// at most one texel and one light contribution, which the wave's LDS tables
+// merge before one FP32 atomic per distinct key.
+template <bool ADJ>
+__global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bounce) {
     const DevScene& sc = a.sc;
     const Queue& cur = a.q[bounce & 1];
     const Queue& nxt = a.q[(bounce + 1) & 1];
     const int32_t n = a.qcount[bounce];
     const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    const int64_t first = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    // wave-uniform trip count so that every lane reaches the ballot
-    const int64_t wave_first = first - lane;
-    for (int64_t sbase = wave_first; sbase < n; sbase += stride) {
-        const int64_t s = sbase + lane;
+    __shared__ int   s_tkeys[ADJ ? SHADE_WAVES * TEX_SLOTS : 1];
+    __shared__ float s_tvals[ADJ ? SHADE_WAVES * TEX_SLOTS * 3 : 1];
+    __shared__ int   s_lkeys[ADJ ? SHADE_WAVES * LIGHT_SLOTS : 1];
+    __shared__ float s_lvals[ADJ ? SHADE_WAVES * LIGHT_SLOTS * 3 : 1];
+    WaveTable<TEX_SLOTS> ttab{s_tkeys + (ADJ ? wave * TEX_SLOTS : 0), s_tvals + (ADJ ? wave * TEX_SLOTS * 3 : 0)};
+    WaveTable<LIGHT_SLOTS> ltab{s_lkeys + (ADJ ? wave * LIGHT_SLOTS : 0), s_lvals + (ADJ ? wave * LIGHT_SLOTS * 3 : 0)};
+    if constexpr (ADJ) {
+        ttab.clear_lane(lane);
+        ltab.clear_lane(lane);
+        __syncthreads();
+    }
+    // block-uniform trip count: every lane reaches the ballot and the barriers
+    for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
+        const int64_t s = bbase + threadIdx.x;
         const bool active = s < n;
         bool pending = false;
         int32_t p = 0;
@@ -252,6 +338,11 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
                bv4 = org;
         int32_t pick_light = 0;
         float4 beta4 = org;
+        // adjoint: this lane's contributions and the factors of the vertex shaded now
+        int32_t tkey = -1, lkey = -1;
+        V3 tval = v3(0.f, 0.f, 0.f), lval = v3(0.f, 0.f, 0.f);
+        int2 par = make_int2(-1, 0);
+        float4 sL4 = org, eL4 = org, sB4 = org, eB4 = org, sC4 = org;
         if (active) {
             p = cur.path[s];
             org = cur.org[s];
@@ -260,16 +351,24 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
             V3 L = v3f4(rad4);
             beta4 = a.beta[p];
             V3 beta = v3f4(beta4);
+            V3 A = v3(0.f, 0.f, 0.f);
+            if constexpr (ADJ) A = v3f4(g.A[p]);
             // -- resolve the previous vertex's NEE
             const float4 pv = cur.bv[s];
             if (bounce > 0 && pv.w > 0.0f) {
                 V3 Ld = v3(0.f, 0.f, 0.f);
                 const float4 sd = cur.shd_dir[s];
-                if (sd.w >= 0.0f && cur.occ[s].x == 0) Ld = Ld + v3f4(cur.lt[s]);
+                const int32_t li = cur.light[s];
+                if (sd.w >= 0.0f && cur.occ[s].x == 0) {
+                    Ld = Ld + v3f4(cur.lt[s]);
+                    if constexpr (ADJ) {
+                        tval = tval + A * v3f4(g.q[bounce & 1].sL[s]);
+                        lval = lval + A * v3f4(g.q[bounce & 1].eL[s]);
+                    }
+                }
                 const float4 md = cur.mis_dir[s];
                 if (md.w >= 0.0f) {
                     const int2 mh = cur.mis_hit[s];
-                    const int32_t li = cur.light[s];
                     if (mh.x >= 0) {
                         const int32_t prim = static_cast<int32_t>(__float_as_uint(sc.tris[3 * mh.x].w) & 0x7FFFFFFFu);
                         if (sc.prim_info[prim].y == li) {
@@ -280,10 +379,26 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
                             const float lp = light_pdfW(lg, h, P0);
                             const float w = mis_power(f4.w, lp);
                             Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
+                            if constexpr (ADJ) {
+                                if (can_emit(lg, h.wi, h.Ns)) {
+                                    tval = tval + (A * v3f4(g.q[bounce & 1].sB[s])) * (load3(lg.radiance) * w);
+                                    lval = lval + A * (v3f4(g.q[bounce & 1].eB[s]) * w);
+                                }
+                            }
                         }
                     }
                 }
                 L = L + v3f4(pv) * divs(Ld, pv.w);
+                if constexpr (ADJ) {
+                    const float4 sc4 = g.q[bounce & 1].sC[s];
+                    if (sc4.w > 0.0f) {
+                        const V3 R = v3f4(g.samples[p]) - L;
+                        tval = tval + (A * R) * v3f4(sc4);
+                    }
+                    const int2 pp = g.q[bounce & 1].par[s];
+                    if (pp.x >= 0) tkey = 2 * pp.x + (pp.y == 3 ? 1 : 0);
+                    lkey = li;
+                }
             }
             // -- shade the continuation hit
             cdir = cur.dir[s];
@@ -295,7 +410,20 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
                 const SurfaceHit h = surface_detail(sc, ch.x, P0, v3f4(cdir), __int_as_float(ch.y));
                 const int4 info = sc.prim_info[h.prim];
                 const DevMaterial mat = sc.mats[info.x];
-                if (bounce == 0 && info.y >= 0) L = L + beta * light_emittance(sc.lights[info.y], h);
+                if (bounce == 0 && info.y >= 0) {
+                    L = L + beta * light_emittance(sc.lights[info.y], h);
+                    if constexpr (ADJ) {
+                        if (can_emit(sc.lights[info.y], h.wi, h.Ns)) {
+                            lkey = info.y;
+                            lval = A * beta;
+                        }
+                    }
+                }
+                V3 alb = v3(0.f, 0.f, 0.f);
+                if constexpr (ADJ) {
+                    par = vertex_param(sc, mat, h.uvw);
+                    if (mat.type == 0) alb = mat_albedo(sc, mat, h.uvw);
+                }
                 // light pick
                 float pick;
                 {
@@ -331,6 +459,18 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
                     const float w = mis_power(ls.pdfW, bpdf);
                     const V3 lt = divs((ls.radiance * w) * factor, ls.pdfW);
                     lt4 = make_float4(lt.x, lt.y, lt.z, 0.f);
+                    if constexpr (ADJ) {
+                        const float k = w * absdot(Ldir, h.Ns) / ls.pdfW / pick;
+                        const V3 e = (beta * f) * k;
+                        eL4 = make_float4(e.x, e.y, e.z, 0.f);
+                        V3 sl = v3(0.f, 0.f, 0.f);
+                        if (mat.type == 0) {
+                            if (dot(h.wi, h.Ns) * dot(Ldir, h.Ns) > 0.0f) sl = (beta * ls.radiance) * (k * K_INV_PI);
+                        } else if (!is_zero(f)) {
+                            sl = (e * ls.radiance) * ggx_dlnf_dr(sc, mat, h, Ldir);
+                        }
+                        sL4 = make_float4(sl.x, sl.y, sl.z, 0.f);
+                    }
                 }
                 // MIS-BSDF ray
                 mdir = make_float4(0.f, 0.f, 0.f, -1.f);
@@ -342,6 +482,13 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
                         mdir = make_float4(md.x, md.y, md.z, FLT_MAX);
                         const V3 fc = bs.f * absdot(bs.L, h.Ns);
                         fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
+                        if constexpr (ADJ) {
+                            const float k = absdot(bs.L, h.Ns) / bs.pdf / pick;
+                            const V3 e = (beta * bs.f) * k;
+                            eB4 = make_float4(e.x, e.y, e.z, 0.f);
+                            const V3 sb = mat.type == 0 ? beta * (k * K_INV_PI) : e * ggx_dlnf_dr(sc, mat, h, bs.L);
+                            sB4 = make_float4(sb.x, sb.y, sb.z, 0.f);
+                        }
                     }
                 }
                 bv4 = make_float4(beta.x, beta.y, beta.z, pick);
@@ -361,6 +508,15 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
                         if (alive && bounce + 1 < a.max_depth) {
                             const V3 nd = normalize(cs.L);
                             cdir = make_float4(nd.x, nd.y, nd.z, FLT_MAX);
+                            if constexpr (ADJ) {
+                                if (mat.type == 0) {
+                                    sC4 = make_float4(alb.x > 0.f ? 1.0f / alb.x : 0.f, alb.y > 0.f ? 1.0f / alb.y : 0.f,
+                                                      alb.z > 0.f ? 1.0f / alb.z : 0.f, 1.f);
+                                } else {
+                                    const float d = ggx_dlnf_dr(sc, mat, h, cs.L);
+                                    sC4 = make_float4(d, d, d, 1.f);
+                                }
+                            }
                         }
                     }
                 }
@@ -369,6 +525,38 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
             }
             a.rad[p] = make_float4(L.x, L.y, L.z, 0.f);
             a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+        }
+        if constexpr (ADJ) {
+            // per-wave pre-reduction, then one atomic per distinct texel / light
+            if (tkey >= 0 && !is_zero(tval)) ttab.insert(tkey, tval);
+            if (lkey >= 0 && !is_zero(lval)) ltab.insert(lkey, lval);
+            __syncthreads();
+            for (int i = lane; i < TEX_SLOTS; i += WAVE) {
+                const int k = ttab.keys[i];
+                if (k < 0) continue;
+                const float x = ttab.vals[3 * i], y = ttab.vals[3 * i + 1], z = ttab.vals[3 * i + 2];
+                float* dst = g.gtex + (k >> 1);
+                if (k & 1) {
+                    unsafeAtomicAdd(dst, x);
+                    unsafeAtomicAdd(dst + 1, y);
+                    unsafeAtomicAdd(dst + 2, z);
+                } else {
+                    unsafeAtomicAdd(dst, (x + y) + z);
+                }
+                ttab.keys[i] = -1;
+                ttab.vals[3 * i] = ttab.vals[3 * i + 1] = ttab.vals[3 * i + 2] = 0.f;
+            }
+            for (int i = lane; i < LIGHT_SLOTS; i += WAVE) {
+                const int k = ltab.keys[i];
+                if (k < 0) continue;
+                float* dst = g.glight + 3 * k;
+                unsafeAtomicAdd(dst, ltab.vals[3 * i]);
+                unsafeAtomicAdd(dst + 1, ltab.vals[3 * i + 1]);
+                unsafeAtomicAdd(dst + 2, ltab.vals[3 * i + 2]);
+                ltab.keys[i] = -1;
+                ltab.vals[3 * i] = ltab.vals[3 * i + 1] = ltab.vals[3 * i + 2] = 0.f;
+            }
+            __syncthreads();
         }
         // wave-ballot compaction into the next queue
         const unsigned long long mask = __ballot(pending);
@@ -386,36 +574,25 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, int32_t bounce) {
             nxt.fac[slot] = fac4;
             nxt.bv[slot] = bv4;
             nxt.light[slot] = pick_light;
+            if constexpr (ADJ) {
+                const AdjQueue& an = g.q[(bounce + 1) & 1];
+                an.par[slot] = par;
+                an.sL[slot] = sL4;
+                an.eL[slot] = eL4;
+                an.sB[slot] = sB4;
+                an.eB[slot] = eB4;
+                an.sC[slot] = sC4;
+            }
         }
     }
 }
+
+template __global__ void k_camera<false>(PassArgs, AdjArgs);
+template __global__ void k_camera<true>(PassArgs, AdjArgs);
+template __global__ void k_shade<false>(PassArgs, AdjArgs, int32_t);
+template __global__ void k_shade<true>(PassArgs, AdjArgs, int32_t);
 
 // ---- film ------------------------------------------------------------------------
-
-// Filter::evaluate of the four reference filters (boxFilter.cpp:9-11,
-// gaussianFilter.cpp:25-33, coneFilter.cpp:13-16, mitchellFilter.cpp:16-40).
-__device__ __forceinline__ float mitchell1d(float b, float c, float d) {
-    const float ad = fabsf(d);
-    const float den = 1.0f / 6.0f;
-    if (ad < 1.0f)
-        return ((12.0f - 9.0f * b - 6.0f * c) * ad * ad * ad + (-18.0f + 12.0f * b + 6.0f * c) * ad * ad +
-                (6.0f - 2.0f * b)) * den;
-    if (ad < 2.0f)
-        return ((-b - 6.0f * c) * ad * ad * ad + (6.0f * b + 30.0f * c) * ad * ad + (-12.0f * b - 48.0f * c) * ad +
-                (8.0f * b + 24.0f * c)) * den;
-    return 0.0f;
-}
-__device__ __forceinline__ float filter_eval(const DevScene& sc, float x, float y) {
-    switch (sc.filter) {
-        case 1: {
-            const float v = sc.amplitude * expf(sc.g_expx * x * x + sc.g_expy * y * y);
-            return mmax(0.0f, v - sc.g_edge);
-        }
-        case 2: return mmax(sc.filter_hw - fabsf(x), 0.0f) * mmax(sc.filter_hh - fabsf(y), 0.0f);
-        case 3: return mitchell1d(sc.mitchell_b, sc.mitchell_c, x) * mitchell1d(sc.mitchell_b, sc.mitchell_c, y);
-        default: return 1.0f;
-    }
-}
 
 // FilmTile::addSample (filmTile.cpp:21-66) as a gather: one block per film
 // tile; each pixel sums the pass's samples of its tile whose filter footprint,

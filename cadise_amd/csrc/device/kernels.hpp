@@ -63,6 +63,30 @@ struct Queue {
     int2*    occ;      // shadow: x = occluded
 };
 
+// Per-vertex factors of the path-replay adjoint, stored beside Queue when a
+// vertex is shaded and applied when its NEE rays and continuation resolve at the
+// next bounce. "Texel" is the vertex's differentiable parameter: the albedo
+// texel of a Lambertian vertex (3 channels) or the roughness texel of a GGX
+// vertex (1 channel).
+struct AdjQueue {
+    int2*   par;   // (texture element index of the texel, channels) or (-1, 0)
+    float4* sL;    // light term:   dLterm / dtexel (per channel)
+    float4* eL;    // light term:   dLterm / dLe
+    float4* sB;    // MIS-BSDF term: dLterm / dtexel = sB * Le * w (w, Le known at resolution)
+    float4* eB;    // MIS-BSDF term: dLterm / dLe = eB * w
+    float4* sC;    // continuation: d ln f / dtexel (xyz), w = 1 when the continuation ray was traced
+};
+
+struct AdjArgs {
+    AdjQueue      q[2];
+    const float4* samples;  // each path's radiance in this pass, as the forward pass computed it
+    float4*       A;        // per-path adjoint weight dLoss / dL_sample
+    const float*  dimg;     // dLoss / dI, npix x 3
+    const float4* film;     // the forward film (sum w per pixel)
+    float*        gtex;     // dLoss / d(texture element)
+    float*        glight;   // dLoss / dLe, nlights x 3
+};
+
 struct PassArgs {
     DevScene sc;
     uint2    key;          // Philox key (seed)

@@ -29,12 +29,15 @@ namespace mpt {
 
 template <int W, int S, bool ANY>
 __global__ void k_trace(TraceArgs a);
-__global__ void k_camera(PassArgs a);
-__global__ void k_shade(PassArgs a, int32_t bounce);
+template <bool ADJ>
+__global__ void k_camera(PassArgs a, AdjArgs g);
+template <bool ADJ>
+__global__ void k_shade(PassArgs a, AdjArgs g, int32_t bounce);
 __global__ void k_film(PassArgs a);
 __global__ void k_srgb8(const float4* film, int64_t npix, uint8_t* out);
 __global__ void k_zero_u64(unsigned long long* p, int n);
 __global__ void k_zero_f4(float4* p, int64_t n);
+__global__ void k_zero_i32(int32_t* p, int n);
 
 constexpr int STACK_LDS = 24;
 constexpr int TRACE_BLOCK = 256;
@@ -81,6 +84,20 @@ struct mpt_scene {
     mpt::DevBuf film, beta, rad, fpos, qcount, work, stats, spill, srgb;
     mpt::DevBuf qbuf[2][12];
     int64_t npaths = 0;
+    // parameters that can change after the upload (differentiable rendering)
+    std::vector<mpt::DevLight> h_lights;
+    std::vector<mpt::DevTexture> h_texs;
+    std::vector<mpt::F3> h_rad;
+    int64_t tex_elems = 0;
+    uint64_t param_version = 0;
+    // per-sample radiance kept by a forward render for the adjoint
+    mpt::DevBuf samples;
+    bool keep_valid = false;
+    uint64_t keep_seed = 0, keep_version = 0;
+    int32_t keep_begin = 0, keep_end = 0, keep_depth = 0;
+    // adjoint buffers
+    mpt::DevBuf adjq[2][6], adjA, dimg, gtex, glight;
+    int64_t adj_npaths = 0;
     int64_t spill_lanes = 0;
     int32_t max_depth_alloc = -1;
     mpt_scene_stats sstats{};
@@ -154,6 +171,10 @@ int build_and_upload(mpt_scene* S) {
     S->sstats.image_height = h.height;
     S->sstats.spp = h.spp;
     S->sstats.max_depth = h.max_depth;
+    S->sstats.ntextures = static_cast<int64_t>(h.textures.size());
+    S->sstats.tex_elements = 0;
+    for (const Texture& t : h.textures)
+        S->sstats.tex_elements += (t.data.empty() ? 1 : static_cast<int64_t>(t.width) * t.height) * t.channels;
     if (host_only) return MPT_OK;
     // leaf-ordered primitive records
     std::vector<float4> tris(static_cast<size_t>(3 * n));
@@ -279,6 +300,10 @@ int build_and_upload(mpt_scene* S) {
     if ((rc = upload_vec(ctx, S->texs, texs)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, S->tex_data, texdata)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, S->lights, lights)) != MPT_OK) return rc;
+    S->h_lights = lights;
+    S->h_texs = texs;
+    S->h_rad = rad;
+    S->tex_elems = static_cast<int64_t>(texdata.size());
     if ((rc = upload_vec(ctx, S->power_cdf, cdf)) != MPT_OK) return rc;
     HIP_TRY(ctx, S->stats.alloc(sizeof(unsigned long long) * ST_COUNT));
 
@@ -491,6 +516,12 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
         HIP_TRY(ctx, hipGetLastError());
     }
     if ((rc = zero_stats(S)) != MPT_OK) return rc;
+    const bool keep = p && (p->flags & MPT_RENDER_KEEP_SAMPLES);
+    S->keep_valid = false;
+    if (keep) {
+        const size_t want = sizeof(float4) * static_cast<size_t>(npix) * static_cast<size_t>(std::max(1, spp_end - spp_begin));
+        if (S->samples.bytes < want) HIP_TRY(ctx, S->samples.alloc(want));
+    }
     const int tblocks = trace_blocks(S, 2 * npix);
     if ((rc = ensure_spill(S, tblocks)) != MPT_OK) return rc;
     const int sblocks = static_cast<int>(std::min<int64_t>((npix + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
@@ -517,7 +548,7 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
         PassArgs a = pass_args(S, seed, max_depth);
         a.sample = k;
         HIP_TRY(ctx, mark(EV_CAMERA));
-        hipLaunchKernelGGL(k_camera, dim3((cam_threads + 255) / 256), dim3(256), 0, S->stream, a);
+        hipLaunchKernelGGL(k_camera<false>, dim3((cam_threads + 255) / 256), dim3(256), 0, S->stream, a, AdjArgs{});
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, stop());
         for (int32_t b = 0; b <= max_depth; ++b) {
@@ -551,7 +582,7 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
                 HIP_TRY(ctx, stop());
             }
             HIP_TRY(ctx, mark(EV_SHADE));
-            hipLaunchKernelGGL(k_shade, dim3(sblocks), dim3(256), 0, S->stream, a, b);
+            hipLaunchKernelGGL(k_shade<false>, dim3(sblocks), dim3(256), 0, S->stream, a, AdjArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, stop());
         }
@@ -559,6 +590,9 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
         hipLaunchKernelGGL(k_film, dim3(ntiles), dim3(256), 0, S->stream, a);
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, stop());
+        if (keep)
+            HIP_TRY(ctx, hipMemcpyAsync(S->samples.as<float4>() + static_cast<int64_t>(k - spp_begin) * npix, S->rad.p,
+                                        sizeof(float4) * static_cast<size_t>(npix), hipMemcpyDeviceToDevice, S->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(S->stream));
     unsigned long long st[ST_COUNT];
@@ -588,6 +622,135 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     r.ms_camera = ms[EV_CAMERA];
     r.ms_film = ms[EV_FILM];
     r.ms_total = ms_since(t0);
+    if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    if (keep && !(p->accumulate)) {
+        S->keep_valid = true;
+        S->keep_seed = seed;
+        S->keep_begin = spp_begin;
+        S->keep_end = spp_end;
+        S->keep_depth = max_depth;
+        S->keep_version = S->param_version;
+    }
+    return MPT_OK;
+}
+
+// Path-replay adjoint: the forward pass's kernels in their ADJ form over the
+// same passes, reading the kept per-sample radiance and the film's weights.
+int do_adjoint(mpt_scene* S, const mpt_render_params* p, const float* d_image, float* d_tex, float* d_light) {
+    mpt_ctx* ctx = S->ctx;
+    const auto t0 = std::chrono::steady_clock::now();
+    const HostScene& h = S->host;
+    const int32_t max_depth = (p && p->max_depth > 0) ? p->max_depth : h.max_depth;
+    const int32_t spp_begin = p ? std::max(0, p->spp_begin) : 0;
+    const int32_t spp_end = (p && p->spp_end > 0) ? p->spp_end : h.spp;
+    const uint64_t seed = p ? p->seed : 0;
+    const int64_t npix = static_cast<int64_t>(h.width) * h.height;
+    if (S->tex_elems >= (int64_t{1} << 30)) return fail(ctx, MPT_ERR_UNSUPPORTED, "textures above 2^30 elements");
+    int rc;
+    if (!(S->keep_valid && S->keep_seed == seed && S->keep_begin == spp_begin && S->keep_end == spp_end &&
+          S->keep_depth == max_depth && S->keep_version == S->param_version)) {
+        mpt_render_params fp{seed, spp_begin, spp_end, max_depth, 0, MPT_RENDER_KEEP_SAMPLES};
+        if ((rc = do_render(S, &fp)) != MPT_OK) return rc;
+    }
+    const mpt_stats fwd = S->rstats;
+    if ((rc = ensure_pass_buffers(S, npix, max_depth)) != MPT_OK) return rc;
+    if (S->adj_npaths != npix) {
+        const size_t sizes[6] = {8, 16, 16, 16, 16, 16};
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 6; ++k) HIP_TRY(ctx, S->adjq[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
+        HIP_TRY(ctx, S->adjA.alloc(sizeof(float4) * static_cast<size_t>(npix)));
+        HIP_TRY(ctx, S->dimg.alloc(sizeof(float) * 3 * static_cast<size_t>(npix)));
+        S->adj_npaths = npix;
+    }
+    HIP_TRY(ctx, S->gtex.alloc(sizeof(float) * static_cast<size_t>(std::max<int64_t>(1, S->tex_elems))));
+    HIP_TRY(ctx, S->glight.alloc(sizeof(float) * 3 * static_cast<size_t>(std::max<size_t>(1, h.lights.size()))));
+    HIP_TRY(ctx, hipMemcpyAsync(S->dimg.p, d_image, sizeof(float) * 3 * static_cast<size_t>(npix), hipMemcpyHostToDevice,
+                                S->stream));
+    {
+        const int64_t nt = std::max<int64_t>(1, S->tex_elems);
+        hipLaunchKernelGGL(k_zero_i32, dim3((nt + 255) / 256), dim3(256), 0, S->stream, S->gtex.as<int32_t>(),
+                           static_cast<int>(nt));
+        const int nl = static_cast<int>(3 * std::max<size_t>(1, h.lights.size()));
+        hipLaunchKernelGGL(k_zero_i32, dim3((nl + 255) / 256), dim3(256), 0, S->stream, S->glight.as<int32_t>(), nl);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if ((rc = zero_stats(S)) != MPT_OK) return rc;
+    const int tblocks = trace_blocks(S, 2 * npix);
+    if ((rc = ensure_spill(S, tblocks)) != MPT_OK) return rc;
+    const int sblocks = static_cast<int>(std::min<int64_t>((npix + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
+    const int cam_threads = static_cast<int>(std::max<int64_t>(npix, 3 * (max_depth + 2) + 1));
+    while (S->events.size() < 2) {
+        hipEvent_t e;
+        HIP_TRY(ctx, hipEventCreate(&e));
+        S->events.push_back(e);
+    }
+    HIP_TRY(ctx, hipEventRecord(S->events[0], S->stream));
+    for (int32_t k = spp_begin; k < spp_end; ++k) {
+        PassArgs a = pass_args(S, seed, max_depth);
+        a.sample = k;
+        AdjArgs g{};
+        for (int q = 0; q < 2; ++q) {
+            g.q[q].par = S->adjq[q][0].as<int2>();
+            g.q[q].sL = S->adjq[q][1].as<float4>();
+            g.q[q].eL = S->adjq[q][2].as<float4>();
+            g.q[q].sB = S->adjq[q][3].as<float4>();
+            g.q[q].eB = S->adjq[q][4].as<float4>();
+            g.q[q].sC = S->adjq[q][5].as<float4>();
+        }
+        g.samples = S->samples.as<float4>() + static_cast<int64_t>(k - spp_begin) * npix;
+        g.A = S->adjA.as<float4>();
+        g.dimg = S->dimg.as<float>();
+        g.film = S->film.as<float4>();
+        g.gtex = S->gtex.as<float>();
+        g.glight = S->glight.as<float>();
+        hipLaunchKernelGGL(k_camera<true>, dim3((cam_threads + 255) / 256), dim3(256), 0, S->stream, a, g);
+        HIP_TRY(ctx, hipGetLastError());
+        for (int32_t b = 0; b <= max_depth; ++b) {
+            const Queue& Q = a.q[b & 1];
+            TraceArgs t{};
+            t.sc = S->sc;
+            t.org = Q.org;
+            t.dir = Q.dir;
+            t.dir2 = Q.mis_dir;
+            t.stride = 1;
+            t.count = a.qcount + b;
+            t.dual = 1;
+            t.hit = Q.hit;
+            t.hit2 = Q.mis_hit;
+            t.want_prim = 0;
+            t.work = a.work + 3 * b;
+            t.stats = S->stats.as<unsigned long long>();
+            t.stat_slot = ST_CLOSEST_RAYS;
+            if ((rc = launch_trace(S, t, false, tblocks)) != MPT_OK) return rc;
+            if (b > 0) {
+                t.dir = Q.shd_dir;
+                t.dual = 0;
+                t.hit = Q.occ;
+                t.hit2 = nullptr;
+                t.work = a.work + 3 * b + 1;
+                t.stat_slot = ST_SHADOW_RAYS;
+                if ((rc = launch_trace(S, t, true, tblocks)) != MPT_OK) return rc;
+            }
+            hipLaunchKernelGGL(k_shade<true>, dim3(sblocks), dim3(256), 0, S->stream, a, g, b);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    }
+    HIP_TRY(ctx, hipEventRecord(S->events[1], S->stream));
+    if (d_tex && S->tex_elems > 0)
+        HIP_TRY(ctx, hipMemcpyAsync(d_tex, S->gtex.p, sizeof(float) * static_cast<size_t>(S->tex_elems),
+                                    hipMemcpyDeviceToHost, S->stream));
+    if (d_light && !h.lights.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(d_light, S->glight.p, sizeof(float) * 3 * h.lights.size(), hipMemcpyDeviceToHost,
+                                    S->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(S->stream));
+    unsigned long long st[ST_COUNT];
+    if ((rc = read_stats(S, st)) != MPT_OK) return rc;
+    float adj_ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&adj_ms, S->events[0], S->events[1]));
+    mpt_stats& r = S->rstats;
+    r = fwd;
+    r.ms_total = ms_since(t0);
+    r.ms_adjoint = adj_ms;
     if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
     return MPT_OK;
 }
@@ -686,8 +849,14 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
             return fail(ctx, MPT_ERR_INVALID, "bad primitive type");
         if (h.prim_mat[i] < 0 || h.prim_mat[i] >= d->nmats) return fail(ctx, MPT_ERR_INVALID, "bad material index");
     }
+    std::vector<std::pair<const float*, int32_t>> seen;   // the same array in several materials is one texture
     auto add_tex = [&](const mpt_texture_desc& t, int channels) -> int32_t {
         if (!t.data) return -1;
+        for (const auto& sn : seen)
+            if (sn.first == t.data && h.textures[sn.second].channels == channels &&
+                h.textures[sn.second].width == t.width && h.textures[sn.second].height == t.height)
+                return sn.second;
+        seen.push_back({t.data, static_cast<int32_t>(h.textures.size())});
         Texture x;
         x.width = t.width;
         x.height = t.height;
@@ -907,6 +1076,70 @@ int mpt_film_save(mpt_scene* s, const char* path) {
         return err.empty() ? MPT_OK : fail(s->ctx, MPT_ERR_IO, err);
     }
     return fail(s->ctx, MPT_ERR_UNSUPPORTED, "output format of " + p + " (png, ppm, pfm)");
+}
+
+int mpt_scene_texture_info(const mpt_scene* s, int32_t index, int64_t* width, int64_t* height, int64_t* channels,
+                           int64_t* offset) {
+    if (!s) return MPT_ERR_INVALID;
+    const auto& T = s->host.textures;
+    if (index < 0 || index >= static_cast<int32_t>(T.size())) return fail(s->ctx, MPT_ERR_INVALID, "texture index");
+    int64_t off = 0;
+    for (int32_t i = 0; i <= index; ++i) {
+        const int64_t w = T[i].data.empty() ? 1 : T[i].width, hh = T[i].data.empty() ? 1 : T[i].height;
+        if (i == index) {
+            if (width) *width = w;
+            if (height) *height = hh;
+            if (channels) *channels = T[i].channels;
+            if (offset) *offset = off;
+        }
+        off += w * hh * T[i].channels;
+    }
+    return MPT_OK;
+}
+
+int mpt_scene_set_texture(mpt_scene* s, int32_t index, const float* data) {
+    if (!s || !data) return MPT_ERR_INVALID;
+    int64_t w = 0, hh = 0, c = 0, off = 0;
+    const int rc = mpt_scene_texture_info(s, index, &w, &hh, &c, &off);
+    if (rc != MPT_OK) return rc;
+    const size_t n = static_cast<size_t>(w * hh * c);
+    Texture& t = s->host.textures[index];
+    if (t.data.empty()) {
+        t.value = {data[0], c == 3 ? data[1] : data[0], c == 3 ? data[2] : data[0]};
+    } else {
+        t.data.assign(data, data + n);
+    }
+    ++s->param_version;
+    if (s->ctx->device < 0) return MPT_OK;
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    HIP_TRY(s->ctx, hipMemcpyAsync(s->tex_data.as<float>() + off, data, n * sizeof(float), hipMemcpyHostToDevice,
+                                   s->stream));
+    HIP_TRY(s->ctx, hipStreamSynchronize(s->stream));
+    return MPT_OK;
+}
+
+int mpt_scene_set_light_radiance(mpt_scene* s, int32_t light, const float* rgb) {
+    if (!s || !rgb) return MPT_ERR_INVALID;
+    if (light < 0 || light >= static_cast<int32_t>(s->host.lights.size())) return fail(s->ctx, MPT_ERR_INVALID, "light index");
+    ++s->param_version;
+    if (s->ctx->device < 0) return MPT_OK;
+    s->h_rad[light] = F3{rgb[0], rgb[1], rgb[2]};
+    DevLight& d = s->h_lights[light];
+    d.radiance[0] = rgb[0]; d.radiance[1] = rgb[1]; d.radiance[2] = rgb[2];
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    HIP_TRY(s->ctx, hipMemcpy(s->lights.p, s->h_lights.data(), sizeof(DevLight) * s->h_lights.size(),
+                              hipMemcpyHostToDevice));
+    // the power cluster's CDF follows the lights' flux
+    const std::vector<float> cdf = power_cdf(s->host, s->h_rad);
+    HIP_TRY(s->ctx, hipMemcpy(s->power_cdf.p, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice));
+    return MPT_OK;
+}
+
+int mpt_render_adjoint(mpt_scene* s, const mpt_render_params* p, const float* d_image, float* d_tex, float* d_light) {
+    if (!s || !d_image) return MPT_ERR_INVALID;
+    REQUIRE_DEVICE(s);
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    return do_adjoint(s, p, d_image, d_tex, d_light);
 }
 
 int mpt_stats_get(const mpt_scene* s, mpt_stats* out) {

@@ -56,12 +56,16 @@ class SceneStats(C.Structure):
                 ("wide_nodes", C.c_int64), ("binary_depth", C.c_int32), ("wide_depth", C.c_int32),
                 ("width", C.c_int32), ("leaf_max", C.c_int32), ("build_ms", C.c_double), ("collapse_ms", C.c_double),
                 ("device_bytes", C.c_int64), ("image_width", C.c_int32), ("image_height", C.c_int32),
-                ("spp", C.c_int32), ("max_depth", C.c_int32)]
+                ("spp", C.c_int32), ("max_depth", C.c_int32), ("ntextures", C.c_int64),
+                ("tex_elements", C.c_int64)]
 
 
 class RenderParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("spp_begin", C.c_int32), ("spp_end", C.c_int32), ("max_depth", C.c_int32),
-                ("accumulate", C.c_int32)]
+                ("accumulate", C.c_int32), ("flags", C.c_int32)]
+
+
+MPT_RENDER_KEEP_SAMPLES = 1
 
 
 class Stats(C.Structure):
@@ -70,7 +74,8 @@ class Stats(C.Structure):
                 ("box_tests_any", C.c_int64), ("tri_tests_any", C.c_int64), ("closest_launches", C.c_int64),
                 ("anyhit_launches", C.c_int64),
                 ("ms_total", C.c_double), ("ms_closest", C.c_double), ("ms_anyhit", C.c_double),
-                ("ms_shade", C.c_double), ("ms_camera", C.c_double), ("ms_film", C.c_double)]
+                ("ms_shade", C.c_double), ("ms_camera", C.c_double), ("ms_film", C.c_double),
+                ("ms_adjoint", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -79,7 +84,8 @@ class Stats(C.Structure):
 EXPORTS = ["mpt_create", "mpt_destroy", "mpt_last_error", "mpt_version", "mpt_scene_load_crsd", "mpt_scene_parse_crsd",
            "mpt_scene_create", "mpt_scene_destroy", "mpt_scene_stats_get", "mpt_scene_binary_bvh", "mpt_scene_camera",
            "mpt_scene_light_radiance", "mpt_trace",
-           "mpt_trace_device", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_save", "mpt_stats_get"]
+           "mpt_trace_device", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_save", "mpt_stats_get",
+           "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_set_light_radiance", "mpt_render_adjoint"]
 
 _lib = None
 
@@ -112,5 +118,9 @@ def load_library():
     L.mpt_film_srgb8.argtypes = [vp, vp]
     L.mpt_film_save.argtypes = [vp, C.c_char_p]
     L.mpt_stats_get.argtypes = [vp, C.POINTER(Stats)]
+    L.mpt_scene_texture_info.argtypes = [vp, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
+    L.mpt_scene_set_texture.argtypes = [vp, i32, vp]
+    L.mpt_scene_set_light_radiance.argtypes = [vp, i32, vp]
+    L.mpt_render_adjoint.argtypes = [vp, C.POINTER(RenderParams), vp, vp, vp]
     _lib = L
     return L

@@ -486,3 +486,108 @@ def surface_rays(spec: SceneSpec, n: int, seed: int) -> np.ndarray:
     rays[:, 4:7] = d
     rays[:, 7] = np.finfo(np.float32).max
     return rays
+
+
+# -- differentiable-rendering scene ----------------------------------------------------
+
+def smooth_noise_texture(res: int, channels: int, lo: float, hi: float, seed: int, freq: float = 4.0) -> np.ndarray:
+    """res x res x channels float32 in [lo, hi]: two octaves of Perlin noise,
+    tileable in neither direction (the texture is sampled nearest-texel)."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm, perm]).astype(np.int64)
+    y, x = np.meshgrid(np.arange(res) + 0.5, np.arange(res) + 0.5, indexing="ij")
+    out = np.empty((res, res, channels), np.float32)
+    for c in range(channels):
+        off = rng.uniform(0, 100, 3)
+        p = np.stack([x.ravel() / res * freq + off[0], y.ravel() / res * freq + off[1], np.full(x.size, off[2])], 1)
+        n = _perlin3(p, perm) + 0.5 * _perlin3(p * 2.0 + 31.0, perm)
+        n = (n - n.min()) / max(n.max() - n.min(), 1e-9)
+        out[..., c] = (lo + (hi - lo) * n).reshape(res, res)
+    return out
+
+
+def sphere_uvs(unit_verts: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """Per-corner (u, v, 0) of a spherical mapping, u = atan2(z, x) / 2pi + 1/2,
+    v = acos(y) / pi; triangles straddling the u seam get u + 1 on their small
+    corners (repeat wrap brings them back), so no triangle sweeps the texture."""
+    u = np.arctan2(unit_verts[:, 2], unit_verts[:, 0]) / (2 * np.pi) + 0.5
+    v = np.arccos(np.clip(unit_verts[:, 1], -1, 1)) / np.pi
+    fu = u[faces]
+    seam = fu.max(1) - fu.min(1) > 0.5
+    fu[seam] = np.where(fu[seam] < 0.5, fu[seam] + 1.0, fu[seam])
+    uvw = np.zeros((len(faces), 3, 3), np.float64)
+    uvw[:, :, 0] = fu
+    uvw[:, :, 1] = v[faces]
+    return uvw.reshape(-1, 9).astype(np.float32)
+
+
+def textured_spheres(seed=0x20261203 ^ 0x5, n_spheres=10, level=5, tex_res=256, width=512, height=512, spp=16,
+                     max_depth=4, albedo=None, roughness=None, light_radiance=None, light_half=2.0) -> SceneSpec:
+    """Differentiation scene: n_spheres displaced icospheres (20 * 4^level
+    triangles each; 10 x level 5 = 204,800) on a ground quad, spherical uvs;
+    odd spheres Lambertian with the shared albedo texture (tex_res^2 RGB, smooth
+    noise in [0.15, 0.85]), even spheres GGX conductors with the shared roughness
+    texture (tex_res^2, smooth noise in [0.1, 0.9]); the ground uses the albedo
+    texture too. One rectangle light whose emitted radiance is a parameter
+    (watt chosen so Le = light_radiance, default (10, 10, 10)). Every coordinate is
+    positive."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm, perm]).astype(np.int64)
+    uv, uf = icosphere(level)
+    o = np.array([20.0, 0.0, 20.0])
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="box", fov=50.0,
+                  cam_pos=tuple(o + [0.0, 4.5, 11.0]), cam_dir=(0.0, -0.38, -1.0), cam_up=(0.0, 1.0, 0.0))
+    alb = smooth_noise_texture(tex_res, 3, 0.15, 0.85, seed + 1) if albedo is None else albedo
+    rough = smooth_noise_texture(tex_res, 1, 0.1, 0.9, seed + 2) if roughness is None else roughness
+    s.textures.append(Texture("albedo", np.ascontiguousarray(alb, np.float32), kind="spectrum"))
+    s.textures.append(Texture("rough", np.ascontiguousarray(rough, np.float32).reshape(tex_res, tex_res, 1),
+                              kind="real"))
+    s.materials.append(Material("ground", albedo_texture="albedo"))
+    s.materials.append(Material("lightmat", albedo=(0.8, 0.8, 0.8)))
+    s.materials.append(Material("diffuse", albedo_texture="albedo"))
+    centres, radii = [], []
+    extent = 4.5
+    while len(centres) < n_spheres:
+        r = float(np.exp(rng.uniform(np.log(0.5), np.log(1.3))))
+        c = np.array([o[0] + rng.uniform(-extent, extent), r * 1.05, o[2] + rng.uniform(-extent, extent * 0.6)])
+        if all(np.linalg.norm(c - c2) > (r + r2) * 1.1 for c2, r2 in zip(centres, radii)):
+            centres.append(c)
+            radii.append(r)
+    uvw = sphere_uvs(uv, uf)
+    for i, (c, r) in enumerate(zip(centres, radii)):
+        n = _perlin3(uv * 2.0 + rng.uniform(0, 100, 3), perm)
+        pos = uv * (r * (1.0 + 0.05 * n))[:, None] + c
+        tri = pos[uf].reshape(-1, 9).astype(np.float32)
+        fn = np.cross(pos[uf[:, 1]] - pos[uf[:, 0]], pos[uf[:, 2]] - pos[uf[:, 0]])
+        vn = np.zeros_like(pos)
+        for k in range(3):
+            np.add.at(vn, uf[:, k], fn)
+        vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+        nrm = vn[uf].reshape(-1, 9).astype(np.float32)
+        if i % 2:
+            mname = "diffuse"
+        else:
+            mname = f"ggx{i}"
+            s.materials.append(Material(mname, type="conductor-microfacet", f0=tuple(rng.uniform(0.3, 0.9, 3)),
+                                        roughness_texture="rough"))
+        s.meshes.append(Mesh(f"sphere{i}", mname, v=tri, normals=nrm, uvws=uvw, as_mesh=True))
+    g = 14.0
+    ground = quad_tris((o[0] - g, 0.0, o[2] - g), (o[0] - g, 0.0, o[2] + g), (o[0] + g, 0.0, o[2] + g),
+                       (o[0] + g, 0.0, o[2] - g))
+    guv = ((ground.reshape(-1, 3)[:, [0, 2]] - (o[[0, 2]] - g)) / (2 * g) * 2.0).reshape(-1, 3, 2)
+    gu = np.zeros((len(ground), 3, 3), np.float32)
+    gu[:, :, :2] = guv
+    s.meshes.append(Mesh("ground", "ground", v=ground, uvws=gu.reshape(-1, 9)))
+    # square rectangle light (side 2 * light_half), facing down, 7 above the ground
+    h = light_half
+    y = 7.0
+    rect = np.array([[o[0] - h, y, o[2] - h, o[0] - h, y, o[2] + h, o[0] + h, y, o[2] + h]], np.float32)
+    s.meshes.append(Mesh("lamp", "lightmat", kind=PRIM_RECTANGLE, v=rect))
+    Le = (10.0, 10.0, 10.0) if light_radiance is None else tuple(light_radiance)
+    # Le = color / sum(color) * watt / (area * pi) (singleAreaLight.cpp:32-36) with color = Le / sum(Le) * 3
+    area = (2 * h) * (2 * h)
+    tot = float(sum(Le))
+    s.lights.append(Light("lamp", tuple(3.0 * x / tot for x in Le), tot * area * np.pi))
+    return s

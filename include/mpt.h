@@ -125,6 +125,8 @@ typedef struct {
     double  collapse_ms;
     int64_t device_bytes;
     int32_t image_width, image_height, spp, max_depth;
+    int64_t ntextures;         /* image and constant textures, in creation order */
+    int64_t tex_elements;      /* floats of all textures together (the texture-gradient length) */
 } mpt_scene_stats;
 
 typedef struct {
@@ -132,7 +134,10 @@ typedef struct {
     int32_t  spp_begin, spp_end; /* sample passes [begin, end); end <= 0 takes the scene's spp */
     int32_t  max_depth;          /* <= 0 takes the scene's max-depth */
     int32_t  accumulate;         /* 1 = add into the film, 0 = clear it first */
+    int32_t  flags;              /* MPT_RENDER_KEEP_SAMPLES: keep each sample's radiance for mpt_render_adjoint */
 } mpt_render_params;
+
+enum { MPT_RENDER_KEEP_SAMPLES = 1 };
 
 /* Counters of the last render or trace call. */
 typedef struct {
@@ -143,6 +148,8 @@ typedef struct {
     int64_t closest_launches, anyhit_launches;
     double  ms_total;             /* wall time of the call */
     double  ms_closest, ms_anyhit, ms_shade, ms_camera, ms_film; /* summed kernel time (events) */
+    double  ms_adjoint;           /* mpt_render_adjoint: GPU time of the replay (events); the other
+                                     fields then describe the forward render it replays */
 } mpt_stats;
 
 /* device >= 0: a GPU context. device == -1: a host-only context, in which
@@ -193,6 +200,28 @@ int  mpt_film_srgb8(mpt_scene* scene, uint8_t* out);
 int  mpt_film_save(mpt_scene* scene, const char* path);
 
 int  mpt_stats_get(const mpt_scene* scene, mpt_stats* out);
+
+/* ---- differentiable rendering (no counterpart in the reference) -------------------
+ * Parameters: the texels of every texture (albedo of matte-lambertian, roughness
+ * of conductor-microfacet; constant textures are 1x1) and the emitted radiance
+ * Le of every light (SingleAreaLight's radiance, singleAreaLight.cpp:32-36).
+ * Texture i occupies elements [offset, offset + W*H*C) of the flat texture
+ * array (row-major H x W x C). */
+int  mpt_scene_texture_info(const mpt_scene* scene, int32_t index, int64_t* width, int64_t* height,
+                            int64_t* channels, int64_t* offset);
+/* Replace texture `index` (W x H x C host floats) / one light's radiance (3 floats). */
+int  mpt_scene_set_texture(mpt_scene* scene, int32_t index, const float* data);
+int  mpt_scene_set_light_radiance(mpt_scene* scene, int32_t light, const float* rgb);
+/* Path-replay adjoint of an image loss: given d_image = dLoss/dI (W x H x 3,
+ * host; I = the resolved film sum w*rgb / sum w), accumulate dLoss/d(texel)
+ * into d_tex (tex_elements floats, host, overwritten) and dLoss/dLe into
+ * d_light (nlights x 3, overwritten); either may be NULL. Replays exactly the
+ * paths of mpt_render with the same seed, passes and depth (it renders them
+ * first, keeping samples, unless the last mpt_render call did so with the same
+ * parameters). Sampling, MIS weights and Russian roulette are held fixed;
+ * only BSDF values and emitted radiance are differentiated. */
+int  mpt_render_adjoint(mpt_scene* scene, const mpt_render_params* params, const float* d_image, float* d_tex,
+                        float* d_light);
 
 #ifdef __cplusplus
 }

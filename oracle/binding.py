@@ -82,6 +82,11 @@ def oracle_lib():
         L.or_time_render.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_uint64, _i64p, _i64p]
         L.or_philox.argtypes = [np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")] * 3
         L.or_film_to_srgb8.argtypes = [_f32p, C.c_int64, _u8p]
+        L.or_adjoint.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int, _f32p, _f32p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_set_light_radiance.argtypes = [C.c_void_p, C.c_int, _f32p]
+        L.or_bsdf_dfdtheta_query.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64,
+                                             np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
         _oracle = L
     return _oracle
 
@@ -99,6 +104,7 @@ class OracleScene:
         t, v, n, u, m = spec.prim_arrays()
         self._keep = [t, v, n, u, m]
         tex = {x.name: x for x in spec.textures}
+        self.tex_arrays = {}
 
         def texture(const, name, channels):
             T = OrTexture()
@@ -107,8 +113,9 @@ class OracleScene:
                 T.value[:] = [float(np.float32(x)) for x in c]
                 T.data = None
                 return T
-            arr = np.ascontiguousarray(tex[name].data, np.float32)
-            self._keep.append(arr)
+            if name not in self.tex_arrays:
+                self.tex_arrays[name] = np.array(tex[name].data, np.float32, order="C", copy=True)
+            arr = self.tex_arrays[name]
             T.data = arr.ctypes.data
             T.height, T.width = arr.shape[0], arr.shape[1]
             T.channels = channels
@@ -206,6 +213,12 @@ class OracleScene:
         self.L.or_bsdf_query(self.h, mat_index, q, len(q), out)
         return out
 
+    def bsdf_dfdtheta(self, mat_index, q):
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.zeros((len(q), 3), np.float64)
+        self.L.or_bsdf_dfdtheta_query(self.h, mat_index, q, len(q), out)
+        return out
+
     def light_sample(self, li, q, nu):
         q = np.ascontiguousarray(q, np.float32)
         out = np.zeros((len(q), 11), np.float32)
@@ -242,6 +255,41 @@ class OracleScene:
         samples = np.zeros(1, np.int64)
         secs = self.L.or_time_render(self.h, begin, end, spp, threads, seed, rays, samples)
         return secs, rays, int(samples[0])
+
+    # -- differentiable-rendering check (FP64 host gradient) --------------------------
+    def set_texture(self, name, data):
+        """Overwrite texture `name` in place (every material using it sees the change)."""
+        self.tex_arrays[name][...] = np.asarray(data, np.float32).reshape(self.tex_arrays[name].shape)
+
+    def set_light_radiance(self, li, rgb):
+        self.L.or_set_light_radiance(self.h, li, np.ascontiguousarray(rgb, np.float32))
+
+    def adjoint(self, seed, film, d_image, spp_begin=0, spp_end=None, threads=8):
+        """dLoss/d(texture) per texture name (summed over the materials sharing it)
+        and dLoss/dLe (nlights x 3), float64."""
+        spp_end = self.spec.spp if spp_end is None else spp_end
+        nm = len(self.spec.materials)
+        alb = [None] * nm
+        rough = [None] * nm
+        for i, m in enumerate(self.spec.materials):
+            if m.albedo_texture is not None:
+                alb[i] = np.zeros(self.tex_arrays[m.albedo_texture].shape[:2] + (3,), np.float64)
+            if m.roughness_texture is not None:
+                rough[i] = np.zeros(self.tex_arrays[m.roughness_texture].shape[:2], np.float64)
+        palb = (C.c_void_p * max(1, nm))(*[a.ctypes.data if a is not None else None for a in alb])
+        prough = (C.c_void_p * max(1, nm))(*[a.ctypes.data if a is not None else None for a in rough])
+        dlight = np.zeros((max(1, len(self.spec.lights)), 3), np.float64)
+        self.L.or_adjoint(self.h, seed, spp_begin, spp_end, threads, np.ascontiguousarray(film, np.float32),
+                          np.ascontiguousarray(d_image, np.float32), C.cast(palb, C.c_void_p),
+                          C.cast(prough, C.c_void_p), dlight.ctypes.data)
+        out = {}
+        for i, m in enumerate(self.spec.materials):
+            for name, g in ((m.albedo_texture, alb[i]), (m.roughness_texture, rough[i])):
+                if name is None:
+                    continue
+                g = g.reshape(self.tex_arrays[name].shape)
+                out[name] = out[name] + g if name in out else g
+        return out, dlight[: len(self.spec.lights)]
 
 
 def philox(ctr, key):
@@ -373,6 +421,12 @@ class RefScene:
         out = np.zeros((len(q), 12), np.float32)
         if self.L.ref_bsdf_query(self.h, material.encode(), q, len(q), out) != 0:
             raise KeyError(material)
+        return out
+
+    def bsdf_dfdtheta(self, mat_index, q):
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.zeros((len(q), 3), np.float64)
+        self.L.or_bsdf_dfdtheta_query(self.h, mat_index, q, len(q), out)
         return out
 
     def light_sample(self, li, q, nu):

@@ -1090,3 +1090,369 @@ void or_film_to_srgb8(const float* film, int64_t npix, uint8_t* out) {
         }
     }
 }
+
+/* ---- path-replay gradient, FP64 host reference -------------------------------------------
+ * Not a reference function: the reference has no derivatives. This is the host check of
+ * the GPU adjoint. Each path of the Philox layout is re-run exactly as estimate() runs it
+ * (same floats, same control flow); every factor of its contribution is recorded per
+ * vertex; the derivative of the sample's radiance is then formed by a backward pass in
+ * double precision over the suffix sums S_j = E_j + N_j + c_j * S_{j+1}:
+ *     dL/dtheta_j = beta_j * (dN_j/dtheta + dc_j/dtheta * S_{j+1})
+ * with sampling pdfs, MIS weights and the Russian-roulette factor held fixed, so only the
+ * BSDF value f (albedo of a Lambertian vertex, roughness of a GGX vertex) and the emitted
+ * radiance Le are differentiated. The loss enters through A = sum_q dLoss/dI_q w(q, s) / W_q
+ * over the sample's (tile-clamped) filter footprint, W_q = the film's sum of weights. */
+
+typedef struct { double x, y, z; } v3d;
+static v3d d3(v3 a) { v3d r = {a.x, a.y, a.z}; return r; }
+static v3d d3_make(double x, double y, double z) { v3d r = {x, y, z}; return r; }
+static double d3_dot(v3d a, v3d b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3d d3_norm(v3d a) { const double l = sqrt(d3_dot(a, a)); return d3_make(a.x / l, a.y / l, a.z / l); }
+
+/* d ln f / d alpha of f = F G D / (4 |V.N| |L.N|) (isotropicGgx.cpp:11-69): D and both G1
+ * factors depend on alpha, F does not. In double from the float vectors. */
+static double ggx_dlnf_dalpha(double a, v3 Vf, v3 Lf, v3 Nf) {
+    const v3d V = d3(Vf), L = d3(Lf), N = d3(Nf);
+    const double vn = d3_dot(V, N), ln = d3_dot(L, N);
+    const double sg = vn > 0 ? 1.0 : (vn < 0 ? -1.0 : 0.0);
+    const v3d H = d3_norm(d3_make((V.x + L.x) * sg, (V.y + L.y) * sg, (V.z + L.z) * sg));
+    const double c = d3_dot(N, H), c2 = c * c;
+    const double tan2 = (1.0 - c2) / c2;
+    double r = 2.0 / a - 4.0 * a / (a * a + tan2);
+    const double xs[2] = {vn, ln};
+    for (int i = 0; i < 2; ++i) {
+        const double t2 = (1.0 - xs[i] * xs[i]) / (xs[i] * xs[i]);
+        const double sq = sqrt(1.0 + a * a * t2);
+        r -= a * t2 / (sq * (1.0 + sq));
+    }
+    return r;
+}
+/* d alpha / d r of TRoughnessMapper<SQUARE> (alpha = max(max(r, 1e-3)^2, 1e-3)) */
+static double dalpha_dr(float r) {
+    const float safe = m_max(r, 0.001f);
+    return (r > 0.001f && safe * safe > 0.001f) ? 2.0 * (double)r : 0.0;
+}
+
+static int64_t tex_texel(const or_texture* t, v3 uvw) {
+    if (!t->data) return -1;
+    float u, v;
+    if (t->wrap_clamp) { u = m_clamp(uvw.x, 0.0f, 1.0f); v = m_clamp(uvw.y, 0.0f, 1.0f); }
+    else { u = uvw.x - floorf(uvw.x); v = uvw.y - floorf(uvw.y); }
+    int32_t x = (int32_t)(u * (float)t->width);
+    int32_t y = (int32_t)(v * (float)t->height);
+    x = (x < t->width) ? x : x - 1;
+    y = (y < t->height) ? y : y - 1;
+    return (int64_t)y * t->width + x;
+}
+
+/* df/dtheta of the vertex's differentiable parameter at direction L (wo in si):
+ * Lambertian: per channel, df_c/da_c; GGX: df_c/dr. Zero where f vanishes identically. */
+static v3d bsdf_dfdtheta(const or_material* m, const or_si* si, v3 f) {
+    const float vn = v3_dot(si->wi, si->Ns), ln = v3_dot(si->wo, si->Ns);
+    if (vn * ln <= 0.0f) return d3_make(0, 0, 0);
+    if (m->type == OR_MAT_LAMBERT) return d3_make(OR_INV_PI, OR_INV_PI, OR_INV_PI);
+    if (v3_is_zero(f)) return d3_make(0, 0, 0);
+    const float r = tex_eval(&m->roughness, si->uvw).x;
+    const double k = ggx_dlnf_dalpha((double)rough_to_alpha(r), si->wi, si->wo, si->Ns) * dalpha_dr(r);
+    return d3_make(f.x * k, f.y * k, f.z * k);
+}
+
+typedef struct {
+    int     mat;
+    int64_t texel;          /* texel of the vertex's differentiable texture, -1 none */
+    int     emit_li;        /* emission counted here (bounce 0), light index or -1 */
+    int     li;             /* picked light */
+    double  pick;
+    int     lt_on, mt_on, cont;
+    v3d     Le;             /* radiance of the picked light */
+    v3d     lt_f, lt_df, mt_f, mt_df, c_f, c_df;
+    double  lt_k, mt_k, c_k; /* w |cos| / pdf (light, MIS); |cos| / pdf * rr (continuation) */
+} adj_vertex;
+
+#define OR_ADJ_MAXV 64
+
+typedef struct {
+    const or_scene* s;
+    uint64_t seed;
+    int spp_begin, spp_end;
+    const float* film;
+    const float* dimg;
+    int64_t ntiles;
+    volatile int64_t next;
+    pthread_mutex_t lock;
+    int64_t nparams;
+    const int64_t* alb_off;   /* per material: offset in the flat parameter array or -1 */
+    const int64_t* rough_off;
+    int64_t light_off;
+} adj_job;
+
+typedef struct { adj_job* J; double* acc; } adj_worker_arg;
+
+/* One path: estimate() with every factor recorded (ptEstimator.cpp:25-123,
+ * directLightEvaluator.cpp:18-109). Returns the number of vertices. */
+static int adj_record(const or_scene* s, v3 o, v3 d, or_rng* rng, adj_vertex* vs) {
+    v3 ro = o, rd = d;
+    v3 beta = v3_make(1, 1, 1);
+    int n = 0;
+    for (int bounce = 0; bounce < s->desc.max_depth && n < OR_ADJ_MAXV; ++bounce) {
+        or_si si;
+        if (!scene_intersect(s, ro, rd, OR_RAY_EPS, FLT_MAX, &si, NULL)) break;
+        const or_prim* p = &s->prims[si.prim];
+        const or_material* m = &s->mats[p->mat];
+        adj_vertex* V = &vs[n++];
+        memset(V, 0, sizeof(*V));
+        V->mat = p->mat;
+        V->texel = tex_texel(m->type == OR_MAT_LAMBERT ? &m->albedo : &m->roughness, si.uvw);
+        V->emit_li = -1;
+        if (p->light >= 0 && bounce == 0 && can_emit(&s->lights[p->light], si.wi, si.Ns)) V->emit_li = p->light;
+        float pickPdf;
+        const int li = pick_light(s, rng, bounce, &pickPdf);
+        V->li = li;
+        V->pick = pickPdf;
+        V->Le = d3(s->lights[li].radiance);
+        or_si sl = si;
+        {
+            const or_light_smp ls = light_sample(s, li, si.P, rng, bounce);
+            if (ls.valid) {
+                const v3 LV = v3_sub(ls.pos, si.P);
+                const float dist = v3_len(LV);
+                const v3 L = v3_divs(LV, dist);
+                sl.wo = L;
+                const v3 sd = v3_normalize(L);
+                if (!or_occluded(s, si.P, sd, OR_RAY_EPS, dist - OR_RAY_EPS)) {
+                    const v3 f = bsdf_eval(m, &sl);
+                    const float bpdf = bsdf_pdf(m, &sl);
+                    const float w = mis_power(ls.pdfW, bpdf);
+                    V->lt_on = 1;
+                    V->lt_f = d3(f);
+                    V->lt_df = bsdf_dfdtheta(m, &sl, f);
+                    V->lt_k = (double)w * (double)v3_absdot(L, si.Ns) / (double)ls.pdfW;
+                }
+            }
+        }
+        {
+            float u0, u1;
+            rng_u2(rng, bounce, SL_MIS0, SL_MIS1, &u0, &u1);
+            const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+            if (bs.valid) {
+                or_si hit;
+                if (scene_intersect(s, si.P, v3_normalize(bs.L), OR_RAY_EPS, FLT_MAX, &hit, NULL) &&
+                    s->prims[hit.prim].light == li && can_emit(&s->lights[li], hit.wi, hit.Ns)) {
+                    const float lpdf = light_pdfW(s, li, &hit, si.P);
+                    const float w = mis_power(bs.pdf, lpdf);
+                    or_si sb = si;
+                    sb.wo = bs.L;
+                    V->mt_on = 1;
+                    V->mt_f = d3(bs.f);
+                    V->mt_df = bsdf_dfdtheta(m, &sb, bs.f);
+                    V->mt_k = (double)w * (double)v3_absdot(bs.L, si.Ns) / (double)bs.pdf;
+                }
+            }
+        }
+        float u0, u1;
+        rng_u2(rng, bounce, SL_CONT0, SL_CONT1, &u0, &u1);
+        const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+        if (!bs.valid) break;
+        const float LdotN = v3_absdot(bs.L, si.Ns);
+        beta = v3_mul(beta, v3_divs(v3_scale(bs.f, LdotN), bs.pdf));
+        double rrw = 1.0;
+        if (bounce > 2) {
+            const float q = m_clamp(1.0f - v3_max_comp(beta), 0.05f, 1.0f);
+            const float u = rng_u(rng, bounce, SL_RR);
+            if (u > q) { beta = v3_scale(beta, 1.0f / (1.0f - q)); rrw = (double)(1.0f / (1.0f - q)); }
+            else break;
+        }
+        if (v3_is_zero(beta)) break;
+        if (bounce + 1 >= s->desc.max_depth) break;
+        or_si sc = si;
+        sc.wo = bs.L;
+        V->cont = 1;
+        V->c_f = d3(bs.f);
+        V->c_df = bsdf_dfdtheta(m, &sc, bs.f);
+        V->c_k = (double)LdotN / (double)bs.pdf * rrw;
+        ro = si.P;
+        rd = v3_normalize(bs.L);
+    }
+    return n;
+}
+
+static void adj_path(adj_job* J, double* acc, uint32_t pixel, int k, float fx, float fy, v3d A) {
+    const or_scene* s = J->s;
+    adj_vertex vs[OR_ADJ_MAXV];
+    or_rng rng;
+    rng_init(&rng, OR_RNG_PHILOX, J->seed, 0, pixel, (uint32_t)k);
+    const int n = adj_record(s, s->cam_pos, camera_dir(s, fx, fy), &rng, vs);
+    /* suffix sums S_j (per channel) */
+    double S[OR_ADJ_MAXV + 1][3];
+    S[n][0] = S[n][1] = S[n][2] = 0.0;
+    for (int j = n - 1; j >= 0; --j) {
+        const adj_vertex* V = &vs[j];
+        const double Lec[3] = {V->Le.x, V->Le.y, V->Le.z};
+        const double ltf[3] = {V->lt_f.x, V->lt_f.y, V->lt_f.z}, mtf[3] = {V->mt_f.x, V->mt_f.y, V->mt_f.z};
+        const double cf[3] = {V->c_f.x, V->c_f.y, V->c_f.z};
+        for (int c = 0; c < 3; ++c) {
+            double e = 0.0;
+            if (V->emit_li >= 0) {
+                const v3 le = s->lights[V->emit_li].radiance;
+                e = c == 0 ? le.x : (c == 1 ? le.y : le.z);
+            }
+            const double nee = (V->lt_on * Lec[c] * ltf[c] * V->lt_k + V->mt_on * Lec[c] * mtf[c] * V->mt_k) / V->pick;
+            S[j][c] = e + nee + (V->cont ? cf[c] * V->c_k * S[j + 1][c] : 0.0);
+        }
+    }
+    /* forward over the vertices with beta_j */
+    double beta[3] = {1.0, 1.0, 1.0};
+    const double Ad[3] = {A.x, A.y, A.z};
+    for (int j = 0; j < n; ++j) {
+        const adj_vertex* V = &vs[j];
+        const double Lec[3] = {V->Le.x, V->Le.y, V->Le.z};
+        const double ltf[3] = {V->lt_f.x, V->lt_f.y, V->lt_f.z}, mtf[3] = {V->mt_f.x, V->mt_f.y, V->mt_f.z};
+        const double ltd[3] = {V->lt_df.x, V->lt_df.y, V->lt_df.z}, mtd[3] = {V->mt_df.x, V->mt_df.y, V->mt_df.z};
+        const double cd[3] = {V->c_df.x, V->c_df.y, V->c_df.z}, cf[3] = {V->c_f.x, V->c_f.y, V->c_f.z};
+        const or_material* m = &s->mats[V->mat];
+        const int64_t off = m->type == OR_MAT_LAMBERT ? J->alb_off[V->mat] : J->rough_off[V->mat];
+        for (int c = 0; c < 3; ++c) {
+            /* texel of this vertex */
+            const double dth = (V->lt_on * Lec[c] * ltd[c] * V->lt_k + V->mt_on * Lec[c] * mtd[c] * V->mt_k) / V->pick +
+                               (V->cont ? cd[c] * V->c_k * S[j + 1][c] : 0.0);
+            if (off >= 0 && V->texel >= 0) {
+                if (m->type == OR_MAT_LAMBERT) acc[off + 3 * V->texel + c] += Ad[c] * beta[c] * dth;
+                else acc[off + V->texel] += Ad[c] * beta[c] * dth;
+            }
+            /* emitted radiance */
+            if (V->emit_li >= 0) acc[J->light_off + 3 * V->emit_li + c] += Ad[c] * beta[c];
+            const double dle = (V->lt_on * ltf[c] * V->lt_k + V->mt_on * mtf[c] * V->mt_k) / V->pick;
+            acc[J->light_off + 3 * V->li + c] += Ad[c] * beta[c] * dle;
+        }
+        if (V->cont)
+            for (int c = 0; c < 3; ++c) beta[c] *= cf[c] * V->c_k;
+    }
+}
+
+/* The sample's adjoint weight A: its filter footprint (FilmTile::addSample's, clamped to
+ * the tile; NaN / inf samples never reach the film and get none) against dLoss/dI / W. */
+static v3d adj_weight(const adj_job* J, int tx0, int ty0, int tx1, int ty1, float fx, float fy) {
+    const or_scene_desc* d = &J->s->desc;
+    const float hx = d->filter_w / 2.0f, hy = d->filter_h / 2.0f;
+    float mnx = std_max(fx - hx, (float)tx0), mny = std_max(fy - hy, (float)ty0);
+    float mxx = std_min(fx + hx, (float)tx1), mxy = std_min(fy + hy, (float)ty1);
+    const int x0 = (int)ceilf(mnx - 0.5f), y0 = (int)ceilf(mny - 0.5f);
+    const int x1 = (int)(floorf(mxx - 0.5f) + 1), y1 = (int)(floorf(mxy - 0.5f) + 1);
+    v3d A = d3_make(0, 0, 0);
+    for (int iy = y0; iy < y1; ++iy)
+        for (int ix = x0; ix < x1; ++ix) {
+            const size_t q = (size_t)iy * (size_t)d->width + (size_t)ix;
+            const double W = J->film[4 * q + 3];
+            if (W == 0.0) continue;
+            const double w = filter_eval(d, (float)ix - (fx - 0.5f), (float)iy - (fy - 0.5f)) / W;
+            A.x += J->dimg[3 * q] * w;
+            A.y += J->dimg[3 * q + 1] * w;
+            A.z += J->dimg[3 * q + 2] * w;
+        }
+    return A;
+}
+
+static void* adj_worker(void* arg) {
+    adj_worker_arg* W = (adj_worker_arg*)arg;
+    adj_job* J = W->J;
+    const or_scene_desc* d = &J->s->desc;
+    const int ntx = (d->width + d->tile_x - 1) / d->tile_x;
+    while (1) {
+        pthread_mutex_lock(&J->lock);
+        const int64_t t = J->next++;
+        pthread_mutex_unlock(&J->lock);
+        if (t >= J->ntiles) break;
+        const int x0 = (int)(t % ntx) * d->tile_x, y0 = (int)(t / ntx) * d->tile_y;
+        const int x1 = x0 + d->tile_x < d->width ? x0 + d->tile_x : d->width;
+        const int y1 = y0 + d->tile_y < d->height ? y0 + d->tile_y : d->height;
+        for (int iy = y0; iy < y1; ++iy)
+            for (int ix = x0; ix < x1; ++ix)
+                for (int k = J->spp_begin; k < J->spp_end; ++k) {
+                    float fx, fy;
+                    philox_film_pos(J->s, J->seed, ix, iy, k, &fx, &fy);
+                    const v3d A = adj_weight(J, x0, y0, x1, y1, fx, fy);
+                    if (A.x == 0.0 && A.y == 0.0 && A.z == 0.0) continue;
+                    adj_path(J, W->acc, (uint32_t)((size_t)iy * (size_t)d->width + (size_t)ix), k, fx, fy, A);
+                }
+    }
+    return NULL;
+}
+
+void or_adjoint(const or_scene* s, uint64_t seed, int spp_begin, int spp_end, int nthreads, const float* film,
+                const float* d_image, double* const* d_albedo, double* const* d_rough, double* d_light) {
+    adj_job J;
+    memset(&J, 0, sizeof(J));
+    J.s = s; J.seed = seed; J.spp_begin = spp_begin; J.spp_end = spp_end; J.film = film; J.dimg = d_image;
+    const or_scene_desc* d = &s->desc;
+    J.ntiles = (int64_t)((d->width + d->tile_x - 1) / d->tile_x) * ((d->height + d->tile_y - 1) / d->tile_y);
+    int64_t* alb = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->nmats + 1));
+    int64_t* rough = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->nmats + 1));
+    int64_t np = 0;
+    for (int m = 0; m < s->nmats; ++m) {
+        const or_texture* a = &s->mats[m].albedo;
+        const or_texture* r = &s->mats[m].roughness;
+        alb[m] = a->data ? np : -1;
+        if (a->data) np += (int64_t)a->width * a->height * 3;
+        rough[m] = r->data ? np : -1;
+        if (r->data) np += (int64_t)r->width * r->height;
+    }
+    J.alb_off = alb; J.rough_off = rough; J.light_off = np;
+    np += 3 * (int64_t)s->nlights;
+    J.nparams = np;
+    pthread_mutex_init(&J.lock, NULL);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    adj_worker_arg* args = (adj_worker_arg*)malloc(sizeof(adj_worker_arg) * (size_t)nthreads);
+    for (int i = 0; i < nthreads; ++i) {
+        args[i].J = &J;
+        args[i].acc = (double*)calloc((size_t)np, sizeof(double));
+        pthread_create(&th[i], NULL, adj_worker, &args[i]);
+    }
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    for (int m = 0; m < s->nmats; ++m) {
+        const or_texture* a = &s->mats[m].albedo;
+        const or_texture* r = &s->mats[m].roughness;
+        if (d_albedo && d_albedo[m] && alb[m] >= 0)
+            for (int64_t e = 0; e < (int64_t)a->width * a->height * 3; ++e) {
+                double v = 0.0;
+                for (int i = 0; i < nthreads; ++i) v += args[i].acc[alb[m] + e];
+                d_albedo[m][e] = v;
+            }
+        if (d_rough && d_rough[m] && rough[m] >= 0)
+            for (int64_t e = 0; e < (int64_t)r->width * r->height; ++e) {
+                double v = 0.0;
+                for (int i = 0; i < nthreads; ++i) v += args[i].acc[rough[m] + e];
+                d_rough[m][e] = v;
+            }
+    }
+    if (d_light)
+        for (int64_t e = 0; e < 3 * (int64_t)s->nlights; ++e) {
+            double v = 0.0;
+            for (int i = 0; i < nthreads; ++i) v += args[i].acc[J.light_off + e];
+            d_light[e] = v;
+        }
+    for (int i = 0; i < nthreads; ++i) free(args[i].acc);
+    free(args); free(th); free(alb); free(rough);
+    pthread_mutex_destroy(&J.lock);
+}
+
+void or_set_light_radiance(or_scene* s, int li, const float* rgb) {
+    if (li >= 0 && li < s->nlights) s->lights[li].radiance = v3_make(rgb[0], rgb[1], rgb[2]);
+}
+
+/* df/dtheta of material mi at the or_bsdf_query inputs (Ns, wi, wo, uvw): Lambertian
+ * df_c/da_c, GGX df_c/dr, double (the derivative the path-replay gradient uses). */
+void or_bsdf_dfdtheta_query(const or_scene* s, int mi, const float* in, int64_t n, double* out) {
+    const or_material* m = &s->mats[mi];
+    for (int64_t i = 0; i < n; ++i) {
+        const float* q = in + 14 * i;
+        or_si si;
+        memset(&si, 0, sizeof(si));
+        si.Ns = si.Ng = v3_load(q);
+        si.wi = v3_load(q + 3);
+        si.wo = v3_load(q + 6);
+        si.uvw = v3_load(q + 9);
+        const v3d g = bsdf_dfdtheta(m, &si, bsdf_eval(m, &si));
+        out[3 * i] = g.x; out[3 * i + 1] = g.y; out[3 * i + 2] = g.z;
+    }
+}

@@ -136,6 +136,21 @@ void or_render_linear(const or_scene* s, int rng_mode, uint64_t seed, int spp_be
 double or_time_render(const or_scene* s, int64_t begin, int64_t end, int spp, int nthreads, uint64_t seed,
                       int64_t* out_rays, int64_t* out_samples);
 
+/* Gradient of an image loss by path replay, FP64 accumulation (host check of the GPU
+ * adjoint; not a reference function). Paths of the Philox layout, passes
+ * [spp_begin, spp_end); film = or_render_linear's film of the same passes (its sum of
+ * weights normalizes dLoss/dI); d_image = dLoss/dI (w x h x 3). Outputs per material
+ * (arrays of nmats pointers, entries NULL to skip): d_albedo[m] (albedo texels x 3,
+ * image textures only), d_rough[m] (roughness texels); d_light = dLoss/dLe (nlights x 3).
+ * Sampling pdfs, MIS weights and Russian roulette are held fixed. */
+void or_adjoint(const or_scene* s, uint64_t seed, int spp_begin, int spp_end, int nthreads, const float* film,
+                const float* d_image, double* const* d_albedo, double* const* d_rough, double* d_light);
+/* df/dtheta (Lambertian: d f_c / d albedo_c; GGX: d f_c / d roughness) at the
+ * or_bsdf_query inputs, n x 3 doubles. */
+void or_bsdf_dfdtheta_query(const or_scene* s, int m, const float* in, int64_t n, double* out);
+/* Set the emitted radiance of light li (3 floats), as the GPU path's parameter update. */
+void or_set_light_radiance(or_scene* s, int li, const float* rgb);
+
 /* Philox4x32-10 block: ctr[4], key[2] -> out[4]. */
 void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 

@@ -1,0 +1,105 @@
+"""Pins of the oracle's FP64 path-replay gradient (or_adjoint), the host check
+of the GPU adjoint. The reference has no derivatives, so the pins are:
+
+* the BSDF parameter derivative against central differences of the oracle's
+  own BSDF evaluation (conductorMicrofacet.cpp:32-67, lambertianDiffuse.cpp:27-44);
+* the whole-path gradient against central differences of the oracle's renders
+  with common random numbers, for Lambertian albedo texels and emitted radiance:
+  neither changes any sampled direction, so at max-depth 3 (no Russian
+  roulette) each sample's radiance is a polynomial in them and the difference
+  quotient equals the derivative up to O(eps^2) and float rounding.
+"""
+import numpy as np
+import pytest
+
+from cadise_amd import scenes
+from oracle.binding import OR_RNG_PHILOX, OracleScene
+
+
+def _ggx_scene(rough):
+    s = scenes.cornell_box(width=4, height=4, spp=1)
+    s.materials.append(scenes.Material("m0", type="conductor-microfacet", f0=(0.9, 0.6, 0.3), roughness=rough))
+    return s
+
+
+@pytest.mark.parametrize("rough", [0.15, 0.4, 0.8])
+def test_ggx_roughness_derivative_matches_fd(rough):
+    rng = np.random.default_rng(int(rough * 100))
+    n = 400
+    N = np.tile([0.0, 1.0, 0.0], (n, 1))
+    V = rng.normal(size=(n, 3)); V[:, 1] = np.abs(V[:, 1]) + 0.3
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    # half vectors within a few alpha of the normal, so that f is well away from zero
+    a = max(rough * rough, 1e-3)
+    H = N + 2.0 * a * rng.normal(size=(n, 3)) * [1.0, 0.0, 1.0]
+    H /= np.linalg.norm(H, axis=1, keepdims=True)
+    L = 2.0 * np.sum(V * H, 1, keepdims=True) * H - V
+    L[:, 1] = np.maximum(L[:, 1], 0.05)
+    L /= np.linalg.norm(L, axis=1, keepdims=True)
+    q = np.zeros((n, 14), np.float32)
+    q[:, 0:3], q[:, 3:6], q[:, 6:9] = N, V, L
+    q[:, 12:14] = 0.5
+    mi = len(_ggx_scene(rough).materials) - 1
+
+    def central(eps):
+        fp = OracleScene(_ggx_scene(rough + eps)).bsdf_query(mi, q)[:, 0:3].astype(np.float64)
+        fm = OracleScene(_ggx_scene(rough - eps)).bsdf_query(mi, q)[:, 0:3].astype(np.float64)
+        return (fp - fm) / (2 * eps)
+
+    fd = (4.0 * central(1e-3) - central(2e-3)) / 3.0     # Richardson: O(eps^4)
+    an = OracleScene(_ggx_scene(rough)).bsdf_dfdtheta(mi, q)
+    big = np.abs(an) > 1e-3 * np.abs(an).max()
+    assert big.sum() > 600
+    # near the lobe peak the float evaluation of tan^2 = (1 - c^2) / c^2 (the
+    # reference's operation order) carries ~1e-3 relative rounding that the
+    # difference quotient inherits and the double-precision derivative does not
+    rel = np.abs(an[big] - fd[big]) / np.abs(fd[big])
+    assert np.quantile(rel, 0.99) < 3e-3 and rel.max() < 1e-2, (np.quantile(rel, 0.99), rel.max())
+
+
+def _small(max_depth=3):
+    return scenes.textured_spheres(n_spheres=4, level=2, tex_res=8, width=24, height=24, spp=4, max_depth=max_depth)
+
+
+def _loss_and_film(O, spec, G, seed):
+    film = O.render_linear(seed, OR_RNG_PHILOX, threads=8).astype(np.float64)
+    w = film[..., 3:4]
+    img = np.where(w > 0, film[..., :3] / np.where(w > 0, w, 1), 0.0)
+    return float((G * img).sum()), film
+
+
+def test_oracle_gradient_matches_fd_albedo_and_emitter():
+    spec = _small()
+    O = OracleScene(spec)
+    rng = np.random.default_rng(5)
+    G = rng.normal(size=(spec.height, spec.width, 3))
+    seed = 77
+    _, film = _loss_and_film(O, spec, G, seed)
+    grads, glight = O.adjoint(seed, film.astype(np.float32), G.astype(np.float32), threads=8)
+    ga = grads["albedo"]
+    base = O.tex_arrays["albedo"].copy()
+    # the texels with the largest gradients, plus random ones
+    flat = np.argsort(-np.abs(ga).ravel())[:10].tolist() + rng.integers(0, ga.size, 6).tolist()
+    eps = 1e-2
+    for e in flat:
+        idx = np.unravel_index(e, ga.shape)
+        vals = []
+        for sgn in (1, -1):
+            t = base.copy()
+            t[idx] += sgn * eps
+            O.set_texture("albedo", t)
+            vals.append(_loss_and_film(O, spec, G, seed)[0])
+        O.set_texture("albedo", base)
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        assert abs(fd - ga[idx]) <= 2e-3 * abs(ga[idx]) + 1e-6 * np.abs(ga).max(), (idx, fd, ga[idx])
+    Le = O.light_radiance()[0].copy()
+    for c in range(3):
+        vals = []
+        for sgn in (1, -1):
+            x = Le.copy()
+            x[c] += sgn * eps
+            O.set_light_radiance(0, x)
+            vals.append(_loss_and_film(O, spec, G, seed)[0])
+        O.set_light_radiance(0, Le)
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        assert abs(fd - glight[0, c]) <= 1e-3 * abs(glight[0, c]), (c, fd, glight[0, c])

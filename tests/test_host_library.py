@@ -135,3 +135,22 @@ def test_reference_tree_matches_oracle_on_stress_instances(ctx):
     assert np.array_equal(bits(b["bounds"]), bits(o["bounds"]))
     for k in ("leaf", "a", "b", "ordered"):
         assert np.array_equal(b[k], o[k]), k
+
+
+def test_texture_layout_and_sharing(ctx):
+    """Materials that share a texture share one parameter block; offsets tile the
+    flat texture array in creation order."""
+    spec = scenes.textured_spheres(n_spheres=4, level=1, tex_res=8, width=8, height=8, spp=1)
+    sc = Scene.from_spec(ctx, spec)
+    st = sc.stats()
+    assert st["ntextures"] == 2 and set(sc.tex_index) == {"albedo", "rough"}
+    total = 0
+    for i in range(st["ntextures"]):
+        w, h, c, o = sc.texture_info(i)
+        assert o == total and (w, h) == (8, 8)
+        total += w * h * c
+    assert total == st["tex_elements"] == 8 * 8 * 4
+    sc.set_texture(sc.tex_index["albedo"], np.full((8, 8, 3), 0.5, np.float32))
+    with pytest.raises(MptError) as e:
+        sc.adjoint(np.zeros((8, 8, 3), np.float32))
+    assert e.value.code == -4
