@@ -16,8 +16,8 @@ import ctypes as C
 import numpy as np
 
 from . import native
-from .native import (MPT_RENDER_KEEP_SAMPLES, BvhOpts, LightDesc, MaterialDesc, MptError, RenderParams, SceneDesc,
-                     SceneStats, Stats, Texture)
+from .native import (MPT_RENDER_KEEP_SAMPLES, BatchOpts, BatchTrial, BvhOpts, LightDesc, MaterialDesc, MptError,
+                     RenderParams, SceneDesc, SceneStats, Stats, Texture)
 
 _FILTERS = {"box": 0, "gaussian": 1, "cone": 2, "mitchell": 3}
 _FILTER_W = {"box": 1.0, "gaussian": 4.0, "cone": 4.0, "mitchell": 4.0}
@@ -261,6 +261,71 @@ class Scene:
 
     def save(self, path: str):
         self.ctx.check(self.L.mpt_film_save(self.h, path.encode()))
+
+
+def batch_opts(width: int = 4, stack_lds: int = 24, block: int = 256, capacity_log2: int = 20,
+               graph_steps: int = 16) -> BatchOpts:
+    return BatchOpts(width, stack_lds, block, capacity_log2, graph_steps)
+
+
+class Batch:
+    """Many scenes rendered by one streaming wavefront (mpt_batch_*). The
+    scenes must outlive the batch (it keeps references)."""
+
+    def __init__(self, ctx: Context, scenes, opts: BatchOpts | None = None):
+        self.ctx, self.L = ctx, ctx.L
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.h for s in self.scenes])
+        h = C.c_void_p()
+        ctx.check(self.L.mpt_batch_create(ctx.h, arr, len(self.scenes), C.byref(opts) if opts else None, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mpt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _seeds(self, seeds):
+        if seeds is None:
+            return None
+        a = np.ascontiguousarray(seeds, np.uint64)
+        assert len(a) == len(self.scenes)
+        return a
+
+    def render(self, seeds=None, spp_override: int = 0) -> dict:
+        sd = self._seeds(seeds)
+        st = Stats()
+        self.ctx.check(self.L.mpt_batch_render(self.h, sd.ctypes.data if sd is not None else None, spp_override,
+                                               C.byref(st)))
+        return st.as_dict()
+
+    def film(self, i: int) -> np.ndarray:
+        sc = self.scenes[i]
+        out = np.zeros((sc.height, sc.width, 4), np.float32)
+        self.ctx.check(self.L.mpt_batch_film(self.h, i, out.ctypes.data))
+        return out
+
+    @property
+    def opts(self) -> dict:
+        o = BatchOpts()
+        self.ctx.check(self.L.mpt_batch_get_opts(self.h, C.byref(o)))
+        return o.as_dict()
+
+    def set_opts(self, **kw):
+        o = BatchOpts(**{**self.opts, **kw})
+        self.ctx.check(self.L.mpt_batch_set_opts(self.h, C.byref(o)))
+
+    def autotune(self, seeds=None, trial_spp: int = 1, max_trials: int = 64):
+        sd = self._seeds(seeds)
+        best = BatchOpts()
+        trials = (BatchTrial * max_trials)()
+        nt = C.c_int32()
+        self.ctx.check(self.L.mpt_batch_autotune(self.h, sd.ctypes.data if sd is not None else None, trial_spp,
+                                                 C.byref(best), trials, max_trials, C.byref(nt)))
+        return best.as_dict(), [(trials[i].opts.as_dict(), trials[i].ms) for i in range(min(nt.value, max_trials))]
 
 
 def film_mean(film: np.ndarray) -> np.ndarray:

@@ -1,0 +1,626 @@
+// Batched renderer: many scenes in one streaming wavefront on one GPU.
+//
+// The scenes' device tables are merged into one DevScene (node child indices,
+// leaf positions, primitive ids, material / texture / light indices offset),
+// so the traversal kernel is the single-scene one with a per-ray root node.
+// Path slots are recycled: each step regenerates camera paths into the free
+// slots from one global work counter over (scene, sample, pixel) (k_bregen /
+// k_bcommit), traces the queue's continuation + MIS-BSDF rays and shadow rays,
+// and shades (k_shade<SHADE_BATCH>), where finished paths splat into their
+// scene's film and free their slot. Steps are recorded once into a HIP graph
+// and the graph is relaunched until the device's done flag is raised, so the
+// GPU stays full across scene boundaries and path-length tails instead of
+// draining at the end of every sample pass of every scene.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "driver.hpp"
+
+namespace mpt {
+
+// ---- merging kernels ------------------------------------------------------------------------
+
+__global__ void k_fix_tris(float4* tris, int64_t n, int32_t prim_base) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t bits = __float_as_uint(tris[3 * j].w);
+    const uint32_t id = (bits & 0x7FFFFFFFu) + static_cast<uint32_t>(prim_base);
+    tris[3 * j].w = __uint_as_float(id | (bits & 0x80000000u));
+}
+
+__global__ void k_fix_info(int4* info, int64_t n, int32_t mat_base, int32_t light_base) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int4 v = info[j];
+    v.x += mat_base;
+    v.y = v.y >= 0 ? v.y + light_base : -1;
+    info[j] = v;
+}
+
+__global__ void k_batch_init(int32_t* free_ids, int32_t capacity, int32_t* ctr, int32_t* qcount,
+                             unsigned long long* work_next, unsigned long long* stats) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < capacity) free_ids[i] = static_cast<int32_t>(i);
+    if (i == 0) {
+        ctr[0] = capacity;
+        ctr[1] = 0;
+        ctr[2] = ctr[3] = 0;
+        qcount[0] = qcount[1] = 0;
+        *work_next = 0ull;
+    }
+    if (i < ST_COUNT) stats[i] = 0ull;
+}
+
+}  // namespace mpt
+
+struct mpt_batch {
+    mpt_ctx* ctx = nullptr;
+    std::vector<mpt_scene*> scenes;
+    hipStream_t stream = nullptr;
+    mpt_batch_opts opts{};
+    // merged scene; nodes per BVH width (index 0: 4, 1: 8), built on demand
+    mpt::DevBuf nodes[2];
+    int64_t nnodes[2] = {0, 0};
+    int32_t max_stack[2] = {0, 0};
+    std::vector<int32_t> roots[2];
+    mpt::DevBuf tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, views, film;
+    mpt::DevScene sc{};
+    std::vector<mpt::BatchView> hviews;
+    int64_t nprims = 0, total_px = 0;
+    // streaming state
+    mpt::DevBuf beta, rad, fpos, meta, free_ids, ctr, work_next, qcount, stats, spill;
+    mpt::DevBuf qbuf[2][12], rootq[2];
+    int32_t capacity_alloc = 0;
+    int64_t spill_lanes = 0;
+    // recorded graph and the shape it was recorded for
+    hipGraphExec_t exec = nullptr;
+    mpt_batch_opts exec_opts{};
+    int64_t exec_total = -1;
+    int32_t* done_host = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    mpt_stats rstats{};
+    ~mpt_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (done_host) (void)hipHostFree(done_host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+using namespace mpt;
+
+void normalize_opts(mpt_batch_opts& o) {
+    if (o.width != 8) o.width = 4;
+    if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
+    if (o.block != 128) o.block = 256;
+    if (o.capacity_log2 < 16 || o.capacity_log2 > 22) o.capacity_log2 = 20;
+    if (o.graph_steps < 2 || o.graph_steps > 64) o.graph_steps = 16;
+    o.graph_steps &= ~1;
+}
+
+// Merged wide-BVH nodes for one width: each scene's nodes (collapsed again
+// from its binary tree when the scene was built with the other width) with
+// interior children offset by the scene's first node and leaves by its first
+// primitive record.
+int merge_nodes(mpt_batch* B, int wi) {
+    if (B->nnodes[wi] > 0) return MPT_OK;
+    const int width = wi == 0 ? 4 : 8;
+    const size_t nb = static_cast<size_t>(width == 4 ? sizeof(WideNode<4>) : sizeof(WideNode<8>));
+    const size_t ns = B->scenes.size();
+    std::vector<WideBvh> other(ns);
+    std::vector<std::thread> th;
+    for (size_t i = 0; i < ns; ++i) {
+        mpt_scene* S = B->scenes[i];
+        if (S->wide.width != width)
+            th.emplace_back([S, &other, i, width]() { collapse_wide(S->host, S->bin, width, S->opts.leaf_max, &other[i]); });
+    }
+    for (auto& t : th) t.join();
+    int64_t total = 0;
+    std::vector<int64_t> nbase(ns), tbase(ns);
+    int64_t tb = 0;
+    int32_t ms = 0;
+    for (size_t i = 0; i < ns; ++i) {
+        const WideBvh& w = B->scenes[i]->wide.width == width ? B->scenes[i]->wide : other[i];
+        nbase[i] = total;
+        tbase[i] = tb;
+        total += w.nnodes;
+        tb += B->scenes[i]->host.nprims();
+        ms = std::max(ms, w.max_stack);
+    }
+    if (total >= (int64_t{1} << 31) || tb >= (int64_t{1} << 31))
+        return fail(B->ctx, MPT_ERR_UNSUPPORTED, "batch larger than 2^31 nodes or primitives");
+    std::vector<uint8_t> merged(static_cast<size_t>(total) * nb);
+    for (size_t i = 0; i < ns; ++i) {
+        const WideBvh& w = B->scenes[i]->wide.width == width ? B->scenes[i]->wide : other[i];
+        uint8_t* dst = merged.data() + static_cast<size_t>(nbase[i]) * nb;
+        std::memcpy(dst, w.nodes.data(), static_cast<size_t>(w.nnodes) * nb);
+        for (int64_t k = 0; k < w.nnodes; ++k) {
+            int32_t* child = reinterpret_cast<int32_t*>(dst + static_cast<size_t>(k) * nb + 24 * width);
+            int32_t* count = child + width;
+            for (int c = 0; c < width; ++c) {
+                if (count[c] > 0) child[c] += static_cast<int32_t>(tbase[i]);
+                else if (child[c] >= 0) child[c] += static_cast<int32_t>(nbase[i]);
+            }
+        }
+    }
+    HIP_TRY(B->ctx, B->nodes[wi].alloc(merged.size()));
+    HIP_TRY(B->ctx, hipMemcpy(B->nodes[wi].p, merged.data(), merged.size(), hipMemcpyHostToDevice));
+    B->nnodes[wi] = total;
+    B->max_stack[wi] = ms;
+    B->roots[wi].assign(nbase.begin(), nbase.end());
+    return MPT_OK;
+}
+
+int merge_tables(mpt_batch* B) {
+    mpt_ctx* ctx = B->ctx;
+    const size_t ns = B->scenes.size();
+    int64_t np = 0, nmat = 0, ntexel = 0, nlight = 0, ncdf = 0, npx = 0;
+    bool any_n = false, any_u = false;
+    std::vector<DevMaterial> mats;
+    std::vector<DevTexture> texs;
+    std::vector<DevLight> lights;
+    std::vector<float> cdf;
+    std::vector<int64_t> pbase(ns), tdbase(ns);
+    for (size_t i = 0; i < ns; ++i) {
+        mpt_scene* S = B->scenes[i];
+        any_n |= !S->host.prim_n.empty();
+        any_u |= !S->host.prim_uvw.empty();
+    }
+    B->hviews.assign(ns, BatchView{});
+    int64_t work = 0;
+    for (size_t i = 0; i < ns; ++i) {
+        mpt_scene* S = B->scenes[i];
+        const HostScene& h = S->host;
+        pbase[i] = np;
+        tdbase[i] = ntexel;
+        const int32_t tex0 = static_cast<int32_t>(texs.size());
+        for (DevMaterial m : S->h_mats) {
+            if (m.albedo_tex >= 0) m.albedo_tex += tex0;
+            if (m.rough_tex >= 0) m.rough_tex += tex0;
+            mats.push_back(m);
+        }
+        for (DevTexture t : S->h_texs) {
+            t.offset += ntexel;
+            texs.push_back(t);
+        }
+        for (DevLight l : S->h_lights) {
+            l.prim += static_cast<int32_t>(np);
+            lights.push_back(l);
+        }
+        BatchView& v = B->hviews[i];
+        const DevScene& d = S->sc;
+        v.light_base = static_cast<int32_t>(nlight);
+        v.nlights = static_cast<int32_t>(h.lights.size());
+        v.cluster = h.cluster;
+        v.cdf_base = static_cast<int32_t>(ncdf);
+        cdf.insert(cdf.end(), S->h_cdf.begin(), S->h_cdf.end());
+        v.img_w = d.img_w;
+        v.img_h = d.img_h;
+        v.tile_x = d.tile_x;
+        v.tile_y = d.tile_y;
+        v.filter = d.filter;
+        v.filter_hw = d.filter_hw;
+        v.filter_hh = d.filter_hh;
+        v.g_expx = d.g_expx;
+        v.g_expy = d.g_expy;
+        v.g_edge = d.g_edge;
+        v.amplitude = d.amplitude;
+        v.mitchell_b = d.mitchell_b;
+        v.mitchell_c = d.mitchell_c;
+        v.sampler = d.sampler;
+        v.spp = d.spp;
+        v.strata = d.strata;
+        v.max_depth = h.max_depth;
+        v.film_base = npx;
+        v.work_base = work;
+        std::memcpy(v.f2c, d.f2c, sizeof(v.f2c));
+        std::memcpy(v.c2w, d.c2w, sizeof(v.c2w));
+        std::memcpy(v.cam_pos, d.cam_pos, sizeof(v.cam_pos));
+        np += h.nprims();
+        nmat += static_cast<int64_t>(S->h_mats.size());
+        ntexel += S->tex_elems;
+        nlight += static_cast<int64_t>(h.lights.size());
+        ncdf += static_cast<int64_t>(S->h_cdf.size());
+        npx += static_cast<int64_t>(h.width) * h.height;
+        work += static_cast<int64_t>(h.width) * h.height * h.spp;
+        if (h.max_depth > 4096) return fail(ctx, MPT_ERR_INVALID, "max-depth above 4096 is not supported on the device");
+    }
+    if (np >= (int64_t{1} << 31) || nmat >= (int64_t{1} << 31) || ntexel >= (int64_t{1} << 30))
+        return fail(ctx, MPT_ERR_UNSUPPORTED, "batch tables too large");
+    B->nprims = np;
+    B->total_px = npx;
+    HIP_TRY(ctx, B->tris.alloc(sizeof(float4) * 3 * static_cast<size_t>(np)));
+    HIP_TRY(ctx, B->prim_info.alloc(sizeof(int4) * static_cast<size_t>(np)));
+    HIP_TRY(ctx, B->prim_n.alloc(any_n ? sizeof(float) * 9 * static_cast<size_t>(np) : 16));
+    HIP_TRY(ctx, B->prim_uvw.alloc(any_u ? sizeof(float) * 9 * static_cast<size_t>(np) : 16));
+    HIP_TRY(ctx, B->tex_data.alloc(sizeof(float) * static_cast<size_t>(std::max<int64_t>(1, ntexel))));
+    for (size_t i = 0; i < ns; ++i) {
+        mpt_scene* S = B->scenes[i];
+        const int64_t n = S->host.nprims();
+        if (n == 0) continue;
+        float4* t = B->tris.as<float4>() + 3 * pbase[i];
+        int4* inf = B->prim_info.as<int4>() + pbase[i];
+        HIP_TRY(ctx, hipMemcpyAsync(t, S->tris.p, sizeof(float4) * 3 * static_cast<size_t>(n), hipMemcpyDeviceToDevice, B->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(inf, S->prim_info.p, sizeof(int4) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, B->stream));
+        const int blocks = static_cast<int>((n + 255) / 256);
+        hipLaunchKernelGGL(k_fix_tris, dim3(blocks), dim3(256), 0, B->stream, t, n, static_cast<int32_t>(pbase[i]));
+        int32_t mat0 = 0, light0 = 0;
+        for (size_t k = 0; k < i; ++k) {
+            mat0 += static_cast<int32_t>(B->scenes[k]->h_mats.size());
+            light0 += static_cast<int32_t>(B->scenes[k]->host.lights.size());
+        }
+        hipLaunchKernelGGL(k_fix_info, dim3(blocks), dim3(256), 0, B->stream, inf, n, mat0, light0);
+        HIP_TRY(ctx, hipGetLastError());
+        if (!S->host.prim_n.empty())
+            HIP_TRY(ctx, hipMemcpyAsync(B->prim_n.as<float>() + 9 * pbase[i], S->prim_n.p, sizeof(float) * 9 * static_cast<size_t>(n),
+                                        hipMemcpyDeviceToDevice, B->stream));
+        if (!S->host.prim_uvw.empty())
+            HIP_TRY(ctx, hipMemcpyAsync(B->prim_uvw.as<float>() + 9 * pbase[i], S->prim_uvw.p,
+                                        sizeof(float) * 9 * static_cast<size_t>(n), hipMemcpyDeviceToDevice, B->stream));
+        if (S->tex_elems > 0)
+            HIP_TRY(ctx, hipMemcpyAsync(B->tex_data.as<float>() + tdbase[i], S->tex_data.p,
+                                        sizeof(float) * static_cast<size_t>(S->tex_elems), hipMemcpyDeviceToDevice, B->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(B->stream));
+    int rc;
+    if ((rc = upload_vec(ctx, B->mats, mats)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, B->texs, texs)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, B->lights, lights)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, B->power_cdf, cdf)) != MPT_OK) return rc;
+    HIP_TRY(ctx, B->film.alloc(sizeof(float4) * static_cast<size_t>(std::max<int64_t>(1, npx))));
+    DevScene& d = B->sc;
+    d = DevScene{};
+    d.nprims = np;
+    d.tris = B->tris.as<float4>();
+    d.prim_info = B->prim_info.as<int4>();
+    d.prim_n = B->prim_n.as<float>();
+    d.prim_uvw = B->prim_uvw.as<float>();
+    d.mats = B->mats.as<DevMaterial>();
+    d.texs = B->texs.as<DevTexture>();
+    d.tex_data = B->tex_data.as<float>();
+    d.lights = B->lights.as<DevLight>();
+    d.power_cdf = B->power_cdf.as<float>();
+    d.nlights = static_cast<int32_t>(nlight);
+    return MPT_OK;
+}
+
+// Streaming buffers for `cap` path slots.
+int ensure_stream_buffers(mpt_batch* B, int32_t cap) {
+    mpt_ctx* ctx = B->ctx;
+    if (cap == B->capacity_alloc) return MPT_OK;
+    const size_t n = static_cast<size_t>(cap);
+    HIP_TRY(ctx, B->beta.alloc(sizeof(float4) * n));
+    HIP_TRY(ctx, B->rad.alloc(sizeof(float4) * n));
+    HIP_TRY(ctx, B->fpos.alloc(sizeof(float2) * n));
+    HIP_TRY(ctx, B->meta.alloc(sizeof(int4) * n));
+    HIP_TRY(ctx, B->free_ids.alloc(sizeof(int32_t) * n));
+    const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
+    for (int q = 0; q < 2; ++q) {
+        for (int k = 0; k < 12; ++k) HIP_TRY(ctx, B->qbuf[q][k].alloc(sizes[k] * n));
+        HIP_TRY(ctx, B->rootq[q].alloc(sizeof(int32_t) * n));
+    }
+    if (!B->ctr.p) {
+        HIP_TRY(ctx, B->ctr.alloc(sizeof(int32_t) * 4));
+        HIP_TRY(ctx, B->work_next.alloc(sizeof(unsigned long long)));
+        HIP_TRY(ctx, B->qcount.alloc(sizeof(int32_t) * 2));
+        HIP_TRY(ctx, B->stats.alloc(sizeof(unsigned long long) * ST_COUNT));
+    }
+    B->capacity_alloc = cap;
+    return MPT_OK;
+}
+
+int trace_blocks_for(const mpt_batch* B, int64_t nrays, int block) {
+    const int64_t want = (nrays + block - 1) / block;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(B->ctx->num_cus) * 6)));
+}
+
+// One wavefront step on queue qc (enqueue only; the caller records it into a graph).
+void enqueue_step(mpt_batch* B, const TraceCfg& cfg, int tblocks, int sblocks, int rblocks, PassArgs& a, BatchArgs& bt,
+                  int qc) {
+    hipLaunchKernelGGL(k_bregen, dim3(rblocks), dim3(256), 0, B->stream, a, bt, qc);
+    hipLaunchKernelGGL(k_bcommit, dim3(1), dim3(64), 0, B->stream, a, bt, qc);
+    const Queue& Q = a.q[qc];
+    TraceArgs t{};
+    t.sc = a.sc;
+    t.org = Q.org;
+    t.dir = Q.dir;
+    t.dir2 = Q.mis_dir;
+    t.stride = 1;
+    t.count = a.qcount + qc;
+    t.dual = 1;
+    t.hit = Q.hit;
+    t.hit2 = Q.mis_hit;
+    t.want_prim = 0;
+    t.root = B->rootq[qc].as<int32_t>();
+    t.work = bt.ctr + 2;
+    t.spill = a.sc.max_stack > cfg.stack ? B->spill.as<int32_t>() : nullptr;
+    t.stats = bt.stats;
+    t.stat_slot = ST_CLOSEST_RAYS;
+    launch_trace_cfg(cfg, t, false, tblocks, B->stream);
+    t.dir = Q.shd_dir;
+    t.dual = 0;
+    t.hit = Q.occ;
+    t.hit2 = nullptr;
+    t.work = bt.ctr + 3;
+    t.stat_slot = ST_SHADOW_RAYS;
+    launch_trace_cfg(cfg, t, true, tblocks, B->stream);
+    hipLaunchKernelGGL(k_shade<SHADE_BATCH>, dim3(sblocks), dim3(256), 0, B->stream, a, AdjArgs{}, bt, qc);
+}
+
+int do_batch_render(mpt_batch* B, const uint64_t* seeds, int32_t spp_override, double* gpu_ms) {
+    mpt_ctx* ctx = B->ctx;
+    const auto t0 = std::chrono::steady_clock::now();
+    mpt_batch_opts& o = B->opts;
+    normalize_opts(o);
+    const int wi = o.width == 8 ? 1 : 0;
+    int rc;
+    if ((rc = merge_nodes(B, wi)) != MPT_OK) return rc;
+    // views for this call: roots of the chosen width, seeds, sample counts
+    int64_t work = 0;
+    int32_t maxd = 0;
+    for (size_t i = 0; i < B->scenes.size(); ++i) {
+        BatchView& v = B->hviews[i];
+        const HostScene& h = B->scenes[i]->host;
+        v.root = B->roots[wi][i];
+        const uint64_t sd = seeds ? seeds[i] : 0;
+        v.key_lo = static_cast<uint32_t>(sd);
+        v.key_hi = static_cast<uint32_t>(sd >> 32);
+        v.spp = spp_override > 0 ? spp_override : h.spp;
+        v.strata = strata_base(v.spp);
+        v.work_base = work;
+        work += static_cast<int64_t>(h.width) * h.height * v.spp;
+        maxd = std::max(maxd, h.max_depth);
+    }
+    // the views buffer keeps its address (the recorded graph holds it)
+    const size_t vbytes = sizeof(BatchView) * B->hviews.size();
+    if (B->views.bytes != vbytes) HIP_TRY(ctx, B->views.alloc(vbytes));
+    HIP_TRY(ctx, hipMemcpy(B->views.p, B->hviews.data(), vbytes, hipMemcpyHostToDevice));
+    DevScene& d = B->sc;
+    d.nodes = B->nodes[wi].p;
+    d.width = o.width;
+    d.nnodes = B->nnodes[wi];
+    d.max_stack = B->max_stack[wi];
+    TraceCfg cfg;
+    cfg.width = o.width;
+    cfg.stack = o.stack_lds;
+    cfg.block = o.block;
+    const int32_t cap = 1 << o.capacity_log2;
+    if ((rc = ensure_stream_buffers(B, cap)) != MPT_OK) return rc;
+    const int tblocks = trace_blocks_for(B, 2 * static_cast<int64_t>(cap), cfg.block);
+    const int sblocks = static_cast<int>(std::min<int64_t>((cap + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
+    const int rblocks = (cap + 255) / 256;
+    {
+        const int64_t extra = d.max_stack - cfg.stack;
+        const int64_t lanes = static_cast<int64_t>(tblocks) * cfg.block;
+        if (extra > 0 && (lanes > B->spill_lanes || B->spill.bytes < sizeof(int32_t) * static_cast<size_t>(lanes * extra))) {
+            HIP_TRY(ctx, B->spill.alloc(sizeof(int32_t) * static_cast<size_t>(lanes * extra)));
+            B->spill_lanes = lanes;
+            if (B->exec) {   // the recorded graph holds the old spill pointer
+                (void)hipGraphExecDestroy(B->exec);
+                B->exec = nullptr;
+            }
+        }
+    }
+    PassArgs a{};
+    a.sc = d;
+    a.beta = B->beta.as<float4>();
+    a.rad = B->rad.as<float4>();
+    a.fpos = B->fpos.as<float2>();
+    for (int q = 0; q < 2; ++q) {
+        Queue& Q = a.q[q];
+        Q.path = B->qbuf[q][0].as<int32_t>();
+        Q.org = B->qbuf[q][1].as<float4>();
+        Q.dir = B->qbuf[q][2].as<float4>();
+        Q.mis_dir = B->qbuf[q][3].as<float4>();
+        Q.shd_dir = B->qbuf[q][4].as<float4>();
+        Q.lt = B->qbuf[q][5].as<float4>();
+        Q.fac = B->qbuf[q][6].as<float4>();
+        Q.bv = B->qbuf[q][7].as<float4>();
+        Q.light = B->qbuf[q][8].as<int32_t>();
+        Q.hit = B->qbuf[q][9].as<int2>();
+        Q.mis_hit = B->qbuf[q][10].as<int2>();
+        Q.occ = B->qbuf[q][11].as<int2>();
+    }
+    a.qcount = B->qcount.as<int32_t>();
+    a.npaths = cap;
+    BatchArgs bt{};
+    bt.views = B->views.as<BatchView>();
+    bt.nviews = static_cast<int32_t>(B->scenes.size());
+    bt.capacity = cap;
+    bt.total_work = work;
+    bt.meta = B->meta.as<int4>();
+    bt.root[0] = B->rootq[0].as<int32_t>();
+    bt.root[1] = B->rootq[1].as<int32_t>();
+    bt.film = B->film.as<float4>();
+    bt.free_ids = B->free_ids.as<int32_t>();
+    bt.ctr = B->ctr.as<int32_t>();
+    bt.work_next = B->work_next.as<unsigned long long>();
+    bt.stats = B->stats.as<unsigned long long>();
+
+    // record the graph for this shape (pointers and total work are baked in)
+    const bool same = B->exec && B->exec_total == work && std::memcmp(&B->exec_opts, &o, sizeof(o)) == 0 &&
+                      B->views.p == reinterpret_cast<const void*>(bt.views);
+    if (!same) {
+        if (B->exec) (void)hipGraphExecDestroy(B->exec);
+        B->exec = nullptr;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(ctx, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < o.graph_steps; ++k) enqueue_step(B, cfg, tblocks, sblocks, rblocks, a, bt, k & 1);
+        const hipError_t ec = hipStreamEndCapture(B->stream, &graph);
+        if (ec != hipSuccess) return fail(ctx, MPT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(ctx, MPT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        B->exec_opts = o;
+        B->exec_total = work;
+    }
+    // reset: free stack, counters, film
+    hipLaunchKernelGGL(k_batch_init, dim3((cap + 255) / 256), dim3(256), 0, B->stream, bt.free_ids, cap, bt.ctr, a.qcount,
+                       bt.work_next, bt.stats);
+    hipLaunchKernelGGL(k_zero_f4, dim3((B->total_px + 255) / 256), dim3(256), 0, B->stream, B->film.as<float4>(),
+                       B->total_px);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(B->ev[0], B->stream));
+    // every path in flight finishes within max-depth + 2 steps, after which its
+    // slot is refilled: ceil(work / cap) + 1 such windows bound the step count
+    const int64_t bound = (work / cap + 2) * static_cast<int64_t>(maxd + 2) + 2;
+    const int64_t max_launches = 2 * (bound / o.graph_steps + 2);
+    int64_t launches = 0;
+    while (true) {
+        HIP_TRY(ctx, hipGraphLaunch(B->exec, B->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(B->done_host, bt.ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, B->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(B->stream));
+        ++launches;
+        if (*B->done_host) break;
+        if (launches > max_launches) return fail(ctx, MPT_ERR_DEVICE, "batched renderer did not drain");
+    }
+    HIP_TRY(ctx, hipEventRecord(B->ev[1], B->stream));
+    HIP_TRY(ctx, hipEventSynchronize(B->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
+    if (gpu_ms) *gpu_ms = ms;
+    unsigned long long st[ST_COUNT];
+    HIP_TRY(ctx, hipMemcpy(st, B->stats.p, sizeof(st), hipMemcpyDeviceToHost));
+    mpt_stats& r = B->rstats;
+    r = mpt_stats{};
+    r.samples = work;
+    r.camera_rays = 0;
+    for (size_t i = 0; i < B->scenes.size(); ++i)
+        if (B->scenes[i]->host.max_depth > 0)
+            r.camera_rays += static_cast<int64_t>(B->hviews[i].img_w) * B->hviews[i].img_h * B->hviews[i].spp;
+    r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
+    r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
+    r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
+    r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
+    r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
+    r.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
+    r.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
+    r.closest_launches = launches * o.graph_steps;
+    r.anyhit_launches = launches * o.graph_steps;
+    r.ms_total = ms_since(t0);
+    r.ms_gpu = ms;
+    if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    return MPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpt_batch_create(mpt_ctx* ctx, mpt_scene* const* scenes, int32_t n, const mpt_batch_opts* opts, mpt_batch** out) {
+    if (!ctx || !scenes || n <= 0 || !out) return fail(ctx, MPT_ERR_INVALID, "null argument");
+    if (ctx->device < 0) return fail(ctx, MPT_ERR_DEVICE, "host-only context has no device");
+    std::unique_ptr<mpt_batch> B(new mpt_batch());
+    B->ctx = ctx;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!scenes[i] || scenes[i]->ctx != ctx) return fail(ctx, MPT_ERR_INVALID, "scene of another context");
+        B->scenes.push_back(scenes[i]);
+    }
+    if (opts) B->opts = *opts;
+    normalize_opts(B->opts);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    HIP_TRY(ctx, hipEventCreate(&B->ev[0]));
+    HIP_TRY(ctx, hipEventCreate(&B->ev[1]));
+    HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&B->done_host), sizeof(int32_t), hipHostMallocDefault));
+    *B->done_host = 0;
+    int rc;
+    if ((rc = merge_tables(B.get())) != MPT_OK) return rc;
+    if ((rc = merge_nodes(B.get(), B->opts.width == 8 ? 1 : 0)) != MPT_OK) return rc;
+    *out = B.release();
+    return MPT_OK;
+}
+
+void mpt_batch_destroy(mpt_batch* b) { delete b; }
+
+int mpt_batch_render(mpt_batch* b, const uint64_t* seeds, int32_t spp_override, mpt_stats* stats) {
+    if (!b) return MPT_ERR_INVALID;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    const int rc = do_batch_render(b, seeds, spp_override, nullptr);
+    if (rc == MPT_OK && stats) *stats = b->rstats;
+    return rc;
+}
+
+int mpt_batch_film(mpt_batch* b, int32_t i, float* film_host) {
+    if (!b || !film_host || i < 0 || i >= static_cast<int32_t>(b->scenes.size())) return MPT_ERR_INVALID;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    const BatchView& v = b->hviews[i];
+    HIP_TRY(b->ctx, hipMemcpy(film_host, b->film.as<float4>() + v.film_base,
+                              sizeof(float4) * static_cast<size_t>(v.img_w) * v.img_h, hipMemcpyDeviceToHost));
+    return MPT_OK;
+}
+
+int mpt_batch_get_opts(const mpt_batch* b, mpt_batch_opts* out) {
+    if (!b || !out) return MPT_ERR_INVALID;
+    *out = b->opts;
+    return MPT_OK;
+}
+
+int mpt_batch_set_opts(mpt_batch* b, const mpt_batch_opts* o) {
+    if (!b || !o) return MPT_ERR_INVALID;
+    b->opts = *o;
+    normalize_opts(b->opts);
+    return MPT_OK;
+}
+
+int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, mpt_batch_opts* best,
+                       mpt_batch_trial* trials, int32_t max_trials, int32_t* ntrials) {
+    if (!b) return MPT_ERR_INVALID;
+    HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+    if (trial_spp <= 0) trial_spp = 1;
+    int32_t nt = 0;
+    mpt_batch_opts cur = b->opts;
+    normalize_opts(cur);
+    double best_ms = 1e30;
+    auto trial = [&](const mpt_batch_opts& o, double* ms) -> int {
+        b->opts = o;
+        double t = 0.0;
+        // the first run of a shape records its graph; time the second
+        int rc = do_batch_render(b, seeds, trial_spp, &t);
+        if (rc == MPT_OK) rc = do_batch_render(b, seeds, trial_spp, &t);
+        if (rc != MPT_OK) return rc;
+        *ms = t;
+        if (trials && nt < max_trials) trials[nt] = mpt_batch_trial{b->opts, t};
+        ++nt;
+        return MPT_OK;
+    };
+    int rc;
+    if ((rc = trial(cur, &best_ms)) != MPT_OK) return rc;
+    // coordinate descent: path slots, BVH width, LDS stack entries, block size
+    for (int pass = 0; pass < 2; ++pass) {
+        bool improved = false;
+        for (int dim = 0; dim < 4; ++dim) {
+            std::vector<mpt_batch_opts> cands;
+            if (dim == 0)
+                for (int c : {18, 19, 20, 21}) { mpt_batch_opts o = cur; o.capacity_log2 = c; cands.push_back(o); }
+            if (dim == 1)
+                for (int w : {4, 8}) { mpt_batch_opts o = cur; o.width = w; cands.push_back(o); }
+            if (dim == 2)
+                for (int s : {16, 24, 32}) { mpt_batch_opts o = cur; o.stack_lds = s; cands.push_back(o); }
+            if (dim == 3)
+                for (int k : {128, 256}) { mpt_batch_opts o = cur; o.block = k; cands.push_back(o); }
+            for (const mpt_batch_opts& o : cands) {
+                if (std::memcmp(&o, &cur, sizeof(o)) == 0) continue;
+                double ms = 0.0;
+                if ((rc = trial(o, &ms)) != MPT_OK) return rc;
+                if (ms < best_ms * 0.98) {
+                    best_ms = ms;
+                    cur = b->opts;
+                    improved = true;
+                }
+            }
+        }
+        if (!improved) break;
+    }
+    b->opts = cur;
+    if (best) *best = cur;
+    if (ntrials) *ntrials = nt;
+    return MPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// Host-driver internals shared by render.hip (one scene: render, trace,
+// adjoint, C ABI) and batch.hip (the batched renderer): device buffers,
+// traversal launch shapes, the context and scene objects.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/mpt.h"
+#include "../host/bvh.hpp"
+#include "../host/crsd.hpp"
+#include "../host/scene.hpp"
+#include "kernels.hpp"
+
+namespace mpt {
+
+template <int W, int S, bool ANY>
+__global__ void k_trace(TraceArgs a);
+template <bool ADJ>
+__global__ void k_camera(PassArgs a, AdjArgs g);
+template <int MODE>
+__global__ void k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step);
+__global__ void k_bregen(PassArgs a, BatchArgs bt, int32_t qc);
+__global__ void k_bcommit(PassArgs a, BatchArgs bt, int32_t qc);
+__global__ void k_film(PassArgs a);
+__global__ void k_srgb8(const float4* film, int64_t npix, uint8_t* out);
+__global__ void k_zero_u64(unsigned long long* p, int n);
+__global__ void k_zero_f4(float4* p, int64_t n);
+__global__ void k_zero_i32(int32_t* p, int n);
+
+// Traversal launch shape: BVH width, per-lane stack entries kept in LDS, threads per block.
+struct TraceCfg {
+    int width = 4, stack = 24, block = 256;
+};
+
+template <int W, int S>
+void launch_ws(bool any, int blocks, int block, size_t lds, hipStream_t st, const TraceArgs& a) {
+    if (any) hipLaunchKernelGGL((k_trace<W, S, true>), dim3(blocks), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL((k_trace<W, S, false>), dim3(blocks), dim3(block), lds, st, a);
+}
+
+inline void launch_trace_cfg(const TraceCfg& c, const TraceArgs& a, bool any, int blocks, hipStream_t st) {
+    const size_t lds = sizeof(int32_t) * static_cast<size_t>(c.stack) * static_cast<size_t>(c.block);
+    const int key = c.width * 100 + c.stack;
+    switch (key) {
+        case 416: launch_ws<4, 16>(any, blocks, c.block, lds, st, a); break;
+        case 432: launch_ws<4, 32>(any, blocks, c.block, lds, st, a); break;
+        case 816: launch_ws<8, 16>(any, blocks, c.block, lds, st, a); break;
+        case 824: launch_ws<8, 24>(any, blocks, c.block, lds, st, a); break;
+        case 832: launch_ws<8, 32>(any, blocks, c.block, lds, st, a); break;
+        default: launch_ws<4, 24>(any, blocks, c.block, lds, st, a); break;
+    }
+}
+
+inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
+    if (o.width != 8) o.width = 4;
+    if (o.leaf_max <= 0) o.leaf_max = 2;
+    if (o.leaf_max > 64) o.leaf_max = 64;
+    if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
+    if (o.block != 128) o.block = 256;
+    TraceCfg c;
+    c.width = o.width;
+    c.stack = o.stack_lds;
+    c.block = o.block;
+    return c;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        bytes = n;
+        return hipMalloc(&p, n);
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace mpt
+
+struct mpt_ctx {
+    int device = 0;
+    int num_cus = 0;
+    std::string error;
+};
+
+struct mpt_scene {
+    mpt_ctx* ctx = nullptr;
+    mpt::HostScene host;
+    mpt::BinaryBvh bin;        // the tree the wide BVH is collapsed from
+    mpt::BinaryBvh ref_bin;    // the reference's tree, built on first request
+    bool ref_built = false;
+    mpt::WideBvh wide;
+    mpt::DevScene sc{};
+    hipStream_t stream = nullptr;
+    // scene arrays
+    mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf;
+    // film + pass state
+    mpt::DevBuf film, beta, rad, fpos, qcount, work, stats, spill, srgb;
+    mpt::DevBuf qbuf[2][12];
+    int64_t npaths = 0;
+    // parameters that can change after the upload (differentiable rendering)
+    std::vector<mpt::DevMaterial> h_mats;
+    std::vector<float> h_cdf;
+    std::vector<mpt::DevLight> h_lights;
+    std::vector<mpt::DevTexture> h_texs;
+    std::vector<mpt::F3> h_rad;
+    int64_t tex_elems = 0;
+    uint64_t param_version = 0;
+    // per-sample radiance kept by a forward render for the adjoint
+    mpt::DevBuf samples;
+    bool keep_valid = false;
+    uint64_t keep_seed = 0, keep_version = 0;
+    int32_t keep_begin = 0, keep_end = 0, keep_depth = 0;
+    // adjoint buffers
+    mpt::DevBuf adjq[2][6], adjA, dimg, gtex, glight;
+    int64_t adj_npaths = 0;
+    int64_t spill_lanes = 0;
+    int32_t max_depth_alloc = -1;
+    mpt_scene_stats sstats{};
+    mpt_stats rstats{};
+    mpt_bvh_opts opts{};
+    mpt::TraceCfg cfg;
+    std::vector<hipEvent_t> events;
+    ~mpt_scene() {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace mpt {
+
+inline int fail(mpt_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, MPT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+inline int upload_vec(mpt_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
+    HIP_TRY(ctx, b.alloc(v.size() * sizeof(T)));
+    if (!v.empty()) HIP_TRY(ctx, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MPT_OK;
+}
+
+inline double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace mpt

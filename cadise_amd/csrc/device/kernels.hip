@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                         r.tmin = o4.w;
                         r.tmax = d4.w;
                         r.hit = -1;
-                        r.node = 0;
+                        r.node = a.root ? a.root[q] : 0;
                         st.sp = 0;
                     }
                 }
@@ -137,6 +137,14 @@ template __global__ void k_trace<4, 24, false>(TraceArgs);
 template __global__ void k_trace<4, 24, true>(TraceArgs);
 template __global__ void k_trace<8, 24, false>(TraceArgs);
 template __global__ void k_trace<8, 24, true>(TraceArgs);
+template __global__ void k_trace<4, 16, false>(TraceArgs);
+template __global__ void k_trace<4, 16, true>(TraceArgs);
+template __global__ void k_trace<8, 16, false>(TraceArgs);
+template __global__ void k_trace<8, 16, true>(TraceArgs);
+template __global__ void k_trace<4, 32, false>(TraceArgs);
+template __global__ void k_trace<4, 32, true>(TraceArgs);
+template __global__ void k_trace<8, 32, false>(TraceArgs);
+template __global__ void k_trace<8, 32, true>(TraceArgs);
 
 // ---- camera ----------------------------------------------------------------------
 
@@ -176,6 +184,7 @@ __device__ __forceinline__ uint32_t kensler_permute(uint32_t i, uint32_t l, uint
 
 // Filter::evaluate of the four reference filters (boxFilter.cpp:9-11,
 // gaussianFilter.cpp:25-33, coneFilter.cpp:13-16, mitchellFilter.cpp:16-40).
+// F is DevScene or BatchView (the same filter fields).
 __device__ __forceinline__ float mitchell1d(float b, float c, float d) {
     const float ad = fabsf(d);
     const float den = 1.0f / 6.0f;
@@ -187,7 +196,8 @@ __device__ __forceinline__ float mitchell1d(float b, float c, float d) {
                 (8.0f * b + 24.0f * c)) * den;
     return 0.0f;
 }
-__device__ __forceinline__ float filter_eval(const DevScene& sc, float x, float y) {
+template <class F>
+__device__ __forceinline__ float filter_eval(const F& sc, float x, float y) {
     switch (sc.filter) {
         case 1: {
             const float v = sc.amplitude * expf(sc.g_expx * x * x + sc.g_expy * y * y);
@@ -202,7 +212,8 @@ __device__ __forceinline__ float filter_eval(const DevScene& sc, float x, float 
 // Pixel range [x0, x1) x [y0, y1) a sample at film position (fx, fy) of the
 // tile [tx0, tx1) x [ty0, ty1) reaches: FilmTile::addSample's footprint,
 // clamped to the tile (filmTile.cpp:36-50).
-__device__ __forceinline__ int4 footprint(const DevScene& sc, float fx, float fy, int tx0, int ty0, int tx1, int ty1) {
+template <class F>
+__device__ __forceinline__ int4 footprint(const F& sc, float fx, float fy, int tx0, int ty0, int tx1, int ty1) {
     const float mnx = fmaxf(fx - sc.filter_hw, static_cast<float>(tx0));
     const float mny = fmaxf(fy - sc.filter_hh, static_cast<float>(ty0));
     const float mxx = fminf(fx + sc.filter_hw, static_cast<float>(tx1));
@@ -213,9 +224,40 @@ __device__ __forceinline__ int4 footprint(const DevScene& sc, float fx, float fy
 
 __device__ __forceinline__ bool finite3(float4 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 
-// One sample pass's primary rays: the stratified (or random) film jitter of
-// StratifiedSampler::requestSample2D (stratifiedSampler.cpp:38-73) and
-// PerspectivePinholeCamera::spawnPrimaryRay (perspectivePinholeCamera.cpp:52-73).
+// The film position and primary ray of sample `sample` of pixel p: the
+// stratified (or random) film jitter of StratifiedSampler::requestSample2D
+// (stratifiedSampler.cpp:38-73) and PerspectivePinholeCamera::spawnPrimaryRay
+// (perspectivePinholeCamera.cpp:52-73). F is DevScene or BatchView.
+template <class F>
+__device__ __forceinline__ V3 camera_ray(const F& sc, int32_t p, int32_t sample, uint2 key, float& fx, float& fy) {
+    const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
+    const uint4 r = philox(make_uint4(static_cast<uint32_t>(p), static_cast<uint32_t>(sample), 0xFFFFFFFFu, 0u), key);
+    const float u0 = unit24(r.x), u1 = unit24(r.y);
+    float jx = u0, jy = u1;
+    if (sc.sampler == 1) {
+        const uint4 pk = philox(make_uint4(static_cast<uint32_t>(p), 0u, 0xFFFFFFFEu, 0u), key);
+        const uint32_t base = static_cast<uint32_t>(sc.strata);
+        const uint32_t j = kensler_permute(static_cast<uint32_t>(sample), static_cast<uint32_t>(sc.spp), pk.x);
+        if (j < base * base) {
+            const float dx = 1.0f / static_cast<float>(base);
+            jx = (static_cast<float>(j % base) + u0) * dx;
+            jy = (static_cast<float>(j / base) + u1) * dx;
+        }
+    }
+    fx = static_cast<float>(ix) + jx;
+    fy = static_cast<float>(iy) + jy;
+    // film -> camera point, camera -> world vector (tMatrix4.ipp:180-195)
+    const float* m = sc.f2c;
+    const float z = 0.0f;
+    const float cx = m[0] * fx + m[1] * fy + m[2] * z + m[3];
+    const float cy = m[4] * fx + m[5] * fy + m[6] * z + m[7];
+    const float cz = m[8] * fx + m[9] * fy + m[10] * z + m[11];
+    const float* c = sc.c2w;
+    return normalize(v3(c[0] * cx + c[1] * cy + c[2] * cz, c[4] * cx + c[5] * cy + c[6] * cz,
+                        c[8] * cx + c[9] * cy + c[10] * cz));
+}
+
+// One sample pass's primary rays (one path per pixel).
 // ADJ: also the path's adjoint weight A = sum_q dLoss/dI_q w(q, s) / W_q over the
 // sample's footprint (zero for samples the film drops: NaN / inf).
 template <bool ADJ>
@@ -226,30 +268,8 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     if (gid >= a.npaths) return;
     const int32_t p = static_cast<int32_t>(gid);
     const DevScene& sc = a.sc;
-    const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
-    const uint4 r = philox(make_uint4(static_cast<uint32_t>(p), static_cast<uint32_t>(a.sample), 0xFFFFFFFFu, 0u), a.key);
-    const float u0 = unit24(r.x), u1 = unit24(r.y);
-    float jx = u0, jy = u1;
-    if (sc.sampler == 1) {
-        const uint4 pk = philox(make_uint4(static_cast<uint32_t>(p), 0u, 0xFFFFFFFEu, 0u), a.key);
-        const uint32_t base = static_cast<uint32_t>(sc.strata);
-        const uint32_t j = kensler_permute(static_cast<uint32_t>(a.sample), static_cast<uint32_t>(sc.spp), pk.x);
-        if (j < base * base) {
-            const float dx = 1.0f / static_cast<float>(base);
-            jx = (static_cast<float>(j % base) + u0) * dx;
-            jy = (static_cast<float>(j / base) + u1) * dx;
-        }
-    }
-    const float fx = static_cast<float>(ix) + jx, fy = static_cast<float>(iy) + jy;
-    // film -> camera point, camera -> world vector (tMatrix4.ipp:180-195)
-    const float* m = sc.f2c;
-    const float z = 0.0f;
-    const float cx = m[0] * fx + m[1] * fy + m[2] * z + m[3];
-    const float cy = m[4] * fx + m[5] * fy + m[6] * z + m[7];
-    const float cz = m[8] * fx + m[9] * fy + m[10] * z + m[11];
-    const float* c = sc.c2w;
-    const V3 dir = normalize(v3(c[0] * cx + c[1] * cy + c[2] * cz, c[4] * cx + c[5] * cy + c[6] * cz,
-                                c[8] * cx + c[9] * cy + c[10] * cz));
+    float fx, fy;
+    const V3 dir = camera_ray(sc, p, a.sample, a.key, fx, fy);
     a.fpos[p] = make_float2(fx, fy);
     a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -261,6 +281,7 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     q.shd_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
     q.bv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (ADJ) {
+        const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
         V3 A = v3(0.f, 0.f, 0.f);
         if (finite3(g.samples[p])) {
             const int tx0 = (ix / sc.tile_x) * sc.tile_x, ty0 = (iy / sc.tile_y) * sc.tile_y;
@@ -292,28 +313,61 @@ constexpr int TEX_SLOTS = 128;
 constexpr int LIGHT_SLOTS = 64;
 constexpr int SHADE_WAVES = 256 / WAVE;
 
-// Step `bounce` of every queued path: first the deferred half of
+// Splat a finished sample into the merged film of a batch: the tile-clamped
+// footprint of FilmTile::addSample (filmTile.cpp:21-66) with float atomics;
+// NaN / inf samples are dropped.
+__device__ __forceinline__ void batch_splat(const BatchArgs& bt, const BatchView& v, int32_t pix, float2 fp4, V3 L) {
+    if (!(isfinite(L.x) && isfinite(L.y) && isfinite(L.z))) return;
+    const int32_t ix = pix % v.img_w, iy = pix / v.img_w;
+    const int tx0 = (ix / v.tile_x) * v.tile_x, ty0 = (iy / v.tile_y) * v.tile_y;
+    const int tx1 = min(tx0 + v.tile_x, v.img_w), ty1 = min(ty0 + v.tile_y, v.img_h);
+    const int4 fp = footprint(v, fp4.x, fp4.y, tx0, ty0, tx1, ty1);
+    for (int qy = fp.y; qy < fp.w; ++qy)
+        for (int qx = fp.x; qx < fp.z; ++qx) {
+            const float w = filter_eval(v, static_cast<float>(qx) - (fp4.x - 0.5f), static_cast<float>(qy) - (fp4.y - 0.5f));
+            float* f = reinterpret_cast<float*>(bt.film + v.film_base + static_cast<int64_t>(qy) * v.img_w + qx);
+            unsafeAtomicAdd(f, L.x * w);
+            unsafeAtomicAdd(f + 1, L.y * w);
+            unsafeAtomicAdd(f + 2, L.z * w);
+            unsafeAtomicAdd(f + 3, w);
+        }
+}
+
+// One step of every queued path: first the deferred half of
 // DirectLightEvaluator::evaluate (directLightEvaluator.cpp:18-109) for the
-// vertex shaded at bounce - 1, then, for the continuation hit, one iteration of
-// PtEstimator::estimate's loop body: emission, light pick
+// vertex shaded one step earlier, then, for the continuation hit, one
+// iteration of PtEstimator::estimate's loop body: emission, light pick
 // (uniformLightCluster.cpp:15-24 / powerLightCluster.cpp:27-34), light sample +
 // shadow ray, MIS-BSDF ray, continuation sample, Russian roulette
-// (russianRoulette.cpp:9-28). bounce == max_depth only resolves.
+// (russianRoulette.cpp:9-28). A path at bounce == max-depth only resolves.
 //
-// ADJ replays the same steps with the same numbers (the radiance sum L below is
-// bit-identical to the forward pass's) and adds the path-replay gradient: when
-// vertex b-1's NEE resolves, its light and MIS-BSDF terms give dLoss/dtexel and
-// dLoss/dLe directly; its continuation gives A * (L_sample - L) * d ln f/dtexel,
-// (L_sample - L) being all radiance the path gathers from vertex b on. Sampling
-// pdfs, MIS weights and the roulette factor are held fixed. Each lane then has
-// at most one texel and one light contribution, which the wave's LDS tables
-// merge before one FP32 atomic per distinct key.
-template <bool ADJ>
-__global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bounce) {
+// SHADE_FWD: one sample pass of one scene; every queued path is at bounce
+// `step`, queues alternate by step.
+//
+// SHADE_ADJ replays the same steps with the same numbers (the radiance sum L
+// below is bit-identical to the forward pass's) and adds the path-replay
+// gradient: when vertex b-1's NEE resolves, its light and MIS-BSDF terms give
+// dLoss/dtexel and dLoss/dLe directly; its continuation gives
+// A * (L_sample - L) * d ln f/dtexel, (L_sample - L) being all radiance the path
+// gathers from vertex b on. Sampling pdfs, MIS weights and the roulette factor
+// are held fixed. Each lane then has at most one texel and one light
+// contribution, which the wave's LDS tables merge before one FP32 atomic per
+// distinct key.
+//
+// SHADE_BATCH: paths of many scenes at mixed bounces (each path carries its
+// scene, pixel, sample and bounce); queue `step` (0 / 1) is consumed; a path
+// that does not go on splats its radiance into its scene's film and returns its
+// slot to the free stack.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step) {
+    constexpr bool ADJ = MODE == SHADE_ADJ;
+    constexpr bool BATCH = MODE == SHADE_BATCH;
     const DevScene& sc = a.sc;
-    const Queue& cur = a.q[bounce & 1];
-    const Queue& nxt = a.q[(bounce + 1) & 1];
-    const int32_t n = a.qcount[bounce];
+    const int qc = step & 1;
+    const Queue& cur = a.q[qc];
+    const Queue& nxt = a.q[qc ^ 1];
+    const int32_t n = BATCH ? a.qcount[qc] : a.qcount[step];
+    int32_t* ncount = BATCH ? &a.qcount[qc ^ 1] : &a.qcount[step + 1];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -332,12 +386,13 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
     for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
         const int64_t s = bbase + threadIdx.x;
         const bool active = s < n;
-        bool pending = false;
+        bool pending = false, finished = false;
         int32_t p = 0;
         float4 org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org,
                bv4 = org;
         int32_t pick_light = 0;
         float4 beta4 = org;
+        int32_t sid = 0, root_next = 0;
         // adjoint: this lane's contributions and the factors of the vertex shaded now
         int32_t tkey = -1, lkey = -1;
         V3 tval = v3(0.f, 0.f, 0.f), lval = v3(0.f, 0.f, 0.f);
@@ -345,6 +400,26 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
         float4 sL4 = org, eL4 = org, sB4 = org, eB4 = org, sC4 = org;
         if (active) {
             p = cur.path[s];
+            // the path's step, sample, pixel and scene settings
+            int32_t pb = step, smp = a.sample, pix = p, md = a.max_depth;
+            uint2 key = a.key;
+            int32_t nl = sc.nlights, cluster = sc.cluster, lbase = 0, cdf0 = 0;
+            int4 meta = make_int4(0, 0, 0, 0);
+            if constexpr (BATCH) {
+                meta = bt.meta[p];
+                sid = meta.x;
+                pix = meta.y;
+                smp = meta.z;
+                pb = meta.w;
+                const BatchView& v = bt.views[sid];
+                md = v.max_depth;
+                key = make_uint2(v.key_lo, v.key_hi);
+                nl = v.nlights;
+                cluster = v.cluster;
+                lbase = v.light_base;
+                cdf0 = v.cdf_base;
+                root_next = v.root;
+            }
             org = cur.org[s];
             const V3 P0 = v3f4(org);
             float4 rad4 = a.rad[p];
@@ -355,24 +430,24 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
             if constexpr (ADJ) A = v3f4(g.A[p]);
             // -- resolve the previous vertex's NEE
             const float4 pv = cur.bv[s];
-            if (bounce > 0 && pv.w > 0.0f) {
+            if (pb > 0 && pv.w > 0.0f) {
                 V3 Ld = v3(0.f, 0.f, 0.f);
                 const float4 sd = cur.shd_dir[s];
                 const int32_t li = cur.light[s];
                 if (sd.w >= 0.0f && cur.occ[s].x == 0) {
                     Ld = Ld + v3f4(cur.lt[s]);
                     if constexpr (ADJ) {
-                        tval = tval + A * v3f4(g.q[bounce & 1].sL[s]);
-                        lval = lval + A * v3f4(g.q[bounce & 1].eL[s]);
+                        tval = tval + A * v3f4(g.q[qc].sL[s]);
+                        lval = lval + A * v3f4(g.q[qc].eL[s]);
                     }
                 }
-                const float4 md = cur.mis_dir[s];
-                if (md.w >= 0.0f) {
+                const float4 md4 = cur.mis_dir[s];
+                if (md4.w >= 0.0f) {
                     const int2 mh = cur.mis_hit[s];
                     if (mh.x >= 0) {
                         const int32_t prim = static_cast<int32_t>(__float_as_uint(sc.tris[3 * mh.x].w) & 0x7FFFFFFFu);
                         if (sc.prim_info[prim].y == li) {
-                            const SurfaceHit h = surface_detail(sc, mh.x, P0, v3f4(md), __int_as_float(mh.y));
+                            const SurfaceHit h = surface_detail(sc, mh.x, P0, v3f4(md4), __int_as_float(mh.y));
                             const DevLight& lg = sc.lights[li];
                             const V3 Le = light_emittance(lg, h);
                             const float4 f4 = cur.fac[s];
@@ -381,8 +456,8 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
                             Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
                             if constexpr (ADJ) {
                                 if (can_emit(lg, h.wi, h.Ns)) {
-                                    tval = tval + (A * v3f4(g.q[bounce & 1].sB[s])) * (load3(lg.radiance) * w);
-                                    lval = lval + A * (v3f4(g.q[bounce & 1].eB[s]) * w);
+                                    tval = tval + (A * v3f4(g.q[qc].sB[s])) * (load3(lg.radiance) * w);
+                                    lval = lval + A * (v3f4(g.q[qc].eB[s]) * w);
                                 }
                             }
                         }
@@ -390,12 +465,12 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
                 }
                 L = L + v3f4(pv) * divs(Ld, pv.w);
                 if constexpr (ADJ) {
-                    const float4 sc4 = g.q[bounce & 1].sC[s];
+                    const float4 sc4 = g.q[qc].sC[s];
                     if (sc4.w > 0.0f) {
                         const V3 R = v3f4(g.samples[p]) - L;
                         tval = tval + (A * R) * v3f4(sc4);
                     }
-                    const int2 pp = g.q[bounce & 1].par[s];
+                    const int2 pp = g.q[qc].par[s];
                     if (pp.x >= 0) tkey = 2 * pp.x + (pp.y == 3 ? 1 : 0);
                     lkey = li;
                 }
@@ -403,14 +478,14 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
             // -- shade the continuation hit
             cdir = cur.dir[s];
             const int2 ch = cur.hit[s];
-            if (bounce < a.max_depth && cdir.w >= 0.0f && ch.x >= 0) {
-                const uint32_t pix = static_cast<uint32_t>(p);
-                const uint4 rA = philox(make_uint4(pix, static_cast<uint32_t>(a.sample), 2u * bounce, 0u), a.key);
-                const uint4 rB = philox(make_uint4(pix, static_cast<uint32_t>(a.sample), 2u * bounce + 1u, 0u), a.key);
+            if (pb < md && cdir.w >= 0.0f && ch.x >= 0) {
+                const uint32_t upix = static_cast<uint32_t>(pix);
+                const uint4 rA = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb, 0u), key);
+                const uint4 rB = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb + 1u, 0u), key);
                 const SurfaceHit h = surface_detail(sc, ch.x, P0, v3f4(cdir), __int_as_float(ch.y));
                 const int4 info = sc.prim_info[h.prim];
                 const DevMaterial mat = sc.mats[info.x];
-                if (bounce == 0 && info.y >= 0) {
+                if (pb == 0 && info.y >= 0) {
                     L = L + beta * light_emittance(sc.lights[info.y], h);
                     if constexpr (ADJ) {
                         if (can_emit(sc.lights[info.y], h.wi, h.Ns)) {
@@ -428,18 +503,18 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
                 float pick;
                 {
                     const float u = u01(rA, SL_PICK);
-                    const int32_t nl = sc.nlights;
                     int32_t idx = 0;
-                    if (sc.cluster == 1) {
+                    if (cluster == 1) {
+                        const float* cdf = sc.power_cdf + cdf0;
                         for (int32_t i = nl - 1; i >= 0; --i)
-                            if (sc.power_cdf[i] <= u) { idx = i; break; }
-                        pick = sc.power_cdf[idx + 1] - sc.power_cdf[idx];
+                            if (cdf[i] <= u) { idx = i; break; }
+                        pick = cdf[idx + 1] - cdf[idx];
                     } else {
                         const uint32_t raw = static_cast<uint32_t>(0.0f + u * static_cast<float>(nl));
                         idx = raw < static_cast<uint32_t>(nl - 1) ? static_cast<int32_t>(raw) : nl - 1;
                         pick = 1.0f / static_cast<float>(nl);
                     }
-                    pick_light = idx;
+                    pick_light = lbase + idx;
                 }
                 // light sample + shadow ray
                 const DevLight& lg = sc.lights[pick_light];
@@ -478,8 +553,8 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
                 {
                     const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1));
                     if (bs.valid) {
-                        const V3 md = normalize(bs.L);
-                        mdir = make_float4(md.x, md.y, md.z, FLT_MAX);
+                        const V3 mdv = normalize(bs.L);
+                        mdir = make_float4(mdv.x, mdv.y, mdv.z, FLT_MAX);
                         const V3 fc = bs.f * absdot(bs.L, h.Ns);
                         fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
                         if constexpr (ADJ) {
@@ -499,13 +574,13 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
                     if (cs.valid) {
                         beta = beta * divs(cs.f * absdot(cs.L, h.Ns), cs.pdf);
                         bool alive = true;
-                        if (bounce > 2) {
+                        if (pb > 2) {
                             const float qq = mclamp(1.0f - max_comp(beta), 0.05f, 1.0f);
                             if (u01(rA, SL_RR) > qq) beta = beta * (1.0f / (1.0f - qq));
                             else alive = false;
                         }
                         if (alive && is_zero(beta)) alive = false;
-                        if (alive && bounce + 1 < a.max_depth) {
+                        if (alive && pb + 1 < md) {
                             const V3 nd = normalize(cs.L);
                             cdir = make_float4(nd.x, nd.y, nd.z, FLT_MAX);
                             if constexpr (ADJ) {
@@ -525,6 +600,14 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
             }
             a.rad[p] = make_float4(L.x, L.y, L.z, 0.f);
             a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+            if constexpr (BATCH) {
+                if (pending) {
+                    bt.meta[p] = make_int4(meta.x, meta.y, meta.z, pb + 1);
+                } else {
+                    finished = true;
+                    batch_splat(bt, bt.views[sid], pix, a.fpos[p], L);
+                }
+            }
         }
         if constexpr (ADJ) {
             // per-wave pre-reduction, then one atomic per distinct texel / light
@@ -558,10 +641,18 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
             }
             __syncthreads();
         }
+        if constexpr (BATCH) {
+            // finished paths give their slots back (one atomic per wave)
+            const unsigned long long fmask = __ballot(finished);
+            int32_t fbase = 0;
+            if (lane == 0 && fmask) fbase = atomicAdd(&bt.ctr[0], __popcll(fmask));
+            fbase = __shfl(fbase, 0, WAVE);
+            if (finished) bt.free_ids[fbase + static_cast<int32_t>(__popcll(fmask & ((1ull << lane) - 1ull)))] = p;
+        }
         // wave-ballot compaction into the next queue
         const unsigned long long mask = __ballot(pending);
         int32_t base = 0;
-        if (lane == 0 && mask) base = atomicAdd(&a.qcount[bounce + 1], __popcll(mask));
+        if (lane == 0 && mask) base = atomicAdd(ncount, __popcll(mask));
         base = __shfl(base, 0, WAVE);
         if (pending) {
             const int32_t slot = base + static_cast<int32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
@@ -574,8 +665,9 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
             nxt.fac[slot] = fac4;
             nxt.bv[slot] = bv4;
             nxt.light[slot] = pick_light;
+            if constexpr (BATCH) bt.root[qc ^ 1][slot] = root_next;
             if constexpr (ADJ) {
-                const AdjQueue& an = g.q[(bounce + 1) & 1];
+                const AdjQueue& an = g.q[qc ^ 1];
                 an.par[slot] = par;
                 an.sL[slot] = sL4;
                 an.eL[slot] = eL4;
@@ -587,10 +679,73 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, int32_t bo
     }
 }
 
+// ---- batched renderer: path regeneration ------------------------------------------------
+
+// New camera paths for the free path slots of a batch, appended to queue qc:
+// n_new = min(free slots, work left, queue room), all threads reading the same
+// counters (k_bcommit applies them afterwards). Work item w of scene s (w in
+// [work_base_s, work_base_s+1)) is sample (w - base) / pixels of pixel
+// (w - base) % pixels, so consecutive lanes trace neighbouring pixels.
+__global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, int32_t qc) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int32_t nfree = bt.ctr[0];
+    const int64_t next = static_cast<int64_t>(*bt.work_next);
+    const int32_t qn = a.qcount[qc];
+    const int64_t n_new = min(min(static_cast<int64_t>(nfree), bt.total_work - next), static_cast<int64_t>(bt.capacity - qn));
+    if (gid >= n_new) return;
+    const int64_t w = next + gid;
+    int lo = 0, hi = bt.nviews - 1;    // last view with work_base <= w
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bt.views[mid].work_base <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchView& v = bt.views[lo];
+    const int64_t local = w - v.work_base;
+    const int64_t npx = static_cast<int64_t>(v.img_w) * v.img_h;
+    const int32_t sample = static_cast<int32_t>(local / npx), pix = static_cast<int32_t>(local % npx);
+    const int32_t p = bt.free_ids[nfree - 1 - gid];
+    float fx, fy;
+    const V3 dir = camera_ray(v, pix, sample, make_uint2(v.key_lo, v.key_hi), fx, fy);
+    a.fpos[p] = make_float2(fx, fy);
+    a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    bt.meta[p] = make_int4(lo, pix, sample, 0);
+    const int32_t slot = qn + static_cast<int32_t>(gid);
+    const Queue& q = a.q[qc];
+    q.path[slot] = p;
+    q.org[slot] = make_float4(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2], K_RAY_EPS);
+    q.dir[slot] = make_float4(dir.x, dir.y, dir.z, v.max_depth > 0 ? FLT_MAX : -1.0f);
+    q.mis_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+    q.shd_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+    q.bv[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    bt.root[qc][slot] = v.root;
+}
+
+// Applies k_bregen's counts (one thread): slots taken, work consumed, queue qc
+// grown; clears the other queue's count and this step's traversal work
+// counters; raises the done flag once no work is left and qc is empty.
+__global__ void k_bcommit(PassArgs a, BatchArgs bt, int32_t qc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t nfree = bt.ctr[0];
+    const int64_t next = static_cast<int64_t>(*bt.work_next);
+    const int32_t qn = a.qcount[qc];
+    const int64_t n_new = min(min(static_cast<int64_t>(nfree), bt.total_work - next), static_cast<int64_t>(bt.capacity - qn));
+    bt.ctr[0] = nfree - static_cast<int32_t>(n_new);
+    *bt.work_next = static_cast<unsigned long long>(next + n_new);
+    a.qcount[qc] = qn + static_cast<int32_t>(n_new);
+    a.qcount[qc ^ 1] = 0;
+    bt.ctr[2] = 0;
+    bt.ctr[3] = 0;
+    bt.ctr[1] = (next + n_new >= bt.total_work && qn + n_new == 0) ? 1 : 0;
+}
+
 template __global__ void k_camera<false>(PassArgs, AdjArgs);
 template __global__ void k_camera<true>(PassArgs, AdjArgs);
-template __global__ void k_shade<false>(PassArgs, AdjArgs, int32_t);
-template __global__ void k_shade<true>(PassArgs, AdjArgs, int32_t);
+template __global__ void k_shade<SHADE_FWD>(PassArgs, AdjArgs, BatchArgs, int32_t);
+template __global__ void k_shade<SHADE_ADJ>(PassArgs, AdjArgs, BatchArgs, int32_t);
+template __global__ void k_shade<SHADE_BATCH>(PassArgs, AdjArgs, BatchArgs, int32_t);
+
 
 // ---- film ------------------------------------------------------------------------
 

@@ -40,6 +40,7 @@ struct TraceArgs {
     int2*          hit;          // closest: (leaf or prim, t bits); any: x = 0/1
     int2*          hit2;
     int32_t        want_prim;    // closest: write the creation index instead of the leaf position
+    const int32_t* root;         // per-ray root node (index q of the first set), null = node 0
     int32_t*       work;         // atomic work counter, zero on entry
     int32_t*       spill;        // per-lane stack spill, (max_stack - S) entries per lane
     unsigned long long* stats;
@@ -86,6 +87,41 @@ struct AdjArgs {
     float*        gtex;     // dLoss / d(texture element)
     float*        glight;   // dLoss / dLe, nlights x 3
 };
+
+// One scene of a batch: where its tables sit in the merged scene, and its
+// camera, film and renderer settings (the per-scene fields of DevScene).
+struct BatchView {
+    int32_t root;                // merged node index of the scene's BVH root
+    int32_t light_base, nlights, cluster, cdf_base;
+    int32_t img_w, img_h, tile_x, tile_y;
+    int32_t filter;
+    float   filter_hw, filter_hh, g_expx, g_expy, g_edge, amplitude, mitchell_b, mitchell_c;
+    int32_t sampler, spp, strata, max_depth;
+    uint32_t key_lo, key_hi;     // Philox key (the scene's seed)
+    int64_t film_base;           // first pixel of the scene in the merged film
+    int64_t work_base;           // first work item of the scene (items: sample-major pixel x sample)
+    float   f2c[12], c2w[12], cam_pos[3];
+};
+
+// Streaming state of the batched renderer: path slots are recycled through a
+// free stack; new camera paths are drawn from one global work counter.
+struct BatchArgs {
+    const BatchView* views;
+    int32_t  nviews;
+    int32_t  capacity;           // path slots (= queue entries)
+    int64_t  total_work;
+    int4*    meta;               // per path slot: scene, pixel, sample, bounce
+    int32_t* root[2];            // per queue entry: BVH root of the entry's scene
+    float4*  film;               // merged linear film (sum w*rgb, sum w)
+    int32_t* free_ids;           // stack of free path slots
+    int32_t* ctr;                // [0] free count, [1] done flag, [2..3] trace work counters
+    unsigned long long* work_next;
+    unsigned long long* stats;
+};
+
+// Modes of the shading kernel: one scene's sample pass, its path-replay
+// adjoint, or the batched renderer's mixed-bounce step.
+enum { SHADE_FWD = 0, SHADE_ADJ = 1, SHADE_BATCH = 2 };
 
 struct PassArgs {
     DevScene sc;

@@ -19,123 +19,12 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/mpt.h"
-#include "../host/bvh.hpp"
-#include "../host/crsd.hpp"
-#include "../host/scene.hpp"
-#include "kernels.hpp"
-
-namespace mpt {
-
-template <int W, int S, bool ANY>
-__global__ void k_trace(TraceArgs a);
-template <bool ADJ>
-__global__ void k_camera(PassArgs a, AdjArgs g);
-template <bool ADJ>
-__global__ void k_shade(PassArgs a, AdjArgs g, int32_t bounce);
-__global__ void k_film(PassArgs a);
-__global__ void k_srgb8(const float4* film, int64_t npix, uint8_t* out);
-__global__ void k_zero_u64(unsigned long long* p, int n);
-__global__ void k_zero_f4(float4* p, int64_t n);
-__global__ void k_zero_i32(int32_t* p, int n);
-
-constexpr int STACK_LDS = 24;
-constexpr int TRACE_BLOCK = 256;
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t alloc(size_t n) {
-        release();
-        if (n == 0) n = 16;
-        bytes = n;
-        return hipMalloc(&p, n);
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-}  // namespace mpt
-
-struct mpt_ctx {
-    int device = 0;
-    int num_cus = 0;
-    std::string error;
-};
-
-struct mpt_scene {
-    mpt_ctx* ctx = nullptr;
-    mpt::HostScene host;
-    mpt::BinaryBvh bin;        // the tree the wide BVH is collapsed from
-    mpt::BinaryBvh ref_bin;    // the reference's tree, built on first request
-    bool ref_built = false;
-    mpt::WideBvh wide;
-    mpt::DevScene sc{};
-    hipStream_t stream = nullptr;
-    // scene arrays
-    mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf;
-    // film + pass state
-    mpt::DevBuf film, beta, rad, fpos, qcount, work, stats, spill, srgb;
-    mpt::DevBuf qbuf[2][12];
-    int64_t npaths = 0;
-    // parameters that can change after the upload (differentiable rendering)
-    std::vector<mpt::DevLight> h_lights;
-    std::vector<mpt::DevTexture> h_texs;
-    std::vector<mpt::F3> h_rad;
-    int64_t tex_elems = 0;
-    uint64_t param_version = 0;
-    // per-sample radiance kept by a forward render for the adjoint
-    mpt::DevBuf samples;
-    bool keep_valid = false;
-    uint64_t keep_seed = 0, keep_version = 0;
-    int32_t keep_begin = 0, keep_end = 0, keep_depth = 0;
-    // adjoint buffers
-    mpt::DevBuf adjq[2][6], adjA, dimg, gtex, glight;
-    int64_t adj_npaths = 0;
-    int64_t spill_lanes = 0;
-    int32_t max_depth_alloc = -1;
-    mpt_scene_stats sstats{};
-    mpt_stats rstats{};
-    mpt_bvh_opts opts{};
-    std::vector<hipEvent_t> events;
-    ~mpt_scene() {
-        for (auto e : events) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+#include "driver.hpp"
 
 namespace {
 
 using namespace mpt;
 
-int fail(mpt_ctx* ctx, int code, const std::string& msg) {
-    if (ctx) ctx->error = msg;
-    return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                      \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            return fail(ctx, MPT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-template <typename T>
-int upload_vec(mpt_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
-    HIP_TRY(ctx, b.alloc(v.size() * sizeof(T)));
-    if (!v.empty()) HIP_TRY(ctx, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return MPT_OK;
-}
-
-inline double ms_since(std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-}
 
 // Build the BVHs and upload every scene table.
 int build_and_upload(mpt_scene* S) {
@@ -147,9 +36,7 @@ int build_and_upload(mpt_scene* S) {
         if (!S->stream) HIP_TRY(ctx, hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
     }
     mpt_bvh_opts& o = S->opts;
-    if (o.width != 8) o.width = 4;
-    if (o.leaf_max <= 0) o.leaf_max = 2;
-    if (o.leaf_max > 64) o.leaf_max = 64;
+    S->cfg = normalize_cfg(o);
 
     auto t0 = std::chrono::steady_clock::now();
     build_binary(h, false, &S->bin);
@@ -301,6 +188,8 @@ int build_and_upload(mpt_scene* S) {
     if ((rc = upload_vec(ctx, S->tex_data, texdata)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, S->lights, lights)) != MPT_OK) return rc;
     S->h_lights = lights;
+    S->h_mats = mats;
+    S->h_cdf = cdf;
     S->h_texs = texs;
     S->h_rad = rad;
     S->tex_elems = static_cast<int64_t>(texdata.size());
@@ -360,16 +249,16 @@ int build_and_upload(mpt_scene* S) {
 }
 
 int trace_blocks(mpt_scene* S, int64_t nrays) {
-    const int64_t want = (nrays + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    const int64_t want = (nrays + S->cfg.block - 1) / S->cfg.block;
     const int64_t cap = static_cast<int64_t>(S->ctx->num_cus) * 6;
     return static_cast<int>(std::max<int64_t>(1, std::min(want, cap)));
 }
 
 // Spill space for traversal stacks deeper than the LDS part.
 int ensure_spill(mpt_scene* S, int blocks) {
-    const int64_t extra = S->sc.max_stack - STACK_LDS;
+    const int64_t extra = S->sc.max_stack - S->cfg.stack;
     if (extra <= 0) return MPT_OK;
-    const int64_t lanes = static_cast<int64_t>(blocks) * TRACE_BLOCK;
+    const int64_t lanes = static_cast<int64_t>(blocks) * S->cfg.block;
     if (lanes <= S->spill_lanes) return MPT_OK;
     HIP_TRY(S->ctx, S->spill.alloc(sizeof(int32_t) * static_cast<size_t>(lanes * extra)));
     S->spill_lanes = lanes;
@@ -377,15 +266,8 @@ int ensure_spill(mpt_scene* S, int blocks) {
 }
 
 int launch_trace(mpt_scene* S, TraceArgs a, bool any, int blocks) {
-    a.spill = S->sc.max_stack > STACK_LDS ? S->spill.as<int32_t>() : nullptr;
-    const size_t lds = sizeof(int32_t) * STACK_LDS * TRACE_BLOCK;
-    if (S->sc.width == 8) {
-        if (any) hipLaunchKernelGGL((k_trace<8, STACK_LDS, true>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
-        else hipLaunchKernelGGL((k_trace<8, STACK_LDS, false>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
-    } else {
-        if (any) hipLaunchKernelGGL((k_trace<4, STACK_LDS, true>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
-        else hipLaunchKernelGGL((k_trace<4, STACK_LDS, false>), dim3(blocks), dim3(TRACE_BLOCK), lds, S->stream, a);
-    }
+    a.spill = S->sc.max_stack > S->cfg.stack ? S->spill.as<int32_t>() : nullptr;
+    launch_trace_cfg(S->cfg, a, any, blocks, S->stream);
     HIP_TRY(S->ctx, hipGetLastError());
     return MPT_OK;
 }
@@ -582,7 +464,7 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
                 HIP_TRY(ctx, stop());
             }
             HIP_TRY(ctx, mark(EV_SHADE));
-            hipLaunchKernelGGL(k_shade<false>, dim3(sblocks), dim3(256), 0, S->stream, a, AdjArgs{}, b);
+            hipLaunchKernelGGL(k_shade<SHADE_FWD>, dim3(sblocks), dim3(256), 0, S->stream, a, AdjArgs{}, BatchArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, stop());
         }
@@ -621,6 +503,7 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     r.ms_shade = ms[EV_SHADE];
     r.ms_camera = ms[EV_CAMERA];
     r.ms_film = ms[EV_FILM];
+    r.ms_gpu = ms[EV_CLOSEST] + ms[EV_ANY] + ms[EV_SHADE] + ms[EV_CAMERA] + ms[EV_FILM];
     r.ms_total = ms_since(t0);
     if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
     if (keep && !(p->accumulate)) {
@@ -731,7 +614,7 @@ int do_adjoint(mpt_scene* S, const mpt_render_params* p, const float* d_image, f
                 t.stat_slot = ST_SHADOW_RAYS;
                 if ((rc = launch_trace(S, t, true, tblocks)) != MPT_OK) return rc;
             }
-            hipLaunchKernelGGL(k_shade<true>, dim3(sblocks), dim3(256), 0, S->stream, a, g, b);
+            hipLaunchKernelGGL(k_shade<SHADE_ADJ>, dim3(sblocks), dim3(256), 0, S->stream, a, g, BatchArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
         }
     }
@@ -1131,6 +1014,7 @@ int mpt_scene_set_light_radiance(mpt_scene* s, int32_t light, const float* rgb) 
                               hipMemcpyHostToDevice));
     // the power cluster's CDF follows the lights' flux
     const std::vector<float> cdf = power_cdf(s->host, s->h_rad);
+    s->h_cdf = cdf;
     HIP_TRY(s->ctx, hipMemcpy(s->power_cdf.p, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice));
     return MPT_OK;
 }

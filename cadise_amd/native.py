@@ -75,17 +75,31 @@ class Stats(C.Structure):
                 ("anyhit_launches", C.c_int64),
                 ("ms_total", C.c_double), ("ms_closest", C.c_double), ("ms_anyhit", C.c_double),
                 ("ms_shade", C.c_double), ("ms_camera", C.c_double), ("ms_film", C.c_double),
-                ("ms_adjoint", C.c_double)]
+                ("ms_adjoint", C.c_double), ("ms_gpu", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("width", C.c_int32), ("stack_lds", C.c_int32), ("block", C.c_int32), ("capacity_log2", C.c_int32),
+                ("graph_steps", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchTrial(C.Structure):
+    _fields_ = [("opts", BatchOpts), ("ms", C.c_double)]
 
 
 EXPORTS = ["mpt_create", "mpt_destroy", "mpt_last_error", "mpt_version", "mpt_scene_load_crsd", "mpt_scene_parse_crsd",
            "mpt_scene_create", "mpt_scene_destroy", "mpt_scene_stats_get", "mpt_scene_binary_bvh", "mpt_scene_camera",
            "mpt_scene_light_radiance", "mpt_trace",
            "mpt_trace_device", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_save", "mpt_stats_get",
-           "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_set_light_radiance", "mpt_render_adjoint"]
+           "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_set_light_radiance", "mpt_render_adjoint",
+           "mpt_batch_create", "mpt_batch_destroy", "mpt_batch_render", "mpt_batch_film", "mpt_batch_autotune",
+           "mpt_batch_get_opts", "mpt_batch_set_opts"]
 
 _lib = None
 
@@ -122,5 +136,12 @@ def load_library():
     L.mpt_scene_set_texture.argtypes = [vp, i32, vp]
     L.mpt_scene_set_light_radiance.argtypes = [vp, i32, vp]
     L.mpt_render_adjoint.argtypes = [vp, C.POINTER(RenderParams), vp, vp, vp]
+    L.mpt_batch_create.argtypes = [vp, C.POINTER(vp), i32, C.POINTER(BatchOpts), C.POINTER(vp)]
+    L.mpt_batch_destroy.argtypes = [vp]
+    L.mpt_batch_render.argtypes = [vp, vp, i32, C.POINTER(Stats)]
+    L.mpt_batch_film.argtypes = [vp, i32, vp]
+    L.mpt_batch_autotune.argtypes = [vp, vp, i32, C.POINTER(BatchOpts), C.POINTER(BatchTrial), i32, C.POINTER(i32)]
+    L.mpt_batch_get_opts.argtypes = [vp, C.POINTER(BatchOpts)]
+    L.mpt_batch_set_opts.argtypes = [vp, C.POINTER(BatchOpts)]
     _lib = L
     return L

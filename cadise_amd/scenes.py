@@ -591,3 +591,32 @@ def textured_spheres(seed=0x20261203 ^ 0x5, n_spheres=10, level=5, tex_res=256, 
     tot = float(sum(Le))
     s.lights.append(Light("lamp", tuple(3.0 * x / tot for x in Le), tot * area * np.pi))
     return s
+
+
+# -- batch of scenes -----------------------------------------------------------------------
+
+def batch_scenes(seed=0x20261203 ^ 0x4, n_scenes=64, tris=(1e4, 2e6), res=(128, 1024), spp=(1, 64), depth=(4, 16)):
+    """n_scenes procedural-sphere scenes of uneven size: triangle count and
+    square resolution log-uniform in the given ranges, spp and max depth
+    uniform integers, GGX roughness U[0.05, 1] (each scene as
+    procedural_spheres with its own seed; the sphere level is the largest that
+    keeps at most 100 spheres)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_scenes):
+        t = float(np.exp(rng.uniform(np.log(tris[0]), np.log(tris[1]))))
+        r = int(round(np.exp(rng.uniform(np.log(res[0]), np.log(res[1])))))
+        s = int(rng.integers(spp[0], spp[1] + 1))
+        d = int(rng.integers(depth[0], depth[1] + 1))
+        level = 1
+        for lv in (5, 4, 3, 2, 1):
+            if t / (20 * 4 ** lv) >= 1:
+                level = lv
+                break
+        while t / (20 * 4 ** level) > 100 and level < 6:
+            level += 1
+        n = int(max(1, min(100, round(t / (20 * 4 ** level)))))
+        sp = procedural_spheres(seed=int(rng.integers(1, 2**31)), n_spheres=n, level=level, width=r, height=r, spp=s,
+                                max_depth=d)
+        out.append(sp)
+    return out

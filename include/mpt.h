@@ -150,6 +150,8 @@ typedef struct {
     double  ms_closest, ms_anyhit, ms_shade, ms_camera, ms_film; /* summed kernel time (events) */
     double  ms_adjoint;           /* mpt_render_adjoint: GPU time of the replay (events); the other
                                      fields then describe the forward render it replays */
+    double  ms_gpu;               /* GPU time of the call's kernels (events; mpt_batch_render: the
+                                     streaming loop) */
 } mpt_stats;
 
 /* device >= 0: a GPU context. device == -1: a host-only context, in which
@@ -222,6 +224,44 @@ int  mpt_scene_set_light_radiance(mpt_scene* scene, int32_t light, const float* 
  * only BSDF values and emitted radiance are differentiated. */
 int  mpt_render_adjoint(mpt_scene* scene, const mpt_render_params* params, const float* d_image, float* d_tex,
                         float* d_light);
+
+/* ---- batched rendering (no counterpart in the reference) -----------------------------
+ * Renders many scenes (each with its own camera, film, spp and max-depth) in
+ * one streaming wavefront: the scenes' BVHs and tables are merged into one
+ * device scene, path slots are refilled from a global work counter over
+ * (scene, sample, pixel), and the bounce loop runs as a captured HIP graph.
+ * Each sample is the same path that mpt_render draws for that scene with the
+ * same seed; films differ from mpt_render's only by floating-point summation
+ * order. The scenes must stay alive while the batch exists. */
+typedef struct mpt_batch mpt_batch;
+
+typedef struct {
+    int32_t width;             /* BVH width, 4 or 8 */
+    int32_t stack_lds;         /* per-lane LDS stack entries: 16, 24 or 32 */
+    int32_t block;             /* traversal threads per block: 128 or 256 */
+    int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..22) */
+    int32_t graph_steps;       /* wavefront steps per graph launch (even, 2..64) */
+} mpt_batch_opts;
+
+typedef struct {
+    mpt_batch_opts opts;
+    double ms;                 /* timed trial (HIP events) */
+} mpt_batch_trial;
+
+int  mpt_batch_create(mpt_ctx* ctx, mpt_scene* const* scenes, int32_t n, const mpt_batch_opts* opts, mpt_batch** out);
+void mpt_batch_destroy(mpt_batch* batch);
+/* seeds: one per scene (NULL = all 0); spp_override > 0 renders that many
+ * samples per pixel of every scene instead of each scene's own. */
+int  mpt_batch_render(mpt_batch* batch, const uint64_t* seeds, int32_t spp_override, mpt_stats* stats);
+/* Scene i's linear film (W x H x 4: sum w*rgb, sum w). */
+int  mpt_batch_film(mpt_batch* batch, int32_t i, float* film_host);
+/* Autotuner: times trial renders (spp_override = trial_spp) over launch shapes
+ * and path-slot counts by coordinate descent, keeps the fastest in the batch
+ * and returns it; trials (up to max_trials) receives every timing. */
+int  mpt_batch_autotune(mpt_batch* batch, const uint64_t* seeds, int32_t trial_spp, mpt_batch_opts* best,
+                        mpt_batch_trial* trials, int32_t max_trials, int32_t* ntrials);
+int  mpt_batch_get_opts(const mpt_batch* batch, mpt_batch_opts* out);
+int  mpt_batch_set_opts(mpt_batch* batch, const mpt_batch_opts* opts);
 
 #ifdef __cplusplus
 }
