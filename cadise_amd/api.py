@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import native
-from .native import (MPT_RENDER_KEEP_SAMPLES, BatchOpts, BatchTrial, BvhOpts, LightDesc, MaterialDesc, MptError,
+from .native import (MPT_RENDER_KEEP_SAMPLES, MPT_RENDER_PASSWISE, BatchOpts, BatchTrial, BvhOpts, LightDesc, MaterialDesc, MptError,
                      RenderParams, SceneDesc, SceneStats, Stats, Texture)
 
 _FILTERS = {"box": 0, "gaussian": 1, "cone": 2, "mitchell": 3}
@@ -204,9 +204,9 @@ class Scene:
 
     # -- Renderer -----------------------------------------------------------------------
     def render(self, seed: int = 0, spp_begin: int = 0, spp_end: int = 0, max_depth: int = 0,
-               accumulate: bool = False, readback: bool = True, keep_samples: bool = False):
+               accumulate: bool = False, readback: bool = True, keep_samples: bool = False, passwise: bool = False):
         p = RenderParams(seed, spp_begin, spp_end, max_depth, int(accumulate),
-                         MPT_RENDER_KEEP_SAMPLES if keep_samples else 0)
+                         (MPT_RENDER_KEEP_SAMPLES if keep_samples else 0) | (MPT_RENDER_PASSWISE if passwise else 0))
         film = np.zeros((self.height, self.width, 4), np.float32) if readback else None
         self.ctx.check(self.L.mpt_render(self.h, C.byref(p), film.ctypes.data if readback else None))
         return film
@@ -263,7 +263,7 @@ class Scene:
         self.ctx.check(self.L.mpt_film_save(self.h, path.encode()))
 
 
-def batch_opts(width: int = 4, stack_lds: int = 24, block: int = 256, capacity_log2: int = 20,
+def batch_opts(width: int = 4, stack_lds: int = 24, block: int = 128, capacity_log2: int = 21,
                graph_steps: int = 16) -> BatchOpts:
     return BatchOpts(width, stack_lds, block, capacity_log2, graph_steps)
 

@@ -65,27 +65,13 @@ struct mpt_batch {
     int64_t nnodes[2] = {0, 0};
     int32_t max_stack[2] = {0, 0};
     std::vector<int32_t> roots[2];
-    mpt::DevBuf tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, views, film;
+    mpt::DevBuf tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, film;
     mpt::DevScene sc{};
     std::vector<mpt::BatchView> hviews;
     int64_t nprims = 0, total_px = 0;
-    // streaming state
-    mpt::DevBuf beta, rad, fpos, meta, free_ids, ctr, work_next, qcount, stats, spill;
-    mpt::DevBuf qbuf[2][12], rootq[2];
-    int32_t capacity_alloc = 0;
-    int64_t spill_lanes = 0;
-    // recorded graph and the shape it was recorded for
-    hipGraphExec_t exec = nullptr;
-    mpt_batch_opts exec_opts{};
-    int64_t exec_total = -1;
-    int32_t* done_host = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    mpt::Streamer st;
     mpt_stats rstats{};
     ~mpt_batch() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        for (auto e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (done_host) (void)hipHostFree(done_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -97,8 +83,8 @@ using namespace mpt;
 void normalize_opts(mpt_batch_opts& o) {
     if (o.width != 8) o.width = 4;
     if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
-    if (o.block != 128) o.block = 256;
-    if (o.capacity_log2 < 16 || o.capacity_log2 > 22) o.capacity_log2 = 20;
+    if (o.block != 256) o.block = 128;
+    if (o.capacity_log2 < 16 || o.capacity_log2 > 23) o.capacity_log2 = 21;
     if (o.graph_steps < 2 || o.graph_steps > 64) o.graph_steps = 16;
     o.graph_steps &= ~1;
 }
@@ -290,40 +276,54 @@ int merge_tables(mpt_batch* B) {
 }
 
 // Streaming buffers for `cap` path slots.
-int ensure_stream_buffers(mpt_batch* B, int32_t cap) {
-    mpt_ctx* ctx = B->ctx;
-    if (cap == B->capacity_alloc) return MPT_OK;
+int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
+    if (!S.done_host) {
+        HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&S.done_host), sizeof(int32_t), hipHostMallocDefault));
+        *S.done_host = 0;
+    }
+    if (cap <= S.capacity_alloc) return MPT_OK;   // larger buffers serve smaller jobs
+    if (S.exec) {   // the recorded graph holds the old buffers
+        (void)hipGraphExecDestroy(S.exec);
+        S.exec = nullptr;
+    }
     const size_t n = static_cast<size_t>(cap);
-    HIP_TRY(ctx, B->beta.alloc(sizeof(float4) * n));
-    HIP_TRY(ctx, B->rad.alloc(sizeof(float4) * n));
-    HIP_TRY(ctx, B->fpos.alloc(sizeof(float2) * n));
-    HIP_TRY(ctx, B->meta.alloc(sizeof(int4) * n));
-    HIP_TRY(ctx, B->free_ids.alloc(sizeof(int32_t) * n));
+    HIP_TRY(ctx, S.beta.alloc(sizeof(float4) * n));
+    HIP_TRY(ctx, S.rad.alloc(sizeof(float4) * n));
+    HIP_TRY(ctx, S.fpos.alloc(sizeof(float2) * n));
+    HIP_TRY(ctx, S.meta.alloc(sizeof(int4) * n));
+    HIP_TRY(ctx, S.free_ids.alloc(sizeof(int32_t) * n));
     const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
     for (int q = 0; q < 2; ++q) {
-        for (int k = 0; k < 12; ++k) HIP_TRY(ctx, B->qbuf[q][k].alloc(sizes[k] * n));
-        HIP_TRY(ctx, B->rootq[q].alloc(sizeof(int32_t) * n));
+        for (int k = 0; k < 12; ++k) HIP_TRY(ctx, S.qbuf[q][k].alloc(sizes[k] * n));
+        HIP_TRY(ctx, S.rootq[q].alloc(sizeof(int32_t) * n));
     }
-    if (!B->ctr.p) {
-        HIP_TRY(ctx, B->ctr.alloc(sizeof(int32_t) * 4));
-        HIP_TRY(ctx, B->work_next.alloc(sizeof(unsigned long long)));
-        HIP_TRY(ctx, B->qcount.alloc(sizeof(int32_t) * 2));
-        HIP_TRY(ctx, B->stats.alloc(sizeof(unsigned long long) * ST_COUNT));
+    if (!S.ctr.p) {
+        HIP_TRY(ctx, S.ctr.alloc(sizeof(int32_t) * 4));
+        HIP_TRY(ctx, S.work_next.alloc(sizeof(unsigned long long)));
+        HIP_TRY(ctx, S.qcount.alloc(sizeof(int32_t) * 2));
+        HIP_TRY(ctx, S.stats.alloc(sizeof(unsigned long long) * ST_COUNT));
     }
-    B->capacity_alloc = cap;
+    S.capacity_alloc = cap;
     return MPT_OK;
 }
 
-int trace_blocks_for(const mpt_batch* B, int64_t nrays, int block) {
-    const int64_t want = (nrays + block - 1) / block;
-    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(B->ctx->num_cus) * 6)));
-}
+enum { SK_REGEN = 0, SK_CLOSEST, SK_ANY, SK_SHADE, SK_KINDS };
 
-// One wavefront step on queue qc (enqueue only; the caller records it into a graph).
-void enqueue_step(mpt_batch* B, const TraceCfg& cfg, int tblocks, int sblocks, int rblocks, PassArgs& a, BatchArgs& bt,
-                  int qc) {
-    hipLaunchKernelGGL(k_bregen, dim3(rblocks), dim3(256), 0, B->stream, a, bt, qc);
-    hipLaunchKernelGGL(k_bcommit, dim3(1), dim3(64), 0, B->stream, a, bt, qc);
+struct StepShape {
+    TraceCfg cfg;
+    int tblocks, sblocks, rblocks;
+};
+
+// One wavefront step on queue qc. With events (2 per kernel kind), each kind's
+// launches are bracketed for timing.
+int enqueue_step(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StepShape& sh, PassArgs& a, BatchArgs& bt,
+                 const AdjArgs* g, int qc, hipEvent_t* ev) {
+    auto rec = [&](int k, int e) -> hipError_t { return ev ? hipEventRecord(ev[2 * k + e], stream) : hipSuccess; };
+    HIP_TRY(ctx, rec(SK_REGEN, 0));
+    if (g) hipLaunchKernelGGL(k_bregen<true>, dim3(sh.rblocks), dim3(256), 0, stream, a, bt, *g, qc);
+    else hipLaunchKernelGGL(k_bregen<false>, dim3(sh.rblocks), dim3(256), 0, stream, a, bt, AdjArgs{}, qc);
+    hipLaunchKernelGGL(k_bcommit, dim3(1), dim3(64), 0, stream, a, bt, qc);
+    HIP_TRY(ctx, rec(SK_REGEN, 1));
     const Queue& Q = a.q[qc];
     TraceArgs t{};
     t.sc = a.sc;
@@ -336,31 +336,239 @@ void enqueue_step(mpt_batch* B, const TraceCfg& cfg, int tblocks, int sblocks, i
     t.hit = Q.hit;
     t.hit2 = Q.mis_hit;
     t.want_prim = 0;
-    t.root = B->rootq[qc].as<int32_t>();
+    t.root = S.rootq[qc].as<int32_t>();
     t.work = bt.ctr + 2;
-    t.spill = a.sc.max_stack > cfg.stack ? B->spill.as<int32_t>() : nullptr;
+    t.spill = a.sc.max_stack > sh.cfg.stack ? S.spill.as<int32_t>() : nullptr;
     t.stats = bt.stats;
     t.stat_slot = ST_CLOSEST_RAYS;
-    launch_trace_cfg(cfg, t, false, tblocks, B->stream);
+    HIP_TRY(ctx, rec(SK_CLOSEST, 0));
+    launch_trace_cfg(sh.cfg, t, false, sh.tblocks, stream);
+    HIP_TRY(ctx, rec(SK_CLOSEST, 1));
     t.dir = Q.shd_dir;
     t.dual = 0;
     t.hit = Q.occ;
     t.hit2 = nullptr;
     t.work = bt.ctr + 3;
     t.stat_slot = ST_SHADOW_RAYS;
-    launch_trace_cfg(cfg, t, true, tblocks, B->stream);
-    hipLaunchKernelGGL(k_shade<SHADE_BATCH>, dim3(sblocks), dim3(256), 0, B->stream, a, AdjArgs{}, bt, qc);
+    HIP_TRY(ctx, rec(SK_ANY, 0));
+    launch_trace_cfg(sh.cfg, t, true, sh.tblocks, stream);
+    HIP_TRY(ctx, rec(SK_ANY, 1));
+    HIP_TRY(ctx, rec(SK_SHADE, 0));
+    if (g) hipLaunchKernelGGL(k_shade<SHADE_BATCH_ADJ>, dim3(sh.sblocks), dim3(256), 0, stream, a, *g, bt, qc);
+    else hipLaunchKernelGGL(k_shade<SHADE_BATCH>, dim3(sh.sblocks), dim3(256), 0, stream, a, AdjArgs{}, bt, qc);
+    HIP_TRY(ctx, rec(SK_SHADE, 1));
+    HIP_TRY(ctx, hipGetLastError());
+    return MPT_OK;
 }
+
+}  // namespace
+
+namespace mpt {
+
+int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob& job, mpt_stats* out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    const int32_t cap = 1 << job.capacity_log2;
+    if ((rc = ensure_stream_buffers(ctx, S, cap)) != MPT_OK) return rc;
+    // the views buffer keeps its address while its size does (a recorded graph holds it)
+    const size_t vbytes = sizeof(BatchView) * static_cast<size_t>(job.nviews);
+    if (S.views.bytes != vbytes) {
+        HIP_TRY(ctx, S.views.alloc(vbytes));
+        if (S.exec) {
+            (void)hipGraphExecDestroy(S.exec);
+            S.exec = nullptr;
+        }
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(S.views.p, job.views, vbytes, hipMemcpyHostToDevice, stream));
+    StepShape sh;
+    sh.cfg = job.cfg;
+    {
+        const int64_t want = (2 * static_cast<int64_t>(cap) + sh.cfg.block - 1) / sh.cfg.block;
+        sh.tblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(ctx->num_cus) * 6)));
+        sh.sblocks = static_cast<int>(std::min<int64_t>((cap + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
+        sh.rblocks = (cap + 255) / 256;
+        const int64_t extra = job.sc.max_stack - sh.cfg.stack;
+        const int64_t lanes = static_cast<int64_t>(sh.tblocks) * sh.cfg.block;
+        if (extra > 0 && S.spill.bytes < sizeof(int32_t) * static_cast<size_t>(lanes * extra)) {
+            HIP_TRY(ctx, S.spill.alloc(sizeof(int32_t) * static_cast<size_t>(lanes * extra)));
+            S.spill_lanes = lanes;
+            if (S.exec) {
+                (void)hipGraphExecDestroy(S.exec);
+                S.exec = nullptr;
+            }
+        }
+    }
+    PassArgs a{};
+    a.sc = job.sc;
+    a.beta = S.beta.as<float4>();
+    a.rad = S.rad.as<float4>();
+    a.fpos = S.fpos.as<float2>();
+    for (int q = 0; q < 2; ++q) {
+        Queue& Q = a.q[q];
+        Q.path = S.qbuf[q][0].as<int32_t>();
+        Q.org = S.qbuf[q][1].as<float4>();
+        Q.dir = S.qbuf[q][2].as<float4>();
+        Q.mis_dir = S.qbuf[q][3].as<float4>();
+        Q.shd_dir = S.qbuf[q][4].as<float4>();
+        Q.lt = S.qbuf[q][5].as<float4>();
+        Q.fac = S.qbuf[q][6].as<float4>();
+        Q.bv = S.qbuf[q][7].as<float4>();
+        Q.light = S.qbuf[q][8].as<int32_t>();
+        Q.hit = S.qbuf[q][9].as<int2>();
+        Q.mis_hit = S.qbuf[q][10].as<int2>();
+        Q.occ = S.qbuf[q][11].as<int2>();
+    }
+    a.qcount = S.qcount.as<int32_t>();
+    a.npaths = cap;
+    BatchArgs bt{};
+    bt.views = S.views.as<BatchView>();
+    bt.nviews = job.nviews;
+    bt.capacity = cap;
+    bt.total_work = job.total_work;
+    bt.meta = S.meta.as<int4>();
+    bt.root[0] = S.rootq[0].as<int32_t>();
+    bt.root[1] = S.rootq[1].as<int32_t>();
+    bt.film = job.film;
+    bt.keep = job.keep;
+    bt.free_ids = S.free_ids.as<int32_t>();
+    bt.ctr = S.ctr.as<int32_t>();
+    bt.work_next = S.work_next.as<unsigned long long>();
+    bt.stats = S.stats.as<unsigned long long>();
+    const int steps = job.graph_steps;
+    AdjArgs g = job.adj;
+    const AdjArgs* gp = nullptr;
+    if (job.adjoint) {
+        if (job.use_graph) return fail(ctx, MPT_ERR_INVALID, "the adjoint runs without a recorded graph");
+        if (cap > S.adj_alloc) {
+            const size_t sizes[6] = {8, 16, 16, 16, 16, 16};
+            for (int q = 0; q < 2; ++q)
+                for (int k = 0; k < 6; ++k) HIP_TRY(ctx, S.adjq[q][k].alloc(sizes[k] * static_cast<size_t>(cap)));
+            HIP_TRY(ctx, S.adjA.alloc(sizeof(float4) * static_cast<size_t>(cap)));
+            S.adj_alloc = cap;
+        }
+        for (int q = 0; q < 2; ++q) {
+            g.q[q].par = S.adjq[q][0].as<int2>();
+            g.q[q].sL = S.adjq[q][1].as<float4>();
+            g.q[q].eL = S.adjq[q][2].as<float4>();
+            g.q[q].sB = S.adjq[q][3].as<float4>();
+            g.q[q].eB = S.adjq[q][4].as<float4>();
+            g.q[q].sC = S.adjq[q][5].as<float4>();
+        }
+        g.A = S.adjA.as<float4>();
+        bt.film = nullptr;
+        bt.keep = nullptr;
+        gp = &g;
+    }
+
+    if (job.use_graph) {
+        // re-record when anything the graph holds by value changed
+        std::vector<uint8_t> key(sizeof(PassArgs) + sizeof(BatchArgs) + sizeof(StepShape) + sizeof(int));
+        std::memcpy(key.data(), &a, sizeof(PassArgs));
+        std::memcpy(key.data() + sizeof(PassArgs), &bt, sizeof(BatchArgs));
+        std::memcpy(key.data() + sizeof(PassArgs) + sizeof(BatchArgs), &sh, sizeof(StepShape));
+        std::memcpy(key.data() + sizeof(PassArgs) + sizeof(BatchArgs) + sizeof(StepShape), &steps, sizeof(int));
+        if (!S.exec || key != S.exec_key) {
+            if (S.exec) (void)hipGraphExecDestroy(S.exec);
+            S.exec = nullptr;
+            hipGraph_t graph = nullptr;
+            HIP_TRY(ctx, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            int erc = MPT_OK;
+            for (int k = 0; k < steps && erc == MPT_OK; ++k)
+                erc = enqueue_step(ctx, stream, S, sh, a, bt, nullptr, k & 1, nullptr);
+            const hipError_t ec = hipStreamEndCapture(stream, &graph);
+            if (erc != MPT_OK) return erc;
+            if (ec != hipSuccess) return fail(ctx, MPT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&S.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return fail(ctx, MPT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+            S.exec_key = key;
+        }
+    }
+    const size_t nev = static_cast<size_t>(2 + (job.timing ? 2 * SK_KINDS * steps : 0));
+    while (S.events.size() < nev) {
+        hipEvent_t e;
+        HIP_TRY(ctx, hipEventCreate(&e));
+        S.events.push_back(e);
+    }
+    // reset: free stack, counters, film
+    hipLaunchKernelGGL(k_batch_init, dim3((cap + 255) / 256), dim3(256), 0, stream, bt.free_ids, cap, bt.ctr, a.qcount,
+                       bt.work_next, bt.stats);
+    if (job.clear_film && !job.adjoint)
+        hipLaunchKernelGGL(k_zero_f4, dim3((job.film_px + 255) / 256), dim3(256), 0, stream, job.film, job.film_px);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(S.events[0], stream));
+    // every path in flight finishes within max-depth + 2 steps, after which its
+    // slot is refilled: ceil(work / cap) + 1 such windows bound the step count
+    const int64_t bound = (job.total_work / cap + 2) * static_cast<int64_t>(job.max_depth + 2) + 2;
+    const int64_t max_chunks = 2 * (bound / steps + 2);
+    int64_t chunks = 0;
+    double ms[SK_KINDS] = {0, 0, 0, 0};
+    while (true) {
+        if (job.use_graph) {
+            HIP_TRY(ctx, hipGraphLaunch(S.exec, stream));
+        } else {
+            for (int k = 0; k < steps; ++k) {
+                hipEvent_t* ev = job.timing ? &S.events[2 + 2 * SK_KINDS * k] : nullptr;
+                if ((rc = enqueue_step(ctx, stream, S, sh, a, bt, gp, k & 1, ev)) != MPT_OK) return rc;
+            }
+        }
+        HIP_TRY(ctx, hipMemcpyAsync(S.done_host, bt.ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(ctx, hipStreamSynchronize(stream));
+        ++chunks;
+        if (job.timing && !job.use_graph) {
+            for (int k = 0; k < steps; ++k)
+                for (int kind = 0; kind < SK_KINDS; ++kind) {
+                    float t = 0.f;
+                    const size_t e = 2 + 2 * SK_KINDS * static_cast<size_t>(k) + 2 * kind;
+                    HIP_TRY(ctx, hipEventElapsedTime(&t, S.events[e], S.events[e + 1]));
+                    ms[kind] += t;
+                }
+        }
+        if (*S.done_host) break;
+        if (chunks > max_chunks) return fail(ctx, MPT_ERR_DEVICE, "streaming renderer did not drain");
+    }
+    HIP_TRY(ctx, hipEventRecord(S.events[1], stream));
+    HIP_TRY(ctx, hipEventSynchronize(S.events[1]));
+    float gms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&gms, S.events[0], S.events[1]));
+    unsigned long long st[ST_COUNT];
+    HIP_TRY(ctx, hipMemcpy(st, S.stats.p, sizeof(st), hipMemcpyDeviceToHost));
+    mpt_stats r{};
+    r.samples = job.total_work;
+    for (int32_t i = 0; i < job.nviews; ++i)
+        if (job.views[i].max_depth > 0)
+            r.camera_rays += static_cast<int64_t>(job.views[i].img_w) * job.views[i].img_h * job.views[i].nsamples;
+    r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
+    r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
+    r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
+    r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
+    r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
+    r.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
+    r.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
+    r.closest_launches = chunks * steps;
+    r.anyhit_launches = chunks * steps;
+    r.ms_camera = ms[SK_REGEN];
+    r.ms_closest = ms[SK_CLOSEST];
+    r.ms_anyhit = ms[SK_ANY];
+    r.ms_shade = ms[SK_SHADE];
+    r.ms_gpu = gms;
+    r.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out) *out = r;
+    if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    return MPT_OK;
+}
+
+}  // namespace mpt
+
+namespace {
 
 int do_batch_render(mpt_batch* B, const uint64_t* seeds, int32_t spp_override, double* gpu_ms) {
     mpt_ctx* ctx = B->ctx;
-    const auto t0 = std::chrono::steady_clock::now();
     mpt_batch_opts& o = B->opts;
     normalize_opts(o);
     const int wi = o.width == 8 ? 1 : 0;
     int rc;
     if ((rc = merge_nodes(B, wi)) != MPT_OK) return rc;
-    // views for this call: roots of the chosen width, seeds, sample counts
     int64_t work = 0;
     int32_t maxd = 0;
     for (size_t i = 0; i < B->scenes.size(); ++i) {
@@ -372,139 +580,33 @@ int do_batch_render(mpt_batch* B, const uint64_t* seeds, int32_t spp_override, d
         v.key_hi = static_cast<uint32_t>(sd >> 32);
         v.spp = spp_override > 0 ? spp_override : h.spp;
         v.strata = strata_base(v.spp);
+        v.sample_base = 0;
+        v.nsamples = v.spp;
         v.work_base = work;
-        work += static_cast<int64_t>(h.width) * h.height * v.spp;
+        work += static_cast<int64_t>(h.width) * h.height * v.nsamples;
         maxd = std::max(maxd, h.max_depth);
     }
-    // the views buffer keeps its address (the recorded graph holds it)
-    const size_t vbytes = sizeof(BatchView) * B->hviews.size();
-    if (B->views.bytes != vbytes) HIP_TRY(ctx, B->views.alloc(vbytes));
-    HIP_TRY(ctx, hipMemcpy(B->views.p, B->hviews.data(), vbytes, hipMemcpyHostToDevice));
-    DevScene& d = B->sc;
-    d.nodes = B->nodes[wi].p;
-    d.width = o.width;
-    d.nnodes = B->nnodes[wi];
-    d.max_stack = B->max_stack[wi];
-    TraceCfg cfg;
-    cfg.width = o.width;
-    cfg.stack = o.stack_lds;
-    cfg.block = o.block;
-    const int32_t cap = 1 << o.capacity_log2;
-    if ((rc = ensure_stream_buffers(B, cap)) != MPT_OK) return rc;
-    const int tblocks = trace_blocks_for(B, 2 * static_cast<int64_t>(cap), cfg.block);
-    const int sblocks = static_cast<int>(std::min<int64_t>((cap + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
-    const int rblocks = (cap + 255) / 256;
-    {
-        const int64_t extra = d.max_stack - cfg.stack;
-        const int64_t lanes = static_cast<int64_t>(tblocks) * cfg.block;
-        if (extra > 0 && (lanes > B->spill_lanes || B->spill.bytes < sizeof(int32_t) * static_cast<size_t>(lanes * extra))) {
-            HIP_TRY(ctx, B->spill.alloc(sizeof(int32_t) * static_cast<size_t>(lanes * extra)));
-            B->spill_lanes = lanes;
-            if (B->exec) {   // the recorded graph holds the old spill pointer
-                (void)hipGraphExecDestroy(B->exec);
-                B->exec = nullptr;
-            }
-        }
-    }
-    PassArgs a{};
-    a.sc = d;
-    a.beta = B->beta.as<float4>();
-    a.rad = B->rad.as<float4>();
-    a.fpos = B->fpos.as<float2>();
-    for (int q = 0; q < 2; ++q) {
-        Queue& Q = a.q[q];
-        Q.path = B->qbuf[q][0].as<int32_t>();
-        Q.org = B->qbuf[q][1].as<float4>();
-        Q.dir = B->qbuf[q][2].as<float4>();
-        Q.mis_dir = B->qbuf[q][3].as<float4>();
-        Q.shd_dir = B->qbuf[q][4].as<float4>();
-        Q.lt = B->qbuf[q][5].as<float4>();
-        Q.fac = B->qbuf[q][6].as<float4>();
-        Q.bv = B->qbuf[q][7].as<float4>();
-        Q.light = B->qbuf[q][8].as<int32_t>();
-        Q.hit = B->qbuf[q][9].as<int2>();
-        Q.mis_hit = B->qbuf[q][10].as<int2>();
-        Q.occ = B->qbuf[q][11].as<int2>();
-    }
-    a.qcount = B->qcount.as<int32_t>();
-    a.npaths = cap;
-    BatchArgs bt{};
-    bt.views = B->views.as<BatchView>();
-    bt.nviews = static_cast<int32_t>(B->scenes.size());
-    bt.capacity = cap;
-    bt.total_work = work;
-    bt.meta = B->meta.as<int4>();
-    bt.root[0] = B->rootq[0].as<int32_t>();
-    bt.root[1] = B->rootq[1].as<int32_t>();
-    bt.film = B->film.as<float4>();
-    bt.free_ids = B->free_ids.as<int32_t>();
-    bt.ctr = B->ctr.as<int32_t>();
-    bt.work_next = B->work_next.as<unsigned long long>();
-    bt.stats = B->stats.as<unsigned long long>();
-
-    // record the graph for this shape (pointers and total work are baked in)
-    const bool same = B->exec && B->exec_total == work && std::memcmp(&B->exec_opts, &o, sizeof(o)) == 0 &&
-                      B->views.p == reinterpret_cast<const void*>(bt.views);
-    if (!same) {
-        if (B->exec) (void)hipGraphExecDestroy(B->exec);
-        B->exec = nullptr;
-        hipGraph_t graph = nullptr;
-        HIP_TRY(ctx, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < o.graph_steps; ++k) enqueue_step(B, cfg, tblocks, sblocks, rblocks, a, bt, k & 1);
-        const hipError_t ec = hipStreamEndCapture(B->stream, &graph);
-        if (ec != hipSuccess) return fail(ctx, MPT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
-        const hipError_t ei = hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess) return fail(ctx, MPT_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
-        B->exec_opts = o;
-        B->exec_total = work;
-    }
-    // reset: free stack, counters, film
-    hipLaunchKernelGGL(k_batch_init, dim3((cap + 255) / 256), dim3(256), 0, B->stream, bt.free_ids, cap, bt.ctr, a.qcount,
-                       bt.work_next, bt.stats);
-    hipLaunchKernelGGL(k_zero_f4, dim3((B->total_px + 255) / 256), dim3(256), 0, B->stream, B->film.as<float4>(),
-                       B->total_px);
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipEventRecord(B->ev[0], B->stream));
-    // every path in flight finishes within max-depth + 2 steps, after which its
-    // slot is refilled: ceil(work / cap) + 1 such windows bound the step count
-    const int64_t bound = (work / cap + 2) * static_cast<int64_t>(maxd + 2) + 2;
-    const int64_t max_launches = 2 * (bound / o.graph_steps + 2);
-    int64_t launches = 0;
-    while (true) {
-        HIP_TRY(ctx, hipGraphLaunch(B->exec, B->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(B->done_host, bt.ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, B->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(B->stream));
-        ++launches;
-        if (*B->done_host) break;
-        if (launches > max_launches) return fail(ctx, MPT_ERR_DEVICE, "batched renderer did not drain");
-    }
-    HIP_TRY(ctx, hipEventRecord(B->ev[1], B->stream));
-    HIP_TRY(ctx, hipEventSynchronize(B->ev[1]));
-    float ms = 0.f;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
-    if (gpu_ms) *gpu_ms = ms;
-    unsigned long long st[ST_COUNT];
-    HIP_TRY(ctx, hipMemcpy(st, B->stats.p, sizeof(st), hipMemcpyDeviceToHost));
-    mpt_stats& r = B->rstats;
-    r = mpt_stats{};
-    r.samples = work;
-    r.camera_rays = 0;
-    for (size_t i = 0; i < B->scenes.size(); ++i)
-        if (B->scenes[i]->host.max_depth > 0)
-            r.camera_rays += static_cast<int64_t>(B->hviews[i].img_w) * B->hviews[i].img_h * B->hviews[i].spp;
-    r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
-    r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
-    r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
-    r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
-    r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
-    r.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
-    r.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
-    r.closest_launches = launches * o.graph_steps;
-    r.anyhit_launches = launches * o.graph_steps;
-    r.ms_total = ms_since(t0);
-    r.ms_gpu = ms;
-    if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    StreamJob job;
+    job.sc = B->sc;
+    job.sc.nodes = B->nodes[wi].p;
+    job.sc.width = o.width;
+    job.sc.nnodes = B->nnodes[wi];
+    job.sc.max_stack = B->max_stack[wi];
+    job.views = B->hviews.data();
+    job.nviews = static_cast<int32_t>(B->hviews.size());
+    job.total_work = work;
+    job.film = B->film.as<float4>();
+    job.film_px = B->total_px;
+    job.clear_film = true;
+    job.cfg.width = o.width;
+    job.cfg.stack = o.stack_lds;
+    job.cfg.block = o.block;
+    job.capacity_log2 = o.capacity_log2;
+    job.graph_steps = o.graph_steps;
+    job.use_graph = true;
+    job.max_depth = maxd;
+    if ((rc = stream_render(ctx, B->stream, B->st, job, &B->rstats)) != MPT_OK) return rc;
+    if (gpu_ms) *gpu_ms = B->rstats.ms_gpu;
     return MPT_OK;
 }
 
@@ -525,10 +627,6 @@ int mpt_batch_create(mpt_ctx* ctx, mpt_scene* const* scenes, int32_t n, const mp
     normalize_opts(B->opts);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    HIP_TRY(ctx, hipEventCreate(&B->ev[0]));
-    HIP_TRY(ctx, hipEventCreate(&B->ev[1]));
-    HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&B->done_host), sizeof(int32_t), hipHostMallocDefault));
-    *B->done_host = 0;
     int rc;
     if ((rc = merge_tables(B.get())) != MPT_OK) return rc;
     if ((rc = merge_nodes(B.get(), B->opts.width == 8 ? 1 : 0)) != MPT_OK) return rc;
@@ -597,7 +695,7 @@ int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, m
         for (int dim = 0; dim < 4; ++dim) {
             std::vector<mpt_batch_opts> cands;
             if (dim == 0)
-                for (int c : {18, 19, 20, 21}) { mpt_batch_opts o = cur; o.capacity_log2 = c; cands.push_back(o); }
+                for (int c : {19, 20, 21, 22, 23}) { mpt_batch_opts o = cur; o.capacity_log2 = c; cands.push_back(o); }
             if (dim == 1)
                 for (int w : {4, 8}) { mpt_batch_opts o = cur; o.width = w; cands.push_back(o); }
             if (dim == 2)

@@ -7,6 +7,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -24,7 +25,8 @@ template <bool ADJ>
 __global__ void k_camera(PassArgs a, AdjArgs g);
 template <int MODE>
 __global__ void k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step);
-__global__ void k_bregen(PassArgs a, BatchArgs bt, int32_t qc);
+template <bool ADJ>
+__global__ void k_bregen(PassArgs a, BatchArgs bt, AdjArgs g, int32_t qc);
 __global__ void k_bcommit(PassArgs a, BatchArgs bt, int32_t qc);
 __global__ void k_film(PassArgs a);
 __global__ void k_srgb8(const float4* film, int64_t npix, uint8_t* out);
@@ -61,7 +63,7 @@ inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
     if (o.leaf_max <= 0) o.leaf_max = 2;
     if (o.leaf_max > 64) o.leaf_max = 64;
     if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
-    if (o.block != 128) o.block = 256;
+    if (o.block != 256) o.block = 128;
     TraceCfg c;
     c.width = o.width;
     c.stack = o.stack_lds;
@@ -88,12 +90,51 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Path state of the streaming wavefront (batch.hip): path slots recycled
+// through a free stack, two queues, counters, and the recorded HIP graph.
+struct Streamer {
+    DevBuf beta, rad, fpos, meta, free_ids, ctr, work_next, qcount, stats, spill, views;
+    DevBuf qbuf[2][12], rootq[2];
+    DevBuf adjq[2][6], adjA;          // adjoint jobs
+    int32_t capacity_alloc = 0, adj_alloc = 0;
+    int64_t spill_lanes = 0;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint8_t> exec_key;   // what the recorded graph was recorded for
+    int32_t* done_host = nullptr;    // pinned
+    std::vector<hipEvent_t> events;
+    ~Streamer() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (done_host) (void)hipHostFree(done_host);
+    }
+};
+
+// One streaming render: scenes (views) over one device scene.
+struct StreamJob {
+    DevScene sc;
+    const BatchView* views = nullptr;   // host array of nviews
+    int32_t nviews = 0;
+    int64_t total_work = 0;
+    float4* film = nullptr;
+    int64_t film_px = 0;
+    bool    clear_film = true;
+    float4* keep = nullptr;
+    TraceCfg cfg;
+    int32_t capacity_log2 = 20, graph_steps = 16;
+    bool    use_graph = false;   // record the steps into a HIP graph (else direct launches)
+    bool    timing = false;      // HIP events per kernel kind (direct launches only)
+    int32_t max_depth = 0;       // largest over the views
+    bool    adjoint = false;     // path-replay adjoint: adj.samples / dimg / film / gtex / glight set,
+    AdjArgs adj{};               // film untouched (its weights normalize dLoss/dI)
+};
+
 }  // namespace mpt
 
 struct mpt_ctx {
     int device = 0;
     int num_cus = 0;
     std::string error;
+    std::unique_ptr<mpt::Streamer> streamer;   // path state of single-scene streaming renders
 };
 
 struct mpt_scene {
@@ -153,6 +194,10 @@ inline int fail(mpt_ctx* ctx, int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                                   \
             return fail(ctx, MPT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// The streaming wavefront (batch.hip): runs job on `stream`, fills out's ray
+// counts and times.
+int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& st, const StreamJob& job, mpt_stats* out);
 
 template <typename T>
 inline int upload_vec(mpt_ctx* ctx, DevBuf& b, const std::vector<T>& v) {

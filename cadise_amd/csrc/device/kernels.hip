@@ -360,8 +360,8 @@ __device__ __forceinline__ void batch_splat(const BatchArgs& bt, const BatchView
 // slot to the free stack.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step) {
-    constexpr bool ADJ = MODE == SHADE_ADJ;
-    constexpr bool BATCH = MODE == SHADE_BATCH;
+    constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
+    constexpr bool BATCH = (MODE & SHADE_BATCH) != 0;
     const DevScene& sc = a.sc;
     const int qc = step & 1;
     const Queue& cur = a.q[qc];
@@ -393,6 +393,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         int32_t pick_light = 0;
         float4 beta4 = org;
         int32_t sid = 0, root_next = 0;
+        int64_t sample_idx = 0;   // the forward pass's sample record of this path (adjoint)
         // adjoint: this lane's contributions and the factors of the vertex shaded now
         int32_t tkey = -1, lkey = -1;
         V3 tval = v3(0.f, 0.f, 0.f), lval = v3(0.f, 0.f, 0.f);
@@ -419,6 +420,9 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 lbase = v.light_base;
                 cdf0 = v.cdf_base;
                 root_next = v.root;
+                sample_idx = static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix;
+            } else {
+                sample_idx = p;
             }
             org = cur.org[s];
             const V3 P0 = v3f4(org);
@@ -467,7 +471,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 if constexpr (ADJ) {
                     const float4 sc4 = g.q[qc].sC[s];
                     if (sc4.w > 0.0f) {
-                        const V3 R = v3f4(g.samples[p]) - L;
+                        const V3 R = v3f4(g.samples[sample_idx]) - L;
                         tval = tval + (A * R) * v3f4(sc4);
                     }
                     const int2 pp = g.q[qc].par[s];
@@ -603,9 +607,15 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             if constexpr (BATCH) {
                 if (pending) {
                     bt.meta[p] = make_int4(meta.x, meta.y, meta.z, pb + 1);
+                } else if (ADJ) {
+                    finished = true;
                 } else {
                     finished = true;
-                    batch_splat(bt, bt.views[sid], pix, a.fpos[p], L);
+                    const BatchView& v = bt.views[sid];
+                    batch_splat(bt, v, pix, a.fpos[p], L);
+                    if (bt.keep)
+                        bt.keep[static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix] =
+                            make_float4(L.x, L.y, L.z, 0.f);
                 }
             }
         }
@@ -681,12 +691,35 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
 
 // ---- batched renderer: path regeneration ------------------------------------------------
 
+// The adjoint weight of a sample: A = sum_q dLoss/dI_q w(q, s) / W_q over the
+// sample's tile-clamped footprint; zero for samples the film drops (NaN / inf).
+template <class F>
+__device__ __forceinline__ V3 adjoint_weight(const F& sc, const AdjArgs& g, int64_t film0, int32_t pix, float fx,
+                                             float fy, float4 Ls) {
+    V3 A = v3(0.f, 0.f, 0.f);
+    if (!finite3(Ls)) return A;
+    const int32_t ix = pix % sc.img_w, iy = pix / sc.img_w;
+    const int tx0 = (ix / sc.tile_x) * sc.tile_x, ty0 = (iy / sc.tile_y) * sc.tile_y;
+    const int tx1 = min(tx0 + sc.tile_x, sc.img_w), ty1 = min(ty0 + sc.tile_y, sc.img_h);
+    const int4 fp = footprint(sc, fx, fy, tx0, ty0, tx1, ty1);
+    for (int qy = fp.y; qy < fp.w; ++qy)
+        for (int qx = fp.x; qx < fp.z; ++qx) {
+            const int64_t qi = film0 + static_cast<int64_t>(qy) * sc.img_w + qx;
+            const float W = g.film[qi].w;
+            if (W == 0.0f) continue;
+            const float w = filter_eval(sc, static_cast<float>(qx) - (fx - 0.5f), static_cast<float>(qy) - (fy - 0.5f)) / W;
+            A = A + v3(g.dimg[3 * qi], g.dimg[3 * qi + 1], g.dimg[3 * qi + 2]) * w;
+        }
+    return A;
+}
+
 // New camera paths for the free path slots of a batch, appended to queue qc:
 // n_new = min(free slots, work left, queue room), all threads reading the same
 // counters (k_bcommit applies them afterwards). Work item w of scene s (w in
 // [work_base_s, work_base_s+1)) is sample (w - base) / pixels of pixel
 // (w - base) % pixels, so consecutive lanes trace neighbouring pixels.
-__global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, int32_t qc) {
+template <bool ADJ>
+__global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArgs g, int32_t qc) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int32_t nfree = bt.ctr[0];
     const int64_t next = static_cast<int64_t>(*bt.work_next);
@@ -703,7 +736,7 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, int32_
     const BatchView& v = bt.views[lo];
     const int64_t local = w - v.work_base;
     const int64_t npx = static_cast<int64_t>(v.img_w) * v.img_h;
-    const int32_t sample = static_cast<int32_t>(local / npx), pix = static_cast<int32_t>(local % npx);
+    const int32_t sample = v.sample_base + static_cast<int32_t>(local / npx), pix = static_cast<int32_t>(local % npx);
     const int32_t p = bt.free_ids[nfree - 1 - gid];
     float fx, fy;
     const V3 dir = camera_ray(v, pix, sample, make_uint2(v.key_lo, v.key_hi), fx, fy);
@@ -720,6 +753,13 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, int32_
     q.shd_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
     q.bv[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     bt.root[qc][slot] = v.root;
+    if constexpr (ADJ) {
+        const int64_t si = static_cast<int64_t>(sample - v.sample_base) * npx + pix;
+        const V3 A = adjoint_weight(v, g, v.film_base, pix, fx, fy, g.samples[si]);
+        g.A[p] = make_float4(A.x, A.y, A.z, 0.f);
+        g.q[qc].par[slot] = make_int2(-1, 0);
+        g.q[qc].sC[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
 // Applies k_bregen's counts (one thread): slots taken, work consumed, queue qc
@@ -745,6 +785,9 @@ template __global__ void k_camera<true>(PassArgs, AdjArgs);
 template __global__ void k_shade<SHADE_FWD>(PassArgs, AdjArgs, BatchArgs, int32_t);
 template __global__ void k_shade<SHADE_ADJ>(PassArgs, AdjArgs, BatchArgs, int32_t);
 template __global__ void k_shade<SHADE_BATCH>(PassArgs, AdjArgs, BatchArgs, int32_t);
+template __global__ void k_shade<SHADE_BATCH_ADJ>(PassArgs, AdjArgs, BatchArgs, int32_t);
+template __global__ void k_bregen<false>(PassArgs, BatchArgs, AdjArgs, int32_t);
+template __global__ void k_bregen<true>(PassArgs, BatchArgs, AdjArgs, int32_t);
 
 
 // ---- film ------------------------------------------------------------------------

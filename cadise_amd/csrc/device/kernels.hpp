@@ -80,7 +80,8 @@ struct AdjQueue {
 
 struct AdjArgs {
     AdjQueue      q[2];
-    const float4* samples;  // each path's radiance in this pass, as the forward pass computed it
+    const float4* samples;  // the forward pass's radiance per sample: [pixel] of this pass (pass by
+                            // pass) or [sample - base][pixel] (streaming)
     float4*       A;        // per-path adjoint weight dLoss / dL_sample
     const float*  dimg;     // dLoss / dI, npix x 3
     const float4* film;     // the forward film (sum w per pixel)
@@ -96,7 +97,8 @@ struct BatchView {
     int32_t img_w, img_h, tile_x, tile_y;
     int32_t filter;
     float   filter_hw, filter_hh, g_expx, g_expy, g_edge, amplitude, mitchell_b, mitchell_c;
-    int32_t sampler, spp, strata, max_depth;
+    int32_t sampler, spp, strata, max_depth;   // spp / strata: the sampler's (stratification)
+    int32_t sample_base, nsamples;             // samples [sample_base, sample_base + nsamples) are rendered
     uint32_t key_lo, key_hi;     // Philox key (the scene's seed)
     int64_t film_base;           // first pixel of the scene in the merged film
     int64_t work_base;           // first work item of the scene (items: sample-major pixel x sample)
@@ -113,15 +115,16 @@ struct BatchArgs {
     int4*    meta;               // per path slot: scene, pixel, sample, bounce
     int32_t* root[2];            // per queue entry: BVH root of the entry's scene
     float4*  film;               // merged linear film (sum w*rgb, sum w)
+    float4*  keep;               // one scene: each sample's radiance [sample - base][pixel], or null
     int32_t* free_ids;           // stack of free path slots
     int32_t* ctr;                // [0] free count, [1] done flag, [2..3] trace work counters
     unsigned long long* work_next;
     unsigned long long* stats;
 };
 
-// Modes of the shading kernel: one scene's sample pass, its path-replay
-// adjoint, or the batched renderer's mixed-bounce step.
-enum { SHADE_FWD = 0, SHADE_ADJ = 1, SHADE_BATCH = 2 };
+// Modes of the shading kernel (bit 0: path-replay adjoint, bit 1: streaming
+// wavefront of mixed-bounce paths; else one sample pass of one scene).
+enum { SHADE_FWD = 0, SHADE_ADJ = 1, SHADE_BATCH = 2, SHADE_BATCH_ADJ = 3 };
 
 struct PassArgs {
     DevScene sc;

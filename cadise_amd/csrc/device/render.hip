@@ -381,7 +381,7 @@ PassArgs pass_args(mpt_scene* S, uint64_t seed, int32_t max_depth) {
 
 enum EvKind { EV_CLOSEST = 0, EV_ANY, EV_SHADE, EV_CAMERA, EV_FILM, EV_KINDS };
 
-int do_render(mpt_scene* S, const mpt_render_params* p) {
+int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
     mpt_ctx* ctx = S->ctx;
     const auto t0 = std::chrono::steady_clock::now();
     const HostScene& h = S->host;
@@ -517,9 +517,99 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     return MPT_OK;
 }
 
+// The scene as the one view of a streaming render: samples
+// [spp_begin, spp_begin + nsamples) with the given seed and depth.
+BatchView scene_view(const mpt_scene* S, uint64_t seed, int32_t spp_begin, int32_t nsamples, int32_t max_depth) {
+    BatchView v{};
+    const DevScene& d = S->sc;
+    v.root = 0;
+    v.light_base = 0;
+    v.nlights = d.nlights;
+    v.cluster = d.cluster;
+    v.cdf_base = 0;
+    v.img_w = d.img_w;
+    v.img_h = d.img_h;
+    v.tile_x = d.tile_x;
+    v.tile_y = d.tile_y;
+    v.filter = d.filter;
+    v.filter_hw = d.filter_hw;
+    v.filter_hh = d.filter_hh;
+    v.g_expx = d.g_expx;
+    v.g_expy = d.g_expy;
+    v.g_edge = d.g_edge;
+    v.amplitude = d.amplitude;
+    v.mitchell_b = d.mitchell_b;
+    v.mitchell_c = d.mitchell_c;
+    v.sampler = d.sampler;
+    v.spp = d.spp;
+    v.strata = d.strata;
+    v.max_depth = max_depth;
+    v.sample_base = spp_begin;
+    v.nsamples = nsamples;
+    v.key_lo = static_cast<uint32_t>(seed);
+    v.key_hi = static_cast<uint32_t>(seed >> 32);
+    v.film_base = 0;
+    v.work_base = 0;
+    std::memcpy(v.f2c, d.f2c, sizeof(v.f2c));
+    std::memcpy(v.c2w, d.c2w, sizeof(v.c2w));
+    std::memcpy(v.cam_pos, d.cam_pos, sizeof(v.cam_pos));
+    return v;
+}
+
+// The default render: the scene's sample passes [begin, end) as one streaming
+// wavefront (batch.hip) over the context's path slots.
+int do_render(mpt_scene* S, const mpt_render_params* p) {
+    if (p && (p->flags & MPT_RENDER_PASSWISE)) return do_render_passes(S, p);
+    mpt_ctx* ctx = S->ctx;
+    const HostScene& h = S->host;
+    const int32_t max_depth = (p && p->max_depth > 0) ? p->max_depth : h.max_depth;
+    const int32_t spp_begin = p ? std::max(0, p->spp_begin) : 0;
+    const int32_t spp_end = (p && p->spp_end > 0) ? p->spp_end : h.spp;
+    const uint64_t seed = p ? p->seed : 0;
+    const int64_t npix = static_cast<int64_t>(h.width) * h.height;
+    if (max_depth > 4096) return fail(ctx, MPT_ERR_INVALID, "max-depth above 4096 is not supported on the device");
+    const bool keep = p && (p->flags & MPT_RENDER_KEEP_SAMPLES);
+    const int32_t nsamples = std::max(0, spp_end - spp_begin);
+    S->keep_valid = false;
+    if (keep) {
+        const size_t want = sizeof(float4) * static_cast<size_t>(npix) * static_cast<size_t>(std::max(1, nsamples));
+        if (S->samples.bytes < want) HIP_TRY(ctx, S->samples.alloc(want));
+    }
+    if (!ctx->streamer) ctx->streamer.reset(new Streamer());
+    const BatchView v = scene_view(S, seed, spp_begin, nsamples, max_depth);
+    StreamJob job;
+    job.sc = S->sc;
+    job.views = &v;
+    job.nviews = 1;
+    job.total_work = npix * nsamples;
+    job.film = S->film.as<float4>();
+    job.film_px = npix;
+    job.clear_film = !(p && p->accumulate);
+    job.keep = keep ? S->samples.as<float4>() : nullptr;
+    job.cfg = S->cfg;
+    int lg = 16;
+    while (lg < 22 && (int64_t{1} << lg) < job.total_work) ++lg;   // 4M slots fill the chip
+    job.capacity_log2 = lg;
+    job.graph_steps = 16;
+    job.use_graph = false;
+    job.timing = true;
+    job.max_depth = max_depth;
+    const int rc = stream_render(ctx, S->stream, *ctx->streamer, job, &S->rstats);
+    if (rc != MPT_OK) return rc;
+    if (keep && !(p->accumulate)) {
+        S->keep_valid = true;
+        S->keep_seed = seed;
+        S->keep_begin = spp_begin;
+        S->keep_end = spp_end;
+        S->keep_depth = max_depth;
+        S->keep_version = S->param_version;
+    }
+    return MPT_OK;
+}
+
 // Path-replay adjoint: the forward pass's kernels in their ADJ form over the
 // same passes, reading the kept per-sample radiance and the film's weights.
-int do_adjoint(mpt_scene* S, const mpt_render_params* p, const float* d_image, float* d_tex, float* d_light) {
+int do_adjoint_passes(mpt_scene* S, const mpt_render_params* p, const float* d_image, float* d_tex, float* d_light) {
     mpt_ctx* ctx = S->ctx;
     const auto t0 = std::chrono::steady_clock::now();
     const HostScene& h = S->host;
@@ -635,6 +725,82 @@ int do_adjoint(mpt_scene* S, const mpt_render_params* p, const float* d_image, f
     r.ms_total = ms_since(t0);
     r.ms_adjoint = adj_ms;
     if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
+    return MPT_OK;
+}
+
+// Path-replay adjoint on the streaming wavefront: the forward render's paths
+// (same seed, passes, depth) regenerated into path slots, each carrying its
+// adjoint weight, shaded by k_shade<SHADE_BATCH_ADJ>.
+int do_adjoint(mpt_scene* S, const mpt_render_params* p, const float* d_image, float* d_tex, float* d_light) {
+    if (p && (p->flags & MPT_RENDER_PASSWISE)) return do_adjoint_passes(S, p, d_image, d_tex, d_light);
+    mpt_ctx* ctx = S->ctx;
+    const auto t0 = std::chrono::steady_clock::now();
+    const HostScene& h = S->host;
+    const int32_t max_depth = (p && p->max_depth > 0) ? p->max_depth : h.max_depth;
+    const int32_t spp_begin = p ? std::max(0, p->spp_begin) : 0;
+    const int32_t spp_end = (p && p->spp_end > 0) ? p->spp_end : h.spp;
+    const uint64_t seed = p ? p->seed : 0;
+    const int64_t npix = static_cast<int64_t>(h.width) * h.height;
+    const int32_t nsamples = std::max(0, spp_end - spp_begin);
+    if (S->tex_elems >= (int64_t{1} << 30)) return fail(ctx, MPT_ERR_UNSUPPORTED, "textures above 2^30 elements");
+    int rc;
+    if (!(S->keep_valid && S->keep_seed == seed && S->keep_begin == spp_begin && S->keep_end == spp_end &&
+          S->keep_depth == max_depth && S->keep_version == S->param_version)) {
+        mpt_render_params fp{seed, spp_begin, spp_end, max_depth, 0, MPT_RENDER_KEEP_SAMPLES};
+        if ((rc = do_render(S, &fp)) != MPT_OK) return rc;
+    }
+    const mpt_stats fwd = S->rstats;
+    if (S->dimg.bytes < sizeof(float) * 3 * static_cast<size_t>(npix))
+        HIP_TRY(ctx, S->dimg.alloc(sizeof(float) * 3 * static_cast<size_t>(npix)));
+    HIP_TRY(ctx, S->gtex.alloc(sizeof(float) * static_cast<size_t>(std::max<int64_t>(1, S->tex_elems))));
+    HIP_TRY(ctx, S->glight.alloc(sizeof(float) * 3 * static_cast<size_t>(std::max<size_t>(1, h.lights.size()))));
+    HIP_TRY(ctx, hipMemcpyAsync(S->dimg.p, d_image, sizeof(float) * 3 * static_cast<size_t>(npix), hipMemcpyHostToDevice,
+                                S->stream));
+    {
+        const int64_t nt = std::max<int64_t>(1, S->tex_elems);
+        hipLaunchKernelGGL(k_zero_i32, dim3((nt + 255) / 256), dim3(256), 0, S->stream, S->gtex.as<int32_t>(),
+                           static_cast<int>(nt));
+        const int nl = static_cast<int>(3 * std::max<size_t>(1, h.lights.size()));
+        hipLaunchKernelGGL(k_zero_i32, dim3((nl + 255) / 256), dim3(256), 0, S->stream, S->glight.as<int32_t>(), nl);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (!ctx->streamer) ctx->streamer.reset(new Streamer());
+    const BatchView v = scene_view(S, seed, spp_begin, nsamples, max_depth);
+    StreamJob job;
+    job.sc = S->sc;
+    job.views = &v;
+    job.nviews = 1;
+    job.total_work = npix * nsamples;
+    job.film = S->film.as<float4>();
+    job.film_px = npix;
+    job.clear_film = false;
+    job.cfg = S->cfg;
+    int lg = 16;
+    while (lg < 22 && (int64_t{1} << lg) < job.total_work) ++lg;
+    job.capacity_log2 = lg;
+    job.graph_steps = 16;
+    job.use_graph = false;
+    job.timing = true;
+    job.max_depth = max_depth;
+    job.adjoint = true;
+    job.adj.samples = S->samples.as<float4>();
+    job.adj.dimg = S->dimg.as<float>();
+    job.adj.film = S->film.as<float4>();
+    job.adj.gtex = S->gtex.as<float>();
+    job.adj.glight = S->glight.as<float>();
+    mpt_stats ast{};
+    if ((rc = stream_render(ctx, S->stream, *ctx->streamer, job, &ast)) != MPT_OK) return rc;
+    if (d_tex && S->tex_elems > 0)
+        HIP_TRY(ctx, hipMemcpyAsync(d_tex, S->gtex.p, sizeof(float) * static_cast<size_t>(S->tex_elems),
+                                    hipMemcpyDeviceToHost, S->stream));
+    if (d_light && !h.lights.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(d_light, S->glight.p, sizeof(float) * 3 * h.lights.size(), hipMemcpyDeviceToHost,
+                                    S->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(S->stream));
+    mpt_stats& r = S->rstats;
+    r = fwd;
+    r.ms_total = ms_since(t0);
+    r.ms_adjoint = ast.ms_gpu;
     return MPT_OK;
 }
 

@@ -66,6 +66,7 @@ class RenderParams(C.Structure):
 
 
 MPT_RENDER_KEEP_SAMPLES = 1
+MPT_RENDER_PASSWISE = 2
 
 
 class Stats(C.Structure):

@@ -112,7 +112,7 @@ typedef struct {
 typedef struct {
     int32_t width;             /* children per node: 4 or 8 (default 4) */
     int32_t leaf_max;          /* largest subtree folded into one leaf (default 2) */
-    int32_t block;             /* traversal threads per block (default 256) */
+    int32_t block;             /* traversal threads per block: 128 (default) or 256 */
     int32_t stack_lds;         /* per-lane stack entries kept in LDS (default 24) */
 } mpt_bvh_opts;
 
@@ -137,7 +137,12 @@ typedef struct {
     int32_t  flags;              /* MPT_RENDER_KEEP_SAMPLES: keep each sample's radiance for mpt_render_adjoint */
 } mpt_render_params;
 
-enum { MPT_RENDER_KEEP_SAMPLES = 1 };
+/* MPT_RENDER_PASSWISE: render sample pass by sample pass (each pass's paths
+ * bounce in lock step; the film is gathered per tile) instead of the default
+ * streaming wavefront (path slots refilled as paths finish, samples splatted
+ * as they finish). Both draw the same paths; the films differ only in the order
+ * of float additions. */
+enum { MPT_RENDER_KEEP_SAMPLES = 1, MPT_RENDER_PASSWISE = 2 };
 
 /* Counters of the last render or trace call. */
 typedef struct {
@@ -239,7 +244,7 @@ typedef struct {
     int32_t width;             /* BVH width, 4 or 8 */
     int32_t stack_lds;         /* per-lane LDS stack entries: 16, 24 or 32 */
     int32_t block;             /* traversal threads per block: 128 or 256 */
-    int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..22) */
+    int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..23) */
     int32_t graph_steps;       /* wavefront steps per graph launch (even, 2..64) */
 } mpt_batch_opts;
 

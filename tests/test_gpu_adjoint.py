@@ -67,6 +67,26 @@ def test_adjoint_matches_fp64_host_gradient(ctx, max_depth):
     np.testing.assert_allclose(d_light, olight, rtol=1e-3)
 
 
+def test_streaming_adjoint_matches_passwise(ctx):
+    """The streaming adjoint (default) and the pass-by-pass adjoint replay the
+    same paths: gradients equal up to float summation order."""
+    from cadise_amd.api import Scene
+    spec = scenes.textured_spheres(n_spheres=4, level=3, tex_res=16, width=64, height=48, spp=4, max_depth=6)
+    sc = Scene.from_spec(ctx, spec)
+    G = np.random.default_rng(8).normal(size=(spec.height, spec.width, 3)).astype(np.float32)
+    sc.render(seed=31, keep_samples=True)
+    a_tex, a_light = sc.adjoint(G, seed=31)
+    from cadise_amd.native import MPT_RENDER_PASSWISE, RenderParams
+    import ctypes as C
+    st = sc.stats()
+    b_tex = np.zeros(st["tex_elements"], np.float32)
+    b_light = np.zeros((st["nlights"], 3), np.float32)
+    p = RenderParams(31, 0, 0, 0, 0, MPT_RENDER_PASSWISE)
+    sc.ctx.check(sc.L.mpt_render_adjoint(sc.h, C.byref(p), G.ctypes.data, b_tex.ctypes.data, b_light.ctypes.data))
+    assert np.linalg.norm(a_tex - b_tex) <= 1e-5 * np.linalg.norm(b_tex)
+    np.testing.assert_allclose(a_light, b_light, rtol=1e-5)
+
+
 @pytest.mark.timeout(600)
 def test_adjoint_matches_finite_differences(ctx):
     """Differentiation workload at full size: 204,802 triangles, 256^2 albedo and
@@ -103,7 +123,9 @@ def test_adjoint_matches_finite_differences(ctx):
             sc.set_texture(ir, r)
         if le is not None:
             sc.set_light_radiance(0, le)
-        v = _loss(film_mean(sc.render(seed=seed)), target)
+        # pass by pass: its film sums in a fixed order, so the two renders of a
+        # difference quotient differ only where the parameter acts
+        v = _loss(film_mean(sc.render(seed=seed, passwise=True)), target)
         return v
 
     rng = np.random.default_rng(11)

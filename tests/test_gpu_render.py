@@ -66,15 +66,40 @@ def test_render_spheres_ggx_vs_oracle(ctx, tmp_path):
     _compare(film, orf, "spheres ggx", frac_bar=0.98)
 
 
-def test_render_progressive_accumulate(ctx):
-    """Passes [0, 2) + [2, 4) accumulated equal one render of [0, 4)."""
+@pytest.mark.parametrize("passwise", [False, True])
+def test_render_progressive_accumulate(ctx, passwise):
+    """Passes [0, 2) + [2, 4) accumulated equal one render of [0, 4): bit for
+    bit pass by pass (the film adds passes in order), up to float summation
+    order when streaming (samples splat as their paths finish)."""
     from cadise_amd.api import Scene
     spec = scenes.cornell_box(width=32, height=32, spp=4, max_depth=5)
     sc = Scene.from_crsd_text(ctx, spec.to_crsd())
-    full = sc.render(seed=5)
-    sc.render(seed=5, spp_begin=0, spp_end=2)
-    part = sc.render(seed=5, spp_begin=2, spp_end=4, accumulate=True)
-    np.testing.assert_array_equal(full, part)
+    full = sc.render(seed=5, passwise=passwise)
+    sc.render(seed=5, spp_begin=0, spp_end=2, passwise=passwise)
+    part = sc.render(seed=5, spp_begin=2, spp_end=4, accumulate=True, passwise=passwise)
+    if passwise:
+        np.testing.assert_array_equal(full, part)
+    else:
+        np.testing.assert_allclose(full, part, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("case", ["spheres", "cornell_gaussian", "textured"])
+def test_streaming_matches_passwise(ctx, case):
+    """The streaming wavefront (default) and the pass-by-pass wavefront draw the
+    same paths: ray counts equal exactly, films up to float summation order."""
+    from cadise_amd.api import Scene
+    spec = {"spheres": lambda: small_spheres(n_spheres=6, level=3, width=96, height=64, spp=5, max_depth=7),
+            "cornell_gaussian": lambda: scenes.cornell_box(width=48, height=40, spp=6, max_depth=9, filter="gaussian"),
+            "textured": lambda: scenes.textured_spheres(n_spheres=3, level=2, tex_res=16, width=40, height=40, spp=3,
+                                                        max_depth=5)}[case]()
+    sc = Scene.from_spec(ctx, spec)
+    a = sc.render(seed=21)
+    sa = sc.last_stats()
+    b = sc.render(seed=21, passwise=True)
+    sb = sc.last_stats()
+    for k in ("camera_rays", "continuation_rays", "mis_rays", "shadow_rays", "samples", "box_tests", "tri_tests"):
+        assert sa[k] == sb[k], (k, sa[k], sb[k])
+    np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6)
 
 
 def test_gaussian_filter_vs_oracle(ctx):
