@@ -47,8 +47,9 @@ class Context:
         self.close()
 
 
-def bvh_opts(width=4, leaf_max=2):
-    return BvhOpts(width, leaf_max, 0, 0)
+def bvh_opts(width=4, leaf_max=2, block=0, stack_lds=0, blocks_per_cu=0):
+    """Zero fields take the library defaults (block 128, 24 LDS stack entries, 6 blocks per CU)."""
+    return BvhOpts(width, leaf_max, block, stack_lds, blocks_per_cu)
 
 
 class Scene:
@@ -264,8 +265,8 @@ class Scene:
 
 
 def batch_opts(width: int = 4, stack_lds: int = 24, block: int = 128, capacity_log2: int = 21,
-               graph_steps: int = 16) -> BatchOpts:
-    return BatchOpts(width, stack_lds, block, capacity_log2, graph_steps)
+               graph_steps: int = 16, blocks_per_cu: int = 6) -> BatchOpts:
+    return BatchOpts(width, stack_lds, block, capacity_log2, graph_steps, blocks_per_cu)
 
 
 class Batch:

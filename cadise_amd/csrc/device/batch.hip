@@ -86,6 +86,7 @@ void normalize_opts(mpt_batch_opts& o) {
     if (o.block != 256) o.block = 128;
     if (o.capacity_log2 < 16 || o.capacity_log2 > 23) o.capacity_log2 = 21;
     if (o.graph_steps < 2 || o.graph_steps > 64) o.graph_steps = 16;
+    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = 6;
     o.graph_steps &= ~1;
 }
 
@@ -384,7 +385,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     sh.cfg = job.cfg;
     {
         const int64_t want = (2 * static_cast<int64_t>(cap) + sh.cfg.block - 1) / sh.cfg.block;
-        sh.tblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(ctx->num_cus) * 6)));
+        sh.tblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(ctx->num_cus) * sh.cfg.bpc)));
         sh.sblocks = static_cast<int>(std::min<int64_t>((cap + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
         sh.rblocks = (cap + 255) / 256;
         const int64_t extra = job.sc.max_stack - sh.cfg.stack;
@@ -601,6 +602,7 @@ int do_batch_render(mpt_batch* B, const uint64_t* seeds, int32_t spp_override, d
     job.cfg.width = o.width;
     job.cfg.stack = o.stack_lds;
     job.cfg.block = o.block;
+    job.cfg.bpc = o.blocks_per_cu;
     job.capacity_log2 = o.capacity_log2;
     job.graph_steps = o.graph_steps;
     job.use_graph = true;
@@ -692,7 +694,7 @@ int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, m
     // coordinate descent: path slots, BVH width, LDS stack entries, block size
     for (int pass = 0; pass < 2; ++pass) {
         bool improved = false;
-        for (int dim = 0; dim < 4; ++dim) {
+        for (int dim = 0; dim < 5; ++dim) {
             std::vector<mpt_batch_opts> cands;
             if (dim == 0)
                 for (int c : {19, 20, 21, 22, 23}) { mpt_batch_opts o = cur; o.capacity_log2 = c; cands.push_back(o); }
@@ -702,6 +704,8 @@ int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, m
                 for (int s : {16, 24, 32}) { mpt_batch_opts o = cur; o.stack_lds = s; cands.push_back(o); }
             if (dim == 3)
                 for (int k : {128, 256}) { mpt_batch_opts o = cur; o.block = k; cands.push_back(o); }
+            if (dim == 4)
+                for (int k : {4, 6, 8, 12}) { mpt_batch_opts o = cur; o.blocks_per_cu = k; cands.push_back(o); }
             for (const mpt_batch_opts& o : cands) {
                 if (std::memcmp(&o, &cur, sizeof(o)) == 0) continue;
                 double ms = 0.0;

@@ -36,7 +36,7 @@ __global__ void k_zero_i32(int32_t* p, int n);
 
 // Traversal launch shape: BVH width, per-lane stack entries kept in LDS, threads per block.
 struct TraceCfg {
-    int width = 4, stack = 24, block = 256;
+    int width = 4, stack = 24, block = 128, bpc = 6;   // bpc: persistent blocks per CU
 };
 
 template <int W, int S>
@@ -64,10 +64,12 @@ inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
     if (o.leaf_max > 64) o.leaf_max = 64;
     if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
     if (o.block != 256) o.block = 128;
+    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = 6;
     TraceCfg c;
     c.width = o.width;
     c.stack = o.stack_lds;
     c.block = o.block;
+    c.bpc = o.blocks_per_cu;
     return c;
 }
 

@@ -250,7 +250,7 @@ int build_and_upload(mpt_scene* S) {
 
 int trace_blocks(mpt_scene* S, int64_t nrays) {
     const int64_t want = (nrays + S->cfg.block - 1) / S->cfg.block;
-    const int64_t cap = static_cast<int64_t>(S->ctx->num_cus) * 6;
+    const int64_t cap = static_cast<int64_t>(S->ctx->num_cus) * S->cfg.bpc;
     return static_cast<int>(std::max<int64_t>(1, std::min(want, cap)));
 }
 

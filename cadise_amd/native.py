@@ -48,7 +48,8 @@ class SceneDesc(C.Structure):
 
 
 class BvhOpts(C.Structure):
-    _fields_ = [("width", C.c_int32), ("leaf_max", C.c_int32), ("block", C.c_int32), ("stack_lds", C.c_int32)]
+    _fields_ = [("width", C.c_int32), ("leaf_max", C.c_int32), ("block", C.c_int32), ("stack_lds", C.c_int32),
+                ("blocks_per_cu", C.c_int32)]
 
 
 class SceneStats(C.Structure):
@@ -84,7 +85,7 @@ class Stats(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("width", C.c_int32), ("stack_lds", C.c_int32), ("block", C.c_int32), ("capacity_log2", C.c_int32),
-                ("graph_steps", C.c_int32)]
+                ("graph_steps", C.c_int32), ("blocks_per_cu", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

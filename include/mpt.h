@@ -114,6 +114,7 @@ typedef struct {
     int32_t leaf_max;          /* largest subtree folded into one leaf (default 2) */
     int32_t block;             /* traversal threads per block: 128 (default) or 256 */
     int32_t stack_lds;         /* per-lane stack entries kept in LDS (default 24) */
+    int32_t blocks_per_cu;     /* resident traversal blocks per CU of the persistent grid (default 6) */
 } mpt_bvh_opts;
 
 typedef struct {
@@ -246,6 +247,7 @@ typedef struct {
     int32_t block;             /* traversal threads per block: 128 or 256 */
     int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..23) */
     int32_t graph_steps;       /* wavefront steps per graph launch (even, 2..64) */
+    int32_t blocks_per_cu;     /* resident traversal blocks per CU (2..32) */
 } mpt_batch_opts;
 
 typedef struct {
