@@ -109,8 +109,18 @@ class Scene:
             mats[i].albedo = texture(mat.albedo, mat.albedo_texture, 3)
             mats[i].f0[:] = [float(np.float32(x)) for x in mat.f0]
             mats[i].roughness = texture(mat.roughness, mat.roughness_texture, 1)
-        lights = (LightDesc * len(spec.lights))()
-        for i, (l, p) in enumerate(zip(spec.lights, spec.light_prims())):
+        order = spec.light_order()
+        lights = (LightDesc * max(1, len(order)))()
+        for i, (kind, l, p) in enumerate(order):
+            if kind == "environment":
+                env = np.ascontiguousarray(spec.env, np.float32)
+                keep.append(env)
+                E = lights[i]
+                E.type = 1
+                E.prim = -1
+                E.env.data = env.ctypes.data
+                E.env.height, E.env.width, E.env.channels = env.shape[0], env.shape[1], 3
+                continue
             lights[i].prim = p
             lights[i].color[:] = [float(np.float32(x)) for x in l.color]
             lights[i].watt = float(np.float32(l.watt))
@@ -121,7 +131,7 @@ class Scene:
         d.prim_n = n.ctypes.data if any(x.normals is not None for x in spec.meshes) else None
         d.prim_uvw = u.ctypes.data if any(x.uvws is not None for x in spec.meshes) else None
         d.nmats, d.mats = len(spec.materials), mats
-        d.nlights, d.lights = len(spec.lights), lights
+        d.nlights, d.lights = len(order), lights
         d.cluster = 1 if spec.cluster == "power" else 0
         d.cam_pos[:], d.cam_dir[:], d.cam_up[:] = spec.cam_pos, spec.cam_dir, spec.cam_up
         d.fov_deg, d.sensor_width_mm = float(np.float32(spec.fov)), 36.0
@@ -181,6 +191,13 @@ class Scene:
         self.ctx.check(self.L.mpt_scene_light_radiance(self.h, out.ctypes.data))
         return out[:n]
 
+    def light_pick_pdf(self):
+        """Probability of picking each light (the scene's light cluster)."""
+        n = self.stats()["nlights"]
+        out = np.zeros(max(1, n), np.float32)
+        self.ctx.check(self.L.mpt_scene_light_pick_pdf(self.h, out.ctypes.data))
+        return out[:n]
+
     def last_stats(self) -> dict:
         st = Stats()
         self.ctx.check(self.L.mpt_stats_get(self.h, C.byref(st)))
@@ -218,6 +235,13 @@ class Scene:
         w, h, c, o = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         self.ctx.check(self.L.mpt_scene_texture_info(self.h, index, C.byref(w), C.byref(h), C.byref(c), C.byref(o)))
         return w.value, h.value, c.value, o.value
+
+    def get_texture(self, index: int) -> np.ndarray:
+        """Texture `index` as the renderer samples it (H x W x C float32)."""
+        w, h, c, _ = self.texture_info(index)
+        out = np.zeros((h, w, c), np.float32)
+        self.ctx.check(self.L.mpt_scene_get_texture(self.h, index, out.ctypes.data))
+        return out
 
     def set_texture(self, index: int, data: np.ndarray):
         data = np.ascontiguousarray(data, np.float32)

@@ -65,7 +65,7 @@ struct mpt_batch {
     int64_t nnodes[2] = {0, 0};
     int32_t max_stack[2] = {0, 0};
     std::vector<int32_t> roots[2];
-    mpt::DevBuf tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, film;
+    mpt::DevBuf tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, env_cdf, film;
     mpt::DevScene sc{};
     std::vector<mpt::BatchView> hviews;
     int64_t nprims = 0, total_px = 0;
@@ -151,7 +151,7 @@ int merge_tables(mpt_batch* B) {
     std::vector<DevMaterial> mats;
     std::vector<DevTexture> texs;
     std::vector<DevLight> lights;
-    std::vector<float> cdf;
+    std::vector<float> cdf, envcdf;
     std::vector<int64_t> pbase(ns), tdbase(ns);
     for (size_t i = 0; i < ns; ++i) {
         mpt_scene* S = B->scenes[i];
@@ -175,13 +175,21 @@ int merge_tables(mpt_batch* B) {
             t.offset += ntexel;
             texs.push_back(t);
         }
+        const int32_t env0 = static_cast<int32_t>(envcdf.size());
         for (DevLight l : S->h_lights) {
-            l.prim += static_cast<int32_t>(np);
+            if (l.type == LIGHT_TYPE_ENV) {
+                l.env_tex += tex0;
+                l.env_cdf += env0;
+            } else {
+                l.prim += static_cast<int32_t>(np);
+            }
             lights.push_back(l);
         }
+        envcdf.insert(envcdf.end(), S->h_envcdf.begin(), S->h_envcdf.end());
         BatchView& v = B->hviews[i];
         const DevScene& d = S->sc;
         v.light_base = static_cast<int32_t>(nlight);
+        v.env_light = d.env_light >= 0 ? static_cast<int32_t>(nlight) + d.env_light : -1;
         v.nlights = static_cast<int32_t>(h.lights.size());
         v.cluster = h.cluster;
         v.cdf_base = static_cast<int32_t>(ncdf);
@@ -259,6 +267,7 @@ int merge_tables(mpt_batch* B) {
     if ((rc = upload_vec(ctx, B->texs, texs)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, B->lights, lights)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, B->power_cdf, cdf)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, B->env_cdf, envcdf)) != MPT_OK) return rc;
     HIP_TRY(ctx, B->film.alloc(sizeof(float4) * static_cast<size_t>(std::max<int64_t>(1, npx))));
     DevScene& d = B->sc;
     d = DevScene{};
@@ -272,6 +281,8 @@ int merge_tables(mpt_batch* B) {
     d.tex_data = B->tex_data.as<float>();
     d.lights = B->lights.as<DevLight>();
     d.power_cdf = B->power_cdf.as<float>();
+    d.env_cdf = B->env_cdf.as<float>();
+    d.env_light = -1;   // per scene: BatchView::env_light
     d.nlights = static_cast<int32_t>(nlight);
     return MPT_OK;
 }

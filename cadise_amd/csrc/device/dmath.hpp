@@ -69,6 +69,7 @@ __device__ __forceinline__ float msign(float v) { return static_cast<float>((0.0
 constexpr float K_PI = static_cast<float>(3.1415926535897932);
 constexpr float K_TWO_PI = static_cast<float>(6.2831853071795864);
 constexpr float K_INV_PI = static_cast<float>(0.3183098861837906);
+constexpr float K_INV_TWO_PI = static_cast<float>(0.1591549430918953);
 constexpr float K_RAY_EPS = static_cast<float>(0.001);
 
 // Philox4x32-10 (Salmon et al., SC 2011): counter-based, so any path vertex

@@ -149,7 +149,7 @@ struct mpt_scene {
     mpt::DevScene sc{};
     hipStream_t stream = nullptr;
     // scene arrays
-    mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf;
+    mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, env_cdf;
     // film + pass state
     mpt::DevBuf film, beta, rad, fpos, qcount, work, stats, spill, srgb;
     mpt::DevBuf qbuf[2][12];
@@ -157,6 +157,7 @@ struct mpt_scene {
     // parameters that can change after the upload (differentiable rendering)
     std::vector<mpt::DevMaterial> h_mats;
     std::vector<float> h_cdf;
+    std::vector<float> h_envcdf;
     std::vector<mpt::DevLight> h_lights;
     std::vector<mpt::DevTexture> h_texs;
     std::vector<mpt::F3> h_rad;

@@ -45,7 +45,12 @@ struct DevLight {
     float   e1[3];       // eAB | eA
     float   e2[3];       // eAC | eB
     float   n[9];        // triangle vertex normals nA nB nC (face normal unless set)
+    // environment light (type 2): radiance texture and sampling tables at
+    // env_cdf: marginal CDF (env_h + 1), then env_h conditional CDFs (env_w + 1)
+    int32_t env_tex, env_w, env_h, env_cdf;
 };
+
+constexpr int32_t LIGHT_TYPE_ENV = 2;
 
 struct DevScene {
     const void*    nodes;
@@ -63,6 +68,8 @@ struct DevScene {
     const DevLight* lights;
     const float*   power_cdf;   // nlights + 1
     int32_t        nlights;
+    int32_t        env_light;   // index of the environment light, -1 none
+    const float*   env_cdf;     // environment lights' sampling tables
     int32_t        cluster;     // 0 uniform, 1 power
     // camera
     float f2c[12];              // film -> camera rows 0..2 (row-major)

@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             // the path's step, sample, pixel and scene settings
             int32_t pb = step, smp = a.sample, pix = p, md = a.max_depth;
             uint2 key = a.key;
-            int32_t nl = sc.nlights, cluster = sc.cluster, lbase = 0, cdf0 = 0;
+            int32_t nl = sc.nlights, cluster = sc.cluster, lbase = 0, cdf0 = 0, envl = sc.env_light;
             int4 meta = make_int4(0, 0, 0, 0);
             if constexpr (BATCH) {
                 meta = bt.meta[p];
@@ -419,6 +419,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 cluster = v.cluster;
                 lbase = v.light_base;
                 cdf0 = v.cdf_base;
+                envl = v.env_light;
                 root_next = v.root;
                 sample_idx = static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix;
             } else {
@@ -448,6 +449,15 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 const float4 md4 = cur.mis_dir[s];
                 if (md4.w >= 0.0f) {
                     const int2 mh = cur.mis_hit[s];
+                    if (mh.x < 0 && li == envl) {
+                        // the MIS-BSDF ray reaches the background, which the picked light lights
+                        const DevLight& lg = sc.lights[li];
+                        const V3 Le = env_emittance(sc, lg, v3f4(md4));
+                        const float4 f4 = cur.fac[s];
+                        const float lp = env_pdfW(sc, lg, v3f4(md4));
+                        const float w = mis_power(f4.w, lp);
+                        Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
+                    }
                     if (mh.x >= 0) {
                         const int32_t prim = static_cast<int32_t>(__float_as_uint(sc.tris[3 * mh.x].w) & 0x7FFFFFFFu);
                         if (sc.prim_info[prim].y == li) {
@@ -482,6 +492,11 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             // -- shade the continuation hit
             cdir = cur.dir[s];
             const int2 ch = cur.hit[s];
+            if (pb == 0 && pb < md && cdir.w >= 0.0f && ch.x < 0 && envl >= 0) {
+                // an unbounded camera ray reaches the background: its emission, and
+                // the path ends there (the background absorbs)
+                L = L + beta * env_emittance(sc, sc.lights[envl], v3f4(cdir));
+            }
             if (pb < md && cdir.w >= 0.0f && ch.x >= 0) {
                 const uint32_t upix = static_cast<uint32_t>(pix);
                 const uint4 rA = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb, 0u), key);
@@ -522,7 +537,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 }
                 // light sample + shadow ray
                 const DevLight& lg = sc.lights[pick_light];
-                const LightSample ls = light_sample(lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
+                const LightSample ls = any_light_sample(sc, lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
                 sdir = make_float4(0.f, 0.f, 0.f, -1.f);
                 lt4 = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (ls.valid) {

@@ -95,7 +95,8 @@ int build_and_upload(mpt_scene* S) {
         const bool has_u = !h.prim_uvw.empty() && h.prim_type[i] == PRIM_TRIANGLE && !std::isnan(h.prim_uvw[9 * i]);
         info[i] = make_int4(h.prim_mat[i], -1, has_n ? 1 : 0, has_u ? 1 : 0);
     }
-    for (size_t l = 0; l < h.lights.size(); ++l) info[h.lights[l].prim].y = static_cast<int32_t>(l);
+    for (size_t l = 0; l < h.lights.size(); ++l)
+        if (h.lights[l].type == LIGHT_AREA) info[h.lights[l].prim].y = static_cast<int32_t>(l);
 
     std::vector<DevMaterial> mats(h.materials.size());
     for (size_t i = 0; i < mats.size(); ++i) {
@@ -131,8 +132,24 @@ int build_and_upload(mpt_scene* S) {
     }
     const std::vector<F3> rad = light_radiance(h);
     std::vector<DevLight> lights(h.lights.size());
+    std::vector<float> envcdf;
+    int32_t env_light = -1;
     for (size_t l = 0; l < lights.size(); ++l) {
         DevLight d{};
+        if (h.lights[l].type == LIGHT_ENVIRONMENT) {
+            const Texture& t = h.textures[h.lights[l].env_tex];
+            d.prim = -1;
+            d.type = LIGHT_TYPE_ENV;
+            d.env_tex = h.lights[l].env_tex;
+            d.env_w = t.width;
+            d.env_h = t.height;
+            d.env_cdf = static_cast<int32_t>(envcdf.size());
+            const EnvTables e = env_tables(t);
+            envcdf.insert(envcdf.end(), e.cdf.begin(), e.cdf.end());
+            env_light = static_cast<int32_t>(l);
+            lights[l] = d;
+            continue;
+        }
         const int32_t p = h.lights[l].prim;
         const float* v = &h.prim_v[9 * p];
         d.prim = p;
@@ -187,6 +204,8 @@ int build_and_upload(mpt_scene* S) {
     if ((rc = upload_vec(ctx, S->texs, texs)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, S->tex_data, texdata)) != MPT_OK) return rc;
     if ((rc = upload_vec(ctx, S->lights, lights)) != MPT_OK) return rc;
+    if ((rc = upload_vec(ctx, S->env_cdf, envcdf)) != MPT_OK) return rc;
+    S->h_envcdf = envcdf;
     S->h_lights = lights;
     S->h_mats = mats;
     S->h_cdf = cdf;
@@ -213,6 +232,8 @@ int build_and_upload(mpt_scene* S) {
     sc.lights = S->lights.as<DevLight>();
     sc.power_cdf = S->power_cdf.as<float>();
     sc.nlights = static_cast<int32_t>(h.lights.size());
+    sc.env_light = env_light;
+    sc.env_cdf = S->env_cdf.as<float>();
     sc.cluster = h.cluster;
     float f2c[16], c2w[16];
     camera_matrices(h, f2c, c2w);
@@ -527,6 +548,7 @@ BatchView scene_view(const mpt_scene* S, uint64_t seed, int32_t spp_begin, int32
     v.nlights = d.nlights;
     v.cluster = d.cluster;
     v.cdf_base = 0;
+    v.env_light = d.env_light;
     v.img_w = d.img_w;
     v.img_h = d.img_h;
     v.tile_x = d.tile_x;
@@ -936,6 +958,24 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
     }
     for (int32_t i = 0; i < d->nlights; ++i) {
         const mpt_light_desc& l = d->lights[i];
+        if (l.type == MPT_LIGHT_ENVIRONMENT) {
+            for (const Light& o : h.lights)
+                if (o.type == LIGHT_ENVIRONMENT) return fail(ctx, MPT_ERR_UNSUPPORTED, "more than one environment light");
+            if (!l.env.data || l.env.width <= 0 || l.env.height <= 0 || l.env.channels != 3)
+                return fail(ctx, MPT_ERR_INVALID, "environment light needs an H x W x 3 image");
+            Texture t;
+            t.width = l.env.width;
+            t.height = l.env.height;
+            t.channels = 3;
+            t.data.assign(l.env.data, l.env.data + static_cast<size_t>(t.width) * t.height * 3);
+            h.textures.push_back(std::move(t));
+            Light x;
+            x.type = LIGHT_ENVIRONMENT;
+            x.env_tex = static_cast<int32_t>(h.textures.size() - 1);
+            h.lights.push_back(x);
+            continue;
+        }
+        if (l.type != MPT_LIGHT_AREA) return fail(ctx, MPT_ERR_INVALID, "light type");
         if (l.prim < 0 || l.prim >= n) return fail(ctx, MPT_ERR_INVALID, "light primitive out of range");
         Light x;
         x.prim = l.prim;
@@ -989,6 +1029,19 @@ int mpt_scene_light_radiance(const mpt_scene* s, float* out) {
         out[3 * i] = r[i].x;
         out[3 * i + 1] = r[i].y;
         out[3 * i + 2] = r[i].z;
+    }
+    return MPT_OK;
+}
+
+int mpt_scene_light_pick_pdf(const mpt_scene* s, float* out) {
+    if (!s || !out) return MPT_ERR_INVALID;
+    const size_t n = s->host.lights.size();
+    if (s->host.cluster == 1) {
+        // the radiance set by mpt_scene_set_light_radiance, else the parsed one
+        const std::vector<float> cdf = power_cdf(s->host, s->h_rad.empty() ? light_radiance(s->host) : s->h_rad);
+        for (size_t i = 0; i < n; ++i) out[i] = cdf[i + 1] - cdf[i];
+    } else {
+        for (size_t i = 0; i < n; ++i) out[i] = 1.0f / static_cast<float>(n);
     }
     return MPT_OK;
 }
@@ -1146,12 +1199,33 @@ int mpt_scene_texture_info(const mpt_scene* s, int32_t index, int64_t* width, in
     return MPT_OK;
 }
 
+int mpt_scene_get_texture(const mpt_scene* s, int32_t index, float* out) {
+    if (!s || !out) return MPT_ERR_INVALID;
+    int64_t w = 0, hh = 0, c = 0, off = 0;
+    const int rc = mpt_scene_texture_info(s, index, &w, &hh, &c, &off);
+    if (rc != MPT_OK) return rc;
+    const Texture& t = s->host.textures[index];
+    if (t.data.empty()) {
+        out[0] = t.value.x;
+        if (c == 3) {
+            out[1] = t.value.y;
+            out[2] = t.value.z;
+        }
+    } else {
+        std::copy(t.data.begin(), t.data.end(), out);
+    }
+    return MPT_OK;
+}
+
 int mpt_scene_set_texture(mpt_scene* s, int32_t index, const float* data) {
     if (!s || !data) return MPT_ERR_INVALID;
     int64_t w = 0, hh = 0, c = 0, off = 0;
     const int rc = mpt_scene_texture_info(s, index, &w, &hh, &c, &off);
     if (rc != MPT_OK) return rc;
     const size_t n = static_cast<size_t>(w * hh * c);
+    for (const Light& l : s->host.lights)
+        if (l.type == LIGHT_ENVIRONMENT && l.env_tex == index)
+            return fail(s->ctx, MPT_ERR_UNSUPPORTED, "an environment light's texture is not a parameter");
     Texture& t = s->host.textures[index];
     if (t.data.empty()) {
         t.value = {data[0], c == 3 ? data[1] : data[0], c == 3 ? data[2] : data[0]};
@@ -1170,6 +1244,8 @@ int mpt_scene_set_texture(mpt_scene* s, int32_t index, const float* data) {
 int mpt_scene_set_light_radiance(mpt_scene* s, int32_t light, const float* rgb) {
     if (!s || !rgb) return MPT_ERR_INVALID;
     if (light < 0 || light >= static_cast<int32_t>(s->host.lights.size())) return fail(s->ctx, MPT_ERR_INVALID, "light index");
+    if (s->host.lights[light].type == LIGHT_ENVIRONMENT)
+        return fail(s->ctx, MPT_ERR_UNSUPPORTED, "an environment light's radiance is its texture");
     ++s->param_version;
     if (s->ctx->device < 0) return MPT_OK;
     s->h_rad[light] = F3{rgb[0], rgb[1], rgb[2]};
@@ -1189,6 +1265,8 @@ int mpt_render_adjoint(mpt_scene* s, const mpt_render_params* p, const float* d_
     if (!s || !d_image) return MPT_ERR_INVALID;
     REQUIRE_DEVICE(s);
     HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    // the environment's radiance texture is not a differentiated parameter
+    if (s->sc.env_light >= 0) return fail(s->ctx, MPT_ERR_UNSUPPORTED, "adjoint of a scene with an environment light");
     return do_adjoint(s, p, d_image, d_tex, d_light);
 }
 

@@ -308,6 +308,108 @@ __device__ __forceinline__ float light_pdfW(const DevLight& l, const SurfaceHit&
     return l.pdfA * len2(ev) / c;
 }
 
+// ---- environment light -----------------------------------------------------------
+// The background of a scene with an environment light is an absorbing infinite
+// sphere that every unbounded ray missing the scene hits (scene.cpp:30-63,
+// infiniteSphere.cpp:35-47); its surface position is the ray's direction.
+
+// SphericalMapper::mappingToUvw (sphericalMapper.cpp:14-28): y up, phi from +z
+// towards +x.
+__device__ __forceinline__ V3 spherical_uvw(V3 d) {
+    const V3 u = normalize(d);
+    const float theta = acosf(mclamp(u.y, -1.0f, 1.0f));
+    float phi = atan2f(u.x, u.z);
+    if (phi < 0.0f) phi += K_TWO_PI;
+    return v3(phi * K_INV_TWO_PI, (K_PI - theta) * K_INV_PI, 0.0f);
+}
+
+// Distribution1D::continuousToDiscrete (distribution1D.cpp:115-125): the last
+// i in [0, n) with cdf[i] <= u (a binary search over the nondecreasing CDF).
+__device__ __forceinline__ int32_t cdf_index(const float* cdf, int32_t n, float u) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Distribution1D::sampleContinuous (distribution1D.cpp:60-80)
+__device__ __forceinline__ float cdf_sample(const float* cdf, int32_t n, float u, float& pdf, int32_t& index) {
+    const float delta = 1.0f / static_cast<float>(n);
+    const int32_t i = cdf_index(cdf, n, u);
+    index = i;
+    pdf = (cdf[i + 1] - cdf[i]) / delta;
+    float d = u - cdf[i];
+    if (cdf[i + 1] > cdf[i]) d /= cdf[i + 1] - cdf[i];
+    return mclamp((static_cast<float>(i) + d) * delta, 0.0f, 1.0f);
+}
+
+// Distribution2D::pdfContinuous (distribution2D.cpp:46-53)
+__device__ __forceinline__ float env_pdf_uv(const DevScene& sc, const DevLight& l, float u, float v) {
+    const float* m = sc.env_cdf + l.env_cdf;
+    const int32_t iy = cdf_index(m, l.env_h, v);
+    const float pdf_y = (m[iy + 1] - m[iy]) / (1.0f / static_cast<float>(l.env_h));
+    const float* c = m + (l.env_h + 1) + static_cast<int64_t>(iy) * (l.env_w + 1);
+    const int32_t ix = cdf_index(c, l.env_w, u);
+    const float pdf_x = (c[ix + 1] - c[ix]) / (1.0f / static_cast<float>(l.env_w));
+    return pdf_x * pdf_y;
+}
+
+// EnvironmentLight::emittance at the background hit of direction d
+// (environmentLight.cpp:58-64)
+__device__ __forceinline__ V3 env_emittance(const DevScene& sc, const DevLight& l, V3 d) {
+    return tex_lookup(sc, l.env_tex, spherical_uvw(d));
+}
+
+// EnvironmentLight::evaluateDirectPdfW at the background hit of direction d
+// (environmentLight.cpp:90-104)
+__device__ __forceinline__ float env_pdfW(const DevScene& sc, const DevLight& l, V3 d) {
+    const V3 uvw = spherical_uvw(d);
+    const float sin_theta = sinf((1.0f - uvw.y) * K_PI);
+    if (sin_theta <= 0.0f) return 0.0f;
+    return env_pdf_uv(sc, l, uvw.x, uvw.y) / (((2.0f * K_PI) * K_PI) * sin_theta);
+}
+
+// EnvironmentLight::evaluateDirectSample with sample (sx, sy)
+// (environmentLight.cpp:66-88), Distribution2D::sampleContinuous
+// (distribution2D.cpp:30-44) and InfiniteSphere::uvwToPosition
+// (infiniteSphere.cpp:61-78, z up: the frame disagrees with the lookup's).
+__device__ __forceinline__ LightSample env_light_sample(const DevScene& sc, const DevLight& l, float sx, float sy) {
+    LightSample r;
+    r.valid = false;
+    r.pdfW = 0.0f;
+    r.pos = v3(0.f, 0.f, 0.f);
+    r.radiance = v3(0.f, 0.f, 0.f);
+    const float* m = sc.env_cdf + l.env_cdf;
+    float pdf_y, pdf_x;
+    int32_t iy, ix;
+    const float v = cdf_sample(m, l.env_h, sy, pdf_y, iy);
+    const float* c = m + (l.env_h + 1) + static_cast<int64_t>(iy) * (l.env_w + 1);
+    const float u = cdf_sample(c, l.env_w, sx, pdf_x, ix);
+    const float uv_pdf = pdf_x * pdf_y;
+    const float theta = (1.0f - v) * K_PI;
+    const float phi = u * K_TWO_PI;
+    const float ct = cosf(theta);
+    const float st = sqrtf(1.0f - ct * ct);
+    const V3 dir = v3(cosf(phi) * st, sinf(phi) * st, ct);
+    const V3 pos = normalize(dir) * 1000000.0f;
+    const float sin_theta = sinf((1.0f - v) * K_PI);
+    if (sin_theta <= 0.0f) return r;
+    r.pos = pos;
+    r.radiance = tex_lookup(sc, l.env_tex, v3(u, v, 0.0f));
+    r.pdfW = uv_pdf / (((2.0f * K_PI) * K_PI) * sin_theta);
+    r.valid = !is_zero(r.radiance) && r.pdfW > 0.0f;
+    return r;
+}
+
+// The picked light's direct sample: area or environment.
+__device__ __forceinline__ LightSample any_light_sample(const DevScene& sc, const DevLight& l, V3 target, float u0,
+                                                        float u1) {
+    return l.type == LIGHT_TYPE_ENV ? env_light_sample(sc, l, u0, u1) : light_sample(l, target, u0, u1);
+}
+
 // TMis<POWER>::weight (tMis.h:21-47)
 __device__ __forceinline__ float mis_power(float a, float b) {
     const float a2 = a * a, b2 = b * b;

@@ -80,6 +80,19 @@ Box prim_bound(const HostScene& s, int64_t i) {
     return b;
 }
 
+}  // namespace
+
+void reference_root_bound(const HostScene& s, float lo[3], float hi[3]) {
+    Box b(true);
+    for (int64_t i = 0; i < s.nprims(); ++i) b.add(prim_bound(s, i));
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = b.lo[k];
+        hi[k] = b.hi[k];
+    }
+}
+
+namespace {
+
 struct Builder {
     std::vector<Info>& infos;
     BinaryBvh&         out;

@@ -34,6 +34,10 @@ struct BinaryBvh {
 
 void build_binary(const HostScene& s, bool reference_quirk, BinaryBvh* out);
 
+// Root bound of the reference's tree: the union of every primitive's padded
+// bound accumulated on the reference's empty box (max starts at FLT_MIN).
+void reference_root_bound(const HostScene& s, float lo[3], float hi[3]);
+
 // Wide node, structure-of-arrays over its W children. child[i] >= 0 with
 // count[i] == 0: interior child node index. count[i] > 0: leaf of count[i]
 // primitives starting at ordered position child[i]. Empty slot: count 0,

@@ -2,6 +2,7 @@
 // float, as the reference tokenizer does (tokenizer.cpp:134-246), so a scene
 // parses to the same binary32 values.
 #include "crsd.hpp"
+#include "hdr.hpp"
 
 #include <cctype>
 #include <cmath>
@@ -299,6 +300,7 @@ struct Db {
 
     bool light(const Block& b) {
         const std::string type = b.str("type");
+        if (type == "environment") return environment(b);
         if (type != "area") return fail(MPT_ERR_UNSUPPORTED, "light type <" + type + "> is not on the path-tracing path");
         const std::string pn = b.str("primitive");
         auto it = prims.find(pn);
@@ -308,6 +310,38 @@ struct Db {
         l.color = b.vec("color");
         l.watt = b.real("watt");
         l.back_face = b.boolean("is-back-face-emit") ? 1 : 0;
+        s.lights.push_back(l);
+        return true;
+    }
+
+    // lightInstantiator.cpp:52-76: the radiance image is loaded with its rows
+    // flipped (stbi_set_flip_vertically_on_load) and then flipped horizontally
+    // (TImage::flipHorizontal), and sampled nearest with repeat wrap.
+    bool environment(const Block& b) {
+        for (const Light& l : s.lights)
+            if (l.type == LIGHT_ENVIRONMENT) return fail(MPT_ERR_UNSUPPORTED, "more than one environment light");
+        const std::string fn = b.str("hdr-filename");
+        if (fn.size() < 4 || (fn.compare(fn.size() - 4, 4, ".hdr") != 0 && fn.compare(fn.size() - 4, 4, ".HDR") != 0))
+            return fail(MPT_ERR_UNSUPPORTED, "environment image <" + fn + ">: only Radiance .hdr files are read");
+        std::vector<float> raw;
+        int w = 0, h = 0;
+        std::string e;
+        if (!read_hdr(resolve(base, fn), &raw, &w, &h, &e)) return fail(MPT_ERR_IO, e);
+        Texture t;
+        t.width = w;
+        t.height = h;
+        t.channels = 3;
+        t.wrap_clamp = 0;
+        t.data.resize(raw.size());
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x)
+                for (int c = 0; c < 3; ++c)
+                    t.data[3 * (static_cast<size_t>(y) * w + x) + c] =
+                        raw[3 * (static_cast<size_t>(h - 1 - y) * w + (w - 1 - x)) + c];
+        s.textures.push_back(std::move(t));
+        Light l;
+        l.type = LIGHT_ENVIRONMENT;
+        l.env_tex = static_cast<int32_t>(s.textures.size() - 1);
         s.lights.push_back(l);
         return true;
     }
@@ -384,7 +418,8 @@ struct Db {
         else return fail(MPT_ERR_PARSE, "unknown light-cluster <" + cl + ">");
         if (s.lights.empty()) return fail(MPT_ERR_PARSE, "no light (the light cluster needs one, uniformLightCluster.cpp:12)");
         for (const Light& l : s.lights)
-            if (!(prim_area(s, l.prim) > 0.f)) return fail(MPT_ERR_PARSE, "zero-area emitter (singleAreaLight.cpp:30)");
+            if (l.type == LIGHT_AREA && !(prim_area(s, l.prim) > 0.f))
+                return fail(MPT_ERR_PARSE, "zero-area emitter (singleAreaLight.cpp:30)");
         return true;
     }
 };

@@ -4,6 +4,10 @@
 // -ffp-contract=off) because the device reproduces the reference's rays.
 #include "scene.hpp"
 
+#include <cfloat>
+
+#include "bvh.hpp"
+
 #include <cmath>
 #include <cstring>
 
@@ -89,6 +93,7 @@ std::vector<F3> light_radiance(const HostScene& s) {
     std::vector<F3> out(s.lights.size());
     for (size_t i = 0; i < s.lights.size(); ++i) {
         const Light& l = s.lights[i];
+        if (l.type == LIGHT_ENVIRONMENT) continue;   // radiance from its texture
         float sum = 0.f;
         sum += l.color.x;
         sum += l.color.y;
@@ -102,16 +107,83 @@ std::vector<F3> light_radiance(const HostScene& s) {
     return out;
 }
 
+void distribution_cdf(const float* value, size_t n, float* cdf) {
+    const float delta = 1.f / static_cast<float>(n);
+    cdf[0] = 0.f;
+    for (size_t i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + value[i - 1] * delta;
+    const float sum = cdf[n];
+    if (sum > 0.f) {
+        const float inv = 1.f / sum;
+        for (size_t i = 1; i <= n; ++i) cdf[i] *= inv;
+    } else {
+        for (size_t i = 1; i <= n; ++i) cdf[i] = static_cast<float>(i) * delta;
+    }
+    cdf[n] = 1.f;
+}
+
+EnvTables env_tables(const Texture& t) {
+    EnvTables out;
+    const size_t w = static_cast<size_t>(t.width), h = static_cast<size_t>(t.height);
+    std::vector<float> weighted(w * h);
+    const float pi = static_cast<float>(3.1415926535897932);
+    for (size_t iy = 0; iy < h; ++iy) {
+        const float v = (static_cast<float>(iy) + 0.5f) / static_cast<float>(h);
+        // image rows run bottom to top, spherical theta top to bottom
+        const float sin_theta = std::sin((1.0f - v) * pi);
+        for (size_t ix = 0; ix < w; ++ix) {
+            // nearest lookup at the texel centre is texel (ix, iy)
+            const float* p = &t.data[3 * (iy * w + ix)];
+            const float lum = 0.0126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+            weighted[iy * w + ix] = lum * sin_theta;
+            out.background_flux += weighted[iy * w + ix];
+        }
+    }
+    out.cdf.assign((h + 1) + h * (w + 1), 0.f);
+    std::vector<float> rows(h, 0.f);
+    for (size_t iy = 0; iy < h; ++iy) {
+        for (size_t ix = 0; ix < w; ++ix) rows[iy] += weighted[iy * w + ix];
+        distribution_cdf(&weighted[iy * w], w, &out.cdf[(h + 1) + iy * (w + 1)]);
+    }
+    distribution_cdf(rows.data(), h, out.cdf.data());
+    return out;
+}
+
+float env_scene_radius(const HostScene& s) {
+    float lo[3], hi[3];
+    reference_root_bound(s, lo, hi);
+    const bool empty = (lo[0] == FLT_MAX && lo[1] == FLT_MAX && lo[2] == FLT_MAX) ||
+                       (hi[0] == FLT_MIN && hi[1] == FLT_MIN && hi[2] == FLT_MIN);
+    if (empty) return 5000.f;
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    float l2 = 0.f;
+    l2 += ex * ex;
+    l2 += ey * ey;
+    l2 += ez * ez;
+    return std::sqrt(l2) * 0.5f;
+}
+
 std::vector<float> power_cdf(const HostScene& s, const std::vector<F3>& rad) {
     const size_t n = s.lights.size();
     std::vector<float> cdf(n + 1, 0.f);
     if (n == 0) return cdf;
     const float delta = 1.f / static_cast<float>(n);
     for (size_t i = 1; i <= n; ++i) {
-        const F3 le = rad[i - 1];
-        const float lum = 0.0126f * le.x + 0.7152f * le.y + 0.0722f * le.z;
-        float flux = lum * prim_area(s, s.lights[i - 1].prim) * static_cast<float>(3.1415926535897932);
-        if (flux <= 0.f) flux = 1.f;
+        const Light& l = s.lights[i - 1];
+        float flux;
+        if (l.type == LIGHT_ENVIRONMENT) {
+            // EnvironmentLight::_updateApproxmiatedFlux (environmentLight.cpp:138-140).
+            // The scene-bound radius reaches the environment light only when it is
+            // the last light declared: RenderDatabase::_setUpLight re-marks every
+            // light declared after it as "the environment light"
+            // (renderDatabase.cpp:181-189, 100-111), which keeps the default 5000.
+            const float r = (i == n) ? env_scene_radius(s) : 5000.f;
+            flux = env_tables(s.textures[l.env_tex]).background_flux * static_cast<float>(12.566370614359172) * r * r;
+        } else {
+            const F3 le = rad[i - 1];
+            const float lum = 0.0126f * le.x + 0.7152f * le.y + 0.0722f * le.z;
+            flux = lum * prim_area(s, l.prim) * static_cast<float>(3.1415926535897932);
+            if (flux <= 0.f) flux = 1.f;
+        }
         cdf[i] = cdf[i - 1] + flux * delta;
     }
     const float total = cdf[n];

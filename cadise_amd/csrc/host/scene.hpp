@@ -33,11 +33,15 @@ struct Material {
     float   roughness = 0.f;
 };
 
+enum LightType : int32_t { LIGHT_AREA = 0, LIGHT_ENVIRONMENT = 1 };
+
 struct Light {
-    int32_t prim = -1;
+    int32_t type = LIGHT_AREA;
+    int32_t prim = -1;              // area light: the emitting primitive
     F3      color{1.f, 1.f, 1.f};
     float   watt = 1.f;
     int32_t back_face = 0;
+    int32_t env_tex = -1;           // environment light: radiance texture (HostScene::textures)
 };
 
 struct HostScene {
@@ -77,6 +81,24 @@ void camera_matrices(const HostScene& s, float film_to_camera[16], float camera_
 std::vector<F3> light_radiance(const HostScene& s);
 // Surface area of a primitive (triangle.cpp:199-201, rectangle.cpp:179-181).
 float prim_area(const HostScene& s, int64_t i);
+// Distribution1D's CDF (distribution1D.cpp:10-40) of n values: n + 1 entries.
+void distribution_cdf(const float* value, size_t n, float* cdf);
+
+// EnvironmentLight's sampling tables (environmentLight.cpp:16-56,
+// distribution2D.cpp:9-28): marginal CDF over rows (height + 1), then one
+// conditional CDF per row (width + 1 each), and the background flux (sum of
+// luminance * sin(theta) over texels).
+struct EnvTables {
+    std::vector<float> cdf;
+    float background_flux = 0.f;
+};
+EnvTables env_tables(const Texture& t);
+
+// The reference BVH's root bound (the union of every primitive's padded bound,
+// built on the reference's empty box), and the environment light's scene
+// radius from it (renderDatabase.cpp:99-111): |extent| / 2, or 5000 when empty.
+float env_scene_radius(const HostScene& s);
+
 // Power light-cluster CDF (powerLightCluster.cpp:9-25, distribution1D.cpp:10-40).
 std::vector<float> power_cdf(const HostScene& s, const std::vector<F3>& radiance);
 // StratifiedSampler's base number (stratifiedSampler.cpp:14-20).

@@ -31,7 +31,8 @@ class MaterialDesc(C.Structure):
 
 
 class LightDesc(C.Structure):
-    _fields_ = [("prim", C.c_int32), ("color", C.c_float * 3), ("watt", C.c_float), ("back_face", C.c_int32)]
+    _fields_ = [("prim", C.c_int32), ("color", C.c_float * 3), ("watt", C.c_float), ("back_face", C.c_int32),
+                ("type", C.c_int32), ("env", Texture)]
 
 
 class SceneDesc(C.Structure):
@@ -97,9 +98,10 @@ class BatchTrial(C.Structure):
 
 EXPORTS = ["mpt_create", "mpt_destroy", "mpt_last_error", "mpt_version", "mpt_scene_load_crsd", "mpt_scene_parse_crsd",
            "mpt_scene_create", "mpt_scene_destroy", "mpt_scene_stats_get", "mpt_scene_binary_bvh", "mpt_scene_camera",
-           "mpt_scene_light_radiance", "mpt_trace",
+           "mpt_scene_light_radiance", "mpt_scene_light_pick_pdf", "mpt_trace",
            "mpt_trace_device", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_save", "mpt_stats_get",
-           "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_set_light_radiance", "mpt_render_adjoint",
+           "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_get_texture", "mpt_scene_set_light_radiance",
+           "mpt_render_adjoint",
            "mpt_batch_create", "mpt_batch_destroy", "mpt_batch_render", "mpt_batch_film", "mpt_batch_autotune",
            "mpt_batch_get_opts", "mpt_batch_set_opts"]
 
@@ -127,6 +129,8 @@ def load_library():
     L.mpt_scene_binary_bvh.argtypes = [vp, C.POINTER(i64), vp, vp, vp, vp, vp]
     L.mpt_scene_camera.argtypes = [vp, vp, vp]
     L.mpt_scene_light_radiance.argtypes = [vp, vp]
+    L.mpt_scene_light_pick_pdf.argtypes = [vp, vp]
+    L.mpt_scene_get_texture.argtypes = [vp, i32, vp]
     L.mpt_trace.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
     L.mpt_trace_device.argtypes = [vp, vp, C.c_size_t, C.c_int, vp, vp]
     L.mpt_render.argtypes = [vp, C.POINTER(RenderParams), vp]

@@ -95,6 +95,12 @@ class SceneSpec:
     textures: list = field(default_factory=list)
     meshes: list = field(default_factory=list)
     lights: list = field(default_factory=list)
+    # environment light: H x W x 3 radiance in texture orientation (row 0 = v in
+    # [0, 1/H), the image as the reference's loader leaves it after its vertical
+    # and horizontal flips), RGBE-exact (rgbe_quantize); declared before area
+    # light env_slot (None: after all area lights)
+    env: np.ndarray | None = None
+    env_slot: int | None = None
 
     # -- derived views ------------------------------------------------------
     def num_prims(self) -> int:
@@ -119,6 +125,15 @@ class SceneSpec:
         cat = lambda a, shape: np.concatenate(a) if a else np.zeros(shape, np.float32)
         return (cat(types, (0,)).astype(np.int32), cat(vs, (0, 9)), cat(ns, (0, 9)), cat(us, (0, 9)),
                 cat(mats, (0,)).astype(np.int32))
+
+    def light_order(self):
+        """The lights in declaration order: ("area", Light, primitive index) or
+        ("environment", None, -1)."""
+        area = [("area", l, p) for l, p in zip(self.lights, self.light_prims())]
+        if self.env is None:
+            return area
+        slot = len(area) if self.env_slot is None else self.env_slot
+        return area[:slot] + [("environment", None, -1)] + area[slot:]
 
     def light_prims(self):
         """Primitive index bound to each light: the first primitive carrying the
@@ -195,7 +210,14 @@ class SceneSpec:
                     triples = ", ".join(_v(t) for t in arr.reshape(-1, 3))
                     lines.append(f'    [{key} vector3r-array "{triples}"]')
             out.append("\n".join(lines) + "\n}")
-        for l in self.lights:
+        for kind, l, _ in self.light_order():
+            if kind == "environment":
+                if sidecar_dir is None:
+                    raise ValueError("an environment light needs a side-car directory")
+                path = os.path.join(sidecar_dir, "env.hdr")
+                write_hdr(path, self.env)
+                out.append(f'light {{\n    [type string "environment"]\n    [hdr-filename string "{path}"]\n}}')
+                continue
             out.append(f'light {{\n    [type string "area"]\n    [primitive primitive "{l.primitive}"]\n'
                        f'    [color rgb "{_v(l.color)}"]\n    [watt real "{_f(l.watt)}"]\n'
                        f'    [is-back-face-emit bool "{"true" if l.back_face else "false"}"]\n}}')
@@ -383,6 +405,25 @@ def procedural_spheres(seed=0x20261203, n_spheres=64, level=5, width=1920, heigh
         rect = np.array([[x - h, y, z - h, x - h, y, z + h, x + h, y, z + h]], np.float32)
         s.meshes.append(Mesh(f"lamp{i}", "lightmat", kind=PRIM_RECTANGLE, v=rect))
         s.lights.append(Light(f"lamp{i}", (1.0, 0.95, 0.85), 4000.0))
+    return s
+
+
+def sky_spheres(seed=0x20261203 ^ 0x6, n_spheres=64, level=5, width=1920, height=1080, spp=16, max_depth=8,
+                env_size=(2048, 1024), sun_diameter_deg=0.5, lamps=1, env_slot=None, offset=(64.0, 0.0, 64.0),
+                cluster="power") -> SceneSpec:
+    """procedural_spheres lit by a procedural sky (sky_texture: gradient, ground
+    colour, sun disc) as the environment light plus `lamps` of its rectangle
+    lights; the camera looks nearly level so the horizon and sky are in view.
+    env_slot places the environment light among the lights (None: last)."""
+    s = procedural_spheres(seed=seed, n_spheres=n_spheres, level=level, width=width, height=height, spp=spp,
+                           max_depth=max_depth, offset=offset)
+    s.cam_dir = (0.0, -0.1, -1.0)
+    keep = {f"lamp{i}" for i in range(lamps)}
+    s.meshes = [m for m in s.meshes if not m.name.startswith("lamp") or m.name in keep]
+    s.lights = [l for l in s.lights if l.primitive in keep]
+    s.env = sky_texture(env_size[0], env_size[1], sun_diameter_deg=sun_diameter_deg)
+    s.env_slot = env_slot
+    s.cluster = cluster
     return s
 
 
@@ -620,3 +661,79 @@ def batch_scenes(seed=0x20261203 ^ 0x4, n_scenes=64, tris=(1e4, 2e6), res=(128, 
                                 max_depth=d)
         out.append(sp)
     return out
+
+
+# -- environment light images (Radiance RGBE) ------------------------------------------------
+
+def rgbe_quantize(img: np.ndarray) -> np.ndarray:
+    """The float32 values an RGBE file stores for img (H x W x 3): Ward's
+    encoding (shared exponent of the largest channel, 8-bit mantissas) decoded
+    as m * 2^(e - 136)."""
+    x = np.asarray(img, np.float64)
+    v = x.max(-1)
+    m, e = np.frexp(v)
+    ok = v >= 1e-32
+    scale = np.where(ok, m * 256.0 / np.where(ok, v, 1.0), 0.0)
+    mant = np.floor(x * scale[..., None]).astype(np.int64).clip(0, 255)
+    return (mant * np.ldexp(1.0, (e - 8))[..., None] * ok[..., None]).astype(np.float32)
+
+
+def _rgbe_bytes(img: np.ndarray) -> np.ndarray:
+    x = np.asarray(img, np.float64)
+    v = x.max(-1)
+    m, e = np.frexp(v)
+    ok = v >= 1e-32
+    scale = np.where(ok, m * 256.0 / np.where(ok, v, 1.0), 0.0)
+    out = np.zeros(x.shape[:-1] + (4,), np.uint8)
+    out[..., :3] = np.floor(x * scale[..., None]).clip(0, 255).astype(np.uint8)
+    out[..., 3] = np.where(ok, e + 128, 0).astype(np.uint8)
+    return out
+
+
+def write_hdr(path: str, tex: np.ndarray) -> None:
+    """Radiance .hdr (new-style run-length scanlines of literal runs) such that
+    a loader that flips the image vertically (row 0 = bottom scanline) and then
+    horizontally gets tex back (after rgbe_quantize)."""
+    img = np.ascontiguousarray(np.asarray(tex, np.float32)[::-1, ::-1])   # file rows top to bottom
+    h, w = img.shape[:2]
+    px = _rgbe_bytes(img)
+    chunks = [b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n", f"-Y {h} +X {w}\n".encode()]
+    if w < 8 or w > 0x7FFF:
+        chunks.append(px.tobytes())
+    else:
+        for row in px:
+            buf = bytearray([2, 2, w >> 8, w & 255])
+            for c in range(4):
+                comp = row[:, c]
+                for i in range(0, w, 128):
+                    seg = comp[i:i + 128]
+                    buf.append(len(seg))
+                    buf.extend(seg.tobytes())
+            chunks.append(bytes(buf))
+    with open(path, "wb") as f:
+        f.write(b"".join(chunks))
+
+
+def sky_texture(width=2048, height=1024, sun_elevation_deg=40.0, sun_azimuth=0.3, sun_diameter_deg=0.5,
+                sun_radiance=(2.0e4, 1.8e4, 1.5e4), zenith=(0.25, 0.45, 1.0), horizon=(1.0, 1.0, 1.1),
+                ground=(0.25, 0.22, 0.2)) -> np.ndarray:
+    """Procedural sky as the environment light's texture (RGBE-exact): a
+    gradient from horizon to zenith colour over the upper hemisphere, a dim
+    ground colour below, and a sun disc of the given angular diameter. Texel
+    (x, y) is looked up at u = (x + 0.5) / W, v = (y + 0.5) / H, i.e. the
+    direction with elevation (v - 1/2) pi and azimuth atan2(x, z) = 2 pi u (the
+    environment light's lookup frame, y up)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    elev = (v - 0.5) * np.pi
+    azim = u * 2 * np.pi
+    E, A = np.meshgrid(elev, azim, indexing="ij")
+    up = np.clip(np.sin(E), 0.0, 1.0)[..., None]
+    sky = np.asarray(horizon) * (1 - up) ** 3 + np.asarray(zenith) * (1 - (1 - up) ** 3)
+    img = np.where(E[..., None] >= 0, sky, np.asarray(ground))
+    d = np.stack([np.cos(E) * np.sin(A), np.sin(E), np.cos(E) * np.cos(A)], -1)
+    se, sa = np.radians(sun_elevation_deg), sun_azimuth * 2 * np.pi
+    s = np.array([np.cos(se) * np.sin(sa), np.sin(se), np.cos(se) * np.cos(sa)])
+    ang = np.degrees(np.arccos(np.clip(d @ s, -1, 1)))
+    img = np.where((ang <= sun_diameter_deg / 2)[..., None], np.asarray(sun_radiance), img)
+    return rgbe_quantize(img.astype(np.float32))

@@ -78,11 +78,17 @@ typedef struct {
     mpt_texture_desc roughness;
 } mpt_material_desc;
 
+enum { MPT_LIGHT_AREA = 0, MPT_LIGHT_ENVIRONMENT = 1 };
+
 typedef struct {
-    int32_t prim;              /* emitting primitive (creation index) */
+    int32_t prim;              /* area: emitting primitive (creation index) */
     float   color[3];
     float   watt;
     int32_t back_face;
+    int32_t type;              /* MPT_LIGHT_* */
+    mpt_texture_desc env;      /* environment: radiance image, H x W x 3 in texture orientation
+                                  (row 0 = v in [0, 1/H): the .hdr file's bottom row, columns
+                                  mirrored, as the reference's loader leaves it) */
 } mpt_light_desc;
 
 typedef struct {
@@ -191,6 +197,12 @@ int  mpt_scene_camera(const mpt_scene* scene, float* film_to_camera, float* came
 /* Emitted radiance of each light (nlights x 3) as SingleAreaLight's constructor
  * computes it (singleAreaLight.cpp:20-37). */
 int  mpt_scene_light_radiance(const mpt_scene* scene, float* out);
+/* Probability of picking each light (nlights floats) as the scene's light cluster
+ * gives it: LightCluster::evaluatePickLightPdf (powerLightCluster.cpp:38-46,
+ * uniformLightCluster.cpp:26-29); the power cluster weighs area lights by
+ * SingleAreaLight::approximateFlux and the environment light by
+ * EnvironmentLight::approximateFlux (environmentLight.cpp:125-139). */
+int  mpt_scene_light_pick_pdf(const mpt_scene* scene, float* out);
 
 /* Batch ray queries; host buffers (copied) or device buffers on a stream. */
 int  mpt_trace(mpt_scene* scene, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits);
@@ -217,8 +229,13 @@ int  mpt_stats_get(const mpt_scene* scene, mpt_stats* out);
  * array (row-major H x W x C). */
 int  mpt_scene_texture_info(const mpt_scene* scene, int32_t index, int64_t* width, int64_t* height,
                             int64_t* channels, int64_t* offset);
-/* Replace texture `index` (W x H x C host floats) / one light's radiance (3 floats). */
+/* Replace texture `index` (W x H x C host floats) / one light's radiance (3 floats).
+ * An environment light's texture and radiance are not parameters
+ * (MPT_ERR_UNSUPPORTED). */
 int  mpt_scene_set_texture(mpt_scene* scene, int32_t index, const float* data);
+/* Copy texture `index` (W x H x C floats, as the renderer samples it: an
+ * environment image after the loader's flips) to host memory. */
+int  mpt_scene_get_texture(const mpt_scene* scene, int32_t index, float* out);
 int  mpt_scene_set_light_radiance(mpt_scene* scene, int32_t light, const float* rgb);
 /* Path-replay adjoint of an image loss: given d_image = dLoss/dI (W x H x 3,
  * host; I = the resolved film sum w*rgb / sum w), accumulate dLoss/d(texel)

@@ -32,7 +32,8 @@ class OrMaterial(C.Structure):
 
 
 class OrLight(C.Structure):
-    _fields_ = [("prim", C.c_int32), ("color", C.c_float * 3), ("watt", C.c_float), ("back_face", C.c_int32)]
+    _fields_ = [("prim", C.c_int32), ("color", C.c_float * 3), ("watt", C.c_float), ("back_face", C.c_int32),
+                ("type", C.c_int32), ("env", OrTexture)]
 
 
 class OrSceneDesc(C.Structure):
@@ -85,6 +86,8 @@ def oracle_lib():
         L.or_adjoint.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int, _f32p, _f32p,
                                  C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_set_light_radiance.argtypes = [C.c_void_p, C.c_int, _f32p]
+        L.or_env_info.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_void_p]
+        L.or_power_cdf.argtypes = [C.c_void_p, _f32p]
         L.or_bsdf_dfdtheta_query.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64,
                                              np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
         _oracle = L
@@ -128,8 +131,20 @@ class OracleScene:
             mats[i].albedo = texture(mat.albedo, mat.albedo_texture, 3)
             mats[i].f0[:] = [float(np.float32(x)) for x in mat.f0]
             mats[i].roughness = texture(mat.roughness, mat.roughness_texture, 1)
-        lights = (OrLight * max(1, len(spec.lights)))()
-        for i, (l, p) in enumerate(zip(spec.lights, spec.light_prims())):
+        order = spec.light_order()
+        self.nlights = len(order)
+        self.env_light = next((i for i, o in enumerate(order) if o[0] == "environment"), -1)
+        lights = (OrLight * max(1, len(order)))()
+        for i, (kind, l, p) in enumerate(order):
+            if kind == "environment":
+                env = np.ascontiguousarray(spec.env, np.float32)
+                self._keep.append(env)
+                lights[i].type = 1
+                lights[i].prim = -1
+                lights[i].env.data = env.ctypes.data
+                lights[i].env.height, lights[i].env.width = env.shape[0], env.shape[1]
+                lights[i].env.channels = 3
+                continue
             lights[i].prim = p
             lights[i].color[:] = [float(np.float32(x)) for x in l.color]
             lights[i].watt = float(np.float32(l.watt))
@@ -143,7 +158,7 @@ class OracleScene:
         d.prim_n = n.ctypes.data if has_n else None
         d.prim_uvw = u.ctypes.data if has_u else None
         d.nmats, d.mats = len(spec.materials), mats
-        d.nlights, d.lights = len(spec.lights), lights
+        d.nlights, d.lights = len(order), lights
         d.cluster = 1 if spec.cluster == "power" else 0
         d.cam_pos[:], d.cam_dir[:], d.cam_up[:] = spec.cam_pos, spec.cam_dir, spec.cam_up
         d.fov_deg, d.sensor_width_mm = float(np.float32(spec.fov)), 36.0
@@ -232,9 +247,25 @@ class OracleScene:
         return out
 
     def light_radiance(self):
-        out = np.zeros((max(1, len(self.spec.lights)), 3), np.float32)
+        out = np.zeros((max(1, self.nlights), 3), np.float32)
         self.L.or_light_radiance(self.h, out)
-        return out[: len(self.spec.lights)]
+        return out[: self.nlights]
+
+    def env_info(self, li=None):
+        """(power-cluster flux, background flux, Distribution2D CDF tables) of the
+        environment light."""
+        li = self.env_light if li is None else li
+        h, w = self.spec.env.shape[:2]
+        out = np.zeros(2, np.float32)
+        cdf = np.zeros((h + 1) + h * (w + 1), np.float32)
+        if self.L.or_env_info(self.h, li, out, cdf.ctypes.data) != 0:
+            raise ValueError(f"light {li} is not an environment light")
+        return float(out[0]), float(out[1]), cdf
+
+    def power_cdf(self):
+        out = np.zeros(self.nlights + 1, np.float32)
+        self.L.or_power_cdf(self.h, out)
+        return out
 
     def estimate(self, film_xy, keys, rng_mode=OR_RNG_SEQ, seed=0):
         film_xy = np.ascontiguousarray(film_xy, np.float32)
@@ -340,6 +371,7 @@ def ref_lib():
         L.ref_light_sample.restype = C.c_int
         L.ref_light_sample.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64, C.c_int, _f32p]
         L.ref_light_hit.argtypes = [C.c_void_p, _f32p, C.c_int64, _f32p]
+        L.ref_light_flux.argtypes = [C.c_void_p, _f32p]
         _ref = L
     return _ref
 
@@ -441,6 +473,12 @@ class RefScene:
         out = np.zeros((len(rays), 5), np.float32)
         self.L.ref_light_hit(self.h, rays, len(rays), out)
         return out
+
+    def light_flux(self):
+        """(approximateFlux, pick pdf) per light."""
+        out = np.zeros((max(1, self.nlights), 2), np.float32)
+        self.L.ref_light_flux(self.h, out)
+        return out[: self.nlights]
 
 
 def seq_key(seed: int, a: int, b: int) -> int:

@@ -22,10 +22,23 @@ typedef struct {
     int32_t leaf;
 } or_node;
 
+/* Distribution1D (distribution1D.cpp:10-41): cdf has n + 1 entries. */
+typedef struct {
+    float*  cdf;
+    int64_t n;
+    float   delta;
+} or_dist1;
+
 typedef struct {
     int32_t prim;
     v3      radiance;        /* emit radiance (singleAreaLight.cpp:20-37) */
     int32_t back_face;
+    /* environment light (environmentLight.cpp:16-57) */
+    int32_t    type;
+    or_texture env;
+    or_dist1   marginal;     /* over rows (v) */
+    or_dist1*  conditional;  /* per row, over columns (u) */
+    float      background_flux, radius, flux;
 } or_light_state;
 
 struct or_scene {
@@ -37,6 +50,7 @@ struct or_scene {
     int32_t       nlights;
     or_light_state* lights;
     float*        power_cdf;  /* nlights + 1 (power cluster) */
+    int32_t       env_light;  /* the environment light (background sphere), -1 none */
     or_node*      nodes;
     int64_t       nnodes;
     int32_t*      ordered;    /* ordered primitive indices (BVH leaf order) */

@@ -116,6 +116,131 @@ static void camera_setup(or_scene* s) {
     memcpy(s->film_to_camera, m, sizeof(m));
 }
 
+/* ---- environment light ---------------------------------------------------------- */
+
+/* Distribution1D ctor (distribution1D.cpp:10-41) */
+static void dist1_init(or_dist1* d, const float* value, int64_t n) {
+    d->n = n;
+    d->cdf = (float*)calloc((size_t)n + 1, sizeof(float));
+    d->delta = 1.0f / (float)(size_t)n;
+    d->cdf[0] = 0.0f;
+    for (int64_t i = 1; i <= n; ++i) d->cdf[i] = d->cdf[i - 1] + value[i - 1] * d->delta;
+    const float sum = d->cdf[n];
+    if (sum > 0.0f) {
+        const float inv = 1.0f / sum;
+        for (int64_t i = 1; i <= n; ++i) d->cdf[i] *= inv;
+    } else {
+        for (int64_t i = 1; i <= n; ++i) d->cdf[i] = (float)(size_t)i * d->delta;
+    }
+    d->cdf[n] = 1.0f;
+}
+
+/* Distribution1D::continuousToDiscrete (distribution1D.cpp:115-125): scan down from n - 1 */
+static int64_t dist1_index(const or_dist1* d, float u) {
+    for (int64_t i = d->n - 1; i >= 0; --i)
+        if (d->cdf[i] <= u) return i;
+    return 0;
+}
+
+/* Distribution1D::pdfContinuous(index) (distribution1D.cpp:105-107) */
+static float dist1_pdf(const or_dist1* d, int64_t i) { return (d->cdf[i + 1] - d->cdf[i]) / d->delta; }
+
+/* Distribution1D::sampleContinuous (distribution1D.cpp:59-81) */
+static float dist1_sample(const or_dist1* d, float u, float* pdf, int64_t* index) {
+    const int64_t i = dist1_index(d, u);
+    *index = i;
+    *pdf = dist1_pdf(d, i);
+    float delta = u - d->cdf[i];
+    if (d->cdf[i + 1] > d->cdf[i]) delta /= d->cdf[i + 1] - d->cdf[i];
+    const float x = ((float)(size_t)i + delta) * d->delta;
+    return m_clamp(x, 0.0f, 1.0f);
+}
+
+static v3 tex_eval(const or_texture* t, v3 uvw);
+
+/* EnvironmentLight ctor (environmentLight.cpp:16-57) with RgbSpectrum::luminance
+ * (rgbSpectrum.cpp:33-37) and Distribution2D ctor (distribution2D.cpp:10-28) */
+static void env_init(or_light_state* L, const or_texture* t) {
+    L->env = *t;
+    const int64_t w = t->width > 0 ? t->width : 1, h = t->height > 0 ? t->height : 1;
+    float* weighted = (float*)calloc((size_t)(w * h), sizeof(float));
+    L->background_flux = 0.0f;
+    for (int64_t iy = 0; iy < h; ++iy) {
+        const float v = ((float)(size_t)iy + 0.5f) / (float)(size_t)h;
+        const float sinTheta = sinf((1.0f - v) * OR_PI);
+        for (int64_t ix = 0; ix < w; ++ix) {
+            const float u = ((float)(size_t)ix + 0.5f) / (float)(size_t)w;
+            const v3 rad = tex_eval(t, v3_make(u, v, 0.0f));
+            const float lum = 0.0126f * rad.x + 0.7152f * rad.y + 0.0722f * rad.z;
+            weighted[iy * w + ix] = lum * sinTheta;
+            L->background_flux += weighted[iy * w + ix];
+        }
+    }
+    float* rows = (float*)calloc((size_t)h, sizeof(float));
+    L->conditional = (or_dist1*)calloc((size_t)h, sizeof(or_dist1));
+    for (int64_t iy = 0; iy < h; ++iy) {
+        for (int64_t ix = 0; ix < w; ++ix) rows[iy] += weighted[iy * w + ix];
+        dist1_init(&L->conditional[iy], &weighted[iy * w], w);
+    }
+    dist1_init(&L->marginal, rows, h);
+    L->radius = 5000.0f; /* environmentLight.cpp:55-56 */
+    free(rows);
+    free(weighted);
+}
+
+/* RenderDatabase::prepareRender (renderDatabase.cpp:100-111) with
+ * _setUpLight (renderDatabase.cpp:181-189): the scene-bound radius goes to the
+ * light recorded as the environment light, which is the last light declared
+ * once an environment light exists; it reaches the environment light only when
+ * that light is the last one. Bound = the BVH root's (bvhAccelerator.cpp:34-38),
+ * isEmpty (tAabb.ipp:30-34), extent().length() (tVector.ipp:316-328). */
+static void env_radius(or_scene* s) {
+    if (s->env_light < 0 || s->env_light != s->nlights - 1) return;
+    const or_aabb b = s->nodes[0].bound;
+    const int empty = (b.mn.x == FLT_MAX && b.mn.y == FLT_MAX && b.mn.z == FLT_MAX) ||
+                      (b.mx.x == FLT_MIN && b.mx.y == FLT_MIN && b.mx.z == FLT_MIN);
+    if (empty) return;
+    const v3 e = v3_sub(b.mx, b.mn);
+    float l2 = 0.0f;
+    l2 += e.x * e.x;
+    l2 += e.y * e.y;
+    l2 += e.z * e.z;
+    s->lights[s->env_light].radius = sqrtf(l2) * 0.5f;
+}
+
+/* PowerLightCluster + Distribution1D (powerLightCluster.cpp:9-25, distribution1D.cpp:10-40);
+ * approximateFlux: SingleAreaLight (singleAreaLight.cpp:168-184), EnvironmentLight
+ * (environmentLight.cpp:137-139); luminance (rgbSpectrum.cpp:33-37) */
+static void power_cdf_init(or_scene* s) {
+    const or_scene_desc* desc = &s->desc;
+    s->power_cdf = (float*)calloc((size_t)desc->nlights + 1, sizeof(float));
+    if (desc->nlights <= 0) return;
+    const float delta = 1.0f / (float)(size_t)desc->nlights;
+    s->power_cdf[0] = 0.0f;
+    for (int32_t i = 1; i <= desc->nlights; ++i) {
+        or_light_state* L = &s->lights[i - 1];
+        float flux;
+        if (L->type == OR_LIGHT_ENVIRONMENT) {
+            flux = L->background_flux * (float)12.566370614359172 * L->radius * L->radius;
+        } else {
+            const v3 le = L->radiance;
+            const float lum = 0.0126f * le.x + 0.7152f * le.y + 0.0722f * le.z;
+            flux = lum * prim_area(&s->prims[L->prim]) * OR_PI;
+            if (flux <= 0.0f) flux = 1.0f;
+        }
+        L->flux = flux;
+        s->power_cdf[i] = s->power_cdf[i - 1] + flux * delta;
+    }
+    const float sum = s->power_cdf[desc->nlights];
+    if (sum > 0.0f) {
+        const float inv = 1.0f / sum;
+        for (int32_t i = 1; i <= desc->nlights; ++i) s->power_cdf[i] *= inv;
+    } else {
+        for (int32_t i = 1; i <= desc->nlights; ++i) s->power_cdf[i] = (float)(size_t)i * delta;
+    }
+    s->power_cdf[desc->nlights] = 1.0f;
+}
+
 or_scene* or_scene_create(const or_scene_desc* desc) {
     or_scene* s = (or_scene*)calloc(1, sizeof(or_scene));
     s->desc = *desc;
@@ -160,10 +285,18 @@ or_scene* or_scene_create(const or_scene_desc* desc) {
     s->mats = (or_material*)calloc((size_t)(desc->nmats > 0 ? desc->nmats : 1), sizeof(or_material));
     memcpy(s->mats, desc->mats, sizeof(or_material) * (size_t)desc->nmats);
     s->nlights = desc->nlights;
+    s->env_light = -1;
     s->lights = (or_light_state*)calloc((size_t)(desc->nlights > 0 ? desc->nlights : 1), sizeof(or_light_state));
     for (int32_t i = 0; i < desc->nlights; ++i) {
-        /* SingleAreaLight ctor (singleAreaLight.cpp:20-37) */
         const or_light* L = &desc->lights[i];
+        s->lights[i].type = L->type;
+        s->lights[i].prim = -1;
+        if (L->type == OR_LIGHT_ENVIRONMENT) {
+            env_init(&s->lights[i], &L->env);
+            s->env_light = i; /* the background sphere of the last environment light */
+            continue;
+        }
+        /* SingleAreaLight ctor (singleAreaLight.cpp:20-37) */
         const v3 color = v3_load(L->color);
         const v3 unit = v3_divs(color, v3_sum(color));
         const v3 total = v3_scale(unit, L->watt);
@@ -173,28 +306,6 @@ or_scene* or_scene_create(const or_scene_desc* desc) {
         s->lights[i].back_face = L->back_face;
         s->prims[L->prim].light = i; /* Primitive::setAreaLight: the last binding wins */
     }
-    /* PowerLightCluster + Distribution1D (powerLightCluster.cpp:9-25, distribution1D.cpp:10-40);
-     * approximateFlux (singleAreaLight.cpp:168-184), luminance (rgbSpectrum.cpp:33-37) */
-    s->power_cdf = (float*)calloc((size_t)desc->nlights + 1, sizeof(float));
-    if (desc->nlights > 0) {
-        const float delta = 1.0f / (float)(size_t)desc->nlights;
-        s->power_cdf[0] = 0.0f;
-        for (int32_t i = 1; i <= desc->nlights; ++i) {
-            const v3 le = s->lights[i - 1].radiance;
-            const float lum = 0.0126f * le.x + 0.7152f * le.y + 0.0722f * le.z;
-            float flux = lum * prim_area(&s->prims[s->lights[i - 1].prim]) * OR_PI;
-            if (flux <= 0.0f) flux = 1.0f;
-            s->power_cdf[i] = s->power_cdf[i - 1] + flux * delta;
-        }
-        const float sum = s->power_cdf[desc->nlights];
-        if (sum > 0.0f) {
-            const float inv = 1.0f / sum;
-            for (int32_t i = 1; i <= desc->nlights; ++i) s->power_cdf[i] *= inv;
-        } else {
-            for (int32_t i = 1; i <= desc->nlights; ++i) s->power_cdf[i] = (float)(size_t)i * delta;
-        }
-        s->power_cdf[desc->nlights] = 1.0f;
-    }
     camera_setup(s);
     /* StratifiedSampler ctor (stratifiedSampler.cpp:14-20), math::nearest_lower_square_number (math.cpp:18-23) */
     {
@@ -202,11 +313,19 @@ or_scene* or_scene_create(const or_scene_desc* desc) {
         s->strata_base = (int)(size_t)sqrt((double)(lower * lower));
     }
     or_build_bvh(s);
+    env_radius(s);
+    power_cdf_init(s);
     return s;
 }
 
 void or_scene_destroy(or_scene* s) {
     if (!s) return;
+    for (int32_t i = 0; i < s->nlights; ++i) {
+        if (s->lights[i].type != OR_LIGHT_ENVIRONMENT) continue;
+        for (int32_t r = 0; r < s->lights[i].env.height; ++r) free(s->lights[i].conditional[r].cdf);
+        free(s->lights[i].conditional);
+        free(s->lights[i].marginal.cdf);
+    }
     free(s->prims); free(s->mats); free(s->lights); free(s->power_cdf);
     free(s->nodes); free(s->ordered);
     free(s);
@@ -218,6 +337,8 @@ typedef struct {
     int32_t prim;
     v3 P, Ng, Ns, uvw, wi, wo;
 } or_si;
+
+static int background_hit(const or_scene* s, v3 d, float maxT, or_si* si);
 
 /* Triangle::_positionToBarycentric (triangle.cpp:233-262) */
 static v3 barycentric(const or_prim* p, v3 P) {
@@ -282,6 +403,10 @@ void or_intersect_detail(const or_scene* s, const float* rays, int64_t n, float*
         float t;
         if (scene_intersect(s, v3_load(r), v3_load(r + 4), r[3], r[7], &si, &t)) {
             o[0] = (float)si.prim; o[1] = t;
+            v3_store(o + 2, si.P); v3_store(o + 5, si.Ng); v3_store(o + 8, si.Ns); v3_store(o + 11, si.uvw);
+        } else if (background_hit(s, v3_load(r + 4), r[7], &si)) {
+            /* the background sphere (prim -2): maxT becomes its radius (infiniteSphere.cpp:36-45) */
+            o[0] = -2.0f; o[1] = 1000000.0f;
             v3_store(o + 2, si.P); v3_store(o + 5, si.Ng); v3_store(o + 8, si.Ns); v3_store(o + 11, si.uvw);
         } else {
             memset(o, 0, 14 * sizeof(float));
@@ -487,6 +612,7 @@ void or_bsdf_query(const or_scene* s, int mi, const float* in, int64_t n, float*
 static int can_emit(const or_light_state* L, v3 dir, v3 N) { return (v3_dot(dir, N) >= 0.0f) != (L->back_face != 0); }
 
 typedef struct { v3 pos, Ns, radiance; float pdfW; int valid; } or_light_smp;
+static or_light_smp env_sample(const or_light_state* L, float sx, float sy);
 
 /* SingleAreaLight::evaluateDirectSample (singleAreaLight.cpp:54-85) with
  * Triangle::evaluatePositionSample (triangle.cpp:162-193) + SampleWarp::uniformTriangleUv
@@ -498,6 +624,11 @@ static or_light_smp light_sample(const or_scene* s, int li, v3 target, or_rng* r
     or_light_smp out;
     memset(&out, 0, sizeof(out));
     const or_light_state* L = &s->lights[li];
+    if (L->type == OR_LIGHT_ENVIRONMENT) {
+        float sx, sy;
+        rng_u2(rng, bounce, SL_LPOS0, SL_LPOS1, &sx, &sy);
+        return env_sample(L, sx, sy);
+    }
     const or_prim* p = &s->prims[L->prim];
     v3 P, Ns;
     float pdfA;
@@ -565,8 +696,94 @@ static float light_pdfW(const or_scene* s, int li, const or_si* si, v3 target) {
     return pdfA * v3_len2(emitVec) / c;
 }
 
+/* SphericalMapper::mappingToUvw (sphericalMapper.cpp:14-29), y up */
+static v3 spherical_uvw(v3 d) {
+    const v3 u = v3_normalize(d);
+    const float theta = acosf(m_clamp(u.y, -1.0f, 1.0f));
+    float phi = atan2f(u.x, u.z);
+    if (phi < 0.0f) phi += OR_TWO_PI;
+    return v3_make(phi * (float)0.1591549430918953, (OR_PI - theta) * OR_INV_PI, 0.0f);
+}
+
+/* Scene::isIntersecting's background branch (scene.cpp:47-61) with
+ * InfiniteSphere::isIntersecting / evaluateSurfaceDetail (infiniteSphere.cpp:36-59):
+ * only unbounded rays reach it; the surface position is the ray direction. */
+static int background_hit(const or_scene* s, v3 d, float maxT, or_si* si) {
+    if (s->env_light < 0 || maxT < FLT_MAX) return 0;
+    memset(si, 0, sizeof(*si));
+    si->prim = -1;
+    si->wi = v3_neg(d);
+    si->P = d;
+    si->uvw = spherical_uvw(d);
+    return 1;
+}
+
+/* EnvironmentLight::emittance (environmentLight.cpp:59-65) */
+static v3 env_emittance(const or_light_state* L, const or_si* si) { return tex_eval(&L->env, si->uvw); }
+
+/* Distribution2D::pdfContinuous (distribution2D.cpp:46-53) */
+static float env_pdf_uv(const or_light_state* L, float u, float v) {
+    const int64_t iy = dist1_index(&L->marginal, v);
+    const float pdfY = dist1_pdf(&L->marginal, iy);
+    const or_dist1* c = &L->conditional[iy];
+    const float pdfX = dist1_pdf(c, dist1_index(c, u));
+    return pdfX * pdfY;
+}
+
+/* EnvironmentLight::evaluateDirectPdfW (environmentLight.cpp:91-105) */
+static float env_pdfW(const or_light_state* L, const or_si* si) {
+    const float sinTheta = sinf((1.0f - si->uvw.y) * OR_PI);
+    if (sinTheta <= 0.0f) return 0.0f;
+    return env_pdf_uv(L, si->uvw.x, si->uvw.y) / (2.0f * OR_PI * OR_PI * sinTheta);
+}
+
+/* EnvironmentLight::evaluateDirectSample (environmentLight.cpp:67-89) with seed (sx, sy),
+ * Distribution2D::sampleContinuous (distribution2D.cpp:30-44) and
+ * InfiniteSphere::uvwToPosition (infiniteSphere.cpp:61-76, z up, radius 1e6). */
+static or_light_smp env_sample(const or_light_state* L, float sx, float sy) {
+    or_light_smp out;
+    memset(&out, 0, sizeof(out));
+    float pdfY, pdfX;
+    int64_t iy, ix;
+    const float v = dist1_sample(&L->marginal, sy, &pdfY, &iy);
+    const float u = dist1_sample(&L->conditional[iy], sx, &pdfX, &ix);
+    const float uvPdf = pdfX * pdfY;
+    const float theta = (1.0f - v) * OR_PI;
+    const float phi = u * OR_TWO_PI;
+    const float cosTheta = cosf(theta);
+    const float sinT = sqrtf(1.0f - cosTheta * cosTheta);
+    const v3 dir = v3_make(cosf(phi) * sinT, sinf(phi) * sinT, cosTheta);
+    const v3 pos = v3_scale(v3_normalize(dir), 1000000.0f);
+    const float sinTheta = sinf((1.0f - v) * OR_PI);
+    if (sinTheta <= 0.0f) return out;
+    out.pos = pos;
+    out.radiance = tex_eval(&L->env, v3_make(u, v, 0.0f));
+    out.pdfW = uvPdf / (2.0f * OR_PI * OR_PI * sinTheta);
+    /* DirectLightSample::isValid (directLightSample.cpp:57-59) */
+    out.valid = !v3_is_zero(out.radiance) && out.pdfW > 0.0f;
+    return out;
+}
+
 void or_light_radiance(const or_scene* s, float* out) {
     for (int32_t i = 0; i < s->nlights; ++i) v3_store(out + 3 * i, s->lights[i].radiance);
+}
+
+int or_env_info(const or_scene* s, int li, float* out, float* cdf) {
+    if (li < 0 || li >= s->nlights || s->lights[li].type != OR_LIGHT_ENVIRONMENT) return -1;
+    const or_light_state* L = &s->lights[li];
+    out[0] = L->flux;
+    out[1] = L->background_flux;
+    if (cdf) {
+        const int64_t h = L->marginal.n, w = L->env.width;
+        memcpy(cdf, L->marginal.cdf, sizeof(float) * (size_t)(h + 1));
+        for (int64_t r = 0; r < h; ++r)
+            memcpy(cdf + (h + 1) + r * (w + 1), L->conditional[r].cdf, sizeof(float) * (size_t)(w + 1));
+    }
+    return 0;
+}
+
+void or_power_cdf(const or_scene* s, float* out) {
+    memcpy(out, s->power_cdf, sizeof(float) * (size_t)(s->nlights + 1));
 }
 
 void or_light_sample(const or_scene* s, int li, const float* in, int64_t n, int nu, float* out) {
@@ -574,13 +791,23 @@ void or_light_sample(const or_scene* s, int li, const float* in, int64_t n, int 
      * (y, x) (right-to-left argument evaluation), a rectangle draws (s, t)
      * pairs until t <= short/long. */
     const or_light_state* L = &s->lights[li];
-    const or_prim* p = &s->prims[L->prim];
     for (int64_t i = 0; i < n; ++i) {
         const float* q = in + (3 + nu) * i;
         const float* vals = q + 3;
         float* o = out + 11 * i;
         or_light_smp r;
         memset(&r, 0, sizeof(r));
+        if (L->type == OR_LIGHT_ENVIRONMENT) {
+            /* Vector2R(nextReal(), nextReal()): the first draw is the seed's y */
+            r = env_sample(L, vals[1], vals[0]);
+            v3_store(o, r.pos);
+            v3_store(o + 3, r.Ns);
+            v3_store(o + 6, r.radiance);
+            o[9] = r.pdfW;
+            o[10] = (float)r.valid;
+            continue;
+        }
+        const or_prim* p = &s->prims[L->prim];
         v3 P, Ns;
         if (p->type == OR_PRIM_RECTANGLE) {
             v3 eA = p->e1, eB = p->e2;
@@ -630,7 +857,14 @@ void or_light_hit(const or_scene* s, const float* rays, int64_t n, float* out) {
         float* o = out + 5 * i;
         o[0] = -1.0f; o[1] = o[2] = o[3] = o[4] = 0.0f;
         or_si si;
-        if (!scene_intersect(s, v3_load(r), v3_load(r + 4), r[3], r[7], &si, NULL)) continue;
+        if (!scene_intersect(s, v3_load(r), v3_load(r + 4), r[3], r[7], &si, NULL)) {
+            if (!background_hit(s, v3_load(r + 4), r[7], &si)) continue;
+            const or_light_state* L = &s->lights[s->env_light];
+            o[0] = (float)s->env_light;
+            v3_store(o + 1, env_emittance(L, &si));
+            o[4] = env_pdfW(L, &si);
+            continue;
+        }
         const int li = s->prims[si.prim].light;
         if (li < 0) continue;
         o[0] = (float)li;
@@ -742,6 +976,14 @@ static v3 direct_light(const or_scene* s, const or_si* si0, const or_material* m
                     const float w = mis_power(bs.pdf, lpdf);
                     Ld = v3_add(Ld, v3_divs(v3_mul(v3_scale(rad, w), factor), bs.pdf));
                 }
+            } else if (background_hit(s, d, FLT_MAX, &hit) && s->env_light == li) {
+                /* the background sphere's area light is the environment light */
+                const or_light_state* L = &s->lights[li];
+                const v3 factor = v3_scale(bs.f, v3_absdot(bs.L, Ns));
+                const v3 rad = env_emittance(L, &hit);
+                const float lpdf = env_pdfW(L, &hit);
+                const float w = mis_power(bs.pdf, lpdf);
+                Ld = v3_add(Ld, v3_divs(v3_mul(v3_scale(rad, w), factor), bs.pdf));
             }
         }
     }
@@ -758,7 +1000,18 @@ static v3 estimate(const or_scene* s, v3 o, v3 d, or_rng* rng, or_counts* cnt) {
     for (int bounce = 0; bounce < s->desc.max_depth; ++bounce) {
         or_si si;
         cnt->closest++;
-        if (!scene_intersect(s, ro, rd, OR_RAY_EPS, FLT_MAX, &si, NULL)) break;
+        if (!scene_intersect(s, ro, rd, OR_RAY_EPS, FLT_MAX, &si, NULL)) {
+            /* the background sphere (scene.cpp:47-61): emission when counted; its
+             * absorber BSDF is not specular, so one light is still picked
+             * (ptEstimator.cpp:68-82) but DirectLightEvaluator returns zero for an
+             * absorber (directLightEvaluator.cpp:33-35), and the absorber's sample is
+             * invalid, which ends the path (ptEstimator.cpp:86-90) */
+            if (!background_hit(s, rd, FLT_MAX, &si)) break;
+            if (countEmit) L = v3_add(L, v3_mul(beta, env_emittance(&s->lights[s->env_light], &si)));
+            float pickPdf;
+            (void)pick_light(s, rng, bounce, &pickPdf);
+            break;
+        }
         const or_prim* p = &s->prims[si.prim];
         const or_material* m = &s->mats[p->mat];
         if (p->light >= 0 && countEmit) L = v3_add(L, v3_mul(beta, light_emittance(s, p->light, &si)));

@@ -36,6 +36,7 @@ enum { OR_FILTER_BOX = 0, OR_FILTER_GAUSSIAN = 1, OR_FILTER_CONE = 2, OR_FILTER_
 enum { OR_SAMPLER_RANDOM = 0, OR_SAMPLER_STRATIFIED = 1 };
 enum { OR_CLUSTER_UNIFORM = 0, OR_CLUSTER_POWER = 1 };
 enum { OR_RNG_SEQ = 0, OR_RNG_PHILOX = 1 };
+enum { OR_LIGHT_AREA = 0, OR_LIGHT_ENVIRONMENT = 1 };
 
 /* A texture: constant when data == NULL. */
 typedef struct {
@@ -53,10 +54,12 @@ typedef struct {
 } or_material;
 
 typedef struct {
-    int32_t prim;          /* index of the emitting primitive */
+    int32_t prim;          /* index of the emitting primitive (area lights) */
     float   color[3];
     float   watt;
     int32_t back_face;
+    int32_t    type;       /* OR_LIGHT_* */
+    or_texture env;        /* environment radiance (image, 3 channels, repeat) */
 } or_light;
 
 typedef struct {
@@ -118,8 +121,15 @@ void or_bsdf_query(const or_scene* s, int m, const float* in, int64_t n, float* 
 /* Light li direct sample towards target, explicit uniforms: in n x (3 + nu). */
 void or_light_sample(const or_scene* s, int li, const float* in, int64_t n, int nu, float* out);
 void or_light_hit(const or_scene* s, const float* rays, int64_t n, float* out);
-/* Emitted radiance Le of every light (n x 3). */
+/* Emitted radiance Le of every light (n x 3; environment lights: 0). */
 void or_light_radiance(const or_scene* s, float* out);
+/* Environment light li: the power cluster's flux estimate and background flux
+ * (out[0], out[1]), and its Distribution2D tables: marginal CDF (h + 1) then
+ * the h conditional CDFs (w + 1 each) into cdf when non-null. Returns 0, or -1
+ * when li is not an environment light. */
+int or_env_info(const or_scene* s, int li, float* out, float* cdf);
+/* Power-cluster CDF over the lights (nlights + 1). */
+void or_power_cdf(const or_scene* s, float* out);
 
 /* PtEstimator::estimate from given film positions. Sequential layout: stream
  * keys[i]. Philox layout: keys[i] = (pixel << 32) | sample, seed = seed. */

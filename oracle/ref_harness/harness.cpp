@@ -113,6 +113,11 @@ struct RefScene {
     std::shared_ptr<EnergyEstimator> estimator;
     std::shared_ptr<Sampler>         sampler;
 
+    // the environment light's background sphere and the index RenderDatabase
+    // records for it (renderDatabase.cpp:181-189)
+    std::shared_ptr<Primitive> background;
+    size_t                     envIndex = static_cast<size_t>(-1);
+
     std::unordered_map<const Primitive*, int32_t> primIndex;
     std::unordered_map<const Light*, int32_t>     lightIndex;
     int32_t width = 0, height = 0, tileX = 16, tileY = 16, spp = 0, maxDepth = 0;
@@ -177,7 +182,6 @@ void make_mesh_from_files(RefScene* s, const std::shared_ptr<SdData>& data) {
 }
 
 void set_up(RefScene* s, const std::shared_ptr<SdData>& data, std::string_view text) {
-    std::shared_ptr<Primitive> background;
     switch (data->classType()) {
         case ESdClassType::FILM: s->filmData = data; s->filmText = std::string(text); break;
         case ESdClassType::CAMERA: s->cameraData = data; break;
@@ -201,7 +205,9 @@ void set_up(RefScene* s, const std::shared_ptr<SdData>& data, std::string_view t
             break;
         }
         case ESdClassType::LIGHT: {
-            auto light = instantiator::makeLight(data, s->primitives, background);
+            // RenderDatabase::_setUpLight (renderDatabase.cpp:181-189)
+            auto light = instantiator::makeLight(data, s->primitives, s->background);
+            if (s->background) s->envIndex = s->lights.size();
             s->lights.push_back(light);
             break;
         }
@@ -254,8 +260,15 @@ void* ref_scene_create(const char* text, size_t len) {
     s->accelerator = instantiator::makeAccelerator(s->acceleratorData, s->intersectors);
     s->buildSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     s->counting  = std::make_shared<CountingAccelerator>(s->accelerator);
+    // environment light: scene-bound radius, background sphere (renderDatabase.cpp:100-120)
+    if (s->envIndex != static_cast<size_t>(-1)) {
+        AABB3R sceneBound;
+        s->accelerator->evaluateBound(&sceneBound);
+        if (!sceneBound.isEmpty()) s->lights[s->envIndex]->setSceneBoundRadius(sceneBound.extent().length() * 0.5f);
+    }
     s->cluster   = instantiator::makeLightCluster(s->clusterData, s->lights);
     s->scene     = std::make_shared<Scene>(s->counting, s->cluster);
+    if (s->background) s->scene->setBackgroundSphere(s->background.get());
     s->estimator = instantiator::makeEstimator(s->rendererData);
     s->sampler   = instantiator::makeSampler(s->rendererData);
     s->width     = filmCopy->findInt32("image-width", 1024);
@@ -345,7 +358,8 @@ void ref_trace(void* h, const float* rays, int64_t n, int any_hit, int32_t* out_
 }
 
 // Full surface interaction of a closest hit (Scene::isIntersecting, scene.cpp:30-65):
-// out per ray: prim, t, P(3), Ng(3), Ns(3), uvw(3) -> 14 floats (prim as float).
+// out per ray: prim, t, P(3), Ng(3), Ns(3), uvw(3) -> 14 floats (prim as float;
+// -2 for the background sphere of an environment light, -1 for no hit).
 void ref_intersect_detail(void* h, const float* rays, int64_t n, float* out) {
     auto* s = static_cast<RefScene*>(h);
     for (int64_t i = 0; i < n; ++i) {
@@ -354,7 +368,9 @@ void ref_intersect_detail(void* h, const float* rays, int64_t n, float* out) {
         SurfaceIntersection si;
         float* o = out + 14 * i;
         if (s->scene->isIntersecting(ray, si)) {
-            o[0] = static_cast<float>(s->primIndex.at(si.primitiveInfo().primitive()));
+            // the environment's background sphere is reported as primitive -2
+            const auto it = s->primIndex.find(si.primitiveInfo().primitive());
+            o[0] = it == s->primIndex.end() ? -2.0f : static_cast<float>(it->second);
             o[1] = ray.maxT();
             put3(o + 2, si.surfaceDetail().position());
             put3(o + 5, si.surfaceDetail().geometryNormal());
@@ -580,6 +596,16 @@ void ref_light_hit(void* h, const float* rays, int64_t n, float* out) {
         al->emittance(si).transformToRgb(&le);
         put3(o + 1, le);
         o[4] = al->evaluateDirectPdfW(si, v3(r));
+    }
+}
+
+// Light::approximateFlux and LightCluster::evaluatePickLightPdf of every light.
+// out: nlights x 2 (flux, pick pdf).
+void ref_light_flux(void* h, float* out) {
+    auto* s = static_cast<RefScene*>(h);
+    for (size_t i = 0; i < s->lights.size(); ++i) {
+        out[2 * i]     = s->lights[i]->approximateFlux();
+        out[2 * i + 1] = s->scene->evaluatePickLightPdf(s->lights[i].get());
     }
 }
 

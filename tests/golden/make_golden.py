@@ -5,7 +5,7 @@ compiled by oracle/Makefile from the reference's own sources). Every fixture
 is inputs plus the reference's outputs for them; the tests compare the oracle
 (and, on the GPU, the product) against these files.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [scene ...]
 """
 from __future__ import annotations
 
@@ -49,13 +49,24 @@ def scene_set():
     equal = scenes.procedural_spheres(n_spheres=4, level=2, width=8, height=8, spp=1, seed=7, with_normals=False,
                                       offset=(0.0, 0.0, 0.0))
     equal.split_mode = "equal"
-    return [("cornell", cornell), ("cornell_gauss", cornell_g), ("spheres", spheres), ("spheres_equal", equal)]
+    # environment light (sky texture with a wide sun) + one rectangle light: declared
+    # last with the power cluster (scene-bound radius), and first with the uniform
+    # cluster (default radius; both lights picked)
+    sky = scenes.sky_spheres(n_spheres=4, level=2, width=24, height=16, spp=2, max_depth=6, env_size=(64, 32),
+                             sun_diameter_deg=12.0, offset=(0.0, 0.0, 0.0))
+    sky_first = scenes.sky_spheres(n_spheres=3, level=1, width=16, height=12, spp=2, max_depth=5, env_size=(48, 24),
+                                   sun_diameter_deg=16.0, env_slot=0, offset=(0.0, 0.0, 0.0), cluster="uniform")
+    return [("cornell", cornell), ("cornell_gauss", cornell_g), ("spheres", spheres), ("spheres_equal", equal),
+            ("sky", sky), ("sky_envfirst", sky_first)]
 
 
 def main():
     if not ref_available():
         raise SystemExit("oracle/_ref/libcadise_ref.so is not built (make -C oracle ref)")
+    only = set(sys.argv[1:])
     for name, spec in scene_set():
+        if only and name not in only:
+            continue
         side = tempfile.mkdtemp()
         R = RefScene(spec.to_crsd(sidecar_dir=side))
         out = {}
@@ -80,7 +91,9 @@ def main():
         out["film"] = R.render_linear(seed=31, threads=4)
         out["light_hit"] = R.light_hit(rays_n)
         # single-function vectors
-        if name == "cornell" or name == "spheres":
+        if spec.env is not None:
+            out["light_flux"] = R.light_flux()
+        if name in ("cornell", "spheres", "sky", "sky_envfirst"):
             m = len(spec.materials)
             q = np.zeros((m, 300, 14), np.float32)
             for mi in range(m):
@@ -92,7 +105,7 @@ def main():
                 q[mi, :, 12:14] = rng.uniform(0, 1, (300, 2))
             bsdf = np.stack([R.bsdf_query(spec.materials[mi].name, q[mi]) for mi in range(min(m, 6))])
             out.update(bsdf_in=q[: len(bsdf)], bsdf_out=bsdf)
-            nl = len(spec.lights)
+            nl = R.nlights
             lq = np.zeros((nl, 200, 3 + 16), np.float32)
             lq[:, :, 0:3] = rng.uniform(lo, hi, (nl, 200, 3)) if name == "cornell" else rng.uniform(-10, 10, (nl, 200, 3))
             lq[:, :, 3:] = rng.uniform(0, 1, (nl, 200, 16))

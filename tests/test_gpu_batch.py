@@ -27,6 +27,8 @@ def batch_set(ctx):
     specs = scenes.batch_scenes(seed=99, n_scenes=6, tris=(1e3, 5e4), res=(24, 64), spp=(1, 8), depth=(2, 8))
     specs.append(scenes.cornell_box(width=40, height=32, spp=4, filter="gaussian"))
     specs.append(scenes.textured_spheres(n_spheres=3, level=2, tex_res=16, width=32, height=32, spp=3, max_depth=5))
+    specs.append(scenes.sky_spheres(n_spheres=4, level=2, width=40, height=24, spp=3, max_depth=6, env_size=(64, 32),
+                                    sun_diameter_deg=6.0))
     big = scenes.procedural_spheres(seed=5, n_spheres=6, level=3, width=256, height=200, spp=6, max_depth=9)
     specs.append(big)
     seeds = [1000 + 17 * i for i in range(len(specs))]

@@ -83,7 +83,7 @@ def test_render_progressive_accumulate(ctx, passwise):
         np.testing.assert_allclose(full, part, rtol=2e-5, atol=1e-6)
 
 
-@pytest.mark.parametrize("case", ["spheres", "cornell_gaussian", "textured"])
+@pytest.mark.parametrize("case", ["spheres", "cornell_gaussian", "textured", "sky"])
 def test_streaming_matches_passwise(ctx, case):
     """The streaming wavefront (default) and the pass-by-pass wavefront draw the
     same paths: ray counts equal exactly, films up to float summation order."""
@@ -91,7 +91,9 @@ def test_streaming_matches_passwise(ctx, case):
     spec = {"spheres": lambda: small_spheres(n_spheres=6, level=3, width=96, height=64, spp=5, max_depth=7),
             "cornell_gaussian": lambda: scenes.cornell_box(width=48, height=40, spp=6, max_depth=9, filter="gaussian"),
             "textured": lambda: scenes.textured_spheres(n_spheres=3, level=2, tex_res=16, width=40, height=40, spp=3,
-                                                        max_depth=5)}[case]()
+                                                        max_depth=5),
+            "sky": lambda: scenes.sky_spheres(n_spheres=5, level=2, width=48, height=32, spp=4, max_depth=6,
+                                              env_size=(128, 64), sun_diameter_deg=4.0)}[case]()
     sc = Scene.from_spec(ctx, spec)
     a = sc.render(seed=21)
     sa = sc.last_stats()
@@ -100,6 +102,25 @@ def test_streaming_matches_passwise(ctx, case):
     for k in ("camera_rays", "continuation_rays", "mis_rays", "shadow_rays", "samples", "box_tests", "tri_tests"):
         assert sa[k] == sb[k], (k, sa[k], sb[k])
     np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("variant", ["last_power", "first_uniform"])
+def test_render_sky_environment_vs_oracle(ctx, tmp_path, variant):
+    """N3: a procedural sky (sun disc of 3 degrees) as the environment light plus
+    one rectangle light, loaded from CRSD + .hdr; camera rays that reach the
+    background, NEE towards the sky (Distribution2D sampling, z-up frame) and
+    BSDF-sampled rays reaching it (y-up lookup, MIS)."""
+    from cadise_amd.api import Scene
+    first = variant == "first_uniform"
+    spec = scenes.sky_spheres(n_spheres=6, level=3, width=64, height=40, spp=4, max_depth=6, env_size=(256, 128),
+                              sun_diameter_deg=3.0, env_slot=0 if first else None, offset=(0.0, 0.0, 0.0),
+                              cluster="uniform" if first else "power")
+    path = tmp_path / "sky.crsd"
+    path.write_text(spec.to_crsd(sidecar_dir=str(tmp_path)))
+    sc = Scene.from_crsd(ctx, str(path))
+    film = sc.render(seed=4242)
+    orf = OracleScene(spec).render_linear(4242, rng_mode=OR_RNG_PHILOX)
+    _compare(film, orf, f"sky {variant}", frac_bar=0.98)
 
 
 def test_gaussian_filter_vs_oracle(ctx):

@@ -154,3 +154,51 @@ def test_texture_layout_and_sharing(ctx):
     with pytest.raises(MptError) as e:
         sc.adjoint(np.zeros((8, 8, 3), np.float32))
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("name", ["sky", "sky_envfirst"])
+def test_environment_light_host(ctx, name, tmp_path):
+    """N3 host half: the .hdr reader plus the loader's flips give the oracle's
+    texture bit for bit, and the light cluster's pick probabilities (environment
+    flux with the scene-bound or the default radius) equal the compiled
+    reference's."""
+    spec = dict(scene_set())[name]
+    sc = Scene.from_crsd_text(ctx, spec.to_crsd(sidecar_dir=str(tmp_path)))
+    st = sc.stats()
+    h, w = spec.env.shape[:2]
+    env = [i for i in range(st["ntextures"]) if sc.texture_info(i)[:3] == (w, h, 3)]
+    assert len(env) == 1
+    assert np.array_equal(bits(sc.get_texture(env[0])), bits(spec.env))
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    assert np.array_equal(bits(sc.light_pick_pdf()), bits(g["light_flux"][:, 1]))
+    with pytest.raises(MptError) as e:
+        sc.set_texture(env[0], spec.env)
+    assert e.value.code == -3
+    li = spec.light_order().index(("environment", None, -1))
+    with pytest.raises(MptError) as e:
+        sc.set_light_radiance(li, (1.0, 1.0, 1.0))
+    assert e.value.code == -3
+
+
+def test_environment_light_crsd_errors(ctx, tmp_path):
+    spec = scenes.sky_spheres(n_spheres=1, level=0, width=8, height=8, spp=1, env_size=(16, 8))
+    text = spec.to_crsd(sidecar_dir=str(tmp_path))
+    hdr = os.path.join(str(tmp_path), "env.hdr")
+    with pytest.raises(MptError) as e:       # only Radiance .hdr images
+        Scene.from_crsd_text(ctx, text.replace(hdr, os.path.join(str(tmp_path), "env.png")))
+    assert e.value.code == -3
+    with pytest.raises(MptError) as e:       # missing image
+        Scene.from_crsd_text(ctx, text.replace(hdr, os.path.join(str(tmp_path), "none.hdr")))
+    assert e.value.code == -5
+    block = f'light {{\n    [type string "environment"]\n    [hdr-filename string "{hdr}"]\n}}\n'
+    with pytest.raises(MptError) as e:       # one background sphere
+        Scene.from_crsd_text(ctx, text + block)
+    assert e.value.code == -3
+    with open(hdr, "r+b") as f:              # truncated pixel data
+        data = f.read()
+        f.seek(0)
+        f.truncate()
+        f.write(data[: len(data) // 2])
+    with pytest.raises(MptError) as e:
+        Scene.from_crsd_text(ctx, text)
+    assert e.value.code in (-2, -5)

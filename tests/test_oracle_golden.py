@@ -13,6 +13,10 @@ patterns), on every fixture:
          uniforms; light emittance and pdf at hits
   A10/A18 the sampling renderer's linear film (stratified sampler, box and
          Gaussian filters, tile clamp)
+  N3     environment light: background-sphere hits (uvw of the ray direction),
+         emittance and pdf, Distribution2D sampling, the light's flux estimate
+         with the scene-bound radius (declared last) or the default radius
+         (declared first), and paths through the background
 """
 import os
 
@@ -100,3 +104,23 @@ def test_light_sample_vectors(case):
     for li in range(len(g["light_in"])):
         out = O.light_sample(li, g["light_in"][li], 16)
         assert np.array_equal(bits(out), bits(g["light_out"][li])), (name, li)
+
+
+def test_light_flux_and_pick(case):
+    """Light::approximateFlux of the environment light and the cluster's pick
+    probabilities (power: flux-proportional; uniform: 1/n)."""
+    name, O, g = case
+    if "light_flux" not in g:
+        pytest.skip("no environment light in this scene")
+    flux, _, cdf = O.env_info()
+    assert bits(flux) == bits(g["light_flux"][O.env_light, 0]), name
+    pick = g["light_flux"][:, 1]
+    if O.spec.cluster == "power":
+        assert np.array_equal(bits(np.diff(O.power_cdf())), bits(pick)), name
+    else:
+        assert np.all(pick == np.float32(1.0) / np.float32(O.nlights)), name
+    # the Distribution2D tables: nondecreasing rows ending at exactly 1
+    h, w = O.spec.env.shape[:2]
+    marg, cond = cdf[: h + 1], cdf[h + 1:].reshape(h, w + 1)
+    assert marg[0] == 0 and marg[-1] == 1 and np.all(np.diff(marg) >= 0)
+    assert np.all(cond[:, 0] == 0) and np.all(cond[:, -1] == 1) and np.all(np.diff(cond, axis=1) >= 0)
