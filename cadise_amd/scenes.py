@@ -460,14 +460,15 @@ def _random_rotations(rng, n):
 
 
 def traversal_stress(seed=0x20261203 ^ 0x3, n_instances=400, n_bases=4, grid=(125, 200), size=(12.0, 19.2),
-                     amp=1.0, width=1024, height=1024, spp=4, max_depth=4) -> SceneSpec:
+                     amp=1.0, width=1024, height=1024, spp=4, max_depth=4, light="sky") -> SceneSpec:
     """Traversal stress scene: n_instances copies (flattened to world space) of
     n_bases seeded displaced grids of 2 * 125 * 200 = 50,000 triangles each,
     randomly rotated, scaled by U[0.6, 1.4] and translated into [15, 85]^3 of a
     100^3 box (400 instances: 20,000,000 triangles). Every coordinate is
     positive (the reference's empty-box quirk, tAabb.ipp:13-17, is then inert).
-    Lambertian grey, one rectangle light under the box's lid; the camera looks
-    into the box from z = -35."""
+    Lambertian grey; the camera looks into the box from z = -35. Lit by the
+    procedural sky (sky_texture: 2048 x 1024, gradient + 0.5-degree sun) as the
+    environment light, or (light="rect") by one rectangle light at the top."""
     rng = np.random.default_rng(seed)
     perm = rng.permutation(256)
     perm = np.concatenate([perm, perm, perm]).astype(np.int64)
@@ -483,7 +484,10 @@ def traversal_stress(seed=0x20261203 ^ 0x3, n_instances=400, n_bases=4, grid=(12
     for k in range(n_instances):
         tri = bases[which[k]] @ (rots[k].T * scales[k]) + trans[k]
         s.meshes.append(Mesh(f"inst{k}", "grey", v=tri.reshape(-1, 9).astype(np.float32), as_mesh=True))
-    # one 30 x 30 rectangle light under the lid of the box, facing down
+    if light == "sky":
+        s.env = sky_texture()
+        return s
+    # one 30 x 30 rectangle light at the top of the box, facing down
     s.materials.append(Material("lightmat", albedo=(0.8, 0.8, 0.8)))
     rect = np.array([[35.0, 99.5, 35.0, 35.0, 99.5, 65.0, 65.0, 99.5, 65.0]], np.float32)
     s.meshes.append(Mesh("sun", "lightmat", kind=PRIM_RECTANGLE, v=rect))

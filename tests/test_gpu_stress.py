@@ -5,6 +5,10 @@ cosine directions. Closest-hit ids and t against the oracle's binary-BVH
 traversal (reference bvhAccelerator.cpp:40-97, triangle.cpp:54-89); rays
 with an exact-t tie are counted and listed separately. Any-hit flags
 (bvhAccelerator.cpp:99-156) identical on the same rays with bounded maxT.
+The sky-lit render of the same scene (1024^2, 4 spp, depth 4) against the
+oracle's render with the GPU's random layout, at the bars of
+test_gpu_render.py (per-pixel 1e-4 relative on >= 98% of pixels, image mean
+1e-3).
 """
 import time
 
@@ -33,20 +37,31 @@ def stress():
     oocc, _ = orc.trace(short, any_hit=True)
     print(f"[stress] {spec.num_prims()} prims, {len(rays)} rays, oracle done in {time.time() - t0:.1f} s, "
           f"hits {int((oprim >= 0).sum())}, tied rays {int((ties > 1).sum())}", flush=True)
-    del orc
-    return spec, rays, short, oprim, ot, ties, oocc.astype(bool)
+    return spec, rays, short, oprim, ot, ties, oocc.astype(bool), orc
+
+
+_SCENES = {}
+
+
+def _scene(spec, width, leaf_max):
+    """The product scene for one BVH shape, built once per module."""
+    from cadise_amd.api import Context, Scene, bvh_opts
+    key = (width, leaf_max)
+    if key not in _SCENES:
+        t0 = time.time()
+        _SCENES.clear()
+        _SCENES[key] = Scene.from_spec(Context(0), spec, bvh_opts(width=width, leaf_max=leaf_max))
+        st = _SCENES[key].stats()
+        print(f"[stress bvh{width} leaf{leaf_max}] built in {time.time() - t0:.1f} s: {st['wide_nodes']} nodes, "
+              f"depth {st['wide_depth']}, {st['device_bytes'] / 2**20:.0f} MiB on device", flush=True)
+    return _SCENES[key]
 
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("width,leaf_max", [(4, 2), (8, 4)])
 def test_stress_trace_bit_exact(stress, width, leaf_max):
-    from cadise_amd.api import Context, Scene, bvh_opts
-    spec, rays, short, oprim, ot, ties, oocc = stress
-    t0 = time.time()
-    sc = Scene.from_spec(Context(0), spec, bvh_opts(width=width, leaf_max=leaf_max))
-    st = sc.stats()
-    print(f"[stress bvh{width} leaf{leaf_max}] built in {time.time() - t0:.1f} s: {st['wide_nodes']} nodes, "
-          f"depth {st['wide_depth']}, {st['device_bytes'] / 2**20:.0f} MiB on device", flush=True)
+    spec, rays, short, oprim, ot, ties, oocc, _ = stress
+    sc = _scene(spec, width, leaf_max)
     prim, t = sc.trace(rays)
     unique = ties <= 1
     bad = np.where(unique & ((prim != oprim) | (t.view(np.int32) != ot.view(np.int32))))[0]
@@ -59,3 +74,30 @@ def test_stress_trace_bit_exact(stress, width, leaf_max):
     assert np.array_equal(t[tied].view(np.int32), ot[tied].view(np.int32))
     occ = sc.is_occluded(short)
     assert np.array_equal(occ, oocc), f"{int((occ != oocc).sum())} any-hit mismatches"
+
+
+@pytest.mark.timeout(900)
+def test_stress_sky_render_vs_oracle(stress):
+    from cadise_amd.api import film_mean
+    from oracle.binding import OR_RNG_PHILOX
+    spec, *_, orc = stress
+    sc = _scene(spec, 8, 4)
+    t0 = time.time()
+    film = sc.render(seed=0xC3)
+    st = sc.last_stats()
+    rays = st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["shadow_rays"]
+    print(f"[stress render] {spec.width}x{spec.height} {spec.spp} spp depth {spec.max_depth}: "
+          f"{st['ms_gpu']:.1f} ms GPU, {rays / st['ms_gpu'] / 1e3:.0f} Mrays/s "
+          f"(wall {time.time() - t0:.1f} s)", flush=True)
+    t0 = time.time()
+    orf = orc.render_linear(0xC3, rng_mode=OR_RNG_PHILOX, threads=16)
+    print(f"[stress render] oracle {time.time() - t0:.1f} s", flush=True)
+    g, o = film_mean(film), film_mean(orf)
+    assert np.all(np.isfinite(g))
+    close = np.all(np.abs(g - o) <= 1e-4 * np.abs(o) + 1e-6, axis=-1)
+    rel_mean = abs(g.mean() - o.mean()) / max(o.mean(), 1e-12)
+    sky = float((g.sum(-1) > 1.0).mean())
+    print(f"[stress render] close={close.mean():.5f} mean gpu={g.mean():.5f} oracle={o.mean():.5f} "
+          f"rel={rel_mean:.2e}, bright pixels {sky:.3f}", flush=True)
+    assert close.mean() >= 0.98 and rel_mean < 1e-3
+    assert np.array_equal(film[..., 3], orf[..., 3])
