@@ -16,6 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import native
+from .scenes import MAT_CODES
 from .native import (MPT_RENDER_KEEP_SAMPLES, MPT_RENDER_PASSWISE, BatchOpts, BatchTrial, BvhOpts, LightDesc, MaterialDesc, MptError,
                      RenderParams, SceneDesc, SceneStats, Stats, Texture)
 
@@ -105,10 +106,15 @@ class Scene:
 
         mats = (MaterialDesc * len(spec.materials))()
         for i, mat in enumerate(spec.materials):
-            mats[i].type = 0 if mat.type == "matte-lambertian" else 1
+            mats[i].type = MAT_CODES[mat.type]
             mats[i].albedo = texture(mat.albedo, mat.albedo_texture, 3)
             mats[i].f0[:] = [float(np.float32(x)) for x in mat.f0]
             mats[i].roughness = texture(mat.roughness, mat.roughness_texture, 1)
+            mats[i].exponent = float(np.float32(mat.exponent))
+            mats[i].ratio = float(np.float32(mat.ratio))
+            mats[i].fresnel = 0 if mat.fresnel == "conductor" else 1
+            mats[i].ior_outer = float(np.float32(mat.ior_outer))
+            mats[i].ior_inner = float(np.float32(mat.ior_inner_value()))
         order = spec.light_order()
         lights = (LightDesc * max(1, len(order)))()
         for i, (kind, l, p) in enumerate(order):
@@ -145,6 +151,7 @@ class Scene:
         d.sampler = 1 if spec.sampler == "stratified" else 0
         d.spp, d.max_depth = spec.spp, spec.max_depth
         d.split_sah = 0 if spec.split_mode == "equal" else 1
+        d.estimator = 1 if spec.estimator == "vpt" else 0
         h = C.c_void_p()
         ctx.check(ctx.L.mpt_scene_create(ctx.h, C.byref(d), C.byref(opts) if opts else None, C.byref(h)))
         del keep

@@ -190,6 +190,7 @@ int merge_tables(mpt_batch* B) {
         const DevScene& d = S->sc;
         v.light_base = static_cast<int32_t>(nlight);
         v.env_light = d.env_light >= 0 ? static_cast<int32_t>(nlight) + d.env_light : -1;
+        v.estimator = d.estimator;
         v.nlights = static_cast<int32_t>(h.lights.size());
         v.cluster = h.cluster;
         v.cdf_base = static_cast<int32_t>(ncdf);
@@ -223,7 +224,6 @@ int merge_tables(mpt_batch* B) {
         ncdf += static_cast<int64_t>(S->h_cdf.size());
         npx += static_cast<int64_t>(h.width) * h.height;
         work += static_cast<int64_t>(h.width) * h.height * h.spp;
-        if (h.max_depth > 4096) return fail(ctx, MPT_ERR_INVALID, "max-depth above 4096 is not supported on the device");
     }
     if (np >= (int64_t{1} << 31) || nmat >= (int64_t{1} << 31) || ntexel >= (int64_t{1} << 30))
         return fail(ctx, MPT_ERR_UNSUPPORTED, "batch tables too large");

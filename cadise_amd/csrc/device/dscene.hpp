@@ -16,15 +16,25 @@
 
 namespace mpt {
 
+// Material types (host MatType): 0 lambert, 1 ggx conductor, 2 phong, 3 plastic
+// (mixed lambert + phong), 4 mirror, 5 glass, 6 transmission, 7 absorber.
+enum : int32_t {
+    DM_LAMBERT = 0, DM_GGX = 1, DM_PHONG = 2, DM_PLASTIC = 3, DM_MIRROR = 4, DM_GLASS = 5, DM_TRANSMISSION = 6,
+    DM_ABSORB = 7
+};
+
 struct DevMaterial {
-    int32_t type;        // 0 lambert, 1 ggx
+    int32_t type;        // DM_*
     int32_t albedo_tex;  // -1 constant
     int32_t rough_tex;   // -1 constant
-    int32_t pad;
+    int32_t fresnel;     // mirror: 0 schlick conductor (f0), 1 dielectric (iors)
     float   albedo[3];
     float   roughness;
     float   f0[3];
-    float   pad2;
+    float   exponent;    // phong / plastic
+    float   ratio;       // plastic diffuse ratio
+    float   ior_outer, ior_inner;
+    float   pad;
 };
 
 struct DevTexture {
@@ -71,6 +81,7 @@ struct DevScene {
     int32_t        env_light;   // index of the environment light, -1 none
     const float*   env_cdf;     // environment lights' sampling tables
     int32_t        cluster;     // 0 uniform, 1 power
+    int32_t        estimator;   // 0 pt, 1 vpt
     // camera
     float f2c[12];              // film -> camera rows 0..2 (row-major)
     float c2w[12];              // camera -> world rows 0..2

@@ -16,8 +16,13 @@ namespace mpt {
 
 constexpr int WAVE = 64;
 
-// Philox slots within one bounce b: counter (pixel, sample, 2b + slot / 4, 0).
-enum { SL_PICK = 0, SL_LPOS0 = 1, SL_LPOS1 = 2, SL_RR = 3, SL_MIS0 = 4, SL_MIS1 = 5, SL_CONT0 = 6, SL_CONT1 = 7 };
+// Philox slots within one bounce b: counter (pixel, sample, 2b + slot / 4, 0);
+// the lobe choices of two-lobe samples (plastic, glass) at slots 8 and 9:
+// counter (pixel, sample, 2b, 1).
+enum {
+    SL_PICK = 0, SL_LPOS0 = 1, SL_LPOS1 = 2, SL_RR = 3, SL_MIS0 = 4, SL_MIS1 = 5, SL_CONT0 = 6, SL_CONT1 = 7,
+    SL_MISC = 8, SL_CONTC = 9
+};
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
@@ -271,7 +276,7 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     float fx, fy;
     const V3 dir = camera_ray(sc, p, a.sample, a.key, fx, fy);
     a.fpos[p] = make_float2(fx, fy);
-    a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    a.beta[p] = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
     a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     const Queue& q = a.q[0];
     q.path[p] = p;
@@ -405,6 +410,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             int32_t pb = step, smp = a.sample, pix = p, md = a.max_depth;
             uint2 key = a.key;
             int32_t nl = sc.nlights, cluster = sc.cluster, lbase = 0, cdf0 = 0, envl = sc.env_light;
+            int32_t est = sc.estimator;
             int4 meta = make_int4(0, 0, 0, 0);
             if constexpr (BATCH) {
                 meta = bt.meta[p];
@@ -420,6 +426,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 lbase = v.light_base;
                 cdf0 = v.cdf_base;
                 envl = v.env_light;
+                est = v.estimator;
                 root_next = v.root;
                 sample_idx = static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix;
             } else {
@@ -431,6 +438,10 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             V3 L = v3f4(rad4);
             beta4 = a.beta[p];
             V3 beta = v3f4(beta4);
+            // emission counts at a hit that follows the camera or a specular vertex (pt),
+            // at every hit (vpt) (ptEstimator.cpp:51-56, 80-82; vptEstimator.cpp:48-51)
+            const bool count_emit = beta4.w != 0.0f;
+            bool count_next = false;
             V3 A = v3(0.f, 0.f, 0.f);
             if constexpr (ADJ) A = v3f4(g.A[p]);
             // -- resolve the previous vertex's NEE
@@ -492,9 +503,9 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             // -- shade the continuation hit
             cdir = cur.dir[s];
             const int2 ch = cur.hit[s];
-            if (pb == 0 && pb < md && cdir.w >= 0.0f && ch.x < 0 && envl >= 0) {
-                // an unbounded camera ray reaches the background: its emission, and
-                // the path ends there (the background absorbs)
+            if (count_emit && pb < md && cdir.w >= 0.0f && ch.x < 0 && envl >= 0) {
+                // an unbounded ray reaches the background: its emission, and the path
+                // ends there (the background absorbs)
                 L = L + beta * env_emittance(sc, sc.lights[envl], v3f4(cdir));
             }
             if (pb < md && cdir.w >= 0.0f && ch.x >= 0) {
@@ -504,7 +515,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 const SurfaceHit h = surface_detail(sc, ch.x, P0, v3f4(cdir), __int_as_float(ch.y));
                 const int4 info = sc.prim_info[h.prim];
                 const DevMaterial mat = sc.mats[info.x];
-                if (pb == 0 && info.y >= 0) {
+                if (count_emit && info.y >= 0) {
                     L = L + beta * light_emittance(sc.lights[info.y], h);
                     if constexpr (ADJ) {
                         if (can_emit(sc.lights[info.y], h.wi, h.Ns)) {
@@ -518,70 +529,83 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                     par = vertex_param(sc, mat, h.uvw);
                     if (mat.type == 0) alb = mat_albedo(sc, mat, h.uvw);
                 }
-                // light pick
-                float pick;
-                {
-                    const float u = u01(rA, SL_PICK);
-                    int32_t idx = 0;
-                    if (cluster == 1) {
-                        const float* cdf = sc.power_cdf + cdf0;
-                        for (int32_t i = nl - 1; i >= 0; --i)
-                            if (cdf[i] <= u) { idx = i; break; }
-                        pick = cdf[idx + 1] - cdf[idx];
-                    } else {
-                        const uint32_t raw = static_cast<uint32_t>(0.0f + u * static_cast<float>(nl));
-                        idx = raw < static_cast<uint32_t>(nl - 1) ? static_cast<int32_t>(raw) : nl - 1;
-                        pick = 1.0f / static_cast<float>(nl);
-                    }
-                    pick_light = lbase + idx;
+                // next-event estimation at non-specular vertices of pt; the absorber's
+                // DirectLightEvaluator returns before sampling (directLightEvaluator.cpp:33-35)
+                const bool nee = est == 0 && !mat_specular(mat.type) && mat.type != DM_ABSORB;
+                count_next = est != 0 || mat_specular(mat.type);
+                // lobe choices of the plastic and glass samples: counter (pixel, sample, 2b, 1)
+                float uc_mis = 0.0f, uc_cont = 0.0f;
+                if (mat_needs_choice(mat.type)) {
+                    const uint4 rC = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb, 1u), key);
+                    uc_mis = u01(rC, SL_MISC);
+                    uc_cont = u01(rC, SL_CONTC);
                 }
-                // light sample + shadow ray
-                const DevLight& lg = sc.lights[pick_light];
-                const LightSample ls = any_light_sample(sc, lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
                 sdir = make_float4(0.f, 0.f, 0.f, -1.f);
                 lt4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (ls.valid) {
-                    const V3 LV = ls.pos - h.P;
-                    const float dist = len(LV);
-                    const V3 Ldir = divs(LV, dist);
-                    const V3 sd = normalize(Ldir);
-                    sdir = make_float4(sd.x, sd.y, sd.z, dist - K_RAY_EPS);
-                    V3 f;
-                    float bpdf;
-                    bsdf_eval_pdf(sc, mat, h, Ldir, f, bpdf);
-                    const V3 factor = f * absdot(Ldir, h.Ns);
-                    const float w = mis_power(ls.pdfW, bpdf);
-                    const V3 lt = divs((ls.radiance * w) * factor, ls.pdfW);
-                    lt4 = make_float4(lt.x, lt.y, lt.z, 0.f);
-                    if constexpr (ADJ) {
-                        const float k = w * absdot(Ldir, h.Ns) / ls.pdfW / pick;
-                        const V3 e = (beta * f) * k;
-                        eL4 = make_float4(e.x, e.y, e.z, 0.f);
-                        V3 sl = v3(0.f, 0.f, 0.f);
-                        if (mat.type == 0) {
-                            if (dot(h.wi, h.Ns) * dot(Ldir, h.Ns) > 0.0f) sl = (beta * ls.radiance) * (k * K_INV_PI);
-                        } else if (!is_zero(f)) {
-                            sl = (e * ls.radiance) * ggx_dlnf_dr(sc, mat, h, Ldir);
-                        }
-                        sL4 = make_float4(sl.x, sl.y, sl.z, 0.f);
-                    }
-                }
-                // MIS-BSDF ray
                 mdir = make_float4(0.f, 0.f, 0.f, -1.f);
                 fac4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                {
-                    const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1));
-                    if (bs.valid) {
-                        const V3 mdv = normalize(bs.L);
-                        mdir = make_float4(mdv.x, mdv.y, mdv.z, FLT_MAX);
-                        const V3 fc = bs.f * absdot(bs.L, h.Ns);
-                        fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
+                float pick = 0.0f;
+                if (nee) {
+                    // light pick
+                    {
+                        const float u = u01(rA, SL_PICK);
+                        int32_t idx = 0;
+                        if (cluster == 1) {
+                            const float* cdf = sc.power_cdf + cdf0;
+                            for (int32_t i = nl - 1; i >= 0; --i)
+                                if (cdf[i] <= u) { idx = i; break; }
+                            pick = cdf[idx + 1] - cdf[idx];
+                        } else {
+                            const uint32_t raw = static_cast<uint32_t>(0.0f + u * static_cast<float>(nl));
+                            idx = raw < static_cast<uint32_t>(nl - 1) ? static_cast<int32_t>(raw) : nl - 1;
+                            pick = 1.0f / static_cast<float>(nl);
+                        }
+                        pick_light = lbase + idx;
+                    }
+                    // light sample + shadow ray
+                    const DevLight& lg = sc.lights[pick_light];
+                    const LightSample ls = any_light_sample(sc, lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
+                    if (ls.valid) {
+                        const V3 LV = ls.pos - h.P;
+                        const float dist = len(LV);
+                        const V3 Ldir = divs(LV, dist);
+                        const V3 sd = normalize(Ldir);
+                        sdir = make_float4(sd.x, sd.y, sd.z, dist - K_RAY_EPS);
+                        V3 f;
+                        float bpdf;
+                        bsdf_eval_pdf(sc, mat, h, Ldir, f, bpdf);
+                        const V3 factor = f * absdot(Ldir, h.Ns);
+                        const float w = mis_power(ls.pdfW, bpdf);
+                        const V3 lt = divs((ls.radiance * w) * factor, ls.pdfW);
+                        lt4 = make_float4(lt.x, lt.y, lt.z, 0.f);
                         if constexpr (ADJ) {
-                            const float k = absdot(bs.L, h.Ns) / bs.pdf / pick;
-                            const V3 e = (beta * bs.f) * k;
-                            eB4 = make_float4(e.x, e.y, e.z, 0.f);
-                            const V3 sb = mat.type == 0 ? beta * (k * K_INV_PI) : e * ggx_dlnf_dr(sc, mat, h, bs.L);
-                            sB4 = make_float4(sb.x, sb.y, sb.z, 0.f);
+                            const float k = w * absdot(Ldir, h.Ns) / ls.pdfW / pick;
+                            const V3 e = (beta * f) * k;
+                            eL4 = make_float4(e.x, e.y, e.z, 0.f);
+                            V3 sl = v3(0.f, 0.f, 0.f);
+                            if (mat.type == 0) {
+                                if (dot(h.wi, h.Ns) * dot(Ldir, h.Ns) > 0.0f) sl = (beta * ls.radiance) * (k * K_INV_PI);
+                            } else if (!is_zero(f)) {
+                                sl = (e * ls.radiance) * ggx_dlnf_dr(sc, mat, h, Ldir);
+                            }
+                            sL4 = make_float4(sl.x, sl.y, sl.z, 0.f);
+                        }
+                    }
+                    // MIS-BSDF ray
+                    {
+                        const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1), uc_mis);
+                        if (bs.valid) {
+                            const V3 mdv = normalize(bs.L);
+                            mdir = make_float4(mdv.x, mdv.y, mdv.z, FLT_MAX);
+                            const V3 fc = bs.f * absdot(bs.L, h.Ns);
+                            fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
+                            if constexpr (ADJ) {
+                                const float k = absdot(bs.L, h.Ns) / bs.pdf / pick;
+                                const V3 e = (beta * bs.f) * k;
+                                eB4 = make_float4(e.x, e.y, e.z, 0.f);
+                                const V3 sb = mat.type == 0 ? beta * (k * K_INV_PI) : e * ggx_dlnf_dr(sc, mat, h, bs.L);
+                                sB4 = make_float4(sb.x, sb.y, sb.z, 0.f);
+                            }
                         }
                     }
                 }
@@ -589,7 +613,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 // continuation
                 cdir = make_float4(0.f, 0.f, 0.f, -1.f);
                 {
-                    const BsdfSample cs = bsdf_sample(sc, mat, h, u01(rB, SL_CONT0), u01(rB, SL_CONT1));
+                    const BsdfSample cs = bsdf_sample(sc, mat, h, u01(rB, SL_CONT0), u01(rB, SL_CONT1), uc_cont);
                     if (cs.valid) {
                         beta = beta * divs(cs.f * absdot(cs.L, h.Ns), cs.pdf);
                         bool alive = true;
@@ -618,7 +642,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 pending = true;
             }
             a.rad[p] = make_float4(L.x, L.y, L.z, 0.f);
-            a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+            a.beta[p] = make_float4(beta.x, beta.y, beta.z, count_next ? 1.0f : 0.0f);
             if constexpr (BATCH) {
                 if (pending) {
                     bt.meta[p] = make_int4(meta.x, meta.y, meta.z, pb + 1);
@@ -756,7 +780,7 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArg
     float fx, fy;
     const V3 dir = camera_ray(v, pix, sample, make_uint2(v.key_lo, v.key_hi), fx, fy);
     a.fpos[p] = make_float2(fx, fy);
-    a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    a.beta[p] = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
     a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     bt.meta[p] = make_int4(lo, pix, sample, 0);
     const int32_t slot = qn + static_cast<int32_t>(gid);

@@ -95,6 +95,7 @@ struct BatchView {
     int32_t root;                // merged node index of the scene's BVH root
     int32_t light_base, nlights, cluster, cdf_base;
     int32_t env_light;           // merged index of the scene's environment light, -1 none
+    int32_t estimator;           // 0 pt, 1 vpt
     int32_t img_w, img_h, tile_x, tile_y;
     int32_t filter;
     float   filter_hw, filter_hh, g_expx, g_expy, g_edge, amplitude, mitchell_b, mitchell_c;

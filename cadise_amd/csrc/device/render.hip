@@ -68,6 +68,15 @@ int build_and_upload(mpt_scene* S) {
     for (int64_t j = 0; j < n; ++j) {
         const int32_t p = S->bin.ordered[j];
         const float* v = &h.prim_v[9 * p];
+        if (h.prim_type[p] == PRIM_SPHERE) {
+            const uint32_t bits = static_cast<uint32_t>(p) | 0x80000000u;
+            float w;
+            std::memcpy(&w, &bits, 4);
+            tris[3 * j + 0] = make_float4(v[0], v[1], v[2], w);
+            tris[3 * j + 1] = make_float4(v[3], 0.f, 0.f, 1.f);
+            tris[3 * j + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
         float A[3], e1[3], e2[3];
         const bool rect = h.prim_type[p] == PRIM_RECTANGLE;
         for (int k = 0; k < 3; ++k) {
@@ -108,6 +117,11 @@ int build_and_upload(mpt_scene* S) {
         d.albedo[0] = m.albedo.x; d.albedo[1] = m.albedo.y; d.albedo[2] = m.albedo.z;
         d.f0[0] = m.f0.x; d.f0[1] = m.f0.y; d.f0[2] = m.f0.z;
         d.roughness = m.roughness;
+        d.fresnel = m.fresnel;
+        d.exponent = m.exponent;
+        d.ratio = m.ratio;
+        d.ior_outer = m.ior_outer;
+        d.ior_inner = m.ior_inner;
         mats[i] = d;
     }
     std::vector<DevTexture> texs(h.textures.size());
@@ -233,6 +247,7 @@ int build_and_upload(mpt_scene* S) {
     sc.power_cdf = S->power_cdf.as<float>();
     sc.nlights = static_cast<int32_t>(h.lights.size());
     sc.env_light = env_light;
+    sc.estimator = h.estimator;
     sc.env_cdf = S->env_cdf.as<float>();
     sc.cluster = h.cluster;
     float f2c[16], c2w[16];
@@ -411,7 +426,6 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
     const int32_t spp_end = (p && p->spp_end > 0) ? p->spp_end : h.spp;
     const uint64_t seed = p ? p->seed : 0;
     const int64_t npix = static_cast<int64_t>(h.width) * h.height;
-    if (max_depth > 4096) return fail(ctx, MPT_ERR_INVALID, "max-depth above 4096 is not supported on the device");
     int rc;
     if ((rc = ensure_pass_buffers(S, npix, max_depth)) != MPT_OK) return rc;
     if (!p || !p->accumulate) {
@@ -431,21 +445,24 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
     const int ntiles = ((h.width + h.tile_x - 1) / h.tile_x) * ((h.height + h.tile_y - 1) / h.tile_y);
     const int cam_threads = static_cast<int>(std::max<int64_t>(npix, 3 * (max_depth + 2) + 1));
 
-    // events: (start, stop) per launch, grouped by kind
     const int passes = std::max(0, spp_end - spp_begin);
-    const size_t need = static_cast<size_t>(passes) * (3 * (max_depth + 1) + 2) * 2;
-    while (S->events.size() < need) {
-        hipEvent_t e;
-        HIP_TRY(ctx, hipEventCreate(&e));
-        S->events.push_back(e);
-    }
+    // events: (start, stop) per launch, grouped by kind, created as needed
     std::vector<std::pair<int, size_t>> marks;   // (kind, index of start event)
     size_t ev = 0;
-    auto mark = [&](int kind) -> hipError_t {
-        marks.push_back({kind, ev});
+    auto record = [&]() -> hipError_t {
+        if (ev == S->events.size()) {
+            hipEvent_t e;
+            const hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return err;
+            S->events.push_back(e);
+        }
         return hipEventRecord(S->events[ev++], S->stream);
     };
-    auto stop = [&]() -> hipError_t { return hipEventRecord(S->events[ev++], S->stream); };
+    auto mark = [&](int kind) -> hipError_t {
+        marks.push_back({kind, ev});
+        return record();
+    };
+    auto stop = [&]() -> hipError_t { return record(); };
 
     for (int32_t k = spp_begin; k < spp_end; ++k) {
         PassArgs a = pass_args(S, seed, max_depth);
@@ -488,6 +505,14 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
             hipLaunchKernelGGL(k_shade<SHADE_FWD>, dim3(sblocks), dim3(256), 0, S->stream, a, AdjArgs{}, BatchArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
             HIP_TRY(ctx, stop());
+            // deep limits (the reference's default max-depth is 10000): every 16
+            // bounces, stop once no path is left for the next one
+            if ((b & 15) == 15 && b < max_depth) {
+                int32_t left = 0;
+                HIP_TRY(ctx, hipMemcpyAsync(&left, a.qcount + b + 1, sizeof(int32_t), hipMemcpyDeviceToHost, S->stream));
+                HIP_TRY(ctx, hipStreamSynchronize(S->stream));
+                if (left == 0) break;
+            }
         }
         HIP_TRY(ctx, mark(EV_FILM));
         hipLaunchKernelGGL(k_film, dim3(ntiles), dim3(256), 0, S->stream, a);
@@ -549,6 +574,7 @@ BatchView scene_view(const mpt_scene* S, uint64_t seed, int32_t spp_begin, int32
     v.cluster = d.cluster;
     v.cdf_base = 0;
     v.env_light = d.env_light;
+    v.estimator = d.estimator;
     v.img_w = d.img_w;
     v.img_h = d.img_h;
     v.tile_x = d.tile_x;
@@ -589,7 +615,6 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     const int32_t spp_end = (p && p->spp_end > 0) ? p->spp_end : h.spp;
     const uint64_t seed = p ? p->seed : 0;
     const int64_t npix = static_cast<int64_t>(h.width) * h.height;
-    if (max_depth > 4096) return fail(ctx, MPT_ERR_INVALID, "max-depth above 4096 is not supported on the device");
     const bool keep = p && (p->flags & MPT_RENDER_KEEP_SAMPLES);
     const int32_t nsamples = std::max(0, spp_end - spp_begin);
     S->keep_valid = false;
@@ -916,7 +941,7 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
     if (d->prim_n) h.prim_n.assign(d->prim_n, d->prim_n + 9 * n);
     if (d->prim_uvw) h.prim_uvw.assign(d->prim_uvw, d->prim_uvw + 9 * n);
     for (int64_t i = 0; i < n; ++i) {
-        if (h.prim_type[i] != PRIM_TRIANGLE && h.prim_type[i] != PRIM_RECTANGLE)
+        if (h.prim_type[i] != PRIM_TRIANGLE && h.prim_type[i] != PRIM_RECTANGLE && h.prim_type[i] != PRIM_SPHERE)
             return fail(ctx, MPT_ERR_INVALID, "bad primitive type");
         if (h.prim_mat[i] < 0 || h.prim_mat[i] >= d->nmats) return fail(ctx, MPT_ERR_INVALID, "bad material index");
     }
@@ -953,7 +978,12 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
         x.f0 = {m.f0[0], m.f0[1], m.f0[2]};
         x.roughness = m.roughness.value[0];
         x.rough_tex = add_tex(m.roughness, 1);
-        if (x.type != MAT_LAMBERT && x.type != MAT_GGX) return fail(ctx, MPT_ERR_UNSUPPORTED, "material type");
+        x.exponent = m.exponent;
+        x.ratio = m.ratio;
+        x.fresnel = m.fresnel;
+        x.ior_outer = m.ior_outer;
+        x.ior_inner = m.ior_inner;
+        if (x.type < MAT_LAMBERT || x.type > MAT_ABSORB) return fail(ctx, MPT_ERR_INVALID, "material type");
         h.materials.push_back(x);
     }
     for (int32_t i = 0; i < d->nlights; ++i) {
@@ -977,6 +1007,8 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
         }
         if (l.type != MPT_LIGHT_AREA) return fail(ctx, MPT_ERR_INVALID, "light type");
         if (l.prim < 0 || l.prim >= n) return fail(ctx, MPT_ERR_INVALID, "light primitive out of range");
+        if (h.prim_type[l.prim] == PRIM_SPHERE)
+            return fail(ctx, MPT_ERR_UNSUPPORTED, "area light on a sphere (the reference cannot sample it)");
         Light x;
         x.prim = l.prim;
         x.color = {l.color[0], l.color[1], l.color[2]};
@@ -986,6 +1018,9 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
         if (!(prim_area(h, l.prim) > 0.f)) return fail(ctx, MPT_ERR_INVALID, "zero-area emitter");
     }
     h.cluster = d->cluster;
+    if (d->estimator != MPT_ESTIMATOR_PT && d->estimator != MPT_ESTIMATOR_VPT)
+        return fail(ctx, MPT_ERR_INVALID, "estimator");
+    h.estimator = d->estimator;
     h.cam_pos = {d->cam_pos[0], d->cam_pos[1], d->cam_pos[2]};
     h.cam_dir = {d->cam_dir[0], d->cam_dir[1], d->cam_dir[2]};
     h.cam_up = {d->cam_up[0], d->cam_up[1], d->cam_up[2]};
@@ -1265,8 +1300,13 @@ int mpt_render_adjoint(mpt_scene* s, const mpt_render_params* p, const float* d_
     if (!s || !d_image) return MPT_ERR_INVALID;
     REQUIRE_DEVICE(s);
     HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
-    // the environment's radiance texture is not a differentiated parameter
+    // the environment's radiance texture is not a differentiated parameter; the
+    // replayed derivatives cover Lambertian and GGX vertices of the pt estimator
     if (s->sc.env_light >= 0) return fail(s->ctx, MPT_ERR_UNSUPPORTED, "adjoint of a scene with an environment light");
+    if (s->host.estimator != EST_PT) return fail(s->ctx, MPT_ERR_UNSUPPORTED, "adjoint of the vpt estimator");
+    for (const Material& m : s->host.materials)
+        if (m.type != MAT_LAMBERT && m.type != MAT_GGX)
+            return fail(s->ctx, MPT_ERR_UNSUPPORTED, "adjoint of materials other than matte-lambertian and GGX");
     return do_adjoint(s, p, d_image, d_tex, d_light);
 }
 

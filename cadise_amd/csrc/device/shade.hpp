@@ -33,9 +33,20 @@ __device__ __forceinline__ V3 bary_mix(V3 b, V3 a, V3 c, V3 d) { return (a * b.x
 
 __device__ __forceinline__ V3 load3(const float* p) { return v3(p[0], p[1], p[2]); }
 
+// SphericalMapper::mappingToUvw (sphericalMapper.cpp:14-28): y up, phi from +z
+// towards +x.
+__device__ __forceinline__ V3 spherical_uvw(V3 d) {
+    const V3 u = normalize(d);
+    const float theta = acosf(mclamp(u.y, -1.0f, 1.0f));
+    float phi = atan2f(u.x, u.z);
+    if (phi < 0.0f) phi += K_TWO_PI;
+    return v3(phi * K_INV_TWO_PI, (K_PI - theta) * K_INV_PI, 0.0f);
+}
+
 // Scene::isIntersecting's surface detail (scene.cpp:30-48) with
-// Triangle::evaluateSurfaceDetail (triangle.cpp:125-160) and
-// Rectangle::evaluateSurfaceDetail (rectangle.cpp:102-132).
+// Triangle::evaluateSurfaceDetail (triangle.cpp:125-160),
+// Rectangle::evaluateSurfaceDetail (rectangle.cpp:102-132) and
+// Sphere::evaluateSurfaceDetail (sphere.cpp:109-130).
 __device__ __forceinline__ SurfaceHit surface_detail(const DevScene& sc, int32_t leaf, V3 o, V3 d, float t) {
     SurfaceHit h;
     const float4 r0 = sc.tris[3 * leaf + 0];
@@ -46,6 +57,15 @@ __device__ __forceinline__ SurfaceHit surface_detail(const DevScene& sc, int32_t
     h.wi = -d;
     h.P = o + d * t;
     const V3 A = v3f4(r0), e1 = v3f4(r1), e2 = v3f4(r2);
+    if ((bits & 0x80000000u) && r1.w != 0.0f) {
+        // Sphere::evaluateSurfaceDetail (sphere.cpp:109-130): the outward normal, uvw
+        // by the spherical mapper of it
+        const V3 N = normalize(h.P - A);
+        h.Ng = N;
+        h.Ns = N;
+        h.uvw = spherical_uvw(N);
+        return h;
+    }
     V3 N = cross(e1, e2);
     N = is_zero(N) ? v3(0.f, 1.f, 0.f) : normalize(N);
     if (bits & 0x80000000u) {
@@ -142,18 +162,84 @@ __device__ __forceinline__ V3 schlick(const DevMaterial& m, float cosi) {
     return f0 + ((f0c * c) * (c * c)) * (c * c);
 }
 
-// LambertianDiffuse / ConductorMicrofacet ::evaluate and ::evaluatePdfW
-// (lambertianDiffuse.cpp:27-44, 83-96; conductorMicrofacet.cpp:32-67, 123-156)
+// VanillaDielectricFresnel::evaluateReflectance (vanillaDielectricFresnel.cpp:14-45)
+__device__ __forceinline__ float dielectric_fresnel(float eta_outer, float eta_inner, float cos_i) {
+    float ei = eta_outer, et = eta_inner, c = cos_i;
+    if (c < 0.0f) {
+        const float t = ei;
+        ei = et;
+        et = t;
+        c *= -1.0f;
+    }
+    const float r = ei / et;
+    const float st2 = r * r * (1.0f - c * c);
+    if (st2 >= 1.0f) return 1.0f;
+    const float ct = sqrtf(1.0f - st2);
+    const float rpa = (et * c - ei * ct) / (et * c + ei * ct);
+    const float rpe = (ei * c - et * ct) / (ei * c + et * ct);
+    return 0.5f * (rpa * rpa + rpe * rpe);
+}
+
+// TVector::canRefract (tVector.ipp:467-503) of v about n
+__device__ __forceinline__ bool refract(V3 v, V3 n, float eta_outer, float eta_inner, V3& out) {
+    float ei = eta_outer, et = eta_inner, sf = 1.0f;
+    const float ci = dot(v, n);
+    if (ci < 0.0f) {
+        const float t = ei;
+        ei = et;
+        et = t;
+        sf = -1.0f;
+    }
+    const float r = ei / et;
+    const float st2 = r * r * (1.0f - ci * ci);
+    if (st2 >= 1.0f) return false;
+    const float ct = sqrtf(1.0f - st2);
+    V3 d = v * (-r);
+    d = d + n * (((r * sf) * ci - ct) * sf);
+    out = normalize(d);
+    return true;
+}
+
+// PhongBsdf (phongBsdf.cpp:14-100): (R.V)^n with R = L reflected about Ns, times
+// (n + 2) / 2pi (value) or (n + 1) / 2pi (pdf); zero across the surface
+__device__ __forceinline__ float phong_power(float n, V3 V, V3 L, V3 Ns) {
+    const V3 R = reflect(L, Ns);
+    return powf(mmax(dot(R, V), 0.0f), n);
+}
+
+// Lambertian / GGX / Phong value and pdf for (wi = h.wi, wo = L); the mixed
+// (plastic) BSDF's are ratio-weighted sums (mixedBsdf.cpp:35-63, 157-187);
+// specular BSDFs and the absorber evaluate to zero.
+// (lambertianDiffuse.cpp:27-44, 83-96; conductorMicrofacet.cpp:32-67, 123-156;
+// phongBsdf.cpp:22-39, 83-100)
 __device__ __forceinline__ void bsdf_eval_pdf(const DevScene& sc, const DevMaterial& m, const SurfaceHit& h, V3 L,
                                               V3& f, float& pdf) {
     const V3 Ns = h.Ns, V = h.wi;
     const float vn = dot(V, Ns), ln = dot(L, Ns);
     f = v3(0.f, 0.f, 0.f);
     pdf = 0.0f;
+    if (m.type >= DM_MIRROR) return;
     if (vn * ln <= 0.0f) return;
-    if (m.type == 0) {
+    if (m.type == DM_LAMBERT) {
         f = mat_albedo(sc, m, h.uvw) * K_INV_PI;
         pdf = absdot(L, Ns) * K_INV_PI;
+        return;
+    }
+    if (m.type == DM_PHONG || m.type == DM_PLASTIC) {
+        const float pw = phong_power(m.exponent, V, L, Ns);
+        const float fb = pw * ((m.exponent + 2.0f) * K_INV_TWO_PI);
+        const float pb = pw * ((m.exponent + 1.0f) * K_INV_TWO_PI);
+        if (m.type == DM_PHONG) {
+            f = v3(fb, fb, fb);
+            pdf = pb;
+            return;
+        }
+        const float r = m.ratio, rc = 1.0f - r;
+        const float avg = ((r + r) + r) / 3.0f;
+        const V3 fa = mat_albedo(sc, m, h.uvw) * K_INV_PI;
+        const float pa = absdot(L, Ns) * K_INV_PI;
+        f = fa * r + v3(fb, fb, fb) * rc;
+        pdf = pa * avg + pb * (1.0f - avg);
         return;
     }
     V3 H = (V + L) * msign(vn);
@@ -181,56 +267,183 @@ struct BsdfSample {
     bool valid;
 };
 
-// ::evaluateSample with sample = (u0, u1) (lambertianDiffuse.cpp:46-81 with
-// hemisphere.cpp:30-48; conductorMicrofacet.cpp:69-121 with isotropicGgx.cpp:71-100)
-__device__ __forceinline__ BsdfSample bsdf_sample(const DevScene& sc, const DevMaterial& m, const SurfaceHit& h,
-                                                  float u0, float u1) {
+// Draws of a material's sample: (u0, u1) = sample.x, .y of the two-uniform
+// BSDFs; uc = the lobe choice (plastic: mixedBsdf.cpp:81; glass:
+// specularDielectric.cpp:49).
+__device__ __forceinline__ bool mat_needs_choice(int32_t type) { return type == DM_PLASTIC || type == DM_GLASS; }
+__device__ __forceinline__ bool mat_specular(int32_t type) {
+    return type == DM_MIRROR || type == DM_GLASS || type == DM_TRANSMISSION;
+}
+
+// LambertianDiffuse::evaluateSample (lambertianDiffuse.cpp:46-81) with
+// Hemisphere::cosineWeightedSampling (hemisphere.cpp:30-48)
+__device__ __forceinline__ BsdfSample lambert_sample(const DevScene& sc, const DevMaterial& m, const SurfaceHit& h,
+                                                     V3 x, V3 z, float u0, float u1) {
+    BsdfSample s;
+    const V3 Ns = h.Ns;
+    const float phi = K_TWO_PI * u0;
+    const float theta = acosf(1.0f - 2.0f * u1) * 0.5f;
+    const float ct = cosf(theta);
+    const float st = sqrtf(1.0f - ct * ct);
+    const V3 l = v3(sinf(phi) * st, ct, cosf(phi) * st);
+    V3 L = normalize((x * l.x + Ns * l.y) + z * l.z);
+    if (dot(h.wi, Ns) <= 0.0f) L = -L;
+    s.f = mat_albedo(sc, m, h.uvw) * K_INV_PI;
+    s.L = L;
+    s.pdf = ct * K_INV_PI;
+    s.valid = !is_zero(s.f) && s.pdf > 0.0f;
+    return s;
+}
+
+// PhongBsdf::evaluateSample (phongBsdf.cpp:41-81) with
+// Hemisphere::cosineExpWeightedSampling (hemisphere.cpp:50-68): the lobe is
+// sampled about Ns and weighed about the mirror direction, as the reference does
+__device__ __forceinline__ BsdfSample phong_sample(const DevMaterial& m, const SurfaceHit& h, V3 x, V3 z, float u0,
+                                                   float u1) {
     BsdfSample s;
     s.f = v3(0.f, 0.f, 0.f);
     s.L = v3(0.f, 0.f, 0.f);
     s.pdf = 0.0f;
     s.valid = false;
     const V3 Ns = h.Ns, V = h.wi;
+    const float phi = K_TWO_PI * u0;
+    const float ct = powf(u1, 1.0f / (m.exponent + 1.0f));
+    const float st = sqrtf(1.0f - ct * ct);
+    const V3 l = v3(sinf(phi) * st, ct, cosf(phi) * st);
+    const V3 L = normalize((x * l.x + Ns * l.y) + z * l.z);
+    if (dot(V, Ns) * dot(L, Ns) <= 0.0f) return s;
+    const float pw = phong_power(m.exponent, V, L, Ns);
+    const float fv = pw * ((m.exponent + 2.0f) * K_INV_TWO_PI);
+    s.f = v3(fv, fv, fv);
+    s.L = L;
+    s.pdf = pw * ((m.exponent + 1.0f) * K_INV_TWO_PI);
+    s.valid = !is_zero(s.f) && s.pdf > 0.0f;
+    return s;
+}
+
+// Transmitted direction, value and pdf 1 of SpecularTransmission::evaluateSample
+// (specularTransmission.cpp:35-78) and SpecularDielectric's refraction branch
+// (specularDielectric.cpp:83-112); the radiance-mode factor (etaT / etaI)^2.
+__device__ __forceinline__ bool transmit(const DevScene& sc, const DevMaterial& m, const SurfaceHit& h, BsdfSample& s) {
+    float ei = m.ior_outer, et = m.ior_inner;
+    V3 L;
+    if (!refract(h.wi, h.Ns, ei, et, L)) return false;
+    const float ci = dot(L, h.Ns);
+    const float R = dielectric_fresnel(m.ior_outer, m.ior_inner, ci);
+    if (ci < 0.0f) {
+        const float t = ei;
+        ei = et;
+        et = t;
+    }
+    const float btdf = (et * et) / (ei * ei);
+    const float T = 1.0f - R;
+    s.f = divs((mat_albedo(sc, m, h.uvw) * T) * btdf, fabsf(ci));
+    s.L = L;
+    s.pdf = 1.0f;
+    return true;
+}
+
+// Bsdf::evaluateSample of every material on this path, from uniforms
+// (u0, u1) = sample.(x, y) and the lobe choice uc (lambertianDiffuse.cpp:46-81;
+// conductorMicrofacet.cpp:69-121 with isotropicGgx.cpp:71-100; phongBsdf.cpp:41-81;
+// mixedBsdf.cpp:65-155; specularReflection.cpp:32-57; specularDielectric.cpp:35-123;
+// specularTransmission.cpp:35-78; absorberBsdf.cpp: never valid).
+__device__ __forceinline__ BsdfSample bsdf_sample(const DevScene& sc, const DevMaterial& m, const SurfaceHit& h,
+                                                  float u0, float u1, float uc) {
+    BsdfSample s;
+    s.f = v3(0.f, 0.f, 0.f);
+    s.L = v3(0.f, 0.f, 0.f);
+    s.pdf = 0.0f;
+    s.valid = false;
+    const V3 Ns = h.Ns, V = h.wi;
+    if (m.type == DM_ABSORB) return s;
+    if (m.type >= DM_MIRROR) {
+        const float vn = dot(V, Ns);
+        float R = 0.0f;
+        bool reflect_lobe = m.type == DM_MIRROR;
+        float pdf = 1.0f;
+        if (m.type == DM_GLASS) {
+            R = dielectric_fresnel(m.ior_outer, m.ior_inner, vn);
+            const float prob = ((R + R) + R) / 3.0f;   // reflectance.average()
+            reflect_lobe = uc < prob;
+            pdf = reflect_lobe ? prob : 1.0f - prob;
+        }
+        if (reflect_lobe) {
+            const V3 L = reflect(V, Ns * ((vn > 0.0f) ? 1.0f : -1.0f));
+            const V3 alb = mat_albedo(sc, m, h.uvw);
+            if (m.type == DM_MIRROR) {
+                const float ln = dot(L, Ns);
+                const V3 F = m.fresnel == 0 ? schlick(m, ln) : v3(1.f, 1.f, 1.f) * dielectric_fresnel(m.ior_outer, m.ior_inner, ln);
+                s.f = divs(F * alb, fabsf(ln));
+            } else {
+                s.f = divs(alb * R, absdot(L, Ns));
+            }
+            s.L = L;
+            s.pdf = pdf;
+        } else {
+            if (!transmit(sc, m, h, s)) return s;
+            s.pdf = pdf;
+        }
+        s.valid = !is_zero(s.f) && s.pdf > 0.0f;
+        return s;
+    }
     V3 z, x;
     frame(Ns, z, x);
-    if (m.type == 0) {
-        const float phi = K_TWO_PI * u0;
-        const float theta = acosf(1.0f - 2.0f * u1) * 0.5f;
+    if (m.type == DM_LAMBERT) return lambert_sample(sc, m, h, x, z, u0, u1);
+    if (m.type == DM_PHONG) return phong_sample(m, h, x, z, u0, u1);
+    if (m.type == DM_PLASTIC) {
+        // MixedBsdf(A = Lambertian, B = Phong); choosing B weighs B's value and pdf
+        // by the ratio and A's by its complement, as the reference does
+        const float r = m.ratio, rc = 1.0f - r;
+        const float avg = ((r + r) + r) / 3.0f;
+        if (uc < avg) {
+            const BsdfSample a = lambert_sample(sc, m, h, x, z, u0, u1);
+            if (!a.valid) return s;
+            const float pw = (dot(V, Ns) * dot(a.L, Ns) <= 0.0f) ? 0.0f : phong_power(m.exponent, V, a.L, Ns);
+            const float fb = pw * ((m.exponent + 2.0f) * K_INV_TWO_PI);
+            const float pb = pw * ((m.exponent + 1.0f) * K_INV_TWO_PI);
+            s.f = a.f * r + v3(fb, fb, fb) * rc;
+            s.pdf = avg * a.pdf + (1.0f - avg) * pb;
+            s.L = a.L;
+        } else {
+            const BsdfSample b = phong_sample(m, h, x, z, u0, u1);
+            if (!b.valid) return s;
+            const bool same = dot(V, Ns) * dot(b.L, Ns) > 0.0f;
+            const V3 fa = same ? mat_albedo(sc, m, h.uvw) * K_INV_PI : v3(0.f, 0.f, 0.f);
+            const float pa = same ? absdot(b.L, Ns) * K_INV_PI : 0.0f;
+            s.f = b.f * r + fa * rc;
+            s.pdf = avg * b.pdf + (1.0f - avg) * pa;
+            s.L = b.L;
+        }
+        s.valid = !is_zero(s.f) && s.pdf > 0.0f;
+        return s;
+    }
+    // GGX conductor
+    const float nf = (dot(V, Ns) > 0.0f) ? 1.0f : -1.0f;
+    const float a = mat_alpha(sc, m, h.uvw);
+    const float s0 = mclamp(u0, 0.0f, 0.9999f), s1 = mclamp(u1, 0.0f, 0.9999f);
+    const float phi = K_TWO_PI * s0;
+    const float theta = atanf(a * sqrtf(s1 / (1.0f - s1)));
+    V3 hl;
+    if (!isfinite(theta)) {
+        hl = v3(0.f, 1.f, 0.f);
+    } else {
         const float ct = cosf(theta);
         const float st = sqrtf(1.0f - ct * ct);
-        const V3 l = v3(sinf(phi) * st, ct, cosf(phi) * st);
-        V3 L = normalize((x * l.x + Ns * l.y) + z * l.z);
-        if (dot(V, Ns) <= 0.0f) L = -L;
-        s.f = mat_albedo(sc, m, h.uvw) * K_INV_PI;
-        s.L = L;
-        s.pdf = ct * K_INV_PI;
-    } else {
-        const float nf = (dot(V, Ns) > 0.0f) ? 1.0f : -1.0f;
-        const float a = mat_alpha(sc, m, h.uvw);
-        const float s0 = mclamp(u0, 0.0f, 0.9999f), s1 = mclamp(u1, 0.0f, 0.9999f);
-        const float phi = K_TWO_PI * s0;
-        const float theta = atanf(a * sqrtf(s1 / (1.0f - s1)));
-        V3 hl;
-        if (!isfinite(theta)) {
-            hl = v3(0.f, 1.f, 0.f);
-        } else {
-            const float ct = cosf(theta);
-            const float st = sqrtf(1.0f - ct * ct);
-            hl = v3(sinf(phi) * st, ct, cosf(phi) * st);
-        }
-        const V3 H = normalize((x * hl.x + Ns * hl.y) + z * hl.z);
-        const V3 L = reflect(V, H * nf);
-        const float vn = dot(V, Ns), ln = dot(L, Ns), lh = dot(L, H), nh = dot(Ns, H);
-        if (vn * ln <= 0.0f) return s;
-        const V3 F = schlick(m, lh);
-        const float D = ggx_D(a, Ns, H);
-        const float G = ggx_G(a, V, L, Ns, H);
-        const float pdfL = fabsf(fabsf(D * nh) / (4.0f * lh));
-        if (!isfinite(pdfL)) return s;
-        s.f = divs((F * G) * D, 4.0f * fabsf(vn * ln));
-        s.L = L;
-        s.pdf = pdfL;
+        hl = v3(sinf(phi) * st, ct, cosf(phi) * st);
     }
+    const V3 H = normalize((x * hl.x + Ns * hl.y) + z * hl.z);
+    const V3 L = reflect(V, H * nf);
+    const float vn = dot(V, Ns), ln = dot(L, Ns), lh = dot(L, H), nh = dot(Ns, H);
+    if (vn * ln <= 0.0f) return s;
+    const V3 F = schlick(m, lh);
+    const float D = ggx_D(a, Ns, H);
+    const float G = ggx_G(a, V, L, Ns, H);
+    const float pdfL = fabsf(fabsf(D * nh) / (4.0f * lh));
+    if (!isfinite(pdfL)) return s;
+    s.f = divs((F * G) * D, 4.0f * fabsf(vn * ln));
+    s.L = L;
+    s.pdf = pdfL;
     s.valid = !is_zero(s.f) && s.pdf > 0.0f;   // BsdfSample::isValid (bsdfSample.cpp:39-41)
     return s;
 }
@@ -313,15 +526,6 @@ __device__ __forceinline__ float light_pdfW(const DevLight& l, const SurfaceHit&
 // sphere that every unbounded ray missing the scene hits (scene.cpp:30-63,
 // infiniteSphere.cpp:35-47); its surface position is the ray's direction.
 
-// SphericalMapper::mappingToUvw (sphericalMapper.cpp:14-28): y up, phi from +z
-// towards +x.
-__device__ __forceinline__ V3 spherical_uvw(V3 d) {
-    const V3 u = normalize(d);
-    const float theta = acosf(mclamp(u.y, -1.0f, 1.0f));
-    float phi = atan2f(u.x, u.z);
-    if (phi < 0.0f) phi += K_TWO_PI;
-    return v3(phi * K_INV_TWO_PI, (K_PI - theta) * K_INV_PI, 0.0f);
-}
 
 // Distribution1D::continuousToDiscrete (distribution1D.cpp:115-125): the last
 // i in [0, n) with cdf[i] <= u (a binary search over the nondecreasing CDF).

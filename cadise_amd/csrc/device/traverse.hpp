@@ -6,6 +6,7 @@
 // bit with the reference's binary BVH whenever the closest hit is unique:
 //   Triangle::isIntersecting   (triangle.cpp:54-89)
 //   Rectangle::isIntersecting  (rectangle.cpp:49-75)
+//   Sphere::isIntersecting     (sphere.cpp:42-78, 146-171)
 // Box culling differs from the reference's slab test (tAabb.ipp:68-127): it is
 // the FMA form (lo * inv - o * inv) with a 4-ulp widening of the far distance,
 // over tight boxes that contain each primitive's 1e-4-padded bound, so it
@@ -22,7 +23,31 @@ struct TravCounters {
     uint32_t prims;
 };
 
-// Primitive record at leaf position li: r0 = (A, id bits), r1 = e1, r2 = e2.
+// Sphere::isIntersecting / isOccluded with _isSolutionValid: the Haines et al.
+// form (q = b + sign(b) sqrt(disc), roots c / q and q, taken in that order).
+__device__ __forceinline__ bool sphere_test(V3 G, float r, V3 o, V3 D, float tmin, float& tmax) {
+    const V3 f = o - G;
+    const float b = -dot(f, D);
+    const V3 l = f + D * b;
+    const float disc = r * r - absdot(l, l);
+    if (disc < 0.0f) return false;
+    const float sb = (b > 0.0f) ? 1.0f : -1.0f;
+    const float q = b + sb * sqrtf(disc);
+    const float c = absdot(f, f) - r * r;
+    const float t0 = c / q, t1 = q;
+    if (t0 > tmax || t1 < tmin) return false;
+    float t = t0;
+    if (t < tmin) {
+        t = t1;
+        if (t > tmax) return false;
+    }
+    tmax = t;
+    return true;
+}
+
+// Primitive record at leaf position li: r0 = (A, id bits), r1 = e1, r2 = e2;
+// the id's top bit marks a rectangle, or a sphere when r1.w != 0 (then
+// r0 = centre, r1.x = radius).
 __device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32_t li, V3 o, V3 D, float tmin,
                                           float& tmax) {
     const float4 r0 = tris[3 * li + 0];
@@ -30,6 +55,7 @@ __device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32
     const float4 r2 = tris[3 * li + 2];
     V3 ea = v3f4(r1), eb = v3f4(r2);
     const bool rect = (__float_as_uint(r0.w) & 0x80000000u) != 0;
+    if (rect && r1.w != 0.0f) return sphere_test(v3f4(r0), r1.x, o, D, tmin, tmax);
     if (dot(D, cross(ea, eb)) > 0.0f) {
         const V3 t = ea;
         ea = eb;

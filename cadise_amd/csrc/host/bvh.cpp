@@ -56,11 +56,19 @@ struct Info {
     int64_t index;
 };
 
-// Triangle / Rectangle::evaluateBound: the vertices' box padded by 1e-4
-// (triangle.cpp:45-52, rectangle.cpp:39-47).
+// Triangle / Rectangle / Sphere::evaluateBound: the vertices' box (the
+// sphere's centre -+ radius) padded by 1e-4 (triangle.cpp:45-52,
+// rectangle.cpp:39-47, sphere.cpp:33-40).
 Box prim_bound(const HostScene& s, int64_t i) {
     const float* v = &s.prim_v[9 * i];
     Box b(false);
+    if (s.prim_type[i] == PRIM_SPHERE) {
+        for (int k = 0; k < 3; ++k) {
+            b.lo[k] = (v[k] - v[3]) - 0.0001f;
+            b.hi[k] = (v[k] + v[3]) + 0.0001f;
+        }
+        return b;
+    }
     for (int k = 0; k < 3; ++k) b.lo[k] = b.hi[k] = v[k];
     b.add(v + 3);
     b.add(v + 6);
@@ -306,6 +314,13 @@ void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf
             for (int64_t j = n.a; j < n.a + n.b; ++j) {
                 const int64_t p = bin.ordered[j];
                 const float* v = &s.prim_v[9 * p];
+                if (s.prim_type[p] == PRIM_SPHERE) {
+                    for (int k = 0; k < 3; ++k) {
+                        t.lo[k] = std::min(t.lo[k], (v[k] - v[3]) - 0.0001f);
+                        t.hi[k] = std::max(t.hi[k], (v[k] + v[3]) + 0.0001f);
+                    }
+                    continue;
+                }
                 float pts[4][3];
                 int np = 3;
                 for (int q = 0; q < 3; ++q)

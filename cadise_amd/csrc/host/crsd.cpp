@@ -185,19 +185,27 @@ struct Db {
         return static_cast<int32_t>(s.textures.size() - 1);
     }
 
+    // SdData::getSpectrumTexture / getRealTexture (sdData.cpp:98-140): a named
+    // texture, else a constant (default 0)
+    bool spectrum_param(const Block& b, std::string_view key, int32_t* tex, F3* value) {
+        const std::string name = b.str(key);
+        if (!name.empty()) {
+            auto it = spec_tex.find(name);
+            if (it == spec_tex.end()) return fail(MPT_ERR_PARSE, "unknown texture <" + name + ">");
+            *tex = it->second;
+        } else {
+            *value = b.vec(key, F3{0.f, 0.f, 0.f});
+        }
+        return true;
+    }
+
+    // instantiator::makeBsdf (bsdfInstantiator.cpp:28-221)
     bool material(const Block& b) {
         const std::string type = b.str("type");
         Material m;
         if (type == "matte-lambertian") {
             m.type = MAT_LAMBERT;
-            const std::string tex = b.str("albedo");
-            if (!tex.empty()) {
-                auto it = spec_tex.find(tex);
-                if (it == spec_tex.end()) return fail(MPT_ERR_PARSE, "unknown texture <" + tex + ">");
-                m.albedo_tex = it->second;
-            } else {
-                m.albedo = b.vec("albedo", F3{0.f, 0.f, 0.f});   // getSpectrumTexture default Spectrum(0)
-            }
+            if (!spectrum_param(b, "albedo", &m.albedo_tex, &m.albedo)) return false;
         } else if (type == "conductor-microfacet") {
             m.type = MAT_GGX;
             const std::string mf = b.str("microfacet-type", "ggx");
@@ -212,6 +220,49 @@ struct Db {
             } else {
                 m.roughness = b.real("roughness", 0.f);
             }
+        } else if (type == "phong") {
+            m.type = MAT_PHONG;
+            m.exponent = b.real("exponent", 32.f);
+        } else if (type == "plastic") {
+            m.type = MAT_PLASTIC;
+            if (!spectrum_param(b, "diffuse-albedo", &m.albedo_tex, &m.albedo)) return false;
+            m.exponent = b.real("specular-exponent", 32.f);
+            m.ratio = b.real("diffuse-ratio", 0.7f);
+        } else if (type == "mixed") {
+            // MixedBsdf over two named materials: on this path, Lambertian A and Phong B
+            // (the plastic combination)
+            auto ia = mats.find(b.str("bsdf-a")), ib = mats.find(b.str("bsdf-b"));
+            if (ia == mats.end() || ib == mats.end()) return fail(MPT_ERR_PARSE, "mixed material names an unknown bsdf");
+            const Material& A = s.materials[ia->second];
+            const Material& B = s.materials[ib->second];
+            if (A.type != MAT_LAMBERT || B.type != MAT_PHONG)
+                return fail(MPT_ERR_UNSUPPORTED, "mixed materials other than (matte-lambertian, phong)");
+            m = A;
+            m.type = MAT_PLASTIC;
+            m.exponent = B.exponent;
+            m.ratio = b.real("ratio", 0.5f);
+        } else if (type == "mirror") {
+            m.type = MAT_MIRROR;
+            if (!spectrum_param(b, "albedo", &m.albedo_tex, &m.albedo)) return false;
+            const std::string fr = b.str("fresnel-type", "conductor");
+            if (fr == "conductor") {
+                m.fresnel = 0;
+                m.f0 = b.vec("f0", F3{1.f, 1.f, 1.f});
+            } else if (fr == "dielectric") {
+                m.fresnel = 1;
+                m.ior_outer = b.real("ior-outer", 1.f);
+                m.ior_inner = b.real("ior-inner", 1.5f);
+            } else {
+                return fail(MPT_ERR_PARSE, "unknown fresnel-type <" + fr + "> in mirror");
+            }
+        } else if (type == "glass" || type == "transmission") {
+            m.type = type == "glass" ? MAT_GLASS : MAT_TRANSMISSION;
+            if (!spectrum_param(b, "albedo", &m.albedo_tex, &m.albedo)) return false;
+            m.fresnel = 1;
+            m.ior_outer = b.real("ior-outer", 1.f);
+            m.ior_inner = b.real("ior-inner", type == "glass" ? 1.5f : 1.f);
+        } else if (type == "absorb") {
+            m.type = MAT_ABSORB;
         } else {
             return fail(MPT_ERR_UNSUPPORTED, "material type <" + type + "> is not on the path-tracing path");
         }
@@ -258,7 +309,13 @@ struct Db {
         const std::string name = b.str("name");
         auto mit = mats.find(b.str("bsdf"));
         const int32_t first = static_cast<int32_t>(s.prim_type.size());
-        if (type == "triangle" || type == "rectangle") {
+        if (type == "sphere") {
+            // createSphere (primitiveInstantiator.cpp:13-23)
+            if (mit == mats.end()) return fail(MPT_ERR_PARSE, "sphere <" + name + "> names an unknown material");
+            const F3 c = b.vec("center");
+            const float v[9] = {c.x, c.y, c.z, b.real("radius"), 0.f, 0.f, 0.f, 0.f, 0.f};
+            push_prim(PRIM_SPHERE, v, mit->second, nullptr, nullptr);
+        } else if (type == "triangle" || type == "rectangle") {
             int32_t mat;
             if (mit != mats.end()) mat = mit->second;
             else if (type == "rectangle") mat = default_lambert();
@@ -400,7 +457,9 @@ struct Db {
         const std::string rt = renderer.str("type");
         if (rt != "sampling") return fail(MPT_ERR_UNSUPPORTED, "renderer <" + rt + "> stays on the CPU (this path is sampling + pt)");
         const std::string est = renderer.str("estimator");
-        if (est != "pt") return fail(MPT_ERR_UNSUPPORTED, "estimator <" + est + "> is not on the path-tracing path");
+        if (est == "pt") s.estimator = EST_PT;
+        else if (est == "vpt") s.estimator = EST_VPT;
+        else return fail(MPT_ERR_UNSUPPORTED, "estimator <" + est + "> is not on the path-tracing path");
         const std::string smp = renderer.str("sampler");
         if (smp == "random") s.sampler = 0;
         else if (smp == "stratified") s.sampler = 1;
@@ -417,9 +476,13 @@ struct Db {
         else if (cl == "power") s.cluster = 1;
         else return fail(MPT_ERR_PARSE, "unknown light-cluster <" + cl + ">");
         if (s.lights.empty()) return fail(MPT_ERR_PARSE, "no light (the light cluster needs one, uniformLightCluster.cpp:12)");
-        for (const Light& l : s.lights)
-            if (l.type == LIGHT_AREA && !(prim_area(s, l.prim) > 0.f))
-                return fail(MPT_ERR_PARSE, "zero-area emitter (singleAreaLight.cpp:30)");
+        for (const Light& l : s.lights) {
+            if (l.type != LIGHT_AREA) continue;
+            // Sphere::evaluatePositionSample is unimplemented (sphere.cpp:132-136)
+            if (s.prim_type[l.prim] == PRIM_SPHERE)
+                return fail(MPT_ERR_UNSUPPORTED, "area light on a sphere (the reference cannot sample it)");
+            if (!(prim_area(s, l.prim) > 0.f)) return fail(MPT_ERR_PARSE, "zero-area emitter (singleAreaLight.cpp:30)");
+        }
         return true;
     }
 };

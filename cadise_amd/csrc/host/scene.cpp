@@ -81,6 +81,7 @@ void camera_matrices(const HostScene& s, float f2c[16], float c2w[16]) {
 
 float prim_area(const HostScene& s, int64_t i) {
     const float* v = &s.prim_v[9 * i];
+    if (s.prim_type[i] == PRIM_SPHERE) return static_cast<float>(12.566370614359172) * v[3] * v[3];  // sphere.cpp:142-144
     if (s.prim_type[i] == PRIM_RECTANGLE) {
         const F3 ea = sub(load(v), load(v + 3)), eb = sub(load(v + 6), load(v + 3));
         return len(ea) * len(eb);

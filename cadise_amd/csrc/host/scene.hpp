@@ -13,8 +13,17 @@ struct F3 {
     float x = 0.f, y = 0.f, z = 0.f;
 };
 
-enum PrimType : int32_t { PRIM_TRIANGLE = 0, PRIM_RECTANGLE = 1 };
-enum MatType : int32_t { MAT_LAMBERT = 0, MAT_GGX = 1 };
+// A sphere's 9 values are (centre, radius, 0, ...).
+enum PrimType : int32_t { PRIM_TRIANGLE = 0, PRIM_RECTANGLE = 1, PRIM_SPHERE = 2 };
+// bsdfInstantiator.cpp:177-221: matte-lambertian, conductor-microfacet (ggx),
+// phong, plastic (MixedBsdf of Lambertian diffuse-albedo and Phong), mirror
+// (SpecularReflection), glass (SpecularDielectric), transmission
+// (SpecularTransmission), absorb.
+enum MatType : int32_t {
+    MAT_LAMBERT = 0, MAT_GGX = 1, MAT_PHONG = 2, MAT_PLASTIC = 3, MAT_MIRROR = 4, MAT_GLASS = 5,
+    MAT_TRANSMISSION = 6, MAT_ABSORB = 7
+};
+enum Estimator : int32_t { EST_PT = 0, EST_VPT = 1 };
 enum FilterType : int32_t { FILTER_BOX = 0, FILTER_GAUSSIAN = 1, FILTER_CONE = 2, FILTER_MITCHELL = 3 };
 
 struct Texture {
@@ -27,11 +36,19 @@ struct Texture {
 struct Material {
     int32_t type = MAT_LAMBERT;
     int32_t albedo_tex = -1;    // index into HostScene::textures, -1 = constant
-    F3      albedo{0.5f, 0.5f, 0.5f};
-    F3      f0{1.f, 1.f, 1.f};
+    F3      albedo{0.5f, 0.5f, 0.5f};   // lambertian / plastic diffuse / specular albedo
+    F3      f0{1.f, 1.f, 1.f};          // schlick f0 (conductor-microfacet, conductor mirror)
     int32_t rough_tex = -1;
     float   roughness = 0.f;
+    float   exponent = 32.f;            // phong / plastic specular exponent
+    float   ratio = 0.7f;               // plastic diffuse ratio
+    int32_t fresnel = 0;                // mirror: 0 schlick conductor (f0), 1 dielectric (iors)
+    float   ior_outer = 1.f, ior_inner = 1.5f;
 };
+
+// Lobes with a specular part: the pt estimator does no NEE there and counts
+// emission at the next hit (ptEstimator.cpp:68-93).
+inline bool mat_specular(int32_t type) { return type == MAT_MIRROR || type == MAT_GLASS || type == MAT_TRANSMISSION; }
 
 enum LightType : int32_t { LIGHT_AREA = 0, LIGHT_ENVIRONMENT = 1 };
 
@@ -54,6 +71,7 @@ struct HostScene {
     std::vector<Material> materials;
     std::vector<Texture>  textures;
     std::vector<Light>    lights;
+    int32_t estimator = EST_PT;      // renderer estimator: pt or vpt
     int32_t cluster = 0;             // 0 uniform, 1 power
     // camera (perspective-pinhole)
     F3 cam_pos, cam_dir{0.f, 0.f, -1.f}, cam_up{0.f, 1.f, 0.f};

@@ -27,7 +27,9 @@ class Texture(C.Structure):
 
 
 class MaterialDesc(C.Structure):
-    _fields_ = [("type", C.c_int32), ("albedo", Texture), ("f0", C.c_float * 3), ("roughness", Texture)]
+    _fields_ = [("type", C.c_int32), ("albedo", Texture), ("f0", C.c_float * 3), ("roughness", Texture),
+                ("exponent", C.c_float), ("ratio", C.c_float), ("fresnel", C.c_int32), ("ior_outer", C.c_float),
+                ("ior_inner", C.c_float)]
 
 
 class LightDesc(C.Structure):
@@ -45,7 +47,7 @@ class SceneDesc(C.Structure):
                 ("filter", C.c_int32), ("filter_w", C.c_float), ("filter_h", C.c_float), ("sigma_x", C.c_float),
                 ("sigma_y", C.c_float), ("amplitude", C.c_float), ("mitchell_b", C.c_float),
                 ("mitchell_c", C.c_float), ("sampler", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
-                ("split_sah", C.c_int32)]
+                ("split_sah", C.c_int32), ("estimator", C.c_int32)]
 
 
 class BvhOpts(C.Structure):

@@ -23,6 +23,11 @@ import numpy as np
 
 PRIM_TRIANGLE = 0
 PRIM_RECTANGLE = 1
+PRIM_SPHERE = 2          # rows of (cx, cy, cz, r, 0, 0, 0, 0, 0)
+
+# CRSD material types of the path and their codes in the C ABI (MPT_MAT_*)
+MAT_CODES = {"matte-lambertian": 0, "conductor-microfacet": 1, "phong": 2, "plastic": 3, "mirror": 4, "glass": 5,
+             "transmission": 6, "absorb": 7}
 
 
 def _f(x: float) -> str:
@@ -37,12 +42,22 @@ def _v(v) -> str:
 @dataclass
 class Material:
     name: str
-    type: str = "matte-lambertian"        # or "conductor-microfacet"
-    albedo: tuple = (0.5, 0.5, 0.5)       # lambert albedo (constant)
+    type: str = "matte-lambertian"        # a key of MAT_CODES
+    albedo: tuple = (0.5, 0.5, 0.5)       # lambert albedo / plastic diffuse-albedo / specular albedo
     albedo_texture: str | None = None     # name of a texture-spectrum block
-    f0: tuple = (1.0, 1.0, 1.0)           # schlick f0 (conductor)
+    f0: tuple = (1.0, 1.0, 1.0)           # schlick f0 (conductor, conductor mirror)
     roughness: float = 0.5
     roughness_texture: str | None = None  # name of a texture-real block
+    exponent: float = 32.0                # phong exponent / plastic specular-exponent
+    ratio: float = 0.7                    # plastic diffuse-ratio
+    fresnel: str = "conductor"            # mirror: "conductor" (f0) or "dielectric" (iors)
+    ior_outer: float = 1.0
+    ior_inner: float | None = None        # default 1.5 (glass, dielectric mirror), 1.0 (transmission)
+
+    def ior_inner_value(self) -> float:
+        if self.ior_inner is not None:
+            return float(self.ior_inner)
+        return 1.0 if self.type == "transmission" else 1.5
 
 
 @dataclass
@@ -58,7 +73,7 @@ class Mesh:
     """A group of primitives sharing a name and a material."""
     name: str
     material: str
-    kind: int = PRIM_TRIANGLE             # PRIM_TRIANGLE or PRIM_RECTANGLE
+    kind: int = PRIM_TRIANGLE             # PRIM_TRIANGLE, PRIM_RECTANGLE or PRIM_SPHERE
     v: np.ndarray = None                  # n x 9 float32 (rectangle: v1 v2 v3)
     normals: np.ndarray | None = None     # n x 9 float32 or None
     uvws: np.ndarray | None = None        # n x 9 float32 or None
@@ -101,6 +116,7 @@ class SceneSpec:
     # light env_slot (None: after all area lights)
     env: np.ndarray | None = None
     env_slot: int | None = None
+    estimator: str = "pt"                 # "pt" (NEE + MIS) or "vpt" (BSDF sampling only)
 
     # -- derived views ------------------------------------------------------
     def num_prims(self) -> int:
@@ -161,7 +177,7 @@ class SceneSpec:
                    f'    [up vector3r "{_v(self.cam_up)}"]\n    [fov real "{_f(self.fov)}"]\n}}')
         out.append("renderer {\n    [type string \"sampling\"]\n"
                    f'    [sampler string "{self.sampler}"]\n    [sample-number int32 "{self.spp}"]\n'
-                   f'    [estimator string "pt"]\n    [max-depth int32 "{self.max_depth}"]\n}}')
+                   f'    [estimator string "{self.estimator}"]\n    [max-depth int32 "{self.max_depth}"]\n}}')
         out.append(f'accelerator {{\n    [type string "bvh"]\n    [split-mode string "{self.split_mode}"]\n}}')
         out.append(f'light-cluster {{\n    [type string "{self.cluster}"]\n}}')
         for t in self.textures:
@@ -175,11 +191,28 @@ class SceneSpec:
                        f'    [filename string "{path}"]\n    [width int32 "{arr.shape[1]}"]\n'
                        f'    [height int32 "{arr.shape[0]}"]\n    [wrap-mode string "{t.wrap}"]\n}}')
         for m in self.materials:
+            head = f'material {{\n    [type string "{m.type}"]\n    [name string "{m.name}"]\n'
+
+            def spectrum(key):
+                return (f'[{key} texture "{m.albedo_texture}"]' if m.albedo_texture
+                        else f'[{key} rgb "{_v(m.albedo)}"]')
             if m.type == "matte-lambertian":
-                alb = (f'[albedo texture "{m.albedo_texture}"]' if m.albedo_texture
-                       else f'[albedo rgb "{_v(m.albedo)}"]')
-                out.append(f'material {{\n    [type string "matte-lambertian"]\n    [name string "{m.name}"]\n'
-                           f'    {alb}\n}}')
+                out.append(head + f'    {spectrum("albedo")}\n}}')
+            elif m.type == "phong":
+                out.append(head + f'    [exponent real "{_f(m.exponent)}"]\n}}')
+            elif m.type == "plastic":
+                out.append(head + f'    {spectrum("diffuse-albedo")}\n    [specular-exponent real "{_f(m.exponent)}"]\n'
+                           f'    [diffuse-ratio real "{_f(m.ratio)}"]\n}}')
+            elif m.type == "mirror":
+                fr = (f'[fresnel-type string "conductor"]\n    [f0 rgb "{_v(m.f0)}"]' if m.fresnel == "conductor"
+                      else f'[fresnel-type string "dielectric"]\n    [ior-outer real "{_f(m.ior_outer)}"]\n'
+                           f'    [ior-inner real "{_f(m.ior_inner_value())}"]')
+                out.append(head + f'    {spectrum("albedo")}\n    {fr}\n}}')
+            elif m.type in ("glass", "transmission"):
+                out.append(head + f'    {spectrum("albedo")}\n    [ior-outer real "{_f(m.ior_outer)}"]\n'
+                           f'    [ior-inner real "{_f(m.ior_inner_value())}"]\n}}')
+            elif m.type == "absorb":
+                out.append(head + '}')
             else:
                 rough = (f'[roughness texture "{m.roughness_texture}"]' if m.roughness_texture
                          else f'[roughness real "{_f(m.roughness)}"]')
@@ -188,6 +221,12 @@ class SceneSpec:
                            f'    [f0 rgb "{_v(m.f0)}"]\n    {rough}\n}}')
         for mi, m in enumerate(self.meshes):
             v = np.asarray(m.v, np.float32).reshape(-1, 9)
+            if m.kind == PRIM_SPHERE:
+                for r in v:
+                    out.append(f'primitive {{\n    [type string "sphere"]\n    [name string "{m.name}"]\n'
+                               f'    [bsdf material "{m.material}"]\n    [center vector3r "{_v(r[0:3])}"]\n'
+                               f'    [radius real "{_f(r[3])}"]\n}}')
+                continue
             if m.kind == PRIM_RECTANGLE or not m.as_mesh:
                 kind = "rectangle" if m.kind == PRIM_RECTANGLE else "triangle"
                 for r in v:
@@ -272,6 +311,93 @@ def cornell_box(width=256, height=256, spp=16, max_depth=8, filter="box") -> Sce
     s.meshes.append(Mesh("light1", "white", v=lt[1:2]))
     area = 0.5 * 0.25 * 0.25
     s.lights = [Light("light0", (1, 1, 1), 3 * 17 * math.pi * area), Light("light1", (1, 1, 1), 3 * 17 * math.pi * area)]
+    return s
+
+
+def shipped_cornell(width=640, height=480, spp=32, max_depth=10000) -> SceneSpec:
+    """The reference's shipped scene (scene/cornell-box.crsd) as a SceneSpec: five
+    rectangle walls (red, blue, white Lambertian), a plastic sphere (MixedBsdf of a
+    Lambertian 0.7/0.56/0 and Phong exponent 16, diffuse ratio 0.8), a GGX sphere
+    with roughness 0, one 190 x 165 ceiling rectangle light of 5e6 W; gaussian
+    filter, 32 spp stratified, pt, uniform light cluster, max-depth default 10000."""
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="gaussian",
+                  cam_pos=(278.0, 273.0, 800.0), cam_dir=(0.0, 0.0, -1.0), cam_up=(0.0, 1.0, 0.0), fov=49.0,
+                  output="cornell-box.png")
+    s.materials = [Material("plane", albedo=(0.7, 0.7, 0.7)), Material("redPlane", albedo=(0.75, 0.3, 0.3)),
+                   Material("bluePlane", albedo=(0.3, 0.3, 0.75)),
+                   Material("microfacet", type="conductor-microfacet", f0=(1.0, 1.0, 1.0), roughness=0.0),
+                   Material("plastic", type="plastic", albedo=(0.7, 0.56, 0.0), exponent=16.0, ratio=0.8)]
+    rect = lambda *v: np.array([np.concatenate(v)], np.float32)
+    s.meshes = [
+        Mesh("leftWall", "redPlane", kind=PRIM_RECTANGLE, v=rect((-90.0, 0.0, -649.2), (-90.0, 0.0, 0.0), (-90.0, 548.8, 0.0))),
+        Mesh("rightWall", "bluePlane", kind=PRIM_RECTANGLE,
+             v=rect((642.8, 0.0, 0.0), (642.8, 0.0, -649.2), (642.8, 548.8, -649.2))),
+        Mesh("topWall", "plane", kind=PRIM_RECTANGLE, v=rect((646.0, 548.8, 0.0), (-90.0, 548.8, 0.0), (-90.0, 548.8, -649.2))),
+        Mesh("backWall", "plane", kind=PRIM_RECTANGLE,
+             v=rect((-90.0, 548.8, -649.2), (646.0, 548.8, -649.2), (646.0, 0.0, -649.2))),
+        Mesh("downWall", "plane", kind=PRIM_RECTANGLE, v=rect((642.8, 0.0, 0.0), (-90.0, 0.0, 0.0), (-90.0, 0.0, -649.2))),
+        Mesh("leftSphere", "plastic", kind=PRIM_SPHERE, v=rect((96.0, 102.0, -361.0), (102.0, 0.0, 0.0), (0.0, 0.0, 0.0))),
+        Mesh("rightSphere", "microfacet", kind=PRIM_SPHERE,
+             v=rect((410.0, 111.0, -216.0), (111.0, 0.0, 0.0), (0.0, 0.0, 0.0))),
+        Mesh("ceilingLight", "plane", kind=PRIM_RECTANGLE,
+             v=rect((373.0, 538.7, -167.0), (373.0, 538.7, -332.0), (183.0, 538.7, -332.0))),
+    ]
+    s.lights = [Light("ceilingLight", (1.0, 1.0, 1.0), 5000000.0)]
+    return s
+
+
+def material_zoo(width=64, height=48, spp=4, max_depth=8, estimator="pt", env=None, cluster="uniform",
+                 textured=False) -> SceneSpec:
+    """Every material of the path in one box: a glass sphere, a dielectric
+    mirror, a conductor mirror quad, a transmission sphere, a Phong sphere, a
+    plastic sphere (its diffuse albedo an image texture when textured), a GGX
+    sphere, an absorbing sphere, a sphere inside the glass one, Lambertian walls
+    (open front) and two rectangle lights; optionally an environment image."""
+    s = SceneSpec(width=width, height=height, spp=spp, max_depth=max_depth, filter="box", fov=55.0,
+                  cam_pos=(5.0, 4.0, 16.0), cam_dir=(0.0, -0.12, -1.0), cam_up=(0.0, 1.0, 0.0),
+                  estimator=estimator, cluster=cluster)
+    if textured:
+        tex = smooth_noise_texture(16, 3, 0.2, 0.9, 0x201)
+        s.textures.append(Texture("zooalb", np.ascontiguousarray(tex, np.float32), kind="spectrum"))
+    s.materials = [
+        Material("wall", albedo=(0.65, 0.65, 0.6)), Material("red", albedo=(0.7, 0.15, 0.1)),
+        Material("green", albedo=(0.15, 0.6, 0.2)),
+        Material("glass", type="glass", albedo=(0.95, 0.95, 0.95), ior_outer=1.0, ior_inner=1.5),
+        Material("dmirror", type="mirror", albedo=(0.9, 0.9, 0.9), fresnel="dielectric", ior_outer=1.0, ior_inner=1.33),
+        Material("cmirror", type="mirror", albedo=(0.95, 0.9, 0.8), fresnel="conductor", f0=(0.9, 0.8, 0.6)),
+        Material("clear", type="transmission", albedo=(0.9, 0.95, 0.9), ior_outer=1.0, ior_inner=1.2),
+        Material("phong", type="phong", exponent=24.0),
+        Material("plastic", type="plastic", albedo=(0.2, 0.5, 0.8), albedo_texture="zooalb" if textured else None,
+                 exponent=40.0, ratio=0.6),
+        Material("ggx", type="conductor-microfacet", f0=(0.8, 0.6, 0.3), roughness=0.3),
+        Material("black", type="absorb"),
+        Material("lamp", albedo=(0.8, 0.8, 0.8)),
+    ]
+    rect = lambda a, b, c: np.array([np.concatenate([a, b, c])], np.float32)
+    sph = lambda c, r: np.array([[c[0], c[1], c[2], r, 0, 0, 0, 0, 0]], np.float32)
+    W = 10.0
+    s.meshes = [
+        Mesh("floor", "wall", kind=PRIM_RECTANGLE, v=rect((W, 0, 0), (0, 0, 0), (0, 0, -W))),
+        Mesh("ceiling", "wall", kind=PRIM_RECTANGLE, v=rect((0, W, 0), (W, W, 0), (W, W, -W))),
+        Mesh("back", "wall", kind=PRIM_RECTANGLE, v=rect((0, W, -W), (W, W, -W), (W, 0, -W))),
+        Mesh("left", "red", kind=PRIM_RECTANGLE, v=rect((0, 0, -W), (0, 0, 0), (0, W, 0))),
+        Mesh("right", "green", kind=PRIM_RECTANGLE, v=rect((W, 0, 0), (W, 0, -W), (W, W, -W))),
+        Mesh("cmirror", "cmirror", kind=PRIM_RECTANGLE, v=rect((9.9, 1.0, -2.0), (9.9, 1.0, -8.0), (9.9, 6.0, -8.0))),
+        Mesh("glass", "glass", kind=PRIM_SPHERE, v=sph((2.5, 2.0, -3.0), 1.6)),
+        Mesh("inner", "red", kind=PRIM_SPHERE, v=sph((2.5, 2.0, -3.0), 0.5)),
+        Mesh("dmirror", "dmirror", kind=PRIM_SPHERE, v=sph((7.2, 1.4, -6.5), 1.4)),
+        Mesh("clear", "clear", kind=PRIM_SPHERE, v=sph((6.0, 1.0, -2.0), 1.0)),
+        Mesh("phong", "phong", kind=PRIM_SPHERE, v=sph((4.0, 1.2, -7.5), 1.2)),
+        Mesh("plastic", "plastic", kind=PRIM_SPHERE, v=sph((1.6, 1.1, -7.6), 1.1)),
+        Mesh("ggx", "ggx", kind=PRIM_SPHERE, v=sph((8.3, 4.5, -2.2), 0.9)),
+        Mesh("black", "black", kind=PRIM_SPHERE, v=sph((5.0, 6.5, -5.0), 0.7)),
+        # lamp0 faces down; lamp1's normal faces the left wall and it emits from its back
+        Mesh("lamp0", "lamp", kind=PRIM_RECTANGLE, v=rect((6.5, 9.98, -6.5), (3.5, 9.98, -6.5), (3.5, 9.98, -3.5))),
+        Mesh("lamp1", "lamp", kind=PRIM_RECTANGLE, v=rect((0.02, 5.5, -6.5), (0.02, 5.5, -8.0), (0.02, 7.0, -8.0))),
+    ]
+    s.lights = [Light("lamp0", (1.0, 0.95, 0.9), 600.0), Light("lamp1", (0.8, 0.9, 1.0), 150.0, back_face=True)]
+    if env is not None:
+        s.env = env
     return s
 
 

@@ -61,8 +61,15 @@ typedef struct {
 } mpt_hit;
 
 /* Primitive description for mpt_scene_create (creation order). */
-enum { MPT_PRIM_TRIANGLE = 0, MPT_PRIM_RECTANGLE = 1 };
-enum { MPT_MAT_LAMBERT = 0, MPT_MAT_GGX = 1 };
+/* A sphere's 9 values are (centre, radius, 0, 0, 0, 0, 0). */
+enum { MPT_PRIM_TRIANGLE = 0, MPT_PRIM_RECTANGLE = 1, MPT_PRIM_SPHERE = 2 };
+/* bsdfInstantiator.cpp:177-221: matte-lambertian, conductor-microfacet (GGX),
+ * phong, plastic (MixedBsdf(Lambertian, Phong)), mirror, glass, transmission, absorb */
+enum {
+    MPT_MAT_LAMBERT = 0, MPT_MAT_GGX = 1, MPT_MAT_PHONG = 2, MPT_MAT_PLASTIC = 3, MPT_MAT_MIRROR = 4,
+    MPT_MAT_GLASS = 5, MPT_MAT_TRANSMISSION = 6, MPT_MAT_ABSORB = 7
+};
+enum { MPT_ESTIMATOR_PT = 0, MPT_ESTIMATOR_VPT = 1 };
 
 typedef struct {
     const float* data;         /* H x W x C float32 (row 0 = v in [0, 1/H)), NULL = constant */
@@ -73,9 +80,14 @@ typedef struct {
 
 typedef struct {
     int32_t          type;     /* MPT_MAT_* */
-    mpt_texture_desc albedo;   /* matte-lambertian albedo */
-    float            f0[3];    /* conductor-microfacet schlick f0 */
+    mpt_texture_desc albedo;   /* matte-lambertian albedo; plastic diffuse-albedo; mirror / glass /
+                                  transmission albedo */
+    float            f0[3];    /* schlick f0: conductor-microfacet, mirror with a conductor fresnel */
     mpt_texture_desc roughness;
+    float            exponent;  /* phong exponent, plastic specular-exponent */
+    float            ratio;     /* plastic diffuse-ratio */
+    int32_t          fresnel;   /* mirror: 0 conductor (schlick f0), 1 dielectric */
+    float            ior_outer, ior_inner;   /* dielectric fresnel (mirror, glass, transmission) */
 } mpt_material_desc;
 
 enum { MPT_LIGHT_AREA = 0, MPT_LIGHT_ENVIRONMENT = 1 };
@@ -111,6 +123,7 @@ typedef struct {
     int32_t sampler;           /* 0 random, 1 stratified */
     int32_t spp, max_depth;
     int32_t split_sah;         /* 1 sah, 0 equal */
+    int32_t estimator;         /* MPT_ESTIMATOR_*: pt (NEE + MIS) or vpt (BSDF sampling only) */
 } mpt_scene_desc;
 
 /* Accelerator options (the reference's accelerator block plus the wide-BVH

@@ -21,6 +21,10 @@ _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 OR_RNG_SEQ = 0
 OR_RNG_PHILOX = 1
 
+# OR_MAT_* of oracle.h by CRSD material type
+MAT_CODES = {"matte-lambertian": 0, "conductor-microfacet": 1, "phong": 2, "plastic": 3, "mirror": 4, "glass": 5,
+             "transmission": 6, "absorb": 7}
+
 
 class OrTexture(C.Structure):
     _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
@@ -28,7 +32,9 @@ class OrTexture(C.Structure):
 
 
 class OrMaterial(C.Structure):
-    _fields_ = [("type", C.c_int32), ("albedo", OrTexture), ("f0", C.c_float * 3), ("roughness", OrTexture)]
+    _fields_ = [("type", C.c_int32), ("albedo", OrTexture), ("f0", C.c_float * 3), ("roughness", OrTexture),
+                ("exponent", C.c_float), ("ratio", C.c_float), ("fresnel", C.c_int32), ("ior_outer", C.c_float),
+                ("ior_inner", C.c_float)]
 
 
 class OrLight(C.Structure):
@@ -46,7 +52,7 @@ class OrSceneDesc(C.Structure):
                 ("filter", C.c_int32), ("filter_w", C.c_float), ("filter_h", C.c_float), ("sigma_x", C.c_float),
                 ("sigma_y", C.c_float), ("amplitude", C.c_float), ("mitchell_b", C.c_float),
                 ("mitchell_c", C.c_float), ("sampler", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
-                ("split_sah", C.c_int32)]
+                ("split_sah", C.c_int32), ("estimator", C.c_int32)]
 
 
 def build_oracle() -> None:
@@ -74,6 +80,7 @@ def oracle_lib():
         L.or_camera_rays.argtypes = [C.c_void_p, _f32p, C.c_int64, _f32p]
         L.or_normalize.argtypes = [_f32p, C.c_int64, _f32p]
         L.or_bsdf_query.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64, _f32p]
+        L.or_bsdf_query_n.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64, C.c_int, _f32p]
         L.or_light_sample.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64, C.c_int, _f32p]
         L.or_light_hit.argtypes = [C.c_void_p, _f32p, C.c_int64, _f32p]
         L.or_light_radiance.argtypes = [C.c_void_p, _f32p]
@@ -127,10 +134,15 @@ class OracleScene:
 
         mats = (OrMaterial * max(1, len(spec.materials)))()
         for i, mat in enumerate(spec.materials):
-            mats[i].type = 0 if mat.type == "matte-lambertian" else 1
+            mats[i].type = MAT_CODES[mat.type]
             mats[i].albedo = texture(mat.albedo, mat.albedo_texture, 3)
             mats[i].f0[:] = [float(np.float32(x)) for x in mat.f0]
             mats[i].roughness = texture(mat.roughness, mat.roughness_texture, 1)
+            mats[i].exponent = float(np.float32(mat.exponent))
+            mats[i].ratio = float(np.float32(mat.ratio))
+            mats[i].fresnel = 0 if mat.fresnel == "conductor" else 1
+            mats[i].ior_outer = float(np.float32(mat.ior_outer))
+            mats[i].ior_inner = float(np.float32(mat.ior_inner_value()))
         order = spec.light_order()
         self.nlights = len(order)
         self.env_light = next((i for i, o in enumerate(order) if o[0] == "environment"), -1)
@@ -172,6 +184,7 @@ class OracleScene:
         d.sampler = 1 if spec.sampler == "stratified" else 0
         d.spp, d.max_depth = spec.spp, spec.max_depth
         d.split_sah = 0 if spec.split_mode == "equal" else 1
+        d.estimator = 1 if spec.estimator == "vpt" else 0
         self._desc = d
         self.h = L.or_scene_create(C.byref(d))
         self.L = L
@@ -223,9 +236,10 @@ class OracleScene:
         return out
 
     def bsdf_query(self, mat_index, q):
+        """q: n x (12 + nu), the uniforms in draw order after Ns, wi, wo, uvw."""
         q = np.ascontiguousarray(q, np.float32)
         out = np.zeros((len(q), 12), np.float32)
-        self.L.or_bsdf_query(self.h, mat_index, q, len(q), out)
+        self.L.or_bsdf_query_n(self.h, mat_index, q, len(q), q.shape[1] - 12, out)
         return out
 
     def bsdf_dfdtheta(self, mat_index, q):
@@ -368,6 +382,8 @@ def ref_lib():
         L.ref_time_render.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_uint64, _i64p, _i64p]
         L.ref_bsdf_query.restype = C.c_int
         L.ref_bsdf_query.argtypes = [C.c_void_p, C.c_char_p, _f32p, C.c_int64, _f32p]
+        L.ref_bsdf_query_n.restype = C.c_int
+        L.ref_bsdf_query_n.argtypes = [C.c_void_p, C.c_char_p, _f32p, C.c_int64, C.c_int, _f32p]
         L.ref_light_sample.restype = C.c_int
         L.ref_light_sample.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64, C.c_int, _f32p]
         L.ref_light_hit.argtypes = [C.c_void_p, _f32p, C.c_int64, _f32p]
@@ -449,9 +465,10 @@ class RefScene:
         return secs, rays, int(samples[0])
 
     def bsdf_query(self, material, q):
+        """q: n x (12 + nu), the uniforms in draw order after Ns, wi, wo, uvw."""
         q = np.ascontiguousarray(q, np.float32)
         out = np.zeros((len(q), 12), np.float32)
-        if self.L.ref_bsdf_query(self.h, material.encode(), q, len(q), out) != 0:
+        if self.L.ref_bsdf_query_n(self.h, material.encode(), q, len(q), q.shape[1] - 12, out) != 0:
             raise KeyError(material)
         return out
 

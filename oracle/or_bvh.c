@@ -16,8 +16,13 @@
 
 /* ---- primitives ------------------------------------------------------------ */
 
-/* Triangle::evaluateBound (triangle.cpp:45-52), Rectangle::evaluateBound (rectangle.cpp:39-47) */
+/* Triangle::evaluateBound (triangle.cpp:45-52), Rectangle::evaluateBound (rectangle.cpp:39-47),
+ * Sphere::evaluateBound (sphere.cpp:33-40) */
 or_aabb or_prim_bound(const or_prim* p) {
+    if (p->type == OR_PRIM_SPHERE) {
+        const or_aabb s = {v3_subs(p->vA, p->radius), v3_adds(p->vA, p->radius)};
+        return aabb_expand(s, 0.0001f);
+    }
     or_aabb b = aabb_point(p->vA);
     b = aabb_union_p(b, p->vB);
     b = aabb_union_p(b, p->vC);
@@ -64,7 +69,31 @@ static int rect_intersect(const or_prim* p, v3 o, v3 D, float minT, float* maxT)
     return 1;
 }
 
+/* Sphere::isIntersecting / isOccluded (sphere.cpp:42-107) with _isSolutionValid (sphere.cpp:146-171) */
+static int sphere_intersect(const or_prim* p, v3 o, v3 d, float minT, float* maxT) {
+    const v3 G = p->vA;
+    const v3 f = v3_sub(o, G);
+    const float b = -v3_dot(f, d);
+    const float r = p->radius;
+    const v3 l = v3_add(f, v3_scale(d, b));
+    const float disc = r * r - v3_absdot(l, l);
+    if (disc < 0.0f) return 0;
+    const float signB = (b > 0.0f) ? 1.0f : -1.0f;
+    const float q = b + signB * sqrtf(disc);
+    const float c = v3_absdot(f, f) - r * r;
+    const float t0 = c / q, t1 = q;
+    if (t0 > *maxT || t1 < minT) return 0;
+    float t = t0;
+    if (t < minT) {
+        t = t1;
+        if (t > *maxT) return 0;
+    }
+    *maxT = t;
+    return 1;
+}
+
 int or_prim_intersect(const or_prim* p, v3 o, v3 d, float minT, float* maxT) {
+    if (p->type == OR_PRIM_SPHERE) return sphere_intersect(p, o, d, minT, maxT);
     return p->type == OR_PRIM_RECTANGLE ? rect_intersect(p, o, d, minT, maxT) : tri_intersect(p, o, d, minT, maxT);
 }
 

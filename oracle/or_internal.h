@@ -12,6 +12,7 @@ typedef struct {
     v3      vD;
     v3      nA, nB, nC;      /* triangle vertex normals (face normal unless set) */
     v3      uvwA, uvwB, uvwC, uvwD;
+    float   radius;          /* sphere: centre in vA */
     int32_t mat;
     int32_t light;           /* area light bound to this primitive, or -1 */
 } or_prim;

@@ -34,8 +34,13 @@ void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 
 static inline float u32_to_unit(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
-/* Slots of the Philox layout within one bounce b: counter (pixel, sample, 2b + slot/4, 0). */
-enum { SL_PICK = 0, SL_LPOS0 = 1, SL_LPOS1 = 2, SL_RR = 3, SL_MIS0 = 4, SL_MIS1 = 5, SL_CONT0 = 6, SL_CONT1 = 7 };
+/* Slots of the Philox layout within one bounce b: counter (pixel, sample, 2b + slot/4, 0);
+ * the lobe choices of two-lobe samples (plastic, glass) at slots 8 and 9: counter
+ * (pixel, sample, 2b, 1). */
+enum {
+    SL_PICK = 0, SL_LPOS0 = 1, SL_LPOS1 = 2, SL_RR = 3, SL_MIS0 = 4, SL_MIS1 = 5, SL_CONT0 = 6, SL_CONT1 = 7,
+    SL_MISC = 8, SL_CONTC = 9
+};
 
 typedef struct {
     int      mode;
@@ -48,7 +53,7 @@ typedef struct {
 static float rng_u(or_rng* r, int bounce, int slot) {
     r->draws++;
     if (r->mode == OR_RNG_SEQ) return seqrng_next(&r->state);
-    const uint32_t ctr[4] = {r->pixel, r->sample, (uint32_t)(2 * bounce + slot / 4), 0u};
+    const uint32_t ctr[4] = {r->pixel, r->sample, (uint32_t)(2 * bounce + (slot & 7) / 4), slot >= 8 ? 1u : 0u};
     uint32_t out[4];
     or_philox(ctr, r->key, out);
     return u32_to_unit(out[slot & 3]);
@@ -69,6 +74,7 @@ static void rng_u2(or_rng* r, int bounce, int slot_x, int slot_y, float* x, floa
 /* ---- scene assembly ---------------------------------------------------------- */
 
 static float prim_area(const or_prim* p) {
+    if (p->type == OR_PRIM_SPHERE) return (float)12.566370614359172 * p->radius * p->radius; /* sphere.cpp:142-144 */
     if (p->type == OR_PRIM_RECTANGLE) return v3_len(p->e1) * v3_len(p->e2); /* rectangle.cpp:179-181 */
     return 0.5f * v3_len(v3_cross(p->e1, p->e2));                            /* triangle.cpp:199-201 */
 }
@@ -253,6 +259,11 @@ or_scene* or_scene_create(const or_scene_desc* desc) {
         p->vA = v3_load(v); p->vB = v3_load(v + 3); p->vC = v3_load(v + 6);
         p->mat = desc->prim_mat[i];
         p->light = -1;
+        if (p->type == OR_PRIM_SPHERE) {
+            /* Sphere ctor (sphere.cpp:18-31): centre and radius */
+            p->radius = v[3];
+            continue;
+        }
         if (p->type == OR_PRIM_RECTANGLE) {
             /* Rectangle ctor (rectangle.cpp:14-37) */
             p->e1 = v3_sub(p->vA, p->vB);
@@ -339,6 +350,7 @@ typedef struct {
 } or_si;
 
 static int background_hit(const or_scene* s, v3 d, float maxT, or_si* si);
+static v3 spherical_uvw(v3 d);
 
 /* Triangle::_positionToBarycentric (triangle.cpp:233-262) */
 static v3 barycentric(const or_prim* p, v3 P) {
@@ -364,6 +376,13 @@ static void surface_detail(const or_scene* s, int32_t prim, v3 o, v3 d, float t,
     si->prim = prim;
     si->wi = v3_neg(d);
     si->P = v3_add(o, v3_scale(d, t));
+    if (p->type == OR_PRIM_SPHERE) {
+        /* Sphere::evaluateSurfaceDetail (sphere.cpp:109-130) */
+        const v3 N = v3_normalize(v3_sub(si->P, p->vA));
+        si->Ng = si->Ns = N;
+        si->uvw = spherical_uvw(N);
+        return;
+    }
     if (p->type == OR_PRIM_RECTANGLE) {
         v3 N = v3_cross(p->e1, p->e2);
         N = v3_is_zero(N) ? v3_make(0.0f, 1.0f, 0.0f) : v3_normalize(N);
@@ -487,14 +506,76 @@ static v3 schlick(v3 f0, float cosI) {
 
 static float mat_alpha(const or_material* m, v3 uvw) { return rough_to_alpha(tex_eval(&m->roughness, uvw).x); }
 
-/* LambertianDiffuse::evaluate (lambertianDiffuse.cpp:27-44), ConductorMicrofacet::evaluate
- * (conductorMicrofacet.cpp:32-67) */
-static v3 bsdf_eval(const or_material* m, const or_si* si) {
-    const v3 Ns = si->Ns, V = si->wi, L = si->wo;
-    if (m->type == OR_MAT_LAMBERT) {
-        if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return v3_make(0, 0, 0);
-        return v3_scale(tex_eval(&m->albedo, si->uvw), OR_INV_PI);
+/* VanillaDielectricFresnel::evaluateReflectance (vanillaDielectricFresnel.cpp:14-45) */
+static float dielectric_fresnel(float iorOuter, float iorInner, float cosThetaI) {
+    float etaI = iorOuter, etaT = iorInner, cosI = cosThetaI;
+    if (cosI < 0.0f) {
+        const float t = etaI; etaI = etaT; etaT = t;
+        cosI *= -1.0f;
     }
+    const float etaRatio = etaI / etaT;
+    const float sinT2 = etaRatio * etaRatio * (1.0f - cosI * cosI);
+    if (sinT2 >= 1.0f) return 1.0f;
+    const float cosT = sqrtf(1.0f - sinT2);
+    const float rPar = (etaT * cosI - etaI * cosT) / (etaT * cosI + etaI * cosT);
+    const float rPerp = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
+    return 0.5f * (rPar * rPar + rPerp * rPerp);
+}
+
+/* TVector::canRefract (tVector.ipp:467-503) */
+static int can_refract(v3 v, v3 n, float iorOuter, float iorInner, v3* out) {
+    float etaI = iorOuter, etaT = iorInner, signFactor = 1.0f;
+    const float cosI = v3_dot(v, n);
+    if (cosI < 0.0f) {
+        const float t = etaI; etaI = etaT; etaT = t;
+        signFactor = -1.0f;
+    }
+    const float etaRatio = etaI / etaT;
+    const float sinT2 = etaRatio * etaRatio * (1.0f - cosI * cosI);
+    if (sinT2 >= 1.0f) return 0;
+    const float cosT = sqrtf(1.0f - sinT2);
+    v3 dir = v3_scale(v, -etaRatio);
+    dir = v3_add(dir, v3_scale(n, (etaRatio * signFactor * cosI - cosT) * signFactor));
+    *out = v3_normalize(dir);
+    return 1;
+}
+
+/* RgbSpectrum(r).average() (tVector.ipp:301-313) */
+static float avg3(float r) { return ((r + r) + r) / 3.0f; }
+
+/* PhongBsdf (phongBsdf.cpp:14-20): _brdfFactor, _pdfFactor */
+static float phong_brdf_factor(const or_material* m) { return (m->exponent + 2.0f) * (float)0.1591549430918953; }
+static float phong_pdf_factor(const or_material* m) { return (m->exponent + 1.0f) * (float)0.1591549430918953; }
+
+/* LambertianDiffuse::evaluate (lambertianDiffuse.cpp:27-44) */
+static v3 lambert_eval(const or_material* m, const or_si* si) {
+    if (v3_dot(si->wi, si->Ns) * v3_dot(si->wo, si->Ns) <= 0.0f) return v3_make(0, 0, 0);
+    return v3_scale(tex_eval(&m->albedo, si->uvw), OR_INV_PI);
+}
+/* LambertianDiffuse::evaluatePdfW (lambertianDiffuse.cpp:83-96) */
+static float lambert_pdf(const or_si* si) {
+    if (v3_dot(si->wi, si->Ns) * v3_dot(si->wo, si->Ns) <= 0.0f) return 0.0f;
+    return v3_absdot(si->wo, si->Ns) * OR_INV_PI;
+}
+/* PhongBsdf::evaluate / evaluatePdfW (phongBsdf.cpp:22-39, 83-100) */
+static float phong_eval(const or_material* m, const or_si* si) {
+    const v3 Ns = si->Ns, V = si->wi, L = si->wo;
+    const v3 R = v3_reflect(L, Ns);
+    if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return 0.0f;
+    const float RdotV = m_max(v3_dot(R, V), 0.0f);
+    return powf(RdotV, m->exponent) * phong_brdf_factor(m);
+}
+static float phong_pdf(const or_material* m, const or_si* si) {
+    const v3 Ns = si->Ns, V = si->wi, L = si->wo;
+    const v3 R = v3_reflect(L, Ns);
+    if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return 0.0f;
+    const float RdotV = m_max(v3_dot(R, V), 0.0f);
+    return powf(RdotV, m->exponent) * phong_pdf_factor(m);
+}
+
+/* ConductorMicrofacet::evaluate (conductorMicrofacet.cpp:32-67) */
+static v3 ggx_eval(const or_material* m, const or_si* si) {
+    const v3 Ns = si->Ns, V = si->wi, L = si->wo;
     const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns);
     if (VdotN * LdotN <= 0.0f) return v3_make(0, 0, 0);
     v3 H = v3_scale(v3_add(V, L), m_sign(VdotN));
@@ -507,15 +588,9 @@ static v3 bsdf_eval(const or_material* m, const or_si* si) {
     const float G = ggx_G(a, V, L, Ns, H);
     return v3_divs(v3_scale(v3_scale(F, G), D), 4.0f * fabsf(VdotN * LdotN));
 }
-
-/* LambertianDiffuse::evaluatePdfW (lambertianDiffuse.cpp:83-96), ConductorMicrofacet::evaluatePdfW
- * (conductorMicrofacet.cpp:123-156) */
-static float bsdf_pdf(const or_material* m, const or_si* si) {
+/* ConductorMicrofacet::evaluatePdfW (conductorMicrofacet.cpp:123-156) */
+static float ggx_pdf(const or_material* m, const or_si* si) {
     const v3 Ns = si->Ns, V = si->wi, L = si->wo;
-    if (m->type == OR_MAT_LAMBERT) {
-        if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return 0.0f;
-        return v3_absdot(L, Ns) * OR_INV_PI;
-    }
     const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns);
     if (VdotN * LdotN <= 0.0f) return 0.0f;
     v3 H = v3_scale(v3_add(V, L), m_sign(VdotN));
@@ -530,64 +605,265 @@ static float bsdf_pdf(const or_material* m, const or_si* si) {
     return pdfL;
 }
 
+/* Bsdf::evaluate of each material; MixedBsdf::evaluate (mixedBsdf.cpp:35-63) for
+ * plastic (A = Lambertian, B = Phong, ratio = diffuse-ratio); specular BSDFs and
+ * the absorber evaluate to zero (specularReflection.cpp:25-30, specularDielectric.cpp:28-33,
+ * specularTransmission.cpp:28-33, absorberBsdf.cpp:12-17) */
+static v3 bsdf_eval(const or_material* m, const or_si* si) {
+    switch (m->type) {
+    case OR_MAT_LAMBERT: return lambert_eval(m, si);
+    case OR_MAT_GGX: return ggx_eval(m, si);
+    case OR_MAT_PHONG: { const float f = phong_eval(m, si); return v3_make(f, f, f); }
+    case OR_MAT_PLASTIC: {
+        const float r = m->ratio, rc = 1.0f - r;
+        const v3 fa = lambert_eval(m, si);
+        const float fb = phong_eval(m, si);
+        return v3_add(v3_scale(fa, r), v3_make(fb * rc, fb * rc, fb * rc));
+    }
+    default: return v3_make(0, 0, 0);
+    }
+}
+
+/* Bsdf::evaluatePdfW of each material; MixedBsdf::evaluatePdfW (mixedBsdf.cpp:157-187) */
+static float bsdf_pdf(const or_material* m, const or_si* si) {
+    switch (m->type) {
+    case OR_MAT_LAMBERT: return lambert_pdf(si);
+    case OR_MAT_GGX: return ggx_pdf(m, si);
+    case OR_MAT_PHONG: return phong_pdf(m, si);
+    case OR_MAT_PLASTIC: {
+        const float avg = avg3(m->ratio);
+        return lambert_pdf(si) * avg + phong_pdf(m, si) * (1.0f - avg);
+    }
+    default: return 0.0f;
+    }
+}
+
+/* Materials whose lobes include a specular one (BsdfLobes::hasAtLeastOne in
+ * ptEstimator.cpp:68-70): no NEE there, emission counted at the next hit. */
+static int mat_specular(const or_material* m) {
+    return m->type == OR_MAT_MIRROR || m->type == OR_MAT_GLASS || m->type == OR_MAT_TRANSMISSION;
+}
+
+/* Uniforms an evaluateSample draws, in the reference's order: Vector2R(next, next)
+ * lands its first draw in y (g++ evaluates constructor arguments right to left);
+ * MixedBsdf draws its lobe choice first (mixedBsdf.cpp:81), SpecularDielectric
+ * its choice only (specularDielectric.cpp:49). */
+static int bsdf_draws(const or_material* m) {
+    switch (m->type) {
+    case OR_MAT_LAMBERT: case OR_MAT_GGX: case OR_MAT_PHONG: return 2;
+    case OR_MAT_PLASTIC: return 3;
+    case OR_MAT_GLASS: return 1;
+    default: return 0;
+    }
+}
+
 /* LambertianDiffuse::evaluateSample (lambertianDiffuse.cpp:46-81) with
- * Hemisphere::cosineWeightedSampling (hemisphere.cpp:30-48); ConductorMicrofacet::evaluateSample
- * (conductorMicrofacet.cpp:69-121) with IsotropicGgx::sampleHalfVectorH (isotropicGgx.cpp:71-100).
- * u0, u1 = sample.x, sample.y. */
-static or_bsdf_sample bsdf_sample(const or_material* m, const or_si* si, float u0, float u1) {
+ * Hemisphere::cosineWeightedSampling (hemisphere.cpp:30-48); sample = (x, y) */
+static or_bsdf_sample lambert_sample(const or_material* m, const or_si* si, float x0, float y0) {
     or_bsdf_sample out;
     memset(&out, 0, sizeof(out));
     const v3 Ns = si->Ns, V = si->wi;
     v3 z, x;
     frame(Ns, &z, &x);
-    if (m->type == OR_MAT_LAMBERT) {
-        const float phi = OR_TWO_PI * u0;
-        const float theta = acosf(1.0f - 2.0f * u1) * 0.5f;
+    const float phi = OR_TWO_PI * x0;
+    const float theta = acosf(1.0f - 2.0f * y0) * 0.5f;
+    const float cosT = cosf(theta);
+    const float sinT = sqrtf(1.0f - cosT * cosT);
+    const v3 l = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
+    const float pdf = cosT * OR_INV_PI;
+    v3 L = v3_normalize(to_world(x, Ns, z, l));
+    if (v3_dot(V, Ns) <= 0.0f) L = v3_neg(L);
+    out.f = v3_scale(tex_eval(&m->albedo, si->uvw), OR_INV_PI);
+    out.L = L;
+    out.pdf = pdf;
+    out.valid = !v3_is_zero(out.f) && out.pdf > 0.0f;
+    return out;
+}
+
+/* PhongBsdf::evaluateSample (phongBsdf.cpp:41-81) with
+ * Hemisphere::cosineExpWeightedSampling (hemisphere.cpp:50-68) */
+static or_bsdf_sample phong_sample(const or_material* m, const or_si* si, float x0, float y0) {
+    or_bsdf_sample out;
+    memset(&out, 0, sizeof(out));
+    const v3 Ns = si->Ns, V = si->wi;
+    v3 z, x;
+    frame(Ns, &z, &x);
+    const float phi = OR_TWO_PI * x0;
+    const float cosT = powf(y0, 1.0f / (m->exponent + 1.0f));
+    const float sinT = sqrtf(1.0f - cosT * cosT);
+    const v3 l = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
+    const v3 L = v3_normalize(to_world(x, Ns, z, l));
+    if (v3_dot(V, Ns) * v3_dot(L, Ns) <= 0.0f) return out;
+    const v3 R = v3_reflect(L, Ns);
+    const float RdotV = m_max(v3_dot(R, V), 0.0f);
+    const float powerTerm = powf(RdotV, m->exponent);
+    const float f = powerTerm * phong_brdf_factor(m);
+    out.f = v3_make(f, f, f);
+    out.L = L;
+    out.pdf = powerTerm * phong_pdf_factor(m);
+    out.valid = !v3_is_zero(out.f) && out.pdf > 0.0f;
+    return out;
+}
+
+/* ConductorMicrofacet::evaluateSample (conductorMicrofacet.cpp:69-121) with
+ * IsotropicGgx::sampleHalfVectorH (isotropicGgx.cpp:71-100) */
+static or_bsdf_sample ggx_sample(const or_material* m, const or_si* si, float u0, float u1) {
+    or_bsdf_sample out;
+    memset(&out, 0, sizeof(out));
+    const v3 Ns = si->Ns, V = si->wi;
+    v3 z, x;
+    frame(Ns, &z, &x);
+    const float Nf = (v3_dot(V, Ns) > 0.0f) ? 1.0f : -1.0f;
+    const float a = mat_alpha(m, si->uvw);
+    const float s0 = m_clamp(u0, 0.0f, 0.9999f), s1 = m_clamp(u1, 0.0f, 0.9999f);
+    const float phi = OR_TWO_PI * s0;
+    const float theta = atanf(a * sqrtf(s1 / (1.0f - s1)));
+    v3 h;
+    if (!isfinite(theta)) h = v3_make(0.0f, 1.0f, 0.0f);
+    else {
         const float cosT = cosf(theta);
         const float sinT = sqrtf(1.0f - cosT * cosT);
-        const v3 l = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
-        const float pdf = cosT * OR_INV_PI;
-        v3 L = v3_normalize(to_world(x, Ns, z, l));
-        if (v3_dot(V, Ns) <= 0.0f) L = v3_neg(L);
-        out.f = v3_scale(tex_eval(&m->albedo, si->uvw), OR_INV_PI);
-        out.L = L;
-        out.pdf = pdf;
-    } else {
-        const float Nf = (v3_dot(V, Ns) > 0.0f) ? 1.0f : -1.0f;
-        const float a = mat_alpha(m, si->uvw);
-        const float s0 = m_clamp(u0, 0.0f, 0.9999f), s1 = m_clamp(u1, 0.0f, 0.9999f);
-        const float phi = OR_TWO_PI * s0;
-        const float theta = atanf(a * sqrtf(s1 / (1.0f - s1)));
-        v3 h;
-        if (!isfinite(theta)) h = v3_make(0.0f, 1.0f, 0.0f);
-        else {
-            const float cosT = cosf(theta);
-            const float sinT = sqrtf(1.0f - cosT * cosT);
-            h = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
+        h = v3_make(sinf(phi) * sinT, cosT, cosf(phi) * sinT);
+    }
+    const v3 H = v3_normalize(to_world(x, Ns, z, h));
+    const v3 L = v3_reflect(V, v3_scale(H, Nf));
+    const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns), LdotH = v3_dot(L, H), NdotH = v3_dot(Ns, H);
+    if (VdotN * LdotN <= 0.0f) return out;
+    const v3 F = schlick(v3_load(m->f0), LdotH);
+    const float D = ggx_D(a, Ns, H);
+    const float G = ggx_G(a, V, L, Ns, H);
+    const float pdfH = fabsf(D * NdotH);
+    const float pdfL = fabsf(pdfH / (4.0f * LdotH));
+    if (!isfinite(pdfL)) return out;
+    out.f = v3_divs(v3_scale(v3_scale(F, G), D), 4.0f * fabsf(VdotN * LdotN));
+    out.L = L;
+    out.pdf = pdfL;
+    out.valid = !v3_is_zero(out.f) && out.pdf > 0.0f;
+    return out;
+}
+
+/* Refraction of SpecularTransmission::evaluateSample (specularTransmission.cpp:35-78)
+ * and SpecularDielectric's refraction branch (specularDielectric.cpp:83-112),
+ * radiance transport; pdf set by the caller. */
+static int transmit(const or_material* m, const or_si* si, or_bsdf_sample* out) {
+    float etaI = m->ior_outer, etaT = m->ior_inner;
+    v3 L;
+    if (!can_refract(si->wi, si->Ns, etaI, etaT, &L)) return 0;
+    const float cosThetaI = v3_dot(L, si->Ns);
+    const float R = dielectric_fresnel(m->ior_outer, m->ior_inner, cosThetaI);
+    if (cosThetaI < 0.0f) { const float t = etaI; etaI = etaT; etaT = t; }
+    const float btdf = (etaT * etaT) / (etaI * etaI);
+    const float LdotN = fabsf(cosThetaI);
+    const float T = 1.0f - R;
+    out->f = v3_divs(v3_scale(v3_scale(tex_eval(&m->albedo, si->uvw), T), btdf), LdotN);
+    out->L = L;
+    return 1;
+}
+
+/* Bsdf::evaluateSample of each material from its uniforms u[] in draw order
+ * (bsdf_draws): LambertianDiffuse, ConductorMicrofacet, PhongBsdf as above;
+ * MixedBsdf (mixedBsdf.cpp:65-155: choosing B weighs B by the ratio and A by its
+ * complement, as written there); SpecularReflection (specularReflection.cpp:32-57);
+ * SpecularDielectric (specularDielectric.cpp:35-123); SpecularTransmission;
+ * AbsorberBsdf (never valid). */
+static or_bsdf_sample bsdf_sample_u(const or_material* m, const or_si* si, const float* u) {
+    or_bsdf_sample out;
+    memset(&out, 0, sizeof(out));
+    const v3 Ns = si->Ns, V = si->wi;
+    switch (m->type) {
+    case OR_MAT_LAMBERT: return lambert_sample(m, si, u[1], u[0]);
+    case OR_MAT_GGX: return ggx_sample(m, si, u[1], u[0]);
+    case OR_MAT_PHONG: return phong_sample(m, si, u[1], u[0]);
+    case OR_MAT_PLASTIC: {
+        const float r = m->ratio, rc = 1.0f - r;
+        const float avg = avg3(r);
+        or_si local = *si;
+        if (u[0] < avg) {
+            const or_bsdf_sample a = lambert_sample(m, si, u[2], u[1]);
+            if (!a.valid) return out;
+            local.wo = a.L;
+            const float fb = phong_eval(m, &local);
+            out.f = v3_add(v3_scale(a.f, r), v3_make(rc * fb, rc * fb, rc * fb));
+            out.pdf = avg * a.pdf + (1.0f - avg) * phong_pdf(m, &local);
+            out.L = a.L;
+        } else {
+            const or_bsdf_sample b = phong_sample(m, si, u[2], u[1]);
+            if (!b.valid) return out;
+            local.wo = b.L;
+            const v3 fa = lambert_eval(m, &local);
+            out.f = v3_add(v3_scale(b.f, r), v3_scale(fa, rc));
+            out.pdf = avg * b.pdf + (1.0f - avg) * lambert_pdf(&local);
+            out.L = b.L;
         }
-        const v3 H = v3_normalize(to_world(x, Ns, z, h));
-        const v3 L = v3_reflect(V, v3_scale(H, Nf));
-        const float VdotN = v3_dot(V, Ns), LdotN = v3_dot(L, Ns), LdotH = v3_dot(L, H), NdotH = v3_dot(Ns, H);
-        if (VdotN * LdotN <= 0.0f) return out;
-        const v3 F = schlick(v3_load(m->f0), LdotH);
-        const float D = ggx_D(a, Ns, H);
-        const float G = ggx_G(a, V, L, Ns, H);
-        const float pdfH = fabsf(D * NdotH);
-        const float pdfL = fabsf(pdfH / (4.0f * LdotH));
-        if (!isfinite(pdfL)) return out;
-        out.f = v3_divs(v3_scale(v3_scale(F, G), D), 4.0f * fabsf(VdotN * LdotN));
+        break;
+    }
+    case OR_MAT_MIRROR: {
+        const float VdotN = v3_dot(V, Ns);
+        const float NFactor = (VdotN > 0.0f) ? 1.0f : -1.0f;
+        const v3 L = v3_reflect(V, v3_scale(Ns, NFactor));
+        const float LdotN = v3_dot(L, Ns);
+        v3 F;
+        if (m->fresnel == 0) F = schlick(v3_load(m->f0), LdotN);
+        else { const float R = dielectric_fresnel(m->ior_outer, m->ior_inner, LdotN); F = v3_make(R, R, R); }
+        out.f = v3_divs(v3_mul(F, tex_eval(&m->albedo, si->uvw)), fabsf(LdotN));
         out.L = L;
-        out.pdf = pdfL;
+        out.pdf = 1.0f;
+        break;
+    }
+    case OR_MAT_GLASS: {
+        const float VdotN = v3_dot(V, Ns);
+        const float R = dielectric_fresnel(m->ior_outer, m->ior_inner, VdotN);
+        const float prob = avg3(R);
+        if (u[0] < prob) {
+            const float NFactor = (VdotN > 0.0f) ? 1.0f : -1.0f;
+            const v3 L = v3_reflect(V, v3_scale(Ns, NFactor));
+            const float LdotN = v3_absdot(L, Ns);
+            out.f = v3_divs(v3_scale(tex_eval(&m->albedo, si->uvw), R), LdotN);
+            out.L = L;
+            out.pdf = prob;
+        } else {
+            if (!transmit(m, si, &out)) return out;
+            out.pdf = 1.0f - prob;
+        }
+        break;
+    }
+    case OR_MAT_TRANSMISSION:
+        if (!transmit(m, si, &out)) return out;
+        out.pdf = 1.0f;
+        break;
+    default:
+        return out;
     }
     /* BsdfSample::isValid (bsdfSample.cpp:39-41) */
     out.valid = !v3_is_zero(out.f) && out.pdf > 0.0f;
     return out;
 }
 
-void or_bsdf_query(const or_scene* s, int mi, const float* in, int64_t n, float* out) {
+/* The uniforms of one evaluateSample: the sequential stream in draw order, or the
+ * Philox slots (x, y, lobe choice) of the GPU layout. */
+static or_bsdf_sample bsdf_sample_rng(const or_material* m, const or_si* si, or_rng* rng, int bounce, int slot_x,
+                                      int slot_y, int slot_c) {
+    float u[3] = {0.0f, 0.0f, 0.0f};
+    const int n = bsdf_draws(m);
+    if (rng->mode == OR_RNG_SEQ) {
+        for (int i = 0; i < n; ++i) u[i] = rng_u(rng, bounce, 0);
+    } else if (n == 1) {
+        u[0] = rng_u(rng, bounce, slot_c);
+    } else if (n == 2) {
+        u[0] = rng_u(rng, bounce, slot_y);
+        u[1] = rng_u(rng, bounce, slot_x);
+    } else if (n == 3) {
+        u[0] = rng_u(rng, bounce, slot_c);
+        u[1] = rng_u(rng, bounce, slot_y);
+        u[2] = rng_u(rng, bounce, slot_x);
+    }
+    return bsdf_sample_u(m, si, u);
+}
+
+void or_bsdf_query_n(const or_scene* s, int mi, const float* in, int64_t n, int nu, float* out) {
     const or_material* m = &s->mats[mi];
     for (int64_t i = 0; i < n; ++i) {
-        const float* q = in + 14 * i;
+        const float* q = in + (12 + nu) * i;
         float* o = out + 12 * i;
         or_si si;
         memset(&si, 0, sizeof(si));
@@ -597,13 +873,18 @@ void or_bsdf_query(const or_scene* s, int mi, const float* in, int64_t n, float*
         si.uvw = v3_load(q + 9);
         v3_store(o, bsdf_eval(m, &si));
         o[3] = bsdf_pdf(m, &si);
-        /* q[12] is drawn first and lands in sample.y (right-to-left evaluation) */
-        const or_bsdf_sample b = bsdf_sample(m, &si, q[13], q[12]);
+        float u[3] = {0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < nu && k < 3; ++k) u[k] = q[12 + k];
+        const or_bsdf_sample b = bsdf_sample_u(m, &si, u);
         v3_store(o + 4, b.f);
         v3_store(o + 7, b.L);
         o[10] = b.pdf;
         o[11] = (float)b.valid;
     }
+}
+
+void or_bsdf_query(const or_scene* s, int mi, const float* in, int64_t n, float* out) {
+    or_bsdf_query_n(s, mi, in, n, 2, out);
 }
 
 /* ---- area lights ---------------------------------------------------------------- */
@@ -961,9 +1242,7 @@ static v3 direct_light(const or_scene* s, const or_si* si0, const or_material* m
         }
     }
     {
-        float u0, u1;
-        rng_u2(rng, bounce, SL_MIS0, SL_MIS1, &u0, &u1);
-        const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+        const or_bsdf_sample bs = bsdf_sample_rng(m, &si, rng, bounce, SL_MIS0, SL_MIS1, SL_MISC);
         if (bs.valid) {
             const v3 d = v3_normalize(bs.L);
             or_si hit;
@@ -991,10 +1270,14 @@ static v3 direct_light(const or_scene* s, const or_si* si0, const or_material* m
 }
 
 /* PtEstimator::estimate (ptEstimator.cpp:25-123) with RussianRoulette::isSurvivedOnNextRound
- * (russianRoulette.cpp:9-28). Both materials here are non-specular, so every vertex does
- * next-event estimation and emission is counted only at bounce 0. */
+ * (russianRoulette.cpp:9-28): next-event estimation at non-specular vertices (an
+ * absorber picks a light and returns before sampling it, directLightEvaluator.cpp:33-35);
+ * emission counted at the first hit and after specular vertices.
+ * VptEstimator::estimate (vptEstimator.cpp:24-88): BSDF sampling only, emission at
+ * every hit. */
 static v3 estimate(const or_scene* s, v3 o, v3 d, or_rng* rng, or_counts* cnt) {
     v3 L = v3_make(0, 0, 0), beta = v3_make(1, 1, 1);
+    const int vpt = s->desc.estimator == OR_EST_VPT;
     int countEmit = 1;
     v3 ro = o, rd = d;
     for (int bounce = 0; bounce < s->desc.max_depth; ++bounce) {
@@ -1002,27 +1285,35 @@ static v3 estimate(const or_scene* s, v3 o, v3 d, or_rng* rng, or_counts* cnt) {
         cnt->closest++;
         if (!scene_intersect(s, ro, rd, OR_RAY_EPS, FLT_MAX, &si, NULL)) {
             /* the background sphere (scene.cpp:47-61): emission when counted; its
-             * absorber BSDF is not specular, so one light is still picked
+             * absorber BSDF is not specular, so pt still picks one light
              * (ptEstimator.cpp:68-82) but DirectLightEvaluator returns zero for an
              * absorber (directLightEvaluator.cpp:33-35), and the absorber's sample is
              * invalid, which ends the path (ptEstimator.cpp:86-90) */
             if (!background_hit(s, rd, FLT_MAX, &si)) break;
-            if (countEmit) L = v3_add(L, v3_mul(beta, env_emittance(&s->lights[s->env_light], &si)));
-            float pickPdf;
-            (void)pick_light(s, rng, bounce, &pickPdf);
+            if (countEmit || vpt) L = v3_add(L, v3_mul(beta, env_emittance(&s->lights[s->env_light], &si)));
+            if (!vpt) {
+                float pickPdf;
+                (void)pick_light(s, rng, bounce, &pickPdf);
+            }
             break;
         }
         const or_prim* p = &s->prims[si.prim];
         const or_material* m = &s->mats[p->mat];
-        if (p->light >= 0 && countEmit) L = v3_add(L, v3_mul(beta, light_emittance(s, p->light, &si)));
-        countEmit = 0;
-        float pickPdf;
-        const int li = pick_light(s, rng, bounce, &pickPdf);
-        const v3 Ld = v3_divs(direct_light(s, &si, m, li, rng, bounce, cnt), pickPdf);
-        L = v3_add(L, v3_mul(beta, Ld));
-        float u0, u1;
-        rng_u2(rng, bounce, SL_CONT0, SL_CONT1, &u0, &u1);
-        const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+        if (p->light >= 0 && (countEmit || vpt)) L = v3_add(L, v3_mul(beta, light_emittance(s, p->light, &si)));
+        if (!vpt) {
+            if (!mat_specular(m)) {
+                countEmit = 0;
+                float pickPdf;
+                const int li = pick_light(s, rng, bounce, &pickPdf);
+                if (m->type != OR_MAT_ABSORB) {
+                    const v3 Ld = v3_divs(direct_light(s, &si, m, li, rng, bounce, cnt), pickPdf);
+                    L = v3_add(L, v3_mul(beta, Ld));
+                }
+            } else {
+                countEmit = 1;
+            }
+        }
+        const or_bsdf_sample bs = bsdf_sample_rng(m, &si, rng, bounce, SL_CONT0, SL_CONT1, SL_CONTC);
         if (!bs.valid) break;
         const float LdotN = v3_absdot(bs.L, si.Ns);
         beta = v3_mul(beta, v3_divs(v3_scale(bs.f, LdotN), bs.pdf));
@@ -1484,9 +1775,7 @@ static int adj_record(const or_scene* s, v3 o, v3 d, or_rng* rng, adj_vertex* vs
             }
         }
         {
-            float u0, u1;
-            rng_u2(rng, bounce, SL_MIS0, SL_MIS1, &u0, &u1);
-            const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+            const or_bsdf_sample bs = bsdf_sample_rng(m, &si, rng, bounce, SL_MIS0, SL_MIS1, SL_MISC);
             if (bs.valid) {
                 or_si hit;
                 if (scene_intersect(s, si.P, v3_normalize(bs.L), OR_RAY_EPS, FLT_MAX, &hit, NULL) &&
@@ -1502,9 +1791,7 @@ static int adj_record(const or_scene* s, v3 o, v3 d, or_rng* rng, adj_vertex* vs
                 }
             }
         }
-        float u0, u1;
-        rng_u2(rng, bounce, SL_CONT0, SL_CONT1, &u0, &u1);
-        const or_bsdf_sample bs = bsdf_sample(m, &si, u0, u1);
+        const or_bsdf_sample bs = bsdf_sample_rng(m, &si, rng, bounce, SL_CONT0, SL_CONT1, SL_CONTC);
         if (!bs.valid) break;
         const float LdotN = v3_absdot(bs.L, si.Ns);
         beta = v3_mul(beta, v3_divs(v3_scale(bs.f, LdotN), bs.pdf));

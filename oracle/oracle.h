@@ -30,8 +30,12 @@
 extern "C" {
 #endif
 
-enum { OR_PRIM_TRIANGLE = 0, OR_PRIM_RECTANGLE = 1 };
-enum { OR_MAT_LAMBERT = 0, OR_MAT_GGX = 1 };
+enum { OR_PRIM_TRIANGLE = 0, OR_PRIM_RECTANGLE = 1, OR_PRIM_SPHERE = 2 };
+enum {
+    OR_MAT_LAMBERT = 0, OR_MAT_GGX = 1, OR_MAT_PHONG = 2, OR_MAT_PLASTIC = 3, OR_MAT_MIRROR = 4, OR_MAT_GLASS = 5,
+    OR_MAT_TRANSMISSION = 6, OR_MAT_ABSORB = 7
+};
+enum { OR_EST_PT = 0, OR_EST_VPT = 1 };
 enum { OR_FILTER_BOX = 0, OR_FILTER_GAUSSIAN = 1, OR_FILTER_CONE = 2, OR_FILTER_MITCHELL = 3 };
 enum { OR_SAMPLER_RANDOM = 0, OR_SAMPLER_STRATIFIED = 1 };
 enum { OR_CLUSTER_UNIFORM = 0, OR_CLUSTER_POWER = 1 };
@@ -48,9 +52,13 @@ typedef struct {
 
 typedef struct {
     int32_t    type;       /* OR_MAT_* */
-    or_texture albedo;     /* lambert albedo */
-    float      f0[3];      /* ggx schlick f0 */
+    or_texture albedo;     /* lambert albedo; plastic diffuse-albedo; mirror / glass / transmission albedo */
+    float      f0[3];      /* ggx / conductor-mirror schlick f0 */
     or_texture roughness;  /* ggx roughness (1 channel) */
+    float      exponent;   /* phong / plastic specular exponent */
+    float      ratio;      /* plastic diffuse ratio */
+    int32_t    fresnel;    /* mirror: 0 schlick conductor, 1 dielectric */
+    float      ior_outer, ior_inner;
 } or_material;
 
 typedef struct {
@@ -65,7 +73,7 @@ typedef struct {
 typedef struct {
     int64_t        nprims;
     const int32_t* prim_type;   /* OR_PRIM_* */
-    const float*   prim_v;      /* nprims x 9: v1 v2 v3 (rectangle: vA vB vC) */
+    const float*   prim_v;      /* nprims x 9: v1 v2 v3 (rectangle: vA vB vC; sphere: centre, radius) */
     const float*   prim_n;      /* nprims x 9 vertex normals (triangles) or NULL */
     const float*   prim_uvw;    /* nprims x 9 vertex uvw (triangles) or NULL */
     const int32_t* prim_mat;    /* material per primitive */
@@ -84,6 +92,7 @@ typedef struct {
     /* renderer */
     int32_t sampler, spp, max_depth;
     int32_t split_sah;          /* 1 sah, 0 equal */
+    int32_t estimator;          /* OR_EST_* */
 } or_scene_desc;
 
 typedef struct or_scene or_scene;
@@ -118,6 +127,8 @@ void or_normalize(const float* v, int64_t n, float* out);
 
 /* BSDF of material m at one shading point; in/out layout as ref_bsdf_query. */
 void or_bsdf_query(const or_scene* s, int m, const float* in, int64_t n, float* out);
+/* The same with nu uniforms per query in draw order: in n x (12 + nu). */
+void or_bsdf_query_n(const or_scene* s, int m, const float* in, int64_t n, int nu, float* out);
 /* Light li direct sample towards target, explicit uniforms: in n x (3 + nu). */
 void or_light_sample(const or_scene* s, int li, const float* in, int64_t n, int nu, float* out);
 void or_light_hit(const or_scene* s, const float* rays, int64_t n, float* out);

@@ -514,15 +514,16 @@ double ref_time_render(void* h, int64_t begin, int64_t end, int spp, int nthread
 }
 
 // Bsdf::evaluate / evaluatePdfW / evaluateSample of the named material at one
-// shading point. in: n x 14 (Ns, wi, wo, uvw, u0, u1); out: n x 12
-// (f(3), pdf, sample_f(3), sample_L(3), sample_pdf, valid).
-int ref_bsdf_query(void* h, const char* material, const float* in, int64_t n, float* out) {
+// shading point. in: n x (12 + nu) (Ns, wi, wo, uvw, then the uniforms the sample
+// draws, in draw order); out: n x 12 (f(3), pdf, sample_f(3), sample_L(3),
+// sample_pdf, valid).
+int ref_bsdf_query_n(void* h, const char* material, const float* in, int64_t n, int nu, float* out) {
     auto* s = static_cast<RefScene*>(h);
     const auto it = s->bsdfs.find(std::string(material));
     if (it == s->bsdfs.end()) return -1;
     const Bsdf* bsdf = it->second.get();
     for (int64_t i = 0; i < n; ++i) {
-        const float* q = in + 14 * i;
+        const float* q = in + (12 + nu) * i;
         float*       o = out + 12 * i;
         SurfaceDetail detail;
         detail.setShadingNormal(v3(q));
@@ -536,7 +537,7 @@ int ref_bsdf_query(void* h, const char* material, const float* in, int64_t n, fl
         bsdf->evaluate(TransportInfo(), si).transformToRgb(&f);
         put3(o, f);
         o[3] = bsdf->evaluatePdfW(TransportInfo(), si);
-        refharness::rng_set_queue(q + 12, 2);
+        refharness::rng_set_queue(q + 12, static_cast<size_t>(nu));
         BsdfSample sample;
         bsdf->evaluateSample(TransportInfo(ETransportMode::RADIANCE), si, &sample);
         Vector3R sf;
@@ -548,6 +549,10 @@ int ref_bsdf_query(void* h, const char* material, const float* in, int64_t n, fl
     }
     refharness::rng_set_stream(0);
     return 0;
+}
+
+int ref_bsdf_query(void* h, const char* material, const float* in, int64_t n, float* out) {
+    return ref_bsdf_query_n(h, material, in, n, 2, out);
 }
 
 // Light::evaluateDirectSample for light index li towards target P, with the given

@@ -56,8 +56,42 @@ def scene_set():
                              sun_diameter_deg=12.0, offset=(0.0, 0.0, 0.0))
     sky_first = scenes.sky_spheres(n_spheres=3, level=1, width=16, height=12, spp=2, max_depth=5, env_size=(48, 24),
                                    sun_diameter_deg=16.0, env_slot=0, offset=(0.0, 0.0, 0.0), cluster="uniform")
+    # the reference's shipped scene (plastic and GGX spheres) at a small film, and
+    # every material of the path with pt, vpt, and pt under a sky
+    shipped = scenes.shipped_cornell(width=32, height=24, spp=4)
+    zoo = scenes.material_zoo(width=32, height=24, spp=4)
+    zoo_vpt = scenes.material_zoo(width=32, height=24, spp=4, estimator="vpt")
+    zoo_sky = scenes.material_zoo(width=24, height=20, spp=4, env=scenes.sky_texture(64, 32, sun_diameter_deg=10.0),
+                                  cluster="power")
+    zoo_sky.meshes = [m for m in zoo_sky.meshes if m.name != "back"]   # the sky behind an open back
     return [("cornell", cornell), ("cornell_gauss", cornell_g), ("spheres", spheres), ("spheres_equal", equal),
-            ("sky", sky), ("sky_envfirst", sky_first)]
+            ("sky", sky), ("sky_envfirst", sky_first), ("shipped", shipped), ("zoo", zoo), ("zoo_vpt", zoo_vpt),
+            ("zoo_sky", zoo_sky)]
+
+
+SHIPPED = "/root/reference/scene/cornell-box.crsd"
+
+
+def check_shipped(spec, R):
+    """scenes.shipped_cornell restates the reference's own scene file: the
+    compiled reference gives the same tree, primary rays and per-sample radiance
+    for that file (film and sample count set to the spec's) as for the spec's CRSD."""
+    text = open(SHIPPED).read()
+    for a, b in (('[image-width int32 "640"]', f'[image-width int32 "{spec.width}"]'),
+                 ('[image-height int32 "480"]', f'[image-height int32 "{spec.height}"]'),
+                 ('[sample-number int32 "32"]', f'[sample-number int32 "{spec.spp}"]')):
+        assert text.count(a) == 1, a
+        text = text.replace(a, b)
+    F = RefScene(text)
+    for k, v in F.bvh().items():
+        assert np.array_equal(v, R.bvh()[k]), k
+    rng = np.random.default_rng(5)
+    fxy = np.stack([rng.uniform(0, spec.width, 500), rng.uniform(0, spec.height, 500)], 1).astype(np.float32)
+    assert np.array_equal(F.camera_rays(fxy), R.camera_rays(fxy))
+    keys = np.array([seq_key(77, i, 3) for i in range(len(fxy))], np.uint64)
+    a, b = F.estimate(fxy, keys), R.estimate(fxy, keys)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(F.render_linear(seed=31, threads=4), R.render_linear(seed=31, threads=4))
 
 
 def main():
@@ -73,7 +107,13 @@ def main():
         bvh = R.bvh(sah=spec.split_mode != "equal")
         for k, v in bvh.items():
             out["bvh_" + k] = v
+        if name == "shipped":
+            check_shipped(spec, R)
         lo, hi = (-0.2, 1.2) if name.startswith("cornell") else ((-18, 0, -18), (18, 6, 18))
+        if name.startswith("zoo"):
+            lo, hi = (0.2, 0.2, -9.8), (9.8, 9.8, 3.0)
+        elif name == "shipped":
+            lo, hi = (-80.0, 10.0, -640.0), (630.0, 540.0, 300.0)
         rays = _rays(4000, lo, hi, seed=len(name))
         prim, t, d = R.trace(rays)
         assert np.array_equal(d.view(np.int32), rays[:, 4:7].view(np.int32))
@@ -93,17 +133,19 @@ def main():
         # single-function vectors
         if spec.env is not None:
             out["light_flux"] = R.light_flux()
-        if name in ("cornell", "spheres", "sky", "sky_envfirst"):
+        if name in ("cornell", "spheres", "sky", "sky_envfirst", "shipped", "zoo"):
             m = len(spec.materials)
-            q = np.zeros((m, 300, 14), np.float32)
+            nu = 3 if name in ("shipped", "zoo") else 2     # plastic and glass draw a lobe choice
+            q = np.zeros((m, 300, 12 + nu), np.float32)
             for mi in range(m):
                 ns = normalize_ref(rng.normal(size=(300, 3)).astype(np.float32))
                 wi = normalize_ref(rng.normal(size=(300, 3)).astype(np.float32))
                 wo = normalize_ref(rng.normal(size=(300, 3)).astype(np.float32))
                 q[mi, :, 0:3], q[mi, :, 3:6], q[mi, :, 6:9] = ns, wi, wo
                 q[mi, :, 9:12] = rng.uniform(0, 1, (300, 3))
-                q[mi, :, 12:14] = rng.uniform(0, 1, (300, 2))
-            bsdf = np.stack([R.bsdf_query(spec.materials[mi].name, q[mi]) for mi in range(min(m, 6))])
+                q[mi, :, 12:] = rng.uniform(0, 1, (300, nu))
+            nm = m if nu == 3 else min(m, 6)
+            bsdf = np.stack([R.bsdf_query(spec.materials[mi].name, q[mi]) for mi in range(nm)])
             out.update(bsdf_in=q[: len(bsdf)], bsdf_out=bsdf)
             nl = R.nlights
             lq = np.zeros((nl, 200, 3 + 16), np.float32)

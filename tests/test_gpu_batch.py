@@ -29,6 +29,8 @@ def batch_set(ctx):
     specs.append(scenes.textured_spheres(n_spheres=3, level=2, tex_res=16, width=32, height=32, spp=3, max_depth=5))
     specs.append(scenes.sky_spheres(n_spheres=4, level=2, width=40, height=24, spp=3, max_depth=6, env_size=(64, 32),
                                     sun_diameter_deg=6.0))
+    specs.append(scenes.material_zoo(width=40, height=30, spp=3, max_depth=9, textured=True))
+    specs.append(scenes.material_zoo(width=24, height=20, spp=5, max_depth=7, estimator="vpt"))
     big = scenes.procedural_spheres(seed=5, n_spheres=6, level=3, width=256, height=200, spp=6, max_depth=9)
     specs.append(big)
     seeds = [1000 + 17 * i for i in range(len(specs))]

@@ -93,3 +93,18 @@ def test_spheres_ggx_power_cluster_vs_reference_renderer(ctx, tmp_path):
     frac, dmean, gse = _zstats(g, r, "spheres ggx power 48x32 8spp")
     assert frac < 0.015
     assert dmean < 3 * gse + 1e-9
+
+
+@pytest.mark.skipif(not ref_available(), reason="compiled reference harness (oracle/_ref) not built")
+def test_shipped_scene_vs_reference_renderer(ctx):
+    """The reference's shipped scene (plastic + GGX spheres) at 160 x 120, 32 spp."""
+    from cadise_amd.api import Scene
+    spec = scenes.shipped_cornell(width=160, height=120, spp=32)
+    text = spec.to_crsd()
+    sc = Scene.from_crsd_text(ctx, text)
+    ref = RefScene(text)
+    g = _stack_gpu(sc, 32, range(300, 300 + R))
+    r = _stack_ref(ref, range(400, 400 + R))
+    frac, dmean, gse = _zstats(g, r, "shipped cornell-box 160x120 32spp")
+    assert frac < 0.015
+    assert dmean < 3 * gse + 1e-9

@@ -28,7 +28,7 @@ def ctx():
     return Context(0)
 
 
-def _compare(gpu_film, or_film, label, frac_bar=0.99):
+def _compare(gpu_film, or_film, label, frac_bar=0.99, exact_weights=True):
     from cadise_amd.api import film_mean
     g, o = film_mean(gpu_film), film_mean(or_film)
     close = np.all(np.abs(g - o) <= 1e-4 * np.abs(o) + 1e-6, axis=-1)
@@ -38,7 +38,12 @@ def _compare(gpu_film, or_film, label, frac_bar=0.99):
           f"oracle={o.mean():.6f} rel={rel_mean:.2e}", flush=True)
     assert close.mean() >= frac_bar, f"{label}: only {close.mean():.4f} of pixels within 1e-4"
     assert rel_mean < 1e-3
-    assert np.array_equal(gpu_film[..., 3], or_film[..., 3]), f"{label}: filter weights differ"
+    if exact_weights:
+        assert np.array_equal(gpu_film[..., 3], or_film[..., 3]), f"{label}: filter weights differ"
+    else:
+        # overlapping footprints (Gaussian filter) summed by the streaming renderer's
+        # atomics in a different order
+        np.testing.assert_allclose(gpu_film[..., 3], or_film[..., 3], rtol=1e-5, atol=1e-6)
 
 
 @pytest.mark.parametrize("sampler", ["stratified", "random"])
@@ -83,7 +88,7 @@ def test_render_progressive_accumulate(ctx, passwise):
         np.testing.assert_allclose(full, part, rtol=2e-5, atol=1e-6)
 
 
-@pytest.mark.parametrize("case", ["spheres", "cornell_gaussian", "textured", "sky"])
+@pytest.mark.parametrize("case", ["spheres", "cornell_gaussian", "textured", "sky", "zoo", "zoo_vpt", "shipped"])
 def test_streaming_matches_passwise(ctx, case):
     """The streaming wavefront (default) and the pass-by-pass wavefront draw the
     same paths: ray counts equal exactly, films up to float summation order."""
@@ -93,7 +98,10 @@ def test_streaming_matches_passwise(ctx, case):
             "textured": lambda: scenes.textured_spheres(n_spheres=3, level=2, tex_res=16, width=40, height=40, spp=3,
                                                         max_depth=5),
             "sky": lambda: scenes.sky_spheres(n_spheres=5, level=2, width=48, height=32, spp=4, max_depth=6,
-                                              env_size=(128, 64), sun_diameter_deg=4.0)}[case]()
+                                              env_size=(128, 64), sun_diameter_deg=4.0),
+            "zoo": lambda: scenes.material_zoo(width=48, height=36, spp=4, max_depth=12),
+            "zoo_vpt": lambda: scenes.material_zoo(width=48, height=36, spp=4, max_depth=12, estimator="vpt"),
+            "shipped": lambda: scenes.shipped_cornell(width=64, height=48, spp=4)}[case]()
     sc = Scene.from_spec(ctx, spec)
     a = sc.render(seed=21)
     sa = sc.last_stats()
@@ -121,6 +129,46 @@ def test_render_sky_environment_vs_oracle(ctx, tmp_path, variant):
     film = sc.render(seed=4242)
     orf = OracleScene(spec).render_linear(4242, rng_mode=OR_RNG_PHILOX)
     _compare(film, orf, f"sky {variant}", frac_bar=0.98)
+
+
+def test_render_shipped_scene_vs_oracle(ctx, tmp_path):
+    """N4: the reference's shipped scene (scene/cornell-box.crsd: rectangles, a
+    plastic sphere, a GGX sphere of roughness 0, gaussian filter, 640 x 480, 32 spp,
+    max-depth 10000) end to end from CRSD."""
+    from cadise_amd.api import Scene
+    spec = scenes.shipped_cornell()
+    path = tmp_path / "cornell-box.crsd"
+    path.write_text(spec.to_crsd())
+    sc = Scene.from_crsd(ctx, str(path))
+    film = sc.render(seed=2020)
+    st = sc.last_stats()
+    orf = OracleScene(spec).render_linear(2020, rng_mode=OR_RNG_PHILOX, threads=16)
+    _compare(film, orf, "shipped cornell-box 640x480 32spp", frac_bar=0.98, exact_weights=False)
+    assert st["samples"] == 640 * 480 * 32
+
+
+@pytest.mark.parametrize("variant", ["pt", "vpt", "sky", "textured"])
+def test_render_material_zoo_vs_oracle(ctx, tmp_path, variant):
+    """N4: glass, dielectric and conductor mirrors, transmission, Phong, plastic,
+    GGX, absorber and Lambertian on spheres and rectangles, under pt and vpt and
+    under the sky."""
+    from cadise_amd.api import Scene
+    kw = dict(width=96, height=72, spp=8, max_depth=10)
+    if variant == "vpt":
+        kw["estimator"] = "vpt"
+    if variant == "sky":
+        kw.update(env=scenes.sky_texture(256, 128, sun_diameter_deg=4.0), cluster="power")
+    if variant == "textured":
+        kw["textured"] = True
+    spec = scenes.material_zoo(**kw)
+    if variant == "sky":
+        spec.meshes = [m for m in spec.meshes if m.name != "back"]
+    path = tmp_path / "zoo.crsd"
+    path.write_text(spec.to_crsd(sidecar_dir=str(tmp_path)))
+    sc = Scene.from_crsd(ctx, str(path))
+    film = sc.render(seed=606)
+    orf = OracleScene(spec).render_linear(606, rng_mode=OR_RNG_PHILOX, threads=16)
+    _compare(film, orf, f"material zoo {variant}", frac_bar=0.97)
 
 
 def test_gaussian_filter_vs_oracle(ctx):

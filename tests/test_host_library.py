@@ -92,8 +92,13 @@ light { [type string "area"] [primitive primitive "p"] [color rgb "1 1 1"] [watt
 @pytest.mark.parametrize("renderer,estimator,material,prim,code", [
     ("sampling", "pt", "matte-lambertian", "triangle", 0),
     ("bdpt", "pt", "matte-lambertian", "triangle", -3),
-    ("sampling", "vpt", "matte-lambertian", "triangle", -3),
-    ("sampling", "pt", "plastic", "triangle", -3),
+    ("vpm", "pt", "matte-lambertian", "triangle", -3),
+    ("sampling", "vpt", "matte-lambertian", "triangle", 0),
+    ("sampling", "wdl", "matte-lambertian", "triangle", -3),
+    ("sampling", "pt", "plastic", "triangle", 0),
+    ("sampling", "pt", "glass", "rectangle", 0),
+    ("sampling", "pt", "conductor-dielectric", "triangle", -3),
+    # a sphere light: the reference cannot sample a sphere's surface (sphere.cpp:132-136)
     ("sampling", "pt", "matte-lambertian", "sphere", -3),
 ])
 def test_crsd_support_and_error_codes(ctx, renderer, estimator, material, prim, code):
