@@ -13,6 +13,7 @@
 // draining at the end of every sample pass of every scene.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <thread>
 
@@ -97,7 +98,9 @@ void normalize_opts(mpt_batch_opts& o) {
 int merge_nodes(mpt_batch* B, int wi) {
     if (B->nnodes[wi] > 0) return MPT_OK;
     const int width = wi == 0 ? 4 : 8;
-    const size_t nb = static_cast<size_t>(width == 4 ? sizeof(WideNode<4>) : sizeof(WideNode<8>));
+    const size_t nb = static_cast<size_t>(width == 4 ? sizeof(QNode<4>) : sizeof(QNode<8>));
+    const size_t child_at = width == 4 ? offsetof(QNode<4>, child) : offsetof(QNode<8>, child);
+    const size_t count_at = width == 4 ? offsetof(QNode<4>, count) : offsetof(QNode<8>, count);
     const size_t ns = B->scenes.size();
     std::vector<WideBvh> other(ns);
     std::vector<std::thread> th;
@@ -107,6 +110,9 @@ int merge_nodes(mpt_batch* B, int wi) {
             th.emplace_back([S, &other, i, width]() { collapse_wide(S->host, S->bin, width, S->opts.leaf_max, &other[i]); });
     }
     for (auto& t : th) t.join();
+    for (size_t i = 0; i < ns; ++i)
+        if (B->scenes[i]->wide.width != width && !other[i].ok)
+            return fail(B->ctx, MPT_ERR_UNSUPPORTED, "a BVH node cannot be encoded");
     int64_t total = 0;
     std::vector<int64_t> nbase(ns), tbase(ns);
     int64_t tb = 0;
@@ -127,8 +133,8 @@ int merge_nodes(mpt_batch* B, int wi) {
         uint8_t* dst = merged.data() + static_cast<size_t>(nbase[i]) * nb;
         std::memcpy(dst, w.nodes.data(), static_cast<size_t>(w.nnodes) * nb);
         for (int64_t k = 0; k < w.nnodes; ++k) {
-            int32_t* child = reinterpret_cast<int32_t*>(dst + static_cast<size_t>(k) * nb + 24 * width);
-            int32_t* count = child + width;
+            int32_t* child = reinterpret_cast<int32_t*>(dst + static_cast<size_t>(k) * nb + child_at);
+            const uint16_t* count = reinterpret_cast<const uint16_t*>(dst + static_cast<size_t>(k) * nb + count_at);
             for (int c = 0; c < width; ++c) {
                 if (count[c] > 0) child[c] += static_cast<int32_t>(tbase[i]);
                 else if (child[c] >= 0) child[c] += static_cast<int32_t>(nbase[i]);

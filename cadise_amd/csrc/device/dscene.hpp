@@ -1,7 +1,7 @@
 // Device-resident scene layout (HBM). All arrays are owned by DeviceScene
 // (render.hip); kernels receive this POD by value.
 //
-//   nodes     wide BVH, WideNode<W> (128 B for W = 4, 256 B for W = 8), root at 0
+//   nodes     wide BVH, QNode<W> (64 B for W = 4, 128 B for W = 8), root at 0
 //   tris      3 x float4 per primitive in BVH leaf order:
 //               triangle  (vA, id) (eAB, 0) (eAC, 0)          as Triangle stores them
 //               rectangle (vB, id|1<<31) (eA, 0) (eB, 0)       as Rectangle stores them

@@ -32,15 +32,16 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // ---- traversal -------------------------------------------------------------------
 
-// Persistent traversal with per-lane refill: a lane whose ray has finished
-// takes a new one at the next refill point (wave ballot + one atomic per
-// wave) instead of idling until the whole wave's 64 rays are done, so lanes
-// stay busy despite the very uneven cost of incoherent rays. Between refills
-// every lane runs up to STEPS node visits.
+// Persistent traversal with per-lane prefetch: every lane holds the next ray it
+// will trace (reserved with one atomic per wave, its loads issued long before
+// they are used), and a lane whose ray has finished takes it at once instead of
+// idling until the whole wave's 64 rays are done, so lanes stay busy despite the
+// very uneven cost of incoherent rays. Reservations happen between blocks of up
+// to STEPS node visits.
 template <int W, int S, bool ANY>
 __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     constexpr int STEPS = 8;
-    constexpr int REFILL = 16;   // refill once this many lanes are idle
+    constexpr int REFILL = 16;   // reserve new prefetch slots once this many lanes lack one
     extern __shared__ int32_t lds_stack[];
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -58,47 +59,60 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     const bool no_tree = a.sc.nnodes == 0;
     LaneRay r;
     int64_t rid = -1;            // ray of this lane, -1 = idle
+    int64_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
+    float4 po = make_float4(0.f, 0.f, 0.f, 0.f), pd = po;
     bool exhausted = false;      // wave-uniform: the work counter has run out
+    // a finished lane takes its prefetched ray at once; invalid rays are answered here
+    auto install = [&]() {
+        while (rid < 0 && pid >= 0) {
+            const int64_t i = pid;
+            pid = -1;
+            const bool second = i >= n0;
+            const int64_t q = second ? i - n0 : i;
+            if (pd.w < 0.0f || no_tree) {
+                const int2 out = make_int2(ANY ? 0 : -1, __float_as_int(pd.w));
+                if (second) a.hit2[q] = out;
+                else a.hit[q] = out;
+            } else {
+                if (second) rays2 += 1;
+                else rays += 1;
+                rid = i;
+                r.o = v3f4(po);
+                r.d = v3f4(pd);
+                r.inv = safe_inv(r.d);
+                r.oi = v3(r.o.x * r.inv.x, r.o.y * r.inv.y, r.o.z * r.inv.z);
+                r.tmin = po.w;
+                r.tmax = pd.w;
+                r.hit = -1;
+                r.node = a.root ? a.root[q] : 0;
+                st.sp = 0;
+            }
+        }
+    };
     while (true) {
-        // -- refill idle lanes
-        const unsigned long long idle = __ballot(rid < 0);
-        const int nidle = __popcll(idle);
-        if (!exhausted && (nidle >= REFILL || nidle == WAVE)) {
-            const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+        // -- reserve and start loading rays for lanes without a prefetched one
+        const unsigned long long need = __ballot(pid < 0);
+        const int nneed = __popcll(need);
+        if (!exhausted && (nneed >= REFILL || __ballot(rid >= 0) == 0ull)) {
+            const int leader = __ffsll(static_cast<long long>(need)) - 1;
             int base = 0;
-            if (lane == leader) base = atomicAdd(a.work, nidle);
+            if (lane == leader) base = atomicAdd(a.work, nneed);
             base = __shfl(base, leader, WAVE);
-            if (base + nidle >= n) exhausted = true;
-            if (rid < 0) {
-                const int64_t i = base + __popcll(idle & ((1ull << lane) - 1ull));
+            if (base + nneed >= n) exhausted = true;
+            if (pid < 0) {
+                const int64_t i = base + __popcll(need & ((1ull << lane) - 1ull));
                 if (i < n) {
                     const bool second = i >= n0;
                     const int64_t q = second ? i - n0 : i;
-                    const float4 o4 = a.org[q * a.stride];
-                    const float4 d4 = second ? a.dir2[q * a.stride] : a.dir[q * a.stride];
-                    if (d4.w < 0.0f || no_tree) {
-                        const int2 out = make_int2(ANY ? 0 : -1, __float_as_int(d4.w));
-                        if (second) a.hit2[q] = out;
-                        else a.hit[q] = out;
-                    } else {
-                        if (second) rays2 += 1;
-                        else rays += 1;
-                        rid = i;
-                        r.o = v3f4(o4);
-                        r.d = v3f4(d4);
-                        r.inv = safe_inv(r.d);
-                        r.oi = v3(r.o.x * r.inv.x, r.o.y * r.inv.y, r.o.z * r.inv.z);
-                        r.tmin = o4.w;
-                        r.tmax = d4.w;
-                        r.hit = -1;
-                        r.node = a.root ? a.root[q] : 0;
-                        st.sp = 0;
-                    }
+                    pid = i;
+                    po = a.org[q * a.stride];
+                    pd = second ? a.dir2[q * a.stride] : a.dir[q * a.stride];
                 }
             }
         }
+        install();
         if (__ballot(rid >= 0) == 0ull) {
-            if (exhausted) break;
+            if (exhausted && __ballot(pid >= 0) == 0ull) break;
             continue;
         }
         // -- traversal steps
@@ -122,6 +136,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                 if (second) a.hit2[q] = out;
                 else a.hit[q] = out;
                 rid = -1;
+                install();
             }
             if (__ballot(rid >= 0) == 0ull) break;
         }

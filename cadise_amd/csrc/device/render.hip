@@ -44,6 +44,8 @@ int build_and_upload(mpt_scene* S) {
     t0 = std::chrono::steady_clock::now();
     collapse_wide(h, S->bin, o.width, o.leaf_max, &S->wide);
     S->sstats.collapse_ms = ms_since(t0);
+    if (!S->wide.ok) return fail(ctx, MPT_ERR_UNSUPPORTED, "a BVH node cannot be encoded (a leaf of more than 65535 "
+                                                          "coincident primitives, or coordinates beyond binary32)");
 
     const int64_t n = h.nprims();
     S->sstats.nprims = n;

@@ -9,8 +9,9 @@
 //   Sphere::isIntersecting     (sphere.cpp:42-78, 146-171)
 // Box culling differs from the reference's slab test (tAabb.ipp:68-127): it is
 // the FMA form (lo * inv - o * inv) with a 4-ulp widening of the far distance,
-// over tight boxes that contain each primitive's 1e-4-padded bound, so it
-// never culls a primitive the reference would hit.
+// over 8-bit quantized boxes that contain the tight boxes (which contain each
+// primitive's 1e-4-padded bound), so it never culls a primitive the reference
+// would hit.
 #pragma once
 
 #include "dmath.hpp"
@@ -152,23 +153,38 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         if (st.sp == 0) return true;
         r.node = st.pop();
     }
-    const float4* p = reinterpret_cast<const float4*>(sc.nodes) + static_cast<int64_t>(r.node) * (2 * W);
+    // QNode<W>: origin + exponents, W children, 6 x W box bytes, W 16-bit counts
+    constexpr int NWORD = W == 4 ? 16 : 28;
+    constexpr int QW = 4 + W;                 // first word of the box bytes
+    constexpr int CW = 4 + W + 6 * W / 4;     // first word of the counts
+    const uint4* p = reinterpret_cast<const uint4*>(sc.nodes) + static_cast<int64_t>(r.node) * (W == 4 ? 4 : 8);
+    uint32_t wd[NWORD];
+#pragma unroll
+    for (int k = 0; k < NWORD / 4; ++k) {
+        const uint4 v = p[k];
+        wd[4 * k + 0] = v.x;
+        wd[4 * k + 1] = v.y;
+        wd[4 * k + 2] = v.z;
+        wd[4 * k + 3] = v.w;
+    }
+    const float ox = __uint_as_float(wd[0]), oy = __uint_as_float(wd[1]), oz = __uint_as_float(wd[2]);
+    const float sx = __uint_as_float((wd[3] & 0xFFu) << 23), sy = __uint_as_float(((wd[3] >> 8) & 0xFFu) << 23),
+                sz = __uint_as_float(((wd[3] >> 16) & 0xFFu) << 23);
     float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
     int32_t child[W], count[W];
+    auto qb = [&](int plane, int c) {
+        return static_cast<float>((wd[QW + (plane * W + c) / 4] >> (8 * (c & 3))) & 0xFFu);
+    };
 #pragma unroll
-    for (int q = 0; q < W / 4; ++q) {
-        const float4 a = p[0 * (W / 4) + q], b = p[1 * (W / 4) + q], c = p[2 * (W / 4) + q];
-        const float4 d = p[3 * (W / 4) + q], e = p[4 * (W / 4) + q], f = p[5 * (W / 4) + q];
-        const int4 g = reinterpret_cast<const int4*>(p)[6 * (W / 4) + q];
-        const int4 h = reinterpret_cast<const int4*>(p)[7 * (W / 4) + q];
-        lox[4 * q + 0] = a.x; lox[4 * q + 1] = a.y; lox[4 * q + 2] = a.z; lox[4 * q + 3] = a.w;
-        loy[4 * q + 0] = b.x; loy[4 * q + 1] = b.y; loy[4 * q + 2] = b.z; loy[4 * q + 3] = b.w;
-        loz[4 * q + 0] = c.x; loz[4 * q + 1] = c.y; loz[4 * q + 2] = c.z; loz[4 * q + 3] = c.w;
-        hix[4 * q + 0] = d.x; hix[4 * q + 1] = d.y; hix[4 * q + 2] = d.z; hix[4 * q + 3] = d.w;
-        hiy[4 * q + 0] = e.x; hiy[4 * q + 1] = e.y; hiy[4 * q + 2] = e.z; hiy[4 * q + 3] = e.w;
-        hiz[4 * q + 0] = f.x; hiz[4 * q + 1] = f.y; hiz[4 * q + 2] = f.z; hiz[4 * q + 3] = f.w;
-        child[4 * q + 0] = g.x; child[4 * q + 1] = g.y; child[4 * q + 2] = g.z; child[4 * q + 3] = g.w;
-        count[4 * q + 0] = h.x; count[4 * q + 1] = h.y; count[4 * q + 2] = h.z; count[4 * q + 3] = h.w;
+    for (int c = 0; c < W; ++c) {
+        child[c] = static_cast<int32_t>(wd[4 + c]);
+        count[c] = static_cast<int32_t>((wd[CW + c / 2] >> (16 * (c & 1))) & 0xFFFFu);
+        lox[c] = __builtin_fmaf(qb(0, c), sx, ox);
+        loy[c] = __builtin_fmaf(qb(1, c), sy, oy);
+        loz[c] = __builtin_fmaf(qb(2, c), sz, oz);
+        hix[c] = __builtin_fmaf(qb(3, c), sx, ox);
+        hiy[c] = __builtin_fmaf(qb(4, c), sy, oy);
+        hiz[c] = __builtin_fmaf(qb(5, c), sz, oz);
     }
     cnt.boxes += W;
     float tn[W];
@@ -183,17 +199,45 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
         tn[c] = n;
     }
-    // leaves first: their hits shrink tmax before the interior children are ordered
+    // leaves first: their hits shrink tmax before the interior children are ordered;
+    // the hit leaves' primitives are tested as one flat list so that the wave loops
+    // over the longest list, not over every leaf slot in turn
+    int32_t lf[W], lc[W];
+    int nl = 0, total = 0;
 #pragma unroll
     for (int c = 0; c < W; ++c) {
-        if (hitc[c] && count[c] > 0) {
-            for (int32_t j = 0; j < count[c]; ++j) {
-                cnt.prims += 1;
-                if (prim_test(sc.tris, child[c] + j, r.o, r.d, r.tmin, r.tmax)) {
-                    r.hit = child[c] + j;
-                    if (ANY) return true;
-                }
+        lf[c] = 0;
+        lc[c] = 0;
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        const bool isl = hitc[c] && count[c] > 0;
+#pragma unroll
+        for (int k = 0; k <= c; ++k) {
+            if (isl && nl == k) {
+                lf[k] = child[c];
+                lc[k] = count[c];
             }
+        }
+        total += isl ? count[c] : 0;
+        nl += isl ? 1 : 0;
+    }
+    int cur = 0, j = 0;
+    for (int t = 0; t < total; ++t) {
+        int32_t first = lf[0], cn = lc[0];
+#pragma unroll
+        for (int k = 1; k < W; ++k) {
+            first = (cur == k) ? lf[k] : first;
+            cn = (cur == k) ? lc[k] : cn;
+        }
+        cnt.prims += 1;
+        if (prim_test(sc.tris, first + j, r.o, r.d, r.tmin, r.tmax)) {
+            r.hit = first + j;
+            if (ANY) return true;
+        }
+        if (++j == cn) {
+            ++cur;
+            j = 0;
         }
     }
     float key[W];

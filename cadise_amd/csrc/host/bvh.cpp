@@ -2,6 +2,7 @@
 #include "bvh.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <limits>
@@ -297,6 +298,71 @@ struct Collapser {
     }
 };
 
+// Quantize one float node: origin = the children's minimum corner, the
+// smallest power-of-two step per axis that spans the extent in 255 steps, then
+// q_lo / q_hi moved outwards until fma(q, step, origin) encloses the tight box
+// exactly as the device evaluates it (a coarser step when 255 do not suffice).
+template <int W>
+bool quantize_node(const WideNode<W>& f, QNode<W>* out) {
+    QNode<W> q{};
+    const float* lo[3] = {f.lo_x, f.lo_y, f.lo_z};
+    const float* hi[3] = {f.hi_x, f.hi_y, f.hi_z};
+    uint32_t exps = 0;
+    for (int a = 0; a < 3; ++a) {
+        float o = FLT_MAX, top = -FLT_MAX;
+        for (int c = 0; c < W; ++c) {
+            if (f.child[c] < 0) continue;
+            o = std::min(o, lo[a][c]);
+            top = std::max(top, hi[a][c]);
+        }
+        if (o == FLT_MAX) o = 0.f, top = 0.f;
+        q.origin[a] = o;
+        const double ext = static_cast<double>(top) - static_cast<double>(o);
+        int e = -126;
+        if (ext > 0.0) {
+            int x = 0;
+            std::frexp(ext / 255.0, &x);   // ext / 255 in [2^(x-1), 2^x)
+            e = std::max(-126, x - 1);
+            while (e < 127 && std::ldexp(255.0, e) < ext) ++e;
+        }
+        for (;; ++e) {
+            if (e > 127) return false;
+            const float step = std::ldexp(1.0f, e);
+            bool ok = true;
+            for (int c = 0; c < W && ok; ++c) {
+                if (f.child[c] < 0) {
+                    q.q[a][c] = 255;
+                    q.q[3 + a][c] = 0;
+                    continue;
+                }
+                double dl = std::floor((static_cast<double>(lo[a][c]) - o) / step);
+                int ql = static_cast<int>(std::max(0.0, std::min(255.0, dl)));
+                while (ql > 0 && std::fma(static_cast<float>(ql), step, o) > lo[a][c]) --ql;
+                double dh = std::ceil((static_cast<double>(hi[a][c]) - o) / step);
+                int qh = static_cast<int>(std::max(0.0, std::min(255.0, dh)));
+                while (qh < 255 && std::fma(static_cast<float>(qh), step, o) < hi[a][c]) ++qh;
+                if (std::fma(static_cast<float>(ql), step, o) > lo[a][c] ||
+                    std::fma(static_cast<float>(qh), step, o) < hi[a][c]) {
+                    ok = false;
+                    break;
+                }
+                q.q[a][c] = static_cast<uint8_t>(ql);
+                q.q[3 + a][c] = static_cast<uint8_t>(qh);
+            }
+            if (ok) break;
+        }
+        exps |= static_cast<uint32_t>(e + 127) << (8 * a);
+    }
+    q.exps = exps;
+    for (int c = 0; c < W; ++c) {
+        q.child[c] = f.child[c];
+        if (f.count[c] > 65535) return false;
+        q.count[c] = static_cast<uint16_t>(f.count[c]);
+    }
+    *out = q;
+    return true;
+}
+
 }  // namespace
 
 void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf_max, WideBvh* out) {
@@ -352,15 +418,19 @@ void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf
         if (nn > 0) c.make(0, 0);
         out->nnodes = static_cast<int64_t>(c.nodes.size());
         out->depth = c.depth;
-        out->nodes.resize(c.nodes.size() * sizeof(WideNode<8>));
-        std::memcpy(out->nodes.data(), c.nodes.data(), out->nodes.size());
+        out->nodes.resize(c.nodes.size() * sizeof(QNode<8>));
+        QNode<8>* qn = reinterpret_cast<QNode<8>*>(out->nodes.data());
+        for (size_t i = 0; i < c.nodes.size(); ++i)
+            if (!quantize_node<8>(c.nodes[i], &qn[i])) out->ok = false;
     } else {
         Collapser<4> c{bin, tight, first, count, leaf_max};
         if (nn > 0) c.make(0, 0);
         out->nnodes = static_cast<int64_t>(c.nodes.size());
         out->depth = c.depth;
-        out->nodes.resize(c.nodes.size() * sizeof(WideNode<4>));
-        std::memcpy(out->nodes.data(), c.nodes.data(), out->nodes.size());
+        out->nodes.resize(c.nodes.size() * sizeof(QNode<4>));
+        QNode<4>* qn = reinterpret_cast<QNode<4>*>(out->nodes.data());
+        for (size_t i = 0; i < c.nodes.size(); ++i)
+            if (!quantize_node<4>(c.nodes[i], &qn[i])) out->ok = false;
         out->width = 4;
     }
     out->max_stack = (out->width - 1) * (out->depth + 1) + 1;

@@ -49,15 +49,35 @@ struct alignas(16) WideNode {
     int32_t child[W];
     int32_t count[W];
 };
-static_assert(sizeof(WideNode<4>) == 128, "BVH4 node must be one 128-byte line");
-static_assert(sizeof(WideNode<8>) == 256, "BVH8 node must be two 128-byte lines");
+
+// The device's node: child boxes as 8-bit offsets on a per-node grid with
+// origin o and power-of-two steps 2^(e - 127) per axis (exps: the biased
+// exponents in bytes 0..2). A child's box is [o + q_lo 2^e, o + q_hi 2^e], each
+// corner evaluated as fma(q, 2^e, o) in binary32; the host picks q so that this
+// box contains the child's tight box. child: leaf = first primitive (leaf
+// order), interior = node index, -1 = empty slot; count: primitives of a leaf,
+// 0 for an interior child. BVH4: 64 bytes (two nodes per 128-byte line), BVH8:
+// 128 bytes (one line).
+template <int W>
+struct alignas(16) QNode {
+    float    origin[3];
+    uint32_t exps;
+    int32_t  child[W];
+    uint8_t  q[6][W];            // lo_x, lo_y, lo_z, hi_x, hi_y, hi_z
+    uint16_t count[W];
+    uint8_t  pad[W == 8 ? 16 : 0];
+};
+static_assert(sizeof(QNode<4>) == 64, "BVH4 node must be half a 128-byte line");
+static_assert(sizeof(QNode<8>) == 128, "BVH8 node must be one 128-byte line");
 
 struct WideBvh {
     int32_t width = 4;
-    std::vector<uint8_t> nodes;   // WideNode<width> array, root at 0
+    std::vector<uint8_t> nodes;   // QNode<width> array, root at 0
     int64_t nnodes = 0;
     int32_t depth = 0;
     int32_t max_stack = 0;        // upper bound on traversal stack entries
+    bool    ok = true;            // false: a node could not be encoded (a leaf of more than
+                                  // 65535 primitives, or boxes beyond the binary32 range)
 };
 
 void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf_max, WideBvh* out);
