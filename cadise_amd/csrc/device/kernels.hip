@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     const int lane = tid & (WAVE - 1);
     const int64_t gslot = static_cast<int64_t>(blockIdx.x) * blockDim.x + tid;
     LaneStack<S> st;
-    st.lds = lds_stack;
+    st.lds = (__attribute__((address_space(3))) int32_t*)lds_stack;
     st.lane = tid;
     st.stride = blockDim.x;
     st.spill = a.spill ? a.spill + gslot * (a.sc.max_stack - S) : nullptr;
@@ -61,6 +61,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     int64_t rid = -1;            // ray of this lane, -1 = idle
     int64_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
     float4 po = make_float4(0.f, 0.f, 0.f, 0.f), pd = po;
+    int32_t proot = 0;           // its BVH root
     bool exhausted = false;      // wave-uniform: the work counter has run out
     // a finished lane takes its prefetched ray at once; invalid rays are answered here
     auto install = [&]() {
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                 r.tmin = po.w;
                 r.tmax = pd.w;
                 r.hit = -1;
-                r.node = a.root ? a.root[q] : 0;
+                r.node = proot;
                 st.sp = 0;
             }
         }
@@ -107,6 +108,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                     pid = i;
                     po = a.org[q * a.stride];
                     pd = second ? a.dir2[q * a.stride] : a.dir[q * a.stride];
+                    proot = a.root ? a.root[q] : 0;
                 }
             }
         }

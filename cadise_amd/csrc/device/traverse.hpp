@@ -96,7 +96,7 @@ __device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32
 // slice. A scene whose worst-case depth fits S never touches the spill.
 template <int S>
 struct LaneStack {
-    int32_t* lds;      // base of this block's LDS stack
+    __attribute__((address_space(3))) int32_t* lds;   // this block's LDS stack (LDS-typed: ds_read / ds_write)
     int32_t* spill;    // this lane's spill slice (may be null when max <= S)
     int      lane;     // thread index in block
     int      stride;   // threads per block
@@ -108,7 +108,11 @@ struct LaneStack {
     }
     __device__ __forceinline__ int32_t pop() {
         --sp;
-        return sp < S ? lds[sp * stride + lane] : spill[sp - S];
+        // the LDS read always (clamped index), the spill read only past S: two
+        // loads of their own address spaces instead of one generic (flat) load
+        int32_t v = lds[min(sp, S - 1) * stride + lane];
+        if (sp >= S) v = spill[sp - S];
+        return v;
     }
 };
 
