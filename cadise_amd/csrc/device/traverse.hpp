@@ -49,11 +49,14 @@ __device__ __forceinline__ bool sphere_test(V3 G, float r, V3 o, V3 D, float tmi
 // Primitive record at leaf position li: r0 = (A, id bits), r1 = e1, r2 = e2;
 // the id's top bit marks a rectangle, or a sphere when r1.w != 0 (then
 // r0 = centre, r1.x = radius).
-__device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32_t li, V3 o, V3 D, float tmin,
-                                          float& tmax) {
-    const float4 r0 = tris[3 * li + 0];
-    const float4 r1 = tris[3 * li + 1];
-    const float4 r2 = tris[3 * li + 2];
+struct PrimRec {
+    float4 r0, r1, r2;
+};
+__device__ __forceinline__ PrimRec prim_load(const float4* __restrict__ tris, int32_t li) {
+    return PrimRec{tris[3 * li + 0], tris[3 * li + 1], tris[3 * li + 2]};
+}
+__device__ __forceinline__ bool prim_hit(const PrimRec& pr, V3 o, V3 D, float tmin, float& tmax) {
+    const float4 r0 = pr.r0, r1 = pr.r1, r2 = pr.r2;
     V3 ea = v3f4(r1), eb = v3f4(r2);
     const bool rect = (__float_as_uint(r0.w) & 0x80000000u) != 0;
     if (rect && r1.w != 0.0f) return sphere_test(v3f4(r0), r1.x, o, D, tmin, tmax);
@@ -89,6 +92,11 @@ __device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32
     if (p1 < 0.0f || p1 > len(e1) || p2 < 0.0f || p2 > len(e2)) return false;
     tmax = t;
     return true;
+}
+
+__device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32_t li, V3 o, V3 D, float tmin,
+                                          float& tmax) {
+    return prim_hit(prim_load(tris, li), o, D, tmin, tmax);
 }
 
 // Per-lane traversal stack: the first S entries live in LDS (entry-major, so
@@ -227,21 +235,38 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         nl += isl ? 1 : 0;
     }
     int cur = 0, j = 0;
-    for (int t = 0; t < total; ++t) {
+    auto next_prim = [&]() {
         int32_t first = lf[0], cn = lc[0];
 #pragma unroll
         for (int k = 1; k < W; ++k) {
             first = (cur == k) ? lf[k] : first;
             cn = (cur == k) ? lc[k] : cn;
         }
-        cnt.prims += 1;
-        if (prim_test(sc.tris, first + j, r.o, r.d, r.tmin, r.tmax)) {
-            r.hit = first + j;
-            if (ANY) return true;
-        }
+        const int32_t li = first + j;
         if (++j == cn) {
             ++cur;
             j = 0;
+        }
+        return li;
+    };
+    // two primitives per round: both records' loads in flight together
+    for (int t = 0; t < total; t += 2) {
+        const int32_t la = next_prim();
+        const bool two = t + 1 < total;
+        const int32_t lb = two ? next_prim() : la;
+        const PrimRec A = prim_load(sc.tris, la);
+        const PrimRec B = prim_load(sc.tris, lb);
+        cnt.prims += 1;
+        if (prim_hit(A, r.o, r.d, r.tmin, r.tmax)) {
+            r.hit = la;
+            if (ANY) return true;
+        }
+        if (two) {
+            cnt.prims += 1;
+            if (prim_hit(B, r.o, r.d, r.tmin, r.tmax)) {
+                r.hit = lb;
+                if (ANY) return true;
+            }
         }
     }
     float key[W];
