@@ -46,7 +46,8 @@ void launch_ws(bool any, int blocks, int block, size_t lds, hipStream_t st, cons
 }
 
 inline void launch_trace_cfg(const TraceCfg& c, const TraceArgs& a, bool any, int blocks, hipStream_t st) {
-    const size_t lds = sizeof(int32_t) * static_cast<size_t>(c.stack) * static_cast<size_t>(c.block);
+    const size_t lds = (sizeof(int32_t) * static_cast<size_t>(c.stack) + static_cast<size_t>(coop_lds_bytes(c.width))) *
+                       static_cast<size_t>(c.block);
     const int key = c.width * 100 + c.stack;
     switch (key) {
         case 416: launch_ws<4, 16>(any, blocks, c.block, lds, st, a); break;

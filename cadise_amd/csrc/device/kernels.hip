@@ -52,6 +52,17 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     st.stride = blockDim.x;
     st.spill = a.spill ? a.spill + gslot * (a.sc.max_stack - S) : nullptr;
     st.sp = 0;
+    // after the stacks: per thread two float4 (ray) and a key, then per wave a
+    // queue of 64 * W leaves
+    CoopLds cl;
+    {
+        const int wv = tid / WAVE, nt = static_cast<int>(blockDim.x);
+        char* base = reinterpret_cast<char*>(lds_stack + S * nt);
+        cl.ray = (__attribute__((address_space(3))) lds_f4*)(reinterpret_cast<lds_f4*>(base) + 2 * WAVE * wv);
+        cl.key = (__attribute__((address_space(3))) unsigned long long*)(
+                     reinterpret_cast<unsigned long long*>(base + 32 * nt) + WAVE * wv);
+        cl.queue = (__attribute__((address_space(3))) lds_i2*)(reinterpret_cast<lds_i2*>(base + 40 * nt) + WAVE * W * wv);
+    }
     const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
     const int64_t n = a.dual ? 2 * n0 : n0;
     TravCounters cnt{0, 0};
@@ -84,6 +95,8 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                 r.oi = v3(r.o.x * r.inv.x, r.o.y * r.inv.y, r.o.z * r.inv.z);
                 r.tmin = po.w;
                 r.tmax = pd.w;
+                cl.ray[2 * lane] = lds_f4{po.x, po.y, po.z, po.w};
+                cl.ray[2 * lane + 1] = lds_f4{pd.x, pd.y, pd.z, pd.w};
                 r.hit = -1;
                 r.node = proot;
                 st.sp = 0;
@@ -119,7 +132,8 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         }
         // -- traversal steps
         for (int s = 0; s < STEPS; ++s) {
-            if (rid >= 0 && visit<W, S, ANY>(a.sc, st, r, cnt)) {
+            const bool fin = visit<W, S, ANY>(a.sc, st, r, cnt, rid >= 0, cl, lane);
+            if (rid >= 0 && fin) {
                 if (r.hit == -2) {
                     errors += 1;
                     r.hit = -1;

@@ -23,6 +23,10 @@ enum StatSlot {
     ST_COUNT
 };
 
+// LDS bytes per traversal thread beyond its stack entries: its ray (two
+// float4), its best-hit key and W leaf-queue entries (int2).
+__host__ __device__ constexpr int coop_lds_bytes(int W) { return 32 + 8 + 8 * W; }
+
 // A batch of rays for the persistent traversal kernel. Ray i is
 // (org[i * stride], dir[i * stride]) for i < n0 and (org[(i - n0) * stride],
 // dir2[(i - n0) * stride]) for n0 <= i < n0 + n1 (the second set shares

@@ -94,11 +94,6 @@ __device__ __forceinline__ bool prim_hit(const PrimRec& pr, V3 o, V3 D, float tm
     return true;
 }
 
-__device__ __forceinline__ bool prim_test(const float4* __restrict__ tris, int32_t li, V3 o, V3 D, float tmin,
-                                          float& tmax) {
-    return prim_hit(prim_load(tris, li), o, D, tmin, tmax);
-}
-
 // Per-lane traversal stack: the first S entries live in LDS (entry-major, so
 // lanes of a wave hit distinct banks), the rest spill to a per-lane global
 // slice. A scene whose worst-case depth fits S never touches the spill.
@@ -154,121 +149,185 @@ __device__ __forceinline__ void cmpx(float& da, int32_t& ia, float& db, int32_t&
     ia = ti;
 }
 
-// One node visit: slab-test the W children, test the primitives of hit
-// leaves (shrinking tmax), sort the hit interior children by entry distance,
-// continue with the nearest and push the rest. Returns true when the ray has
-// finished (any hit found, or the stack is empty). All register arrays are
-// indexed with compile-time indices.
+// LDS of the wave-cooperative leaf phase, per wave: every lane's ray
+// (origin, tmin) (direction, tmax) and best-hit key, and a queue of the hit
+// leaves of all lanes (first primitive, count | owner lane << 16), up to W
+// per lane.
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+typedef int32_t lds_i2 __attribute__((ext_vector_type(2)));
+struct CoopLds {
+    __attribute__((address_space(3))) lds_f4* ray;
+    __attribute__((address_space(3))) unsigned long long* key;
+    __attribute__((address_space(3))) lds_i2* queue;
+};
+
+// Best-hit keys ordered as unsigned integers: distance first (the float's
+// order-preserving integer form), then the larger leaf position, so that a
+// wave-wide minimum selects what testing the leaf primitives one by one in
+// leaf order selects (t == tmax is accepted, so the later of equal
+// distances wins). The low word 0xFFFFFFFF means "no hit".
+__device__ __forceinline__ uint32_t ord_dist(float t) {
+    const uint32_t u = __float_as_uint(t);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float dist_of(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+__device__ __forceinline__ unsigned long long hit_key(float t, int32_t li) {
+    return (static_cast<unsigned long long>(ord_dist(t)) << 32) | static_cast<uint32_t>(0x7FFFFFFF - li);
+}
+
+// Exclusive prefix sum over the wave of v in [0, 2^BITS), and the wave total,
+// from BITS ballots (every lane of the wave must be active).
+template <int BITS>
+__device__ __forceinline__ int wave_excl_scan(int v, int& total) {
+    int excl = 0;
+    total = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const unsigned long long m = __ballot((v >> b) & 1);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        excl += static_cast<int>(below) << b;
+        total += __popcll(m) << b;
+    }
+    return excl;
+}
+
+// One traversal step of a wave. Every lane with a ray (act) pops its next
+// node, slab-tests the W children and lists its hit leaves; the primitives of
+// all lanes' hit leaves are then tested by the whole wave, one leaf per lane
+// (so a step costs one round of primitive tests however the leaves are spread
+// over the lanes), the per-ray minimum taken by LDS atomics on hit keys; each
+// lane with a ray then sorts its hit interior children by entry distance,
+// continues with the nearest and pushes the rest. Returns true when the
+// lane's ray has finished (any hit found, or the stack is empty). Must be
+// called by all lanes of the wave together. All register arrays are indexed
+// with compile-time indices.
 template <int W, int S, bool ANY>
-__device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, LaneRay& r, TravCounters& cnt) {
-    if (r.node < 0) {
-        if (st.sp == 0) return true;
-        r.node = st.pop();
-    }
-    // QNode<W>: origin + exponents, W children, 6 x W box bytes, W 16-bit counts
-    constexpr int NWORD = W == 4 ? 16 : 28;
-    constexpr int QW = 4 + W;                 // first word of the box bytes
-    constexpr int CW = 4 + W + 6 * W / 4;     // first word of the counts
-    const uint4* p = reinterpret_cast<const uint4*>(sc.nodes) + static_cast<int64_t>(r.node) * (W == 4 ? 4 : 8);
-    uint32_t wd[NWORD];
-#pragma unroll
-    for (int k = 0; k < NWORD / 4; ++k) {
-        const uint4 v = p[k];
-        wd[4 * k + 0] = v.x;
-        wd[4 * k + 1] = v.y;
-        wd[4 * k + 2] = v.z;
-        wd[4 * k + 3] = v.w;
-    }
-    const float ox = __uint_as_float(wd[0]), oy = __uint_as_float(wd[1]), oz = __uint_as_float(wd[2]);
-    const float sx = __uint_as_float((wd[3] & 0xFFu) << 23), sy = __uint_as_float(((wd[3] >> 8) & 0xFFu) << 23),
-                sz = __uint_as_float(((wd[3] >> 16) & 0xFFu) << 23);
-    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
-    int32_t child[W], count[W];
-    auto qb = [&](int plane, int c) {
-        return static_cast<float>((wd[QW + (plane * W + c) / 4] >> (8 * (c & 3))) & 0xFFu);
-    };
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
-        child[c] = static_cast<int32_t>(wd[4 + c]);
-        count[c] = static_cast<int32_t>((wd[CW + c / 2] >> (16 * (c & 1))) & 0xFFFFu);
-        lox[c] = __builtin_fmaf(qb(0, c), sx, ox);
-        loy[c] = __builtin_fmaf(qb(1, c), sy, oy);
-        loz[c] = __builtin_fmaf(qb(2, c), sz, oz);
-        hix[c] = __builtin_fmaf(qb(3, c), sx, ox);
-        hiy[c] = __builtin_fmaf(qb(4, c), sy, oy);
-        hiz[c] = __builtin_fmaf(qb(5, c), sz, oz);
-    }
-    cnt.boxes += W;
+__device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, LaneRay& r, TravCounters& cnt, bool act,
+                                      const CoopLds& cl, int lane) {
+    bool done = !act;
+    int32_t child[W], count[W], lf[W], lc[W];
     float tn[W];
     bool hitc[W];
+    int nl = 0;
 #pragma unroll
     for (int c = 0; c < W; ++c) {
-        const float tx0 = __builtin_fmaf(lox[c], r.inv.x, -r.oi.x), tx1 = __builtin_fmaf(hix[c], r.inv.x, -r.oi.x);
-        const float ty0 = __builtin_fmaf(loy[c], r.inv.y, -r.oi.y), ty1 = __builtin_fmaf(hiy[c], r.inv.y, -r.oi.y);
-        const float tz0 = __builtin_fmaf(loz[c], r.inv.z, -r.oi.z), tz1 = __builtin_fmaf(hiz[c], r.inv.z, -r.oi.z);
-        const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
-        const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-        hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
-        tn[c] = n;
-    }
-    // leaves first: their hits shrink tmax before the interior children are ordered;
-    // the hit leaves' primitives are tested as one flat list so that the wave loops
-    // over the longest list, not over every leaf slot in turn
-    int32_t lf[W], lc[W];
-    int nl = 0, total = 0;
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
+        child[c] = -1;
+        count[c] = 0;
         lf[c] = 0;
         lc[c] = 0;
+        tn[c] = INFINITY;
+        hitc[c] = false;
     }
+    if (act) {
+        if (r.node < 0) {
+            if (st.sp == 0) done = true;
+            else r.node = st.pop();
+        }
+    }
+    if (!done) {
+        // QNode<W>: origin + exponents, W children, 6 x W box bytes, W 16-bit counts
+        constexpr int NWORD = W == 4 ? 16 : 28;
+        constexpr int QW = 4 + W;                 // first word of the box bytes
+        constexpr int CW = 4 + W + 6 * W / 4;     // first word of the counts
+        const uint4* p = reinterpret_cast<const uint4*>(sc.nodes) + static_cast<int64_t>(r.node) * (W == 4 ? 4 : 8);
+        uint32_t wd[NWORD];
 #pragma unroll
-    for (int c = 0; c < W; ++c) {
-        const bool isl = hitc[c] && count[c] > 0;
+        for (int k = 0; k < NWORD / 4; ++k) {
+            const uint4 v = p[k];
+            wd[4 * k + 0] = v.x;
+            wd[4 * k + 1] = v.y;
+            wd[4 * k + 2] = v.z;
+            wd[4 * k + 3] = v.w;
+        }
+        const float ox = __uint_as_float(wd[0]), oy = __uint_as_float(wd[1]), oz = __uint_as_float(wd[2]);
+        const float sx = __uint_as_float((wd[3] & 0xFFu) << 23), sy = __uint_as_float(((wd[3] >> 8) & 0xFFu) << 23),
+                    sz = __uint_as_float(((wd[3] >> 16) & 0xFFu) << 23);
+        auto qb = [&](int plane, int c) {
+            return static_cast<float>((wd[QW + (plane * W + c) / 4] >> (8 * (c & 3))) & 0xFFu);
+        };
+        cnt.boxes += W;
 #pragma unroll
-        for (int k = 0; k <= c; ++k) {
-            if (isl && nl == k) {
-                lf[k] = child[c];
-                lc[k] = count[c];
+        for (int c = 0; c < W; ++c) {
+            child[c] = static_cast<int32_t>(wd[4 + c]);
+            count[c] = static_cast<int32_t>((wd[CW + c / 2] >> (16 * (c & 1))) & 0xFFFFu);
+            const float lox = __builtin_fmaf(qb(0, c), sx, ox), loy = __builtin_fmaf(qb(1, c), sy, oy),
+                        loz = __builtin_fmaf(qb(2, c), sz, oz), hix = __builtin_fmaf(qb(3, c), sx, ox),
+                        hiy = __builtin_fmaf(qb(4, c), sy, oy), hiz = __builtin_fmaf(qb(5, c), sz, oz);
+            const float tx0 = __builtin_fmaf(lox, r.inv.x, -r.oi.x), tx1 = __builtin_fmaf(hix, r.inv.x, -r.oi.x);
+            const float ty0 = __builtin_fmaf(loy, r.inv.y, -r.oi.y), ty1 = __builtin_fmaf(hiy, r.inv.y, -r.oi.y);
+            const float tz0 = __builtin_fmaf(loz, r.inv.z, -r.oi.z), tz1 = __builtin_fmaf(hiz, r.inv.z, -r.oi.z);
+            const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
+            const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+            hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
+            tn[c] = n;
+        }
+        // the hit leaves, compacted in child order
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            const bool isl = hitc[c] && count[c] > 0;
+#pragma unroll
+            for (int k = 0; k <= c; ++k) {
+                if (isl && nl == k) {
+                    lf[k] = child[c];
+                    lc[k] = count[c];
+                }
+            }
+            nl += isl ? 1 : 0;
+        }
+    }
+    // -- the wave's leaves: queue them, then one leaf per lane
+    int ntot;
+    const int excl = wave_excl_scan<W == 4 ? 3 : 4>(nl, ntot);
+    if (ntot > 0) {
+        if (nl > 0) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                if (k < nl) cl.queue[excl + k] = lds_i2{lf[k], lc[k] | (lane << 16)};
+            }
+            cl.ray[2 * lane + 1].w = r.tmax;
+            cl.key[lane] = (static_cast<unsigned long long>(ord_dist(r.tmax)) << 32) | 0xFFFFFFFFull;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int base = 0; base < ntot; base += 64) {
+            const int e = base + lane;
+            if (e < ntot) {
+                const lds_i2 q = cl.queue[e];
+                const int own = q.y >> 16, n = q.y & 0xFFFF;
+                const lds_f4 ro = cl.ray[2 * own], rd = cl.ray[2 * own + 1];
+                const V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
+                float tmax = rd.w;
+                int32_t best = -1;
+                // two records per round: both loads in flight together
+                for (int j = 0; j < n; j += 2) {
+                    const int32_t la = q.x + j;
+                    const bool two = j + 1 < n;
+                    const int32_t lb = two ? la + 1 : la;
+                    const PrimRec A = prim_load(sc.tris, la);
+                    const PrimRec B = prim_load(sc.tris, lb);
+                    cnt.prims += two ? 2 : 1;
+                    if (prim_hit(A, o, d, ro.w, tmax)) best = la;
+                    if (two && prim_hit(B, o, d, ro.w, tmax)) best = lb;
+                }
+                if (best >= 0)
+                    __hip_atomic_fetch_min(cl.key + own, hit_key(tmax, best), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
         }
-        total += isl ? count[c] : 0;
-        nl += isl ? 1 : 0;
-    }
-    int cur = 0, j = 0;
-    auto next_prim = [&]() {
-        int32_t first = lf[0], cn = lc[0];
-#pragma unroll
-        for (int k = 1; k < W; ++k) {
-            first = (cur == k) ? lf[k] : first;
-            cn = (cur == k) ? lc[k] : cn;
-        }
-        const int32_t li = first + j;
-        if (++j == cn) {
-            ++cur;
-            j = 0;
-        }
-        return li;
-    };
-    // two primitives per round: both records' loads in flight together
-    for (int t = 0; t < total; t += 2) {
-        const int32_t la = next_prim();
-        const bool two = t + 1 < total;
-        const int32_t lb = two ? next_prim() : la;
-        const PrimRec A = prim_load(sc.tris, la);
-        const PrimRec B = prim_load(sc.tris, lb);
-        cnt.prims += 1;
-        if (prim_hit(A, r.o, r.d, r.tmin, r.tmax)) {
-            r.hit = la;
-            if (ANY) return true;
-        }
-        if (two) {
-            cnt.prims += 1;
-            if (prim_hit(B, r.o, r.d, r.tmin, r.tmax)) {
-                r.hit = lb;
-                if (ANY) return true;
+        __builtin_amdgcn_wave_barrier();
+        if (nl > 0) {
+            const unsigned long long k = cl.key[lane];
+            const uint32_t lo = static_cast<uint32_t>(k);
+            if (lo != 0xFFFFFFFFu) {
+                r.hit = 0x7FFFFFFF - static_cast<int32_t>(lo);
+                r.tmax = dist_of(static_cast<uint32_t>(k >> 32));
+                if (ANY) done = true;
             }
         }
     }
+    if (done) return true;
     float key[W];
     int32_t kid[W];
     int nk = 0;
