@@ -85,17 +85,29 @@ class Scene:
         keep = [t, v, n, u, m]
         tex = {x.name: x for x in spec.textures}
         arrays = {}        # one array per texture name: materials sharing it share one device texture
-        tex_index = {}     # texture name -> index in the scene's texture list (creation order)
+        tex_index = {}     # image texture name -> index in the scene's texture list (creation order)
+        created = [0]      # textures the library creates, in its order: a checkerboard's odd and
+                           # even textures (constants included) before the checkerboard itself
 
-        def texture(const, name, channels):
+        def texture(const, name, channels, depth=0):
             T = Texture()
             if name is None:
                 c = const if isinstance(const, (tuple, list)) else (const, const, const)
                 T.value[:] = [float(np.float32(x)) for x in c]
+                created[0] += 1 if depth > 0 else 0
+                return T
+            if tex[name].checker is not None:
+                on, en, odd, even = tex[name].checker
+                kids = [texture(*sub_texture(x), channels, depth + 1) for x in (odd, even)]
+                keep.extend(kids)
+                T.odd, T.even = C.addressof(kids[0]), C.addressof(kids[1])
+                T.odd_number, T.even_number = float(np.float32(on)), float(np.float32(en))
+                created[0] += 1
                 return T
             if name not in arrays:
                 arrays[name] = np.ascontiguousarray(tex[name].data, np.float32)
-                tex_index[name] = len(tex_index)
+                tex_index[name] = created[0]
+                created[0] += 1
             arr = arrays[name]
             keep.append(arr)
             T.data = arr.ctypes.data
@@ -103,6 +115,9 @@ class Scene:
             T.channels = arr.shape[2] if arr.ndim == 3 else 1
             T.wrap_clamp = 1 if tex[name].wrap == "clamp" else 0
             return T
+
+        def sub_texture(x):   # a checkerboard's odd / even: a texture name or a constant
+            return (None, x) if isinstance(x, str) else (x, None)
 
         mats = (MaterialDesc * len(spec.materials))()
         for i, mat in enumerate(spec.materials):

@@ -16,7 +16,7 @@ namespace mpt {
 __device__ __forceinline__ int2 vertex_param(const DevScene& sc, const DevMaterial& m, V3 uvw) {
     const int32_t id = m.type == 0 ? m.albedo_tex : m.rough_tex;
     if (id < 0) return make_int2(-1, 0);
-    const DevTexture t = sc.texs[id];
+    const DevTexture t = tex_resolve(sc, id, uvw);
     float u, v;
     if (t.wrap_clamp) {
         u = mclamp(uvw.x, 0.f, 1.f);

@@ -179,6 +179,10 @@ int merge_tables(mpt_batch* B) {
         }
         for (DevTexture t : S->h_texs) {
             t.offset += ntexel;
+            if (t.odd >= 0) {
+                t.odd += tex0;
+                t.even += tex0;
+            }
             texs.push_back(t);
         }
         const int32_t env0 = static_cast<int32_t>(envcdf.size());

@@ -37,9 +37,13 @@ struct DevMaterial {
     float   pad;
 };
 
+// An image (a constant is a 1 x 1 image) or, when odd >= 0, a checkerboard
+// of textures odd / even (its one texel in tex_data is unused).
 struct DevTexture {
     int64_t offset;      // into tex_data
     int32_t width, height, channels, wrap_clamp;
+    int32_t odd, even;
+    float   odd_size, even_size;
 };
 
 // An area light with the geometry of its primitive (lights are few, so the

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -136,6 +137,10 @@ int build_and_upload(mpt_scene* S) {
         d.height = t.height;
         d.channels = t.channels;
         d.wrap_clamp = t.wrap_clamp;
+        d.odd = t.odd;
+        d.even = t.even;
+        d.odd_size = t.odd_size;
+        d.even_size = t.even_size;
         if (t.data.empty()) {
             // a constant texture as a 1x1 image: u * 1 and v * 1 floor to 0
             d.width = d.height = 1;
@@ -948,8 +953,30 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
         if (h.prim_mat[i] < 0 || h.prim_mat[i] >= d->nmats) return fail(ctx, MPT_ERR_INVALID, "bad material index");
     }
     std::vector<std::pair<const float*, int32_t>> seen;   // the same array in several materials is one texture
-    auto add_tex = [&](const mpt_texture_desc& t, int channels) -> int32_t {
-        if (!t.data) return -1;
+    std::function<int32_t(const mpt_texture_desc&, int, int)> add_tex = [&](const mpt_texture_desc& t, int channels,
+                                                                           int depth) -> int32_t {
+        if (t.odd && t.even) {
+            if (depth >= kMaxTextureNesting) return -2;
+            const int32_t odd = add_tex(*t.odd, channels, depth + 1);
+            const int32_t even = add_tex(*t.even, channels, depth + 1);
+            if (odd == -2 || even == -2) return -2;
+            Texture x;
+            x.channels = channels;
+            x.odd = odd;
+            x.even = even;
+            x.odd_size = 1.0f / t.odd_number;
+            x.even_size = 1.0f / t.even_number;
+            h.textures.push_back(std::move(x));
+            return static_cast<int32_t>(h.textures.size() - 1);
+        }
+        if (!t.data) {
+            if (depth == 0) return -1;   // a material's constant lives in the material
+            Texture x;
+            x.channels = channels;
+            x.value = {t.value[0], channels == 3 ? t.value[1] : t.value[0], channels == 3 ? t.value[2] : t.value[0]};
+            h.textures.push_back(std::move(x));
+            return static_cast<int32_t>(h.textures.size() - 1);
+        }
         for (const auto& sn : seen)
             if (sn.first == t.data && h.textures[sn.second].channels == channels &&
                 h.textures[sn.second].width == t.width && h.textures[sn.second].height == t.height)
@@ -976,10 +1003,12 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
         Material x;
         x.type = m.type;
         x.albedo = {m.albedo.value[0], m.albedo.value[1], m.albedo.value[2]};
-        x.albedo_tex = add_tex(m.albedo, 3);
+        x.albedo_tex = add_tex(m.albedo, 3, 0);
         x.f0 = {m.f0[0], m.f0[1], m.f0[2]};
         x.roughness = m.roughness.value[0];
-        x.rough_tex = add_tex(m.roughness, 1);
+        x.rough_tex = add_tex(m.roughness, 1, 0);
+        if (x.albedo_tex == -2 || x.rough_tex == -2)
+            return fail(ctx, MPT_ERR_UNSUPPORTED, "checkerboards nested deeper than 8");
         x.exponent = m.exponent;
         x.ratio = m.ratio;
         x.fresnel = m.fresnel;
@@ -1264,6 +1293,7 @@ int mpt_scene_set_texture(mpt_scene* s, int32_t index, const float* data) {
         if (l.type == LIGHT_ENVIRONMENT && l.env_tex == index)
             return fail(s->ctx, MPT_ERR_UNSUPPORTED, "an environment light's texture is not a parameter");
     Texture& t = s->host.textures[index];
+    if (t.checker()) return fail(s->ctx, MPT_ERR_UNSUPPORTED, "a checkerboard has no texels of its own");
     if (t.data.empty()) {
         t.value = {data[0], c == 3 ? data[1] : data[0], c == 3 ? data[2] : data[0]};
     } else {

@@ -107,8 +107,21 @@ __device__ __forceinline__ SurfaceHit surface_detail(const DevScene& sc, int32_t
 // TConstantTexture / TNearestPixelSampler with repeat or clamp wrap
 // (tConstantTexture.ipp:14-19, tNearestPixelSampler.ipp:13-31,
 // repeatUvwWrapper.cpp:9-15, clampUvwWrapper.cpp:8-12)
+// The image a texture reads at uvw: checkerboards resolved
+// (TCheckerboardTexture::evaluate, tCheckerboardTexture.ipp:28-40; the host
+// bounds the nesting).
+__device__ __forceinline__ DevTexture tex_resolve(const DevScene& sc, int32_t id, V3 uvw) {
+    DevTexture t = sc.texs[id];
+    for (int k = 0; k < 9 && t.odd >= 0; ++k) {
+        const int32_t uc = static_cast<int32_t>(floorf(uvw.x / t.odd_size));
+        const int32_t vc = static_cast<int32_t>(floorf(uvw.y / t.even_size));
+        t = sc.texs[(uc + vc) % 2 == 0 ? t.even : t.odd];
+    }
+    return t;
+}
+
 __device__ __forceinline__ V3 tex_lookup(const DevScene& sc, int32_t id, V3 uvw) {
-    const DevTexture t = sc.texs[id];
+    const DevTexture t = tex_resolve(sc, id, uvw);
     float u, v;
     if (t.wrap_clamp) {
         u = mclamp(uvw.x, 0.f, 1.f);

@@ -3,7 +3,9 @@
 // parses to the same binary32 values.
 #include "crsd.hpp"
 #include "hdr.hpp"
+#include "image.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <fstream>
@@ -161,6 +163,34 @@ struct Db {
         return false;
     }
 
+    // A checkerboard's odd / even texture: SdData::getSpectrumTexture /
+    // getRealTexture (sdData.cpp:98-140) give the named texture of the same
+    // kind, else a constant of the field's value (default 0).
+    int32_t sub_texture(const Block& b, std::string_view key, int channels) {
+        const std::string name = b.str(key);
+        if (!name.empty()) {
+            const auto& tab = channels == 3 ? spec_tex : real_tex;
+            auto it = tab.find(name);
+            if (it == tab.end()) {
+                fail(MPT_ERR_PARSE, "unknown texture <" + name + ">");
+                return -1;
+            }
+            return it->second;
+        }
+        Texture t;
+        t.channels = channels;
+        if (channels == 3) t.value = b.vec(key, F3{0.f, 0.f, 0.f});
+        else { const float v = b.real(key, 0.0f); t.value = {v, v, v}; }
+        s.textures.push_back(std::move(t));
+        return static_cast<int32_t>(s.textures.size() - 1);
+    }
+
+    int nesting(int32_t id) const {
+        const Texture& t = s.textures[id];
+        return t.checker() ? 1 + std::max(nesting(t.odd), nesting(t.even)) : 0;
+    }
+
+    // instantiator::makeRealTexture / makeSpectrumTexture (textureInstantiator.cpp:118-176)
     int32_t add_texture(const Block& b, int channels) {
         const std::string type = b.str("type");
         Texture t;
@@ -168,6 +198,34 @@ struct Db {
         if (type == "constant") {
             if (channels == 3) t.value = b.vec("value");
             else { const float v = b.real("value"); t.value = {v, v, v}; }
+        } else if (type == "image" && channels == 3) {
+            // createSpectrumImage (textureInstantiator.cpp:70-103): nearest sampling
+            // whatever the sample-mode, repeat wrap unless wrap-mode is "clamp"
+            std::string e;
+            const std::string fn = resolve(base, b.str("filename"));
+            if (!rgb_image_supported(fn)) {
+                fail(MPT_ERR_UNSUPPORTED, "image <" + fn + ">: only .png, .tga and .hdr pictures are read");
+                return -1;
+            }
+            if (!load_rgb_image(fn, &t.width, &t.height, &t.data, &e)) {
+                fail(MPT_ERR_IO, e);
+                return -1;
+            }
+            t.wrap_clamp = b.str("wrap-mode") == "clamp" ? 1 : 0;
+        } else if (type == "checkerboard") {
+            // createRealCheckerboard / createSpectrumCheckerboard (textureInstantiator.cpp:31-68)
+            const int32_t odd = sub_texture(b, "odd-texture", channels);
+            if (odd < 0) return -1;
+            const int32_t even = sub_texture(b, "even-texture", channels);
+            if (even < 0) return -1;
+            if (std::max(nesting(odd), nesting(even)) + 1 > kMaxTextureNesting) {
+                fail(MPT_ERR_UNSUPPORTED, "checkerboards nested deeper than 8");
+                return -1;
+            }
+            t.odd = odd;
+            t.even = even;
+            t.odd_size = 1.0f / b.real("odd-number");
+            t.even_size = 1.0f / b.real("even-number");
         } else if (type == "image-raw") {
             t.width = b.i32("width");
             t.height = b.i32("height");
@@ -177,6 +235,10 @@ struct Db {
                 fail(MPT_ERR_IO, "cannot read image-raw texture " + b.str("filename"));
                 return -1;
             }
+        } else if (type == "image") {
+            fail(MPT_ERR_UNSUPPORTED, "real image textures: the reference leaves them unimplemented "
+                                      "(textureInstantiator.cpp:133-136)");
+            return -1;
         } else {
             fail(MPT_ERR_UNSUPPORTED, "texture type <" + type + "> is not on the path-tracing path");
             return -1;

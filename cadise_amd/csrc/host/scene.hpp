@@ -26,12 +26,23 @@ enum MatType : int32_t {
 enum Estimator : int32_t { EST_PT = 0, EST_VPT = 1 };
 enum FilterType : int32_t { FILTER_BOX = 0, FILTER_GAUSSIAN = 1, FILTER_CONE = 2, FILTER_MITCHELL = 3 };
 
+// An image (data), a constant (value), or a checkerboard of two other
+// textures (odd >= 0): TCheckerboardTexture::evaluate
+// (tCheckerboardTexture.ipp:28-40) takes the even texture where
+// floor(u / odd_size) + floor(v / even_size) is even, else the odd one, with
+// odd_size = 1 / odd-number and even_size = 1 / even-number.
 struct Texture {
-    std::vector<float> data;   // empty = constant
+    std::vector<float> data;   // empty = constant (or checkerboard)
     int32_t width = 0, height = 0, channels = 3;
     int32_t wrap_clamp = 0;
     F3 value;
+    int32_t odd = -1, even = -1;
+    float odd_size = 0.0f, even_size = 0.0f;
+    bool checker() const { return odd >= 0; }
 };
+
+// Deepest nesting of checkerboards in checkerboards.
+constexpr int kMaxTextureNesting = 8;
 
 struct Material {
     int32_t type = MAT_LAMBERT;

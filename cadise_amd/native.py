@@ -23,7 +23,8 @@ class MptError(RuntimeError):
 
 class Texture(C.Structure):
     _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
-                ("wrap_clamp", C.c_int32), ("value", C.c_float * 3)]
+                ("wrap_clamp", C.c_int32), ("value", C.c_float * 3), ("odd", C.c_void_p), ("even", C.c_void_p),
+                ("odd_number", C.c_float), ("even_number", C.c_float)]
 
 
 class MaterialDesc(C.Structure):

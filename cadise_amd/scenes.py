@@ -63,9 +63,13 @@ class Material:
 @dataclass
 class Texture:
     name: str
-    data: np.ndarray                      # H x W x C float32, row 0 = v in [0, 1/H)
+    data: np.ndarray | None               # H x W x C float32, row 0 = v in [0, 1/H) (None: checkerboard)
     kind: str = "spectrum"                # "spectrum" (3 channels) or "real" (1 channel)
     wrap: str = "repeat"
+    file: str | None = None               # picture (.png / .tga / .hdr) written as an "image" texture;
+                                          # data = its texels as the reference decodes them
+    checker: tuple | None = None          # (odd-number, even-number, odd, even): odd / even a texture
+                                          # name or a constant (rgb tuple, or a float for "real")
 
 
 @dataclass
@@ -181,12 +185,28 @@ class SceneSpec:
         out.append(f'accelerator {{\n    [type string "bvh"]\n    [split-mode string "{self.split_mode}"]\n}}')
         out.append(f'light-cluster {{\n    [type string "{self.cluster}"]\n}}')
         for t in self.textures:
+            blk = "texture-spectrum" if t.kind == "spectrum" else "texture-real"
+            if t.checker is not None:
+                on, en, odd, even = t.checker
+
+                def sub(key, x):
+                    if isinstance(x, str):
+                        return f'[{key} texture "{x}"]'
+                    return f'[{key} rgb "{_v(x)}"]' if t.kind == "spectrum" else f'[{key} real "{_f(x)}"]'
+                out.append(f'{blk} {{\n    [type string "checkerboard"]\n    [name string "{t.name}"]\n'
+                           f'    [odd-number real "{_f(on)}"]\n    [even-number real "{_f(en)}"]\n'
+                           f'    {sub("odd-texture", odd)}\n    {sub("even-texture", even)}\n}}')
+                continue
+            if t.file is not None:
+                out.append(f'{blk} {{\n    [type string "image"]\n    [name string "{t.name}"]\n'
+                           f'    [filename string "{os.path.abspath(t.file)}"]\n    [sample-mode string "nearest"]\n'
+                           f'    [wrap-mode string "{t.wrap}"]\n}}')
+                continue
             arr = np.ascontiguousarray(t.data, np.float32)
             if sidecar_dir is None:
                 raise ValueError("image textures need a side-car directory")
             path = os.path.join(sidecar_dir, f"tex_{t.name}.f32")
             arr.tofile(path)
-            blk = "texture-spectrum" if t.kind == "spectrum" else "texture-real"
             out.append(f'{blk} {{\n    [type string "image-raw"]\n    [name string "{t.name}"]\n'
                        f'    [filename string "{path}"]\n    [width int32 "{arr.shape[1]}"]\n'
                        f'    [height int32 "{arr.shape[0]}"]\n    [wrap-mode string "{t.wrap}"]\n}}')
@@ -311,6 +331,28 @@ def cornell_box(width=256, height=256, spp=16, max_depth=8, filter="box") -> Sce
     s.meshes.append(Mesh("light1", "white", v=lt[1:2]))
     area = 0.5 * 0.25 * 0.25
     s.lights = [Light("light0", (1, 1, 1), 3 * 17 * math.pi * area), Light("light1", (1, 1, 1), 3 * 17 * math.pi * area)]
+    return s
+
+
+def checker_cornell(picture: str, texels: np.ndarray, width=32, height=32, spp=4, max_depth=6) -> SceneSpec:
+    """The Cornell box with picture and checkerboard textures: the back wall's albedo
+    is a checkerboard (5 x 3 cells over uv) of an "image" texture read from
+    `picture` (texels: its texels as the reference decodes them, for the paths that
+    take arrays) and a constant; the floor is a GGX conductor whose roughness is a
+    checkerboard of two constants; the tall box's albedo is a checkerboard of a
+    checkerboard and a constant."""
+    s = cornell_box(width=width, height=height, spp=spp, max_depth=max_depth)
+    s.textures = [Texture("pic", np.ascontiguousarray(texels, np.float32), file=picture),
+                  Texture("back", None, checker=(5.0, 3.0, "pic", (0.2, 0.6, 0.3))),
+                  Texture("rough", None, kind="real", checker=(4.0, 4.0, 0.15, 0.7)),
+                  Texture("inner", None, checker=(2.0, 2.0, (0.9, 0.9, 0.2), "pic")),
+                  Texture("nested", None, checker=(3.0, 1.5, "inner", (0.1, 0.1, 0.8)))]
+    s.materials += [Material("backmat", albedo_texture="back"),
+                    Material("floormat", type="conductor-microfacet", f0=(0.9, 0.7, 0.5), roughness_texture="rough"),
+                    Material("boxmat", albedo_texture="nested")]
+    swap = {"back": "backmat", "floor": "floormat", "tallbox": "boxmat"}
+    for m in s.meshes:
+        m.material = swap.get(m.name, m.material)
     return s
 
 

@@ -71,11 +71,18 @@ enum {
 };
 enum { MPT_ESTIMATOR_PT = 0, MPT_ESTIMATOR_VPT = 1 };
 
-typedef struct {
+/* A texture: an image (data), a constant (data NULL), or, when odd and even
+ * are both set, a checkerboard (TCheckerboardTexture, tCheckerboardTexture.ipp:11-40):
+ * even where floor(u / (1 / odd_number)) + floor(v / (1 / even_number)) is
+ * even, else odd (nested up to 8 deep). */
+typedef struct mpt_texture_desc_s {
     const float* data;         /* H x W x C float32 (row 0 = v in [0, 1/H)), NULL = constant */
     int32_t      width, height, channels;
     int32_t      wrap_clamp;   /* 0 repeat, 1 clamp */
     float        value[3];     /* constant value */
+    const struct mpt_texture_desc_s* odd;
+    const struct mpt_texture_desc_s* even;
+    float        odd_number, even_number;
 } mpt_texture_desc;
 
 typedef struct {

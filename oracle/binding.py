@@ -28,7 +28,8 @@ MAT_CODES = {"matte-lambertian": 0, "conductor-microfacet": 1, "phong": 2, "plas
 
 class OrTexture(C.Structure):
     _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
-                ("wrap_clamp", C.c_int32), ("value", C.c_float * 3)]
+                ("wrap_clamp", C.c_int32), ("value", C.c_float * 3), ("odd", C.c_void_p), ("even", C.c_void_p),
+                ("odd_size", C.c_float), ("even_size", C.c_float)]
 
 
 class OrMaterial(C.Structure):
@@ -116,12 +117,24 @@ class OracleScene:
         tex = {x.name: x for x in spec.textures}
         self.tex_arrays = {}
 
+        self._tex_nodes = []   # checkerboards' odd / even textures, kept alive
+
         def texture(const, name, channels):
             T = OrTexture()
             if name is None:
                 c = const if isinstance(const, (tuple, list)) else (const, const, const)
                 T.value[:] = [float(np.float32(x)) for x in c]
                 T.data = None
+                return T
+            if tex[name].checker is not None:
+                on, en, odd, even = tex[name].checker
+                kids = [texture(None, x, channels) if isinstance(x, str) else texture(x, None, channels)
+                        for x in (odd, even)]
+                self._tex_nodes.extend(kids)
+                T.odd, T.even = C.addressof(kids[0]), C.addressof(kids[1])
+                T.odd_size = float(np.float32(1.0) / np.float32(on))
+                T.even_size = float(np.float32(1.0) / np.float32(en))
+                T.channels = channels
                 return T
             if name not in self.tex_arrays:
                 self.tex_arrays[name] = np.array(tex[name].data, np.float32, order="C", copy=True)
@@ -360,6 +373,18 @@ def ref_available() -> bool:
     return os.path.exists(REF_SO)
 
 
+def ref_load_rgb_image(path: str) -> np.ndarray:
+    """The reference's PictureLoader::loadRgbImage of a picture file: H x W x 3
+    float32 texels (row 0 = the picture's bottom row)."""
+    L = ref_lib()
+    w, h = C.c_int32(0), C.c_int32(0)
+    L.ref_load_rgb_image(path.encode(), C.byref(w), C.byref(h), None)
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    if out.size:
+        L.ref_load_rgb_image(path.encode(), C.byref(w), C.byref(h), out.ctypes.data)
+    return out
+
+
 def ref_lib():
     global _ref
     if _ref is None:
@@ -367,6 +392,7 @@ def ref_lib():
         L.ref_scene_create.restype = C.c_void_p
         L.ref_scene_create.argtypes = [C.c_char_p, C.c_size_t]
         L.ref_scene_destroy.argtypes = [C.c_void_p]
+        L.ref_load_rgb_image.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_void_p]
         L.ref_scene_error.restype = C.c_char_p
         L.ref_scene_error.argtypes = [C.c_void_p]
         L.ref_scene_info.argtypes = [C.c_void_p, _i64p, C.POINTER(C.c_double)]

@@ -437,7 +437,20 @@ void or_intersect_detail(const or_scene* s, const float* rays, int64_t n, float*
 /* ---- textures (tConstantTexture.ipp:14-19, tNearestPixelSampler.ipp:13-31,
  *      repeatUvwWrapper.cpp:9-15, clampUvwWrapper.cpp:8-12) ------------------- */
 
+/* TCheckerboardTexture::evaluate (tCheckerboardTexture.ipp:28-40): the even
+ * texture where floor(u / oddSize) + floor(v / evenSize) is even (C++ %, so
+ * negative odd sums count as odd), else the odd texture. */
+static const or_texture* tex_resolve(const or_texture* t, v3 uvw) {
+    while (t->odd && t->even) {
+        const int32_t uc = (int32_t)floorf(uvw.x / t->odd_size);
+        const int32_t vc = (int32_t)floorf(uvw.y / t->even_size);
+        t = ((uc + vc) % 2 == 0) ? t->even : t->odd;
+    }
+    return t;
+}
+
 static v3 tex_eval(const or_texture* t, v3 uvw) {
+    t = tex_resolve(t, uvw);
     if (!t->data) return v3_make(t->value[0], t->value[1], t->value[2]);
     float u, v;
     if (t->wrap_clamp) { u = m_clamp(uvw.x, 0.0f, 1.0f); v = m_clamp(uvw.y, 0.0f, 1.0f); }
@@ -1677,8 +1690,9 @@ static double dalpha_dr(float r) {
     return (r > 0.001f && safe * safe > 0.001f) ? 2.0 * (double)r : 0.0;
 }
 
+/* The texel a vertex reads; checkerboards carry no gradient in the oracle. */
 static int64_t tex_texel(const or_texture* t, v3 uvw) {
-    if (!t->data) return -1;
+    if (!t->data || t->odd) return -1;
     float u, v;
     if (t->wrap_clamp) { u = m_clamp(uvw.x, 0.0f, 1.0f); v = m_clamp(uvw.y, 0.0f, 1.0f); }
     else { u = uvw.x - floorf(uvw.x); v = uvw.y - floorf(uvw.y); }

@@ -42,12 +42,16 @@ enum { OR_CLUSTER_UNIFORM = 0, OR_CLUSTER_POWER = 1 };
 enum { OR_RNG_SEQ = 0, OR_RNG_PHILOX = 1 };
 enum { OR_LIGHT_AREA = 0, OR_LIGHT_ENVIRONMENT = 1 };
 
-/* A texture: constant when data == NULL. */
-typedef struct {
+/* A texture: constant when data == NULL; a checkerboard of odd / even when
+ * both are set (odd_size = 1 / odd-number, even_size = 1 / even-number). */
+typedef struct or_texture_s {
     const float* data;     /* width*height*channels, row-major, row 0 = v 0 */
     int32_t      width, height, channels;
     int32_t      wrap_clamp; /* 0 repeat, 1 clamp */
     float        value[3];   /* constant value */
+    const struct or_texture_s* odd;
+    const struct or_texture_s* even;
+    float        odd_size, even_size;
 } or_texture;
 
 typedef struct {

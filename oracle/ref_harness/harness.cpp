@@ -43,9 +43,13 @@
 #include "core/surface/transportInfo.h"
 #include "core/surfaceDetail.h"
 #include "core/surfaceIntersection.h"
+#include "core/imaging/tImage.h"
+#include "file-io/path.h"
+#include "file-io/pictureLoader.h"
 #include "file-io/scene-description/sdData.h"
 #include "file-io/scene-description/tokenizer.h"
 #include "math/tAabb.h"
+#include "math/type/imageType.h"
 #include "math/type/mapType.h"
 
 #include "ref_random.h"
@@ -602,6 +606,18 @@ void ref_light_hit(void* h, const float* rays, int64_t n, float* out) {
         put3(o + 1, le);
         o[4] = al->evaluateDirectPdfW(si, v3(r));
     }
+}
+
+// PictureLoader::loadRgbImage: the texels an image texture of that file holds
+// (width x height x 3 floats, as the HdrImage stores them). out may be NULL
+// to query the size.
+void ref_load_rgb_image(const char* path, int32_t* width, int32_t* height, float* out) {
+    const HdrImage img = PictureLoader::loadRgbImage(Path(std::string(path)));
+    *width = img.width();
+    *height = img.height();
+    if (!out) return;
+    const real* d = img.rawData();
+    for (size_t i = 0; i < img.dataSize(); ++i) out[i] = d[i];
 }
 
 // Light::approximateFlux and LightCluster::evaluatePickLightPdf of every light.

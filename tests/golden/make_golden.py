@@ -64,9 +64,12 @@ def scene_set():
     zoo_sky = scenes.material_zoo(width=24, height=20, spp=4, env=scenes.sky_texture(64, 32, sun_diameter_deg=10.0),
                                   cluster="power")
     zoo_sky.meshes = [m for m in zoo_sky.meshes if m.name != "back"]   # the sky behind an open back
+    # picture and checkerboard textures (the picture's texels as the reference decodes them)
+    pic = os.path.join(HERE, "images", "rgb8_filters.png")
+    checker = scenes.checker_cornell(pic, np.load(os.path.join(HERE, "images.npz"))["rgb8_filters.png"])
     return [("cornell", cornell), ("cornell_gauss", cornell_g), ("spheres", spheres), ("spheres_equal", equal),
             ("sky", sky), ("sky_envfirst", sky_first), ("shipped", shipped), ("zoo", zoo), ("zoo_vpt", zoo_vpt),
-            ("zoo_sky", zoo_sky)]
+            ("zoo_sky", zoo_sky), ("checker", checker)]
 
 
 SHIPPED = "/root/reference/scene/cornell-box.crsd"

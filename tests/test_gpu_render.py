@@ -183,3 +183,21 @@ def test_gaussian_filter_vs_oracle(ctx):
     np.testing.assert_allclose(film[..., 3], orf[..., 3], rtol=1e-5, atol=1e-6)
     close = np.all(np.abs(g - o) <= 1e-4 * np.abs(o) + 1e-6, axis=-1)
     assert close.mean() >= 0.99
+
+
+@pytest.mark.parametrize("source", ["crsd", "arrays"])
+def test_render_picture_and_checkerboard_textures_vs_oracle(ctx, tmp_path, source):
+    """A20: an "image" texture read from a PNG file by the product's decoder (crsd)
+    or handed over as texels (arrays), inside checkerboards of spectra and of
+    roughness, against the oracle given the texels the reference decodes."""
+    import os
+    from cadise_amd.api import Scene
+    here = os.path.dirname(os.path.abspath(__file__))
+    pics = np.load(os.path.join(here, "golden", "images.npz"))
+    spec = scenes.checker_cornell(os.path.join(here, "golden", "images", "rgb8_filters.png"),
+                                  pics["rgb8_filters.png"], width=96, height=96, spp=8, max_depth=8)
+    sc = (Scene.from_crsd_text(ctx, spec.to_crsd(sidecar_dir=str(tmp_path))) if source == "crsd"
+          else Scene.from_spec(ctx, spec))
+    film = sc.render(seed=4242)
+    orf = OracleScene(spec).render_linear(4242, rng_mode=OR_RNG_PHILOX, threads=16)
+    _compare(film, orf, f"picture + checkerboard textures ({source})", frac_bar=0.98)
