@@ -24,10 +24,11 @@ Workloads (seeded, synthetic; no files):
 Reported (one JSON line):
   value          all ray queries (camera, continuation, MIS-BSDF, shadow) / s of the
                  procedural render, whole job, inputs resident in HBM
-  roofline       the closest-hit traversal kernel: algorithmic bytes per query
-                 (32 B ray + 16 B hit + 24 B per slab test + 36 B per primitive
-                 test, counted by the kernel) / its average launch time (HIP events
-                 on the render stream)
+  roofline       the traversal kernel (one mixed launch per wavefront step: the
+                 continuation and MIS-BSDF rays closest hit, the shadow rays any
+                 hit): algorithmic bytes per query (32 B ray + 16 B hit + 24 B per
+                 slab test + 36 B per primitive test, counted by the kernel) / its
+                 average launch time (HIP events on the render stream)
   cpu_baseline   the compiled reference (oracle/_ref, built from the reference's
                  sources) on this host's cores, on a bounded sample of the same
                  frame at 1 spp; the oracle port when the reference build is absent
@@ -268,16 +269,18 @@ def main():
     closest = st["camera_rays"] + st["continuation_rays"] + st["mis_rays"]
     rays = closest + st["shadow_rays"]
     mrays = rays / dt / 1e6
-    # roofline of the closest-hit traversal kernel
-    bytes_closest = 48.0 * closest + 24.0 * st["box_tests"] + 36.0 * st["tri_tests"]
+    # roofline of the traversal kernel: one mixed launch per step traces every query
+    boxes = st["box_tests"] + st["box_tests_any"]
+    prims = st["tri_tests"] + st["tri_tests_any"]
+    bytes_trace = 48.0 * rays + 24.0 * boxes + 36.0 * prims
     launches = max(1, st["closest_launches"])
-    per_launch_bytes = bytes_closest / launches
+    per_launch_bytes = bytes_trace / launches
     avg_launch_ms = st["ms_closest"] / launches
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
     log(f"procedural: {args.steps} passes in {dt:.3f} s: {rays} rays ({mrays:.1f} Mrays/s), closest "
-        f"{st['ms_closest']:.1f} ms any {st['ms_anyhit']:.1f} ms shade {st['ms_shade']:.1f} ms regen "
+        f"{st['ms_closest']:.1f} ms (closest + shadow) shade {st['ms_shade']:.1f} ms regen "
         f"{st['ms_camera']:.1f} ms over {launches} steps")
-    kernel = f"k_trace<{st_scene['width']},24,false>"
+    kernel = f"k_trace<{st_scene['width']},24,mixed>"
     # the same frame pass by pass (every pass's paths bounce in lock step)
     dt_pw = None
     if not args.no_passwise:
@@ -310,15 +313,16 @@ def main():
         "passwise_mrays_s": round(rays / dt_pw / 1e6, 1) if dt_pw else None,
         "rays": {"camera": st["camera_rays"], "continuation": st["continuation_rays"], "mis_bsdf": st["mis_rays"],
                  "shadow": st["shadow_rays"]},
-        "kernel_ms": {"closest": round(st["ms_closest"], 3), "anyhit": round(st["ms_anyhit"], 3),
+        "kernel_ms": {"trace": round(st["ms_closest"], 3),
                       "shade": round(st["ms_shade"], 3), "regenerate": round(st["ms_camera"], 3),
                       "gpu_total": round(st["ms_gpu"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"{kernel} (closest hit)", "launches": launches,
-                     "per_launch_bytes": round(per_launch_bytes), "avg_launch_ms": round(avg_launch_ms, 4),
-                     "box_tests_per_ray": round(st["box_tests"] / max(closest, 1), 2),
-                     "prim_tests_per_ray": round(st["tri_tests"] / max(closest, 1), 2)},
+                     "kernel": f"{kernel} (closest hit + shadow any hit, one launch per step)",
+                     "launches": launches, "per_launch_bytes": round(per_launch_bytes),
+                     "avg_launch_ms": round(avg_launch_ms, 4),
+                     "box_tests_per_ray": round(boxes / max(rays, 1), 2),
+                     "prim_tests_per_ray": round(prims / max(rays, 1), 2)},
         "setup_s": round(setup_s, 2),
     }
     # HBM bytes per launch of this kernel on this workload, measured with

@@ -363,17 +363,14 @@ int enqueue_step(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StepShape&
     t.spill = a.sc.max_stack > sh.cfg.stack ? S.spill.as<int32_t>() : nullptr;
     t.stats = bt.stats;
     t.stat_slot = ST_CLOSEST_RAYS;
+    // the shadow rays join the same launch (any-hit lanes beside closest-hit lanes)
+    t.dir3 = Q.shd_dir;
+    t.hit3 = Q.occ;
+    t.shadow = 1;
     HIP_TRY(ctx, rec(SK_CLOSEST, 0));
-    launch_trace_cfg(sh.cfg, t, false, sh.tblocks, stream);
+    launch_trace_cfg(sh.cfg, t, TRACE_MIXED, sh.tblocks, stream);
     HIP_TRY(ctx, rec(SK_CLOSEST, 1));
-    t.dir = Q.shd_dir;
-    t.dual = 0;
-    t.hit = Q.occ;
-    t.hit2 = nullptr;
-    t.work = bt.ctr + 3;
-    t.stat_slot = ST_SHADOW_RAYS;
-    HIP_TRY(ctx, rec(SK_ANY, 0));
-    launch_trace_cfg(sh.cfg, t, true, sh.tblocks, stream);
+    HIP_TRY(ctx, rec(SK_ANY, 0));   // no launch of its own: an empty interval
     HIP_TRY(ctx, rec(SK_ANY, 1));
     HIP_TRY(ctx, rec(SK_SHADE, 0));
     if (g) hipLaunchKernelGGL(k_shade<SHADE_BATCH_ADJ>, dim3(sh.sblocks), dim3(256), 0, stream, a, *g, bt, qc);
@@ -405,7 +402,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     StepShape sh;
     sh.cfg = job.cfg;
     {
-        const int64_t want = (2 * static_cast<int64_t>(cap) + sh.cfg.block - 1) / sh.cfg.block;
+        const int64_t want = (3 * static_cast<int64_t>(cap) + sh.cfg.block - 1) / sh.cfg.block;
         sh.tblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(want, static_cast<int64_t>(ctx->num_cus) * sh.cfg.bpc)));
         sh.sblocks = static_cast<int>(std::min<int64_t>((cap + 255) / 256, static_cast<int64_t>(ctx->num_cus) * 8));
         sh.rblocks = (cap + 255) / 256;
@@ -567,8 +564,8 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
     r.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
     r.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
-    r.closest_launches = chunks * steps;
-    r.anyhit_launches = chunks * steps;
+    r.closest_launches = chunks * steps;   // mixed launches: their shadow rays need no launch of their own
+    r.anyhit_launches = 0;
     r.ms_camera = ms[SK_REGEN];
     r.ms_closest = ms[SK_CLOSEST];
     r.ms_anyhit = ms[SK_ANY];

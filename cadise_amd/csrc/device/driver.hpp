@@ -19,7 +19,7 @@
 
 namespace mpt {
 
-template <int W, int S, bool ANY>
+template <int W, int S, int MODE>
 __global__ void k_trace(TraceArgs a);
 template <bool ADJ>
 __global__ void k_camera(PassArgs a, AdjArgs g);
@@ -39,23 +39,25 @@ struct TraceCfg {
     int width = 4, stack = 24, block = 128, bpc = 6;   // bpc: persistent blocks per CU
 };
 
+// mode: TRACE_CLOSEST, TRACE_ANY or TRACE_MIXED
 template <int W, int S>
-void launch_ws(bool any, int blocks, int block, size_t lds, hipStream_t st, const TraceArgs& a) {
-    if (any) hipLaunchKernelGGL((k_trace<W, S, true>), dim3(blocks), dim3(block), lds, st, a);
-    else hipLaunchKernelGGL((k_trace<W, S, false>), dim3(blocks), dim3(block), lds, st, a);
+void launch_ws(int mode, int blocks, int block, size_t lds, hipStream_t st, const TraceArgs& a) {
+    if (mode == TRACE_ANY) hipLaunchKernelGGL((k_trace<W, S, TRACE_ANY>), dim3(blocks), dim3(block), lds, st, a);
+    else if (mode == TRACE_MIXED) hipLaunchKernelGGL((k_trace<W, S, TRACE_MIXED>), dim3(blocks), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL((k_trace<W, S, TRACE_CLOSEST>), dim3(blocks), dim3(block), lds, st, a);
 }
 
-inline void launch_trace_cfg(const TraceCfg& c, const TraceArgs& a, bool any, int blocks, hipStream_t st) {
+inline void launch_trace_cfg(const TraceCfg& c, const TraceArgs& a, int mode, int blocks, hipStream_t st) {
     const size_t lds = (sizeof(int32_t) * static_cast<size_t>(c.stack) + static_cast<size_t>(coop_lds_bytes(c.width))) *
                        static_cast<size_t>(c.block);
     const int key = c.width * 100 + c.stack;
     switch (key) {
-        case 416: launch_ws<4, 16>(any, blocks, c.block, lds, st, a); break;
-        case 432: launch_ws<4, 32>(any, blocks, c.block, lds, st, a); break;
-        case 816: launch_ws<8, 16>(any, blocks, c.block, lds, st, a); break;
-        case 824: launch_ws<8, 24>(any, blocks, c.block, lds, st, a); break;
-        case 832: launch_ws<8, 32>(any, blocks, c.block, lds, st, a); break;
-        default: launch_ws<4, 24>(any, blocks, c.block, lds, st, a); break;
+        case 416: launch_ws<4, 16>(mode, blocks, c.block, lds, st, a); break;
+        case 432: launch_ws<4, 32>(mode, blocks, c.block, lds, st, a); break;
+        case 816: launch_ws<8, 16>(mode, blocks, c.block, lds, st, a); break;
+        case 824: launch_ws<8, 24>(mode, blocks, c.block, lds, st, a); break;
+        case 832: launch_ws<8, 32>(mode, blocks, c.block, lds, st, a); break;
+        default: launch_ws<4, 24>(mode, blocks, c.block, lds, st, a); break;
     }
 }
 

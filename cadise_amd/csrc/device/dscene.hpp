@@ -37,6 +37,10 @@ struct DevMaterial {
     float   pad;
 };
 
+// Kinds of traversal launch: every ray closest hit, every ray any hit, or a
+// mixed launch whose lanes carry their own kind.
+enum { TRACE_CLOSEST = 0, TRACE_ANY = 1, TRACE_MIXED = 2 };
+
 // An image (a constant is a 1 x 1 image) or, when odd >= 0, a checkerboard
 // of textures odd / even (its one texel in tex_data is unused).
 struct DevTexture {

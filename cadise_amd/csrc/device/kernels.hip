@@ -37,8 +37,10 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // they are used), and a lane whose ray has finished takes it at once instead of
 // idling until the whole wave's 64 rays are done, so lanes stay busy despite the
 // very uneven cost of incoherent rays. Reservations happen between blocks of up
-// to STEPS node visits.
-template <int W, int S, bool ANY>
+// to STEPS node visits. A mixed launch (MODE TRACE_MIXED) traces the closest-hit
+// sets and the shadow (any-hit) set from one work counter, so one ramp and one
+// tail of stragglers serve both kinds of query.
+template <int W, int S, int MODE>
 __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     constexpr int STEPS = 8;
     constexpr int REFILL = 16;   // reserve new prefetch slots once this many lanes lack one
@@ -64,9 +66,17 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         cl.queue = (__attribute__((address_space(3))) lds_i2*)(reinterpret_cast<lds_i2*>(base + 40 * nt) + WAVE * W * wv);
     }
     const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
-    const int64_t n = a.dual ? 2 * n0 : n0;
-    TravCounters cnt{0, 0};
-    unsigned long long rays = 0, rays2 = 0, errors = 0;
+    const int64_t n = n0 * (1 + a.dual + (MODE == TRACE_MIXED ? a.shadow : 0));
+    TravCounters cnt{0, 0, 0, 0};
+    unsigned long long rays = 0, rays2 = 0, rays3 = 0, errors = 0;
+    // ray i: set i / n0 (0: dir -> hit, 1: dir2 -> hit2, 2: dir3 -> hit3, any hit), slot i % n0
+    auto set_of = [&](int64_t i) { return i >= 2 * n0 ? 2 : (i >= n0 ? 1 : 0); };
+    auto is_any = [&](int set) { return MODE == TRACE_ANY || (MODE == TRACE_MIXED && set == 2); };
+    auto put = [&](int set, int64_t q, int2 out) {
+        if (set == 0) a.hit[q] = out;
+        else if (set == 1) a.hit2[q] = out;
+        else a.hit3[q] = out;
+    };
     const bool no_tree = a.sc.nnodes == 0;
     LaneRay r;
     int64_t rid = -1;            // ray of this lane, -1 = idle
@@ -79,16 +89,16 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         while (rid < 0 && pid >= 0) {
             const int64_t i = pid;
             pid = -1;
-            const bool second = i >= n0;
-            const int64_t q = second ? i - n0 : i;
+            const int set = set_of(i);
+            const int64_t q = i - set * n0;
             if (pd.w < 0.0f || no_tree) {
-                const int2 out = make_int2(ANY ? 0 : -1, __float_as_int(pd.w));
-                if (second) a.hit2[q] = out;
-                else a.hit[q] = out;
+                put(set, q, make_int2(is_any(set) ? 0 : -1, __float_as_int(pd.w)));
             } else {
-                if (second) rays2 += 1;
-                else rays += 1;
+                rays += set == 0 ? 1 : 0;
+                rays2 += set == 1 ? 1 : 0;
+                rays3 += set == 2 ? 1 : 0;
                 rid = i;
+                r.any = is_any(set);
                 r.o = v3f4(po);
                 r.d = v3f4(pd);
                 r.inv = safe_inv(r.d);
@@ -116,11 +126,11 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
             if (pid < 0) {
                 const int64_t i = base + __popcll(need & ((1ull << lane) - 1ull));
                 if (i < n) {
-                    const bool second = i >= n0;
-                    const int64_t q = second ? i - n0 : i;
+                    const int set = set_of(i);
+                    const int64_t q = i - set * n0;
                     pid = i;
                     po = a.org[q * a.stride];
-                    pd = second ? a.dir2[q * a.stride] : a.dir[q * a.stride];
+                    pd = (set == 0 ? a.dir : set == 1 ? a.dir2 : a.dir3)[q * a.stride];
                     proot = a.root ? a.root[q] : 0;
                 }
             }
@@ -132,16 +142,16 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         }
         // -- traversal steps
         for (int s = 0; s < STEPS; ++s) {
-            const bool fin = visit<W, S, ANY>(a.sc, st, r, cnt, rid >= 0, cl, lane);
+            const bool fin = visit<W, S, MODE>(a.sc, st, r, cnt, rid >= 0, cl, lane);
             if (rid >= 0 && fin) {
                 if (r.hit == -2) {
                     errors += 1;
                     r.hit = -1;
                 }
-                const bool second = rid >= n0;
-                const int64_t q = second ? rid - n0 : rid;
+                const int set = set_of(rid);
+                const int64_t q = rid - set * n0;
                 int2 out;
-                if (ANY) {
+                if (is_any(set)) {
                     out = make_int2(r.hit >= 0 ? 1 : 0, 0);
                 } else {
                     int32_t idx = r.hit;
@@ -149,8 +159,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                         idx = static_cast<int32_t>(__float_as_uint(a.sc.tris[3 * r.hit].w) & 0x7FFFFFFFu);
                     out = make_int2(r.hit >= 0 ? idx : -1, __float_as_int(r.tmax));
                 }
-                if (second) a.hit2[q] = out;
-                else a.hit[q] = out;
+                put(set, q, out);
                 rid = -1;
                 install();
             }
@@ -158,29 +167,39 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         }
     }
     // one atomic per wave for the counters
-    const unsigned long long sr = wave_sum(rays), sr2 = wave_sum(rays2), sb = wave_sum(cnt.boxes),
-                             sp = wave_sum(cnt.prims), se = wave_sum(errors);
+    const unsigned long long sr = wave_sum(rays), sr2 = wave_sum(rays2), sr3 = wave_sum(rays3),
+                             sb = wave_sum(cnt.boxes), sp = wave_sum(cnt.prims), sba = wave_sum(cnt.boxes_any),
+                             spa = wave_sum(cnt.prims_any), se = wave_sum(errors);
     if (lane == 0 && a.stats) {
         atomicAdd(&a.stats[a.stat_slot], sr);
         if (sr2) atomicAdd(&a.stats[ST_MIS_RAYS], sr2);
-        atomicAdd(&a.stats[ANY ? ST_BOXES_ANY : ST_BOXES], sb);
-        atomicAdd(&a.stats[ANY ? ST_PRIMS_ANY : ST_PRIMS], sp);
+        if (sr3) atomicAdd(&a.stats[ST_SHADOW_RAYS], sr3);
+        if (sb) atomicAdd(&a.stats[ST_BOXES], sb);
+        if (sp) atomicAdd(&a.stats[ST_PRIMS], sp);
+        if (sba) atomicAdd(&a.stats[ST_BOXES_ANY], sba);
+        if (spa) atomicAdd(&a.stats[ST_PRIMS_ANY], spa);
         if (se) atomicAdd(&a.stats[ST_ERRORS], se);
     }
 }
 
-template __global__ void k_trace<4, 24, false>(TraceArgs);
-template __global__ void k_trace<4, 24, true>(TraceArgs);
-template __global__ void k_trace<8, 24, false>(TraceArgs);
-template __global__ void k_trace<8, 24, true>(TraceArgs);
-template __global__ void k_trace<4, 16, false>(TraceArgs);
-template __global__ void k_trace<4, 16, true>(TraceArgs);
-template __global__ void k_trace<8, 16, false>(TraceArgs);
-template __global__ void k_trace<8, 16, true>(TraceArgs);
-template __global__ void k_trace<4, 32, false>(TraceArgs);
-template __global__ void k_trace<4, 32, true>(TraceArgs);
-template __global__ void k_trace<8, 32, false>(TraceArgs);
-template __global__ void k_trace<8, 32, true>(TraceArgs);
+template __global__ void k_trace<4, 24, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<4, 24, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<4, 24, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<8, 24, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<8, 24, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<8, 24, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<4, 16, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<4, 16, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<4, 16, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<8, 16, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<8, 16, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<8, 16, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<4, 32, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<4, 32, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<4, 32, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<8, 32, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<8, 32, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<8, 32, TRACE_MIXED>(TraceArgs);
 
 // ---- camera ----------------------------------------------------------------------
 

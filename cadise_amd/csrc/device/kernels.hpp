@@ -43,6 +43,9 @@ struct TraceArgs {
     int32_t        dual;         // 1: n1 = n0 rays from dir2
     int2*          hit;          // closest: (leaf or prim, t bits); any: x = 0/1
     int2*          hit2;
+    const float4*  dir3;         // mixed launch: a third set of n0 any-hit rays (org, dir3) -> hit3
+    int2*          hit3;
+    int32_t        shadow;       // mixed launch: 1 when the third set is present
     int32_t        want_prim;    // closest: write the creation index instead of the leaf position
     const int32_t* root;         // per-ray root node (index q of the first set), null = node 0
     int32_t*       work;         // atomic work counter, zero on entry

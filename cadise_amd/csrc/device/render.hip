@@ -310,7 +310,7 @@ int ensure_spill(mpt_scene* S, int blocks) {
 
 int launch_trace(mpt_scene* S, TraceArgs a, bool any, int blocks) {
     a.spill = S->sc.max_stack > S->cfg.stack ? S->spill.as<int32_t>() : nullptr;
-    launch_trace_cfg(S->cfg, a, any, blocks, S->stream);
+    launch_trace_cfg(S->cfg, a, any ? TRACE_ANY : TRACE_CLOSEST, blocks, S->stream);
     HIP_TRY(S->ctx, hipGetLastError());
     return MPT_OK;
 }

@@ -19,10 +19,11 @@
 
 namespace mpt {
 
+// Slab and primitive tests of closest-hit and of any-hit queries.
 struct TravCounters {
-    uint32_t boxes;
-    uint32_t prims;
+    uint32_t boxes, prims, boxes_any, prims_any;
 };
+
 
 // Sphere::isIntersecting / isOccluded with _isSolutionValid: the Haines et al.
 // form (q = b + sign(b) sqrt(disc), roots c / q and q, taken in that order).
@@ -136,6 +137,7 @@ struct LaneRay {
     float   tmin, tmax;
     int32_t hit;
     int32_t node;
+    bool    any;   // any hit (occlusion) query
 };
 
 // Compare-exchange of a bitonic sorting network on (distance, child) pairs.
@@ -201,12 +203,13 @@ __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
 // over the lanes), the per-ray minimum taken by LDS atomics on hit keys; each
 // lane with a ray then sorts its hit interior children by entry distance,
 // continues with the nearest and pushes the rest. Returns true when the
-// lane's ray has finished (any hit found, or the stack is empty). Must be
-// called by all lanes of the wave together. All register arrays are indexed
-// with compile-time indices.
-template <int W, int S, bool ANY>
+// lane's ray has finished (any hit found for an any-hit query, or the stack
+// is empty). Must be called by all lanes of the wave together. All register
+// arrays are indexed with compile-time indices.
+template <int W, int S, int MODE>
 __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, LaneRay& r, TravCounters& cnt, bool act,
                                       const CoopLds& cl, int lane) {
+    const bool any = MODE == TRACE_ANY || (MODE == TRACE_MIXED && r.any);
     bool done = !act;
     int32_t child[W], count[W], lf[W], lc[W];
     float tn[W];
@@ -248,7 +251,8 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         auto qb = [&](int plane, int c) {
             return static_cast<float>((wd[QW + (plane * W + c) / 4] >> (8 * (c & 3))) & 0xFFu);
         };
-        cnt.boxes += W;
+        if (any) cnt.boxes_any += W;
+        else cnt.boxes += W;
 #pragma unroll
         for (int c = 0; c < W; ++c) {
             child[c] = static_cast<int32_t>(wd[4 + c]);
@@ -285,7 +289,7 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         if (nl > 0) {
 #pragma unroll
             for (int k = 0; k < W; ++k) {
-                if (k < nl) cl.queue[excl + k] = lds_i2{lf[k], lc[k] | (lane << 16)};
+                if (k < nl) cl.queue[excl + k] = lds_i2{lf[k], lc[k] | (any ? 0x8000 : 0) | (lane << 16)};
             }
             cl.ray[2 * lane + 1].w = r.tmax;
             cl.key[lane] = (static_cast<unsigned long long>(ord_dist(r.tmax)) << 32) | 0xFFFFFFFFull;
@@ -295,7 +299,8 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             const int e = base + lane;
             if (e < ntot) {
                 const lds_i2 q = cl.queue[e];
-                const int own = q.y >> 16, n = q.y & 0xFFFF;
+                const int own = q.y >> 16, n = q.y & 0x7FFF;
+                const bool qany = MODE == TRACE_ANY || (MODE == TRACE_MIXED && (q.y & 0x8000) != 0);
                 const lds_f4 ro = cl.ray[2 * own], rd = cl.ray[2 * own + 1];
                 const V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
                 float tmax = rd.w;
@@ -307,7 +312,8 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
                     const int32_t lb = two ? la + 1 : la;
                     const PrimRec A = prim_load(sc.tris, la);
                     const PrimRec B = prim_load(sc.tris, lb);
-                    cnt.prims += two ? 2 : 1;
+                    if (qany) cnt.prims_any += two ? 2 : 1;
+                    else cnt.prims += two ? 2 : 1;
                     if (prim_hit(A, o, d, ro.w, tmax)) best = la;
                     if (two && prim_hit(B, o, d, ro.w, tmax)) best = lb;
                 }
@@ -323,7 +329,7 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             if (lo != 0xFFFFFFFFu) {
                 r.hit = 0x7FFFFFFF - static_cast<int32_t>(lo);
                 r.tmax = dist_of(static_cast<uint32_t>(k >> 32));
-                if (ANY) done = true;
+                if (any) done = true;
             }
         }
     }

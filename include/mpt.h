@@ -177,9 +177,11 @@ typedef struct {
     int64_t samples;
     int64_t box_tests, tri_tests; /* closest-hit traversal work (slab tests, primitive tests) */
     int64_t box_tests_any, tri_tests_any; /* any-hit traversal work */
-    int64_t closest_launches, anyhit_launches;
+    int64_t closest_launches, anyhit_launches;  /* the streaming renderer traces its shadow rays in
+                                                   the closest-hit launches: anyhit_launches 0 */
     double  ms_total;             /* wall time of the call */
-    double  ms_closest, ms_anyhit, ms_shade, ms_camera, ms_film; /* summed kernel time (events) */
+    double  ms_closest, ms_anyhit, ms_shade, ms_camera, ms_film; /* summed kernel time (events); the
+                                     streaming renderer's ms_closest includes its shadow rays */
     double  ms_adjoint;           /* mpt_render_adjoint: GPU time of the replay (events); the other
                                      fields then describe the forward render it replays */
     double  ms_gpu;               /* GPU time of the call's kernels (events; mpt_batch_render: the
