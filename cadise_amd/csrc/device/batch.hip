@@ -309,10 +309,7 @@ int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
         S.exec = nullptr;
     }
     const size_t n = static_cast<size_t>(cap);
-    HIP_TRY(ctx, S.beta.alloc(sizeof(float4) * n));
-    HIP_TRY(ctx, S.rad.alloc(sizeof(float4) * n));
-    HIP_TRY(ctx, S.fpos.alloc(sizeof(float2) * n));
-    HIP_TRY(ctx, S.meta.alloc(sizeof(int4) * n));
+    HIP_TRY(ctx, S.path.alloc(sizeof(PathRec) * n));
     HIP_TRY(ctx, S.free_ids.alloc(sizeof(int32_t) * n));
     const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
     for (int q = 0; q < 2; ++q) {
@@ -419,9 +416,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     }
     PassArgs a{};
     a.sc = job.sc;
-    a.beta = S.beta.as<float4>();
-    a.rad = S.rad.as<float4>();
-    a.fpos = S.fpos.as<float2>();
+    a.path = S.path.as<PathRec>();
     for (int q = 0; q < 2; ++q) {
         Queue& Q = a.q[q];
         Q.path = S.qbuf[q][0].as<int32_t>();
@@ -444,7 +439,6 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     bt.nviews = job.nviews;
     bt.capacity = cap;
     bt.total_work = job.total_work;
-    bt.meta = S.meta.as<int4>();
     bt.root[0] = S.rootq[0].as<int32_t>();
     bt.root[1] = S.rootq[1].as<int32_t>();
     bt.film = job.film;

@@ -98,7 +98,7 @@ struct DevBuf {
 // Path state of the streaming wavefront (batch.hip): path slots recycled
 // through a free stack, two queues, counters, and the recorded HIP graph.
 struct Streamer {
-    DevBuf beta, rad, fpos, meta, free_ids, ctr, work_next, qcount, stats, spill, views;
+    DevBuf path, free_ids, ctr, work_next, qcount, stats, spill, views;
     DevBuf qbuf[2][12], rootq[2];
     DevBuf adjq[2][6], adjA;          // adjoint jobs
     int32_t capacity_alloc = 0, adj_alloc = 0;
@@ -154,7 +154,7 @@ struct mpt_scene {
     // scene arrays
     mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, env_cdf;
     // film + pass state
-    mpt::DevBuf film, beta, rad, fpos, qcount, work, stats, spill, srgb;
+    mpt::DevBuf film, path, qcount, work, stats, spill, srgb;
     mpt::DevBuf qbuf[2][12];
     int64_t npaths = 0;
     // parameters that can change after the upload (differentiable rendering)

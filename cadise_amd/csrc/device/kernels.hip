@@ -325,9 +325,10 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     const DevScene& sc = a.sc;
     float fx, fy;
     const V3 dir = camera_ray(sc, p, a.sample, a.key, fx, fy);
-    a.fpos[p] = make_float2(fx, fy);
-    a.beta[p] = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
-    a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    PathRec& pr = a.path[p];
+    pr.fpos = make_float2(fx, fy);
+    pr.beta = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
+    pr.rad = make_float4(0.f, 0.f, 0.f, 0.f);
     const Queue& q = a.q[0];
     q.path[p] = p;
     q.org[p] = make_float4(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2], K_RAY_EPS);
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             int32_t est = sc.estimator;
             int4 meta = make_int4(0, 0, 0, 0);
             if constexpr (BATCH) {
-                meta = bt.meta[p];
+                meta = a.path[p].meta;
                 sid = meta.x;
                 pix = meta.y;
                 smp = meta.z;
@@ -484,9 +485,9 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             }
             org = cur.org[s];
             const V3 P0 = v3f4(org);
-            float4 rad4 = a.rad[p];
+            float4 rad4 = a.path[p].rad;
             V3 L = v3f4(rad4);
-            beta4 = a.beta[p];
+            beta4 = a.path[p].beta;
             V3 beta = v3f4(beta4);
             // emission counts at a hit that follows the camera or a specular vertex (pt),
             // at every hit (vpt) (ptEstimator.cpp:51-56, 80-82; vptEstimator.cpp:48-51)
@@ -691,17 +692,17 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 org = make_float4(h.P.x, h.P.y, h.P.z, K_RAY_EPS);
                 pending = true;
             }
-            a.rad[p] = make_float4(L.x, L.y, L.z, 0.f);
-            a.beta[p] = make_float4(beta.x, beta.y, beta.z, count_next ? 1.0f : 0.0f);
+            a.path[p].rad = make_float4(L.x, L.y, L.z, 0.f);
+            a.path[p].beta = make_float4(beta.x, beta.y, beta.z, count_next ? 1.0f : 0.0f);
             if constexpr (BATCH) {
                 if (pending) {
-                    bt.meta[p] = make_int4(meta.x, meta.y, meta.z, pb + 1);
+                    a.path[p].meta = make_int4(meta.x, meta.y, meta.z, pb + 1);
                 } else if (ADJ) {
                     finished = true;
                 } else {
                     finished = true;
                     const BatchView& v = bt.views[sid];
-                    batch_splat(bt, v, pix, a.fpos[p], L);
+                    batch_splat(bt, v, pix, a.path[p].fpos, L);
                     if (bt.keep)
                         bt.keep[static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix] =
                             make_float4(L.x, L.y, L.z, 0.f);
@@ -829,10 +830,13 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArg
     const int32_t p = bt.free_ids[nfree - 1 - gid];
     float fx, fy;
     const V3 dir = camera_ray(v, pix, sample, make_uint2(v.key_lo, v.key_hi), fx, fy);
-    a.fpos[p] = make_float2(fx, fy);
-    a.beta[p] = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
-    a.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    bt.meta[p] = make_int4(lo, pix, sample, 0);
+    PathRec pr;
+    pr.rad = make_float4(0.f, 0.f, 0.f, 0.f);
+    pr.beta = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
+    pr.meta = make_int4(lo, pix, sample, 0);
+    pr.fpos = make_float2(fx, fy);
+    pr.pad = make_float2(0.f, 0.f);
+    a.path[p] = pr;
     const int32_t slot = qn + static_cast<int32_t>(gid);
     const Queue& q = a.q[qc];
     q.path[slot] = p;
@@ -899,9 +903,9 @@ __global__ void __launch_bounds__(256) k_film(PassArgs a) {
         for (int sy = max(y0, qy - ry); sy <= min(y1 - 1, qy + ry); ++sy) {
             for (int sx = max(x0, qx - rx); sx <= min(x1 - 1, qx + rx); ++sx) {
                 const int sp = sy * sc.img_w + sx;
-                const float4 L = a.rad[sp];
+                const float4 L = a.path[sp].rad;
                 if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) continue;
-                const float2 fp = a.fpos[sp];
+                const float2 fp = a.path[sp].fpos;
                 const float mnx = fmaxf(fp.x - sc.filter_hw, static_cast<float>(x0));
                 const float mny = fmaxf(fp.y - sc.filter_hh, static_cast<float>(y0));
                 const float mxx = fminf(fp.x + sc.filter_hw, static_cast<float>(x1));

@@ -121,7 +121,6 @@ struct BatchArgs {
     int32_t  nviews;
     int32_t  capacity;           // path slots (= queue entries)
     int64_t  total_work;
-    int4*    meta;               // per path slot: scene, pixel, sample, bounce
     int32_t* root[2];            // per queue entry: BVH root of the entry's scene
     float4*  film;               // merged linear film (sum w*rgb, sum w)
     float4*  keep;               // one scene: each sample's radiance [sample - base][pixel], or null
@@ -135,15 +134,23 @@ struct BatchArgs {
 // wavefront of mixed-bounce paths; else one sample pass of one scene).
 enum { SHADE_FWD = 0, SHADE_ADJ = 1, SHADE_BATCH = 2, SHADE_BATCH_ADJ = 3 };
 
+// The state of one path, one 64-byte record per path slot so that a shading
+// step reads and writes one line per path instead of gathering four arrays.
+struct __align__(64) PathRec {
+    float4 rad;    // radiance so far (xyz)
+    float4 beta;   // throughput (xyz); w: emission counts at the next hit
+    int4   meta;   // streaming: scene, pixel, sample, bounce
+    float2 fpos;   // film position of the path's sample
+    float2 pad;
+};
+
 struct PassArgs {
     DevScene sc;
     uint2    key;          // Philox key (seed)
     int32_t  sample;       // sample index of this pass
     int32_t  max_depth;
     int32_t  npaths;
-    float4*  beta;
-    float4*  rad;
-    float2*  fpos;
+    PathRec* path;         // per path slot (pass by pass: per pixel)
     Queue    q[2];
     int32_t* qcount;       // [max_depth + 2]
     int32_t* work;         // traversal work counters, 3 per bounce

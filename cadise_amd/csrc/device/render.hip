@@ -375,9 +375,7 @@ int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_h
 int ensure_pass_buffers(mpt_scene* S, int64_t npix, int32_t max_depth) {
     mpt_ctx* ctx = S->ctx;
     if (npix != S->npaths) {
-        HIP_TRY(ctx, S->beta.alloc(sizeof(float4) * npix));
-        HIP_TRY(ctx, S->rad.alloc(sizeof(float4) * npix));
-        HIP_TRY(ctx, S->fpos.alloc(sizeof(float2) * npix));
+        HIP_TRY(ctx, S->path.alloc(sizeof(PathRec) * npix));
         const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
         for (int q = 0; q < 2; ++q)
             for (int k = 0; k < 12; ++k) HIP_TRY(ctx, S->qbuf[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
@@ -397,9 +395,7 @@ PassArgs pass_args(mpt_scene* S, uint64_t seed, int32_t max_depth) {
     a.key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
     a.max_depth = max_depth;
     a.npaths = static_cast<int32_t>(S->npaths);
-    a.beta = S->beta.as<float4>();
-    a.rad = S->rad.as<float4>();
-    a.fpos = S->fpos.as<float2>();
+    a.path = S->path.as<PathRec>();
     for (int q = 0; q < 2; ++q) {
         Queue& Q = a.q[q];
         Q.path = S->qbuf[q][0].as<int32_t>();
@@ -526,8 +522,9 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, stop());
         if (keep)
-            HIP_TRY(ctx, hipMemcpyAsync(S->samples.as<float4>() + static_cast<int64_t>(k - spp_begin) * npix, S->rad.p,
-                                        sizeof(float4) * static_cast<size_t>(npix), hipMemcpyDeviceToDevice, S->stream));
+            HIP_TRY(ctx, hipMemcpy2DAsync(S->samples.as<float4>() + static_cast<int64_t>(k - spp_begin) * npix,
+                                          sizeof(float4), S->path.p, sizeof(PathRec), sizeof(float4),
+                                          static_cast<size_t>(npix), hipMemcpyDeviceToDevice, S->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(S->stream));
     unsigned long long st[ST_COUNT];
