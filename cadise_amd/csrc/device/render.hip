@@ -308,9 +308,9 @@ int ensure_spill(mpt_scene* S, int blocks) {
     return MPT_OK;
 }
 
-int launch_trace(mpt_scene* S, TraceArgs a, bool any, int blocks) {
+int launch_trace(mpt_scene* S, TraceArgs a, int mode, int blocks) {
     a.spill = S->sc.max_stack > S->cfg.stack ? S->spill.as<int32_t>() : nullptr;
-    launch_trace_cfg(S->cfg, a, any ? TRACE_ANY : TRACE_CLOSEST, blocks, S->stream);
+    launch_trace_cfg(S->cfg, a, mode, blocks, S->stream);
     HIP_TRY(S->ctx, hipGetLastError());
     return MPT_OK;
 }
@@ -354,7 +354,7 @@ int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_h
         a.work = work.as<int32_t>();
         a.stats = S->stats.as<unsigned long long>();
         a.stat_slot = any_hit ? ST_SHADOW_RAYS : ST_CLOSEST_RAYS;
-        if ((rc = launch_trace(S, a, any_hit != 0, blocks)) != MPT_OK) break;
+        if ((rc = launch_trace(S, a, any_hit != 0 ? TRACE_ANY : TRACE_CLOSEST, blocks)) != MPT_OK) break;
         if (hipStreamSynchronize(S->stream) != hipSuccess) { rc = fail(S->ctx, MPT_ERR_DEVICE, "trace failed"); break; }
         unsigned long long st[ST_COUNT];
         if ((rc = read_stats(S, st)) != MPT_OK) break;
@@ -490,20 +490,14 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
             t.work = a.work + 3 * b;
             t.stats = S->stats.as<unsigned long long>();
             t.stat_slot = ST_CLOSEST_RAYS;
-            HIP_TRY(ctx, mark(EV_CLOSEST));
-            if ((rc = launch_trace(S, t, false, tblocks)) != MPT_OK) return rc;
-            HIP_TRY(ctx, stop());
-            if (b > 0) {
-                t.dir = Q.shd_dir;
-                t.dual = 0;
-                t.hit = Q.occ;
-                t.hit2 = nullptr;
-                t.work = a.work + 3 * b + 1;
-                t.stat_slot = ST_SHADOW_RAYS;
-                HIP_TRY(ctx, mark(EV_ANY));
-                if ((rc = launch_trace(S, t, true, tblocks)) != MPT_OK) return rc;
-                HIP_TRY(ctx, stop());
+            if (b > 0) {   // the previous vertices' shadow rays join the launch
+                t.dir3 = Q.shd_dir;
+                t.hit3 = Q.occ;
+                t.shadow = 1;
             }
+            HIP_TRY(ctx, mark(EV_CLOSEST));
+            if ((rc = launch_trace(S, t, b > 0 ? TRACE_MIXED : TRACE_CLOSEST, tblocks)) != MPT_OK) return rc;
+            HIP_TRY(ctx, stop());
             HIP_TRY(ctx, mark(EV_SHADE));
             hipLaunchKernelGGL(k_shade<SHADE_FWD>, dim3(sblocks), dim3(256), 0, S->stream, a, AdjArgs{}, BatchArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
@@ -547,7 +541,7 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
     r.box_tests_any = static_cast<int64_t>(st[ST_BOXES_ANY]);
     r.tri_tests_any = static_cast<int64_t>(st[ST_PRIMS_ANY]);
     r.closest_launches = static_cast<int64_t>(passes) * (max_depth + 1);
-    r.anyhit_launches = static_cast<int64_t>(passes) * max_depth;
+    r.anyhit_launches = 0;   // shadow rays ride in the closest-hit launches
     r.ms_closest = ms[EV_CLOSEST];
     r.ms_anyhit = ms[EV_ANY];
     r.ms_shade = ms[EV_SHADE];
@@ -745,16 +739,12 @@ int do_adjoint_passes(mpt_scene* S, const mpt_render_params* p, const float* d_i
             t.work = a.work + 3 * b;
             t.stats = S->stats.as<unsigned long long>();
             t.stat_slot = ST_CLOSEST_RAYS;
-            if ((rc = launch_trace(S, t, false, tblocks)) != MPT_OK) return rc;
             if (b > 0) {
-                t.dir = Q.shd_dir;
-                t.dual = 0;
-                t.hit = Q.occ;
-                t.hit2 = nullptr;
-                t.work = a.work + 3 * b + 1;
-                t.stat_slot = ST_SHADOW_RAYS;
-                if ((rc = launch_trace(S, t, true, tblocks)) != MPT_OK) return rc;
+                t.dir3 = Q.shd_dir;
+                t.hit3 = Q.occ;
+                t.shadow = 1;
             }
+            if ((rc = launch_trace(S, t, b > 0 ? TRACE_MIXED : TRACE_CLOSEST, tblocks)) != MPT_OK) return rc;
             hipLaunchKernelGGL(k_shade<SHADE_ADJ>, dim3(sblocks), dim3(256), 0, S->stream, a, g, BatchArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
         }
