@@ -224,44 +224,99 @@ struct Tight {
     float lo[3], hi[3];
 };
 
+// SAH-optimal collapse (Ylitie, Karras and Laine, "Efficient Incoherent Ray
+// Traversal on GPUs Through Compressed Wide BVHs", HPG 2017, section 3.1):
+// cost(n, i) is the least SAH cost of binary subtree n spread over at most i
+// child slots of its wide parent. cost(n, 1) = min(leaf, A(n) c_node +
+// dist(n, W)), a leaf only for at most leaf_max primitives (cost A(n) N c_prim);
+// cost(n, i > 1) = min(cost(n, 1), dist(n, i)); dist(n, j) = min over k of
+// cost(left, k) + cost(right, j - k). The wide tree is rebuilt top-down from
+// the recorded choices. Leaves stay contiguous binary subtrees, so primitive
+// order is unchanged.
 template <int W>
-struct Collapser {
-    const BinaryBvh&         bin;
-    const std::vector<Tight>& tight;   // per binary node, tight (no min() quirk)
-    const std::vector<int64_t>& first; // per binary node, first ordered primitive
-    const std::vector<int64_t>& count; // per binary node, primitive count
-    int leaf_max;
+struct SahCollapser {
+    const BinaryBvh&            bin;
+    const std::vector<Tight>&   tight;
+    const std::vector<int64_t>& first;
+    const std::vector<int64_t>& count;
+    int    leaf_max;
+    double c_node, c_prim;
+    std::vector<double> cost;     // [n * W + i - 1]
+    std::vector<int8_t> split;    // [n * W + i - 1]: 0 = one slot (cost(n, 1)), k = k slots to the left child
+    std::vector<uint8_t> as_leaf; // per n: cost(n, 1) is the leaf
     std::vector<WideNode<W>> nodes;
     int32_t depth = 0;
 
-    float area(int64_t n) const {
+    double area(int64_t n) const {
         const Tight& t = tight[n];
-        const float ex = t.hi[0] - t.lo[0], ey = t.hi[1] - t.lo[1], ez = t.hi[2] - t.lo[2];
-        return ex * ey + ex * ez + ey * ez;
+        const double ex = static_cast<double>(t.hi[0]) - t.lo[0], ey = static_cast<double>(t.hi[1]) - t.lo[1],
+                     ez = static_cast<double>(t.hi[2]) - t.lo[2];
+        return std::max(0.0, ex * ey + ex * ez + ey * ez);
     }
-    bool is_leaf_child(int64_t n) const { return bin.nodes[n].leaf || count[n] <= leaf_max; }
-
-    int32_t make(int64_t root, int32_t d) {
+    void solve() {
+        const int64_t nn = static_cast<int64_t>(bin.nodes.size());
+        cost.assign(static_cast<size_t>(nn) * W, 0.0);
+        split.assign(static_cast<size_t>(nn) * W, 0);
+        as_leaf.assign(static_cast<size_t>(nn), 0);
+        const double inf = std::numeric_limits<double>::infinity();
+        for (int64_t n = nn - 1; n >= 0; --n) {   // pre-order: children after their parent
+            const double A = area(n);
+            const double leaf = count[n] <= leaf_max ? A * c_prim * static_cast<double>(count[n]) : inf;
+            double* c = &cost[static_cast<size_t>(n) * W];
+            int8_t* sp = &split[static_cast<size_t>(n) * W];
+            if (bin.nodes[n].leaf) {
+                as_leaf[n] = 1;
+                for (int i = 0; i < W; ++i) c[i] = leaf;
+                continue;
+            }
+            const double* cl = &cost[static_cast<size_t>(n + 1) * W];
+            const double* cr = &cost[static_cast<size_t>(bin.nodes[n].a) * W];
+            double dist[W + 1];
+            int8_t dk[W + 1];
+            for (int j = 2; j <= W; ++j) {
+                dist[j] = inf;
+                dk[j] = 1;
+                for (int k = 1; k < j; ++k) {
+                    const double v = cl[k - 1] + cr[j - k - 1];
+                    if (v < dist[j]) { dist[j] = v; dk[j] = static_cast<int8_t>(k); }
+                }
+            }
+            const double node = A * c_node + dist[W];
+            as_leaf[n] = leaf <= node ? 1 : 0;
+            c[0] = std::min(leaf, node);
+            sp[0] = 0;
+            for (int i = 2; i <= W; ++i) {
+                if (dist[i] < c[0]) { c[i - 1] = dist[i]; sp[i - 1] = dk[i]; }
+                else { c[i - 1] = c[0]; sp[i - 1] = 0; }
+            }
+        }
+    }
+    void gather(int64_t n, int i, std::vector<int64_t>* out) const {
+        const int k = split[static_cast<size_t>(n) * W + i - 1];
+        if (k == 0) {
+            out->push_back(n);
+            return;
+        }
+        gather(n + 1, k, out);
+        gather(bin.nodes[n].a, i - k, out);
+    }
+    // the wide node of binary node n (n interior, opened over W slots)
+    int32_t make(int64_t n, int32_t d) {
         depth = std::max(depth, d);
         std::vector<int64_t> kids;
-        if (is_leaf_child(root)) {
-            kids.push_back(root);
+        if (bin.nodes[n].leaf) {
+            kids.push_back(n);
         } else {
-            kids.push_back(root + 1);
-            kids.push_back(bin.nodes[root].a);
-            while (static_cast<int>(kids.size()) < W) {
-                int best = -1;
-                float best_area = -1.f;
-                for (int i = 0; i < static_cast<int>(kids.size()); ++i) {
-                    if (is_leaf_child(kids[i])) continue;
-                    const float a = area(kids[i]);
-                    if (a > best_area) { best_area = a; best = i; }
-                }
-                if (best < 0) break;
-                const int64_t c = kids[best];
-                kids[best] = c + 1;
-                kids.insert(kids.begin() + best + 1, bin.nodes[c].a);
+            const double* cl = &cost[static_cast<size_t>(n + 1) * W];
+            const double* cr = &cost[static_cast<size_t>(bin.nodes[n].a) * W];
+            int bk = 1;
+            double best = std::numeric_limits<double>::infinity();
+            for (int k = 1; k < W; ++k) {
+                const double v = cl[k - 1] + cr[W - k - 1];
+                if (v < best) { best = v; bk = k; }
             }
+            gather(n + 1, bk, &kids);
+            gather(bin.nodes[n].a, W - bk, &kids);
         }
         const int32_t idx = static_cast<int32_t>(nodes.size());
         nodes.emplace_back();
@@ -275,7 +330,7 @@ struct Collapser {
         for (int i = 0; i < static_cast<int>(kids.size()); ++i) {
             const int64_t c = kids[i];
             int32_t ch, cnt;
-            if (is_leaf_child(c)) {
+            if (as_leaf[c]) {
                 ch = static_cast<int32_t>(first[c]);
                 cnt = static_cast<int32_t>(count[c]);
                 if (cnt == 0) ch = -1;
@@ -365,6 +420,10 @@ bool quantize_node(const WideNode<W>& f, QNode<W>* out) {
 
 }  // namespace
 
+// SAH costs of the collapse: a wide node visit (W slab tests, sort, push)
+// against one primitive test, in units of a node visit.
+constexpr double kSahNode = 1.0, kSahPrim = 0.3;
+
 void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf_max, WideBvh* out) {
     const int64_t nn = static_cast<int64_t>(bin.nodes.size());
     std::vector<Tight> tight(static_cast<size_t>(nn));
@@ -414,8 +473,11 @@ void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf
     }
     out->width = width;
     if (width == 8) {
-        Collapser<8> c{bin, tight, first, count, leaf_max};
-        if (nn > 0) c.make(0, 0);
+        SahCollapser<8> c{bin, tight, first, count, leaf_max, kSahNode, kSahPrim};
+        if (nn > 0) {
+            c.solve();
+            c.make(0, 0);
+        }
         out->nnodes = static_cast<int64_t>(c.nodes.size());
         out->depth = c.depth;
         out->nodes.resize(c.nodes.size() * sizeof(QNode<8>));
@@ -423,8 +485,11 @@ void collapse_wide(const HostScene& s, const BinaryBvh& bin, int width, int leaf
         for (size_t i = 0; i < c.nodes.size(); ++i)
             if (!quantize_node<8>(c.nodes[i], &qn[i])) out->ok = false;
     } else {
-        Collapser<4> c{bin, tight, first, count, leaf_max};
-        if (nn > 0) c.make(0, 0);
+        SahCollapser<4> c{bin, tight, first, count, leaf_max, kSahNode, kSahPrim};
+        if (nn > 0) {
+            c.solve();
+            c.make(0, 0);
+        }
         out->nnodes = static_cast<int64_t>(c.nodes.size());
         out->depth = c.depth;
         out->nodes.resize(c.nodes.size() * sizeof(QNode<4>));
