@@ -228,7 +228,8 @@ struct Tight {
 // Traversal on GPUs Through Compressed Wide BVHs", HPG 2017, section 3.1):
 // cost(n, i) is the least SAH cost of binary subtree n spread over at most i
 // child slots of its wide parent. cost(n, 1) = min(leaf, A(n) c_node +
-// dist(n, W)), a leaf only for at most leaf_max primitives (cost A(n) N c_prim);
+// dist(n, W)), a leaf only for at most leaf_max primitives (cost A(n) N c_prim)
+// or when n is a binary leaf (kept whole, whatever its size);
 // cost(n, i > 1) = min(cost(n, 1), dist(n, i)); dist(n, j) = min over k of
 // cost(left, k) + cost(right, j - k). The wide tree is rebuilt top-down from
 // the recorded choices. Leaves stay contiguous binary subtrees, so primitive
@@ -261,14 +262,15 @@ struct SahCollapser {
         const double inf = std::numeric_limits<double>::infinity();
         for (int64_t n = nn - 1; n >= 0; --n) {   // pre-order: children after their parent
             const double A = area(n);
-            const double leaf = count[n] <= leaf_max ? A * c_prim * static_cast<double>(count[n]) : inf;
+            const double leaf_cost = A * c_prim * static_cast<double>(count[n]);
             double* c = &cost[static_cast<size_t>(n) * W];
             int8_t* sp = &split[static_cast<size_t>(n) * W];
-            if (bin.nodes[n].leaf) {
+            if (bin.nodes[n].leaf) {   // a binary leaf cannot be split: a leaf whatever its size
                 as_leaf[n] = 1;
-                for (int i = 0; i < W; ++i) c[i] = leaf;
+                for (int i = 0; i < W; ++i) c[i] = leaf_cost;
                 continue;
             }
+            const double leaf = count[n] <= leaf_max ? leaf_cost : inf;
             const double* cl = &cost[static_cast<size_t>(n + 1) * W];
             const double* cr = &cost[static_cast<size_t>(bin.nodes[n].a) * W];
             double dist[W + 1];
@@ -281,8 +283,8 @@ struct SahCollapser {
                     if (v < dist[j]) { dist[j] = v; dk[j] = static_cast<int8_t>(k); }
                 }
             }
-            const double node = A * c_node + dist[W];
-            as_leaf[n] = leaf <= node ? 1 : 0;
+            const double node = A * c_node + dist[W];   // finite: binary leaves cost finitely
+            as_leaf[n] = leaf < node ? 1 : 0;
             c[0] = std::min(leaf, node);
             sp[0] = 0;
             for (int i = 2; i <= W; ++i) {

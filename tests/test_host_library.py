@@ -207,3 +207,25 @@ def test_environment_light_crsd_errors(ctx, tmp_path):
     with pytest.raises(MptError) as e:
         Scene.from_crsd_text(ctx, text)
     assert e.value.code in (-2, -5)
+
+
+def test_wide_collapse_keeps_oversized_binary_leaves(ctx):
+    """Three coincident triangles make one binary leaf of 3 (the reference's SAH
+    cannot split them), above the default leaf size of 2: the collapse keeps that
+    leaf whole and still builds a real tree over the other 70,000 triangles (it
+    once turned every ancestor of such a leaf into one leaf, which no node can
+    encode beyond 65535 primitives)."""
+    rng = np.random.default_rng(3)
+    c = rng.uniform(0, 100, (70000, 1, 3)).astype(np.float32)
+    tris = c + rng.normal(0, 0.2, (70000, 3, 3)).astype(np.float32)
+    dup = np.repeat(np.array([[[50, 50, 50], [51, 50, 50], [50, 51, 50]]], np.float32), 3, 0)
+    spec = scenes.SceneSpec(width=8, height=8, spp=1, cam_pos=(50, 50, 200), cam_dir=(0, 0, -1), cam_up=(0, 1, 0))
+    spec.materials = [scenes.Material("m")]
+    spec.meshes = [scenes.Mesh("soup", "m", v=np.concatenate([tris, dup]).reshape(-1, 9)),
+                   scenes.Mesh("lamp", "m", v=scenes.quad_tris((40, 120, 40), (60, 120, 40), (60, 120, 60), (40, 120, 60)))]
+    spec.lights = [scenes.Light("lamp", (1, 1, 1), 100.0)]
+    from cadise_amd.api import bvh_opts
+    for width in (4, 8):
+        for leaf_max in (1, 2):
+            st = Scene.from_spec(ctx, spec, bvh_opts(width=width, leaf_max=leaf_max)).stats()
+            assert st["wide_nodes"] > 70000 // (2 * width), (width, leaf_max, st["wide_nodes"])
