@@ -216,7 +216,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--width", type=int, default=8, help="BVH width (4 or 8)")
+    ap.add_argument("--width", type=int, default=4, help="BVH width (4 or 8)")
     ap.add_argument("--leaf-max", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -48,7 +48,7 @@ class Context:
         self.close()
 
 
-def bvh_opts(width=8, leaf_max=2, block=0, stack_lds=0, blocks_per_cu=0):
+def bvh_opts(width=4, leaf_max=2, block=0, stack_lds=0, blocks_per_cu=0):
     """Zero fields take the library defaults (block 128, 24 LDS stack entries, 6 blocks per CU)."""
     return BvhOpts(width, leaf_max, block, stack_lds, blocks_per_cu)
 

@@ -62,7 +62,7 @@ inline void launch_trace_cfg(const TraceCfg& c, const TraceArgs& a, int mode, in
 }
 
 inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
-    if (o.width != 4) o.width = 8;
+    if (o.width != 8) o.width = 4;
     if (o.leaf_max <= 0) o.leaf_max = 2;
     if (o.leaf_max > 64) o.leaf_max = 64;
     if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;

@@ -136,7 +136,7 @@ typedef struct {
 /* Accelerator options (the reference's accelerator block plus the wide-BVH
  * layout chosen for gfx950). Zero fields take the defaults. */
 typedef struct {
-    int32_t width;             /* children per node: 4 or 8 (default 8) */
+    int32_t width;             /* children per node: 4 or 8 (default 4) */
     int32_t leaf_max;          /* largest subtree folded into one leaf (default 2) */
     int32_t block;             /* traversal threads per block: 128 (default) or 256 */
     int32_t stack_lds;         /* per-lane stack entries kept in LDS (default 24) */
