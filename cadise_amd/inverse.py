@@ -70,7 +70,7 @@ def recover(scene: Scene, target: np.ndarray, init: dict, truth: dict, radiance_
             rad.clamp_(min=0.1)
         out.losses.append(float(loss.detach()))
         if log and (it % log_every == 0 or it == steps - 1):
-            log(f"[adam] step {it}: loss {float(loss):.4e}")
+            log(f"[adam] step {it}: loss {float(loss.detach()):.4e}")
     out.seconds = time.time() - t0
     out.ms_forward = fwd_ms / steps
     out.ms_adjoint = adj_ms / steps
