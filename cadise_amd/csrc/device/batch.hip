@@ -23,12 +23,14 @@ namespace mpt {
 
 // ---- merging kernels ------------------------------------------------------------------------
 
-__global__ void k_fix_tris(float4* tris, int64_t n, int32_t prim_base) {
+__global__ void k_fix_tris(float4* tris, int64_t n, int32_t prim_base, int32_t light_base) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t bits = __float_as_uint(tris[3 * j].w);
     const uint32_t id = (bits & 0x7FFFFFFFu) + static_cast<uint32_t>(prim_base);
     tris[3 * j].w = __uint_as_float(id | (bits & 0x80000000u));
+    const int32_t l = __float_as_int(tris[3 * j + 2].w);   // the primitive's area light
+    tris[3 * j + 2].w = __int_as_float(l >= 0 ? l + light_base : -1);
 }
 
 __global__ void k_fix_info(int4* info, int64_t n, int32_t mat_base, int32_t light_base) {
@@ -253,12 +255,13 @@ int merge_tables(mpt_batch* B) {
         HIP_TRY(ctx, hipMemcpyAsync(t, S->tris.p, sizeof(float4) * 3 * static_cast<size_t>(n), hipMemcpyDeviceToDevice, B->stream));
         HIP_TRY(ctx, hipMemcpyAsync(inf, S->prim_info.p, sizeof(int4) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, B->stream));
         const int blocks = static_cast<int>((n + 255) / 256);
-        hipLaunchKernelGGL(k_fix_tris, dim3(blocks), dim3(256), 0, B->stream, t, n, static_cast<int32_t>(pbase[i]));
         int32_t mat0 = 0, light0 = 0;
         for (size_t k = 0; k < i; ++k) {
             mat0 += static_cast<int32_t>(B->scenes[k]->h_mats.size());
             light0 += static_cast<int32_t>(B->scenes[k]->host.lights.size());
         }
+        hipLaunchKernelGGL(k_fix_tris, dim3(blocks), dim3(256), 0, B->stream, t, n, static_cast<int32_t>(pbase[i]),
+                           light0);
         hipLaunchKernelGGL(k_fix_info, dim3(blocks), dim3(256), 0, B->stream, inf, n, mat0, light0);
         HIP_TRY(ctx, hipGetLastError());
         if (!S->host.prim_n.empty())

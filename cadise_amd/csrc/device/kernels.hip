@@ -523,8 +523,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                         Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
                     }
                     if (mh.x >= 0) {
-                        const int32_t prim = static_cast<int32_t>(__float_as_uint(sc.tris[3 * mh.x].w) & 0x7FFFFFFFu);
-                        if (sc.prim_info[prim].y == li) {
+                        if (__float_as_int(sc.tris[3 * mh.x + 2].w) == li) {   // the record's area light
                             const SurfaceHit h = surface_detail(sc, mh.x, P0, v3f4(md4), __int_as_float(mh.y));
                             const DevLight& lg = sc.lights[li];
                             const V3 Le = light_emittance(lg, h);

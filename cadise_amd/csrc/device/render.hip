@@ -109,6 +109,14 @@ int build_and_upload(mpt_scene* S) {
     }
     for (size_t l = 0; l < h.lights.size(); ++l)
         if (h.lights[l].type == LIGHT_AREA) info[h.lights[l].prim].y = static_cast<int32_t>(l);
+    // the record's last word: the primitive's area light (-1 none), so that a hit's
+    // light is read from the record the hit already names
+    for (int64_t j = 0; j < n; ++j) {
+        uint32_t bits;
+        std::memcpy(&bits, &tris[3 * j].w, 4);
+        const int32_t l = info[bits & 0x7FFFFFFFu].y;
+        std::memcpy(&tris[3 * j + 2].w, &l, 4);
+    }
 
     std::vector<DevMaterial> mats(h.materials.size());
     for (size_t i = 0; i < mats.size(); ++i) {

@@ -47,7 +47,8 @@ __device__ __forceinline__ bool sphere_test(V3 G, float r, V3 o, V3 D, float tmi
     return true;
 }
 
-// Primitive record at leaf position li: r0 = (A, id bits), r1 = e1, r2 = e2;
+// Primitive record at leaf position li: r0 = (A, id bits), r1 = e1, r2 = e2
+// (r2.w: the primitive's area light, -1 none, read by shading only);
 // the id's top bit marks a rectangle, or a sphere when r1.w != 0 (then
 // r0 = centre, r1.x = radius).
 struct PrimRec {
