@@ -239,8 +239,8 @@ def main():
         print("copy calibration: 3 x 1 GiB device copies (expect 3145728 KiB read, 3145728 KiB written)", flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU design (pixel/sample sharding "
-                                   "with one film reduction) is described in DESIGN.md and not launched"}))
+        print(json.dumps({"error": "bench.py runs on one GPU; sharding pixels and samples over GPUs "
+                                   "(one film reduction at the end) is not launched"}))
         sys.exit(2)
 
     import torch
