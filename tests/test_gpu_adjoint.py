@@ -188,3 +188,50 @@ def test_adam_recovers_textures(ctx):
     assert r.tex_rmse["albedo"][1] < 0.5 * r.tex_rmse["albedo"][0]
     assert r.tex_rmse["rough"][1] < 0.9 * r.tex_rmse["rough"][0]
     assert np.abs(np.array(r.radiance[1]) - 10.0).max() < 1.0
+
+
+def test_adjoint_through_checkerboards_matches_finite_differences(ctx):
+    """A picture texture read through checkerboards (the back wall: a checkerboard
+    of the picture and a constant; the tall box: a checkerboard of a checkerboard
+    that holds the picture): the adjoint sends each vertex's gradient to the
+    picture texel its checkerboards resolve to. Checked by central differences of
+    a fixed linear functional sum(G * image) on the pass-by-pass renderer (fixed
+    summation order, common random numbers), max depth 4 (Russian roulette inert):
+    bar 2% relative per texel."""
+    import os
+    from cadise_amd.api import Scene
+    here = os.path.dirname(os.path.abspath(__file__))
+    pics = np.load(os.path.join(here, "golden", "images.npz"))
+    spec = scenes.checker_cornell(os.path.join(here, "golden", "images", "rgb8_filters.png"),
+                                  pics["rgb8_filters.png"], width=96, height=96, spp=16, max_depth=4)
+    sc = Scene.from_spec(ctx, spec)
+    ip = sc.tex_index["pic"]
+    pic = sc.get_texture(ip)
+    G = np.random.default_rng(5).normal(size=(spec.height, spec.width, 3)).astype(np.float32)
+    seed = 99
+    sc.render(seed=seed, keep_samples=True)
+    d_tex, _ = sc.adjoint(G, seed=seed)
+    g = sc.texture_views(d_tex)[ip].reshape(pic.shape)
+
+    def functional(p):
+        sc.set_texture(ip, p)
+        img = film_mean(sc.render(seed=seed, passwise=True)).astype(np.float64)
+        return float((img * G).sum())
+
+    rng = np.random.default_rng(12)
+    cand = np.argwhere(np.abs(g) > 0.05 * np.abs(g).max())
+    assert len(cand) >= 24
+    eps = 1e-3
+    worst = 0.0
+    for (y, x, c) in cand[rng.choice(len(cand), 24, replace=False)]:
+        p = pic.copy()
+        p[y, x, c] += eps
+        lp = functional(p)
+        p[y, x, c] -= 2 * eps
+        lm = functional(p)
+        fd = (lp - lm) / (2 * eps)
+        rel = abs(fd - g[y, x, c]) / abs(fd)
+        worst = max(worst, rel)
+        assert rel < 0.02, ((y, x, c), fd, float(g[y, x, c]))
+    sc.set_texture(ip, pic)
+    print(f"[fd checkerboard] 24 picture texels: worst relative difference {worst:.2e}", flush=True)
