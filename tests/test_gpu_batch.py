@@ -7,6 +7,8 @@ slots far fewer than the work (slots recycled many times), both BVH widths
 (a width-8 batch over scenes built with width 4 re-collapses their binary trees
 and must equal width-8 scenes node for node, so tie-breaking matches), all LDS
 stack depths and block sizes, and after the autotuner."""
+import os
+
 import numpy as np
 import pytest
 
@@ -31,6 +33,12 @@ def batch_set(ctx):
                                     sun_diameter_deg=6.0))
     specs.append(scenes.material_zoo(width=40, height=30, spp=3, max_depth=9, textured=True))
     specs.append(scenes.material_zoo(width=24, height=20, spp=5, max_depth=7, estimator="vpt"))
+    # picture and checkerboard textures after other scenes' textures: the merge offsets
+    # the checkerboards' odd / even indices
+    here = os.path.dirname(os.path.abspath(__file__))
+    pics = np.load(os.path.join(here, "golden", "images.npz"))
+    specs.append(scenes.checker_cornell(os.path.join(here, "golden", "images", "rgb8_filters.png"),
+                                        pics["rgb8_filters.png"], width=32, height=32, spp=3, max_depth=5))
     big = scenes.procedural_spheres(seed=5, n_spheres=6, level=3, width=256, height=200, spp=6, max_depth=9)
     specs.append(big)
     seeds = [1000 + 17 * i for i in range(len(specs))]
