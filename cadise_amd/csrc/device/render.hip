@@ -641,7 +641,9 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     job.keep = keep ? S->samples.as<float4>() : nullptr;
     job.cfg = S->cfg;
     int lg = 16;
-    while (lg < 22 && (int64_t{1} << lg) < job.total_work) ++lg;   // 4M slots fill the chip
+    // 8M slots: each traversal launch carries ~9M queries, which amortizes the
+    // launch's tail (5% less traversal time than 4M slots at 1080p)
+    while (lg < 23 && (int64_t{1} << lg) < job.total_work) ++lg;
     job.capacity_log2 = lg;
     job.graph_steps = 16;
     job.use_graph = false;
