@@ -308,7 +308,7 @@ def main():
                                "1920x1080, 64 spp, depth 8, GGX, 4 quad lights, NEE+MIS",
                    "primitives": st_scene["nprims"], "width": spec.width, "height": spec.height,
                    "spp_per_step": 1, "max_depth": spec.max_depth, "bvh_width": st_scene["width"],
-                   "leaf_max": st_scene["leaf_max"], "renderer": "streaming wavefront, 2^23 path slots"},
+                   "leaf_max": st_scene["leaf_max"], "renderer": "streaming wavefront, 2^24 path slots"},
         "samples_per_s": round(st["samples"] / dt, 1),
         "passwise_mrays_s": round(rays / dt_pw / 1e6, 1) if dt_pw else None,
         "rays": {"camera": st["camera_rays"], "continuation": st["continuation_rays"], "mis_bsdf": st["mis_rays"],

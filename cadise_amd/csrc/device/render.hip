@@ -641,9 +641,10 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     job.keep = keep ? S->samples.as<float4>() : nullptr;
     job.cfg = S->cfg;
     int lg = 16;
-    // 8M slots: each traversal launch carries ~9M queries, which amortizes the
-    // launch's tail (5% less traversal time than 4M slots at 1080p)
-    while (lg < 23 && (int64_t{1} << lg) < job.total_work) ++lg;
+    // up to 16M slots (~6 GB): larger traversal launches amortize each launch's
+    // ramp and straggler tail (1080p, 32 passes: 99.8 / 94.6 / 91.3 ms of
+    // traversal with 4M / 8M / 16M slots)
+    while (lg < 24 && (int64_t{1} << lg) < job.total_work) ++lg;
     job.capacity_log2 = lg;
     job.graph_steps = 16;
     job.use_graph = false;
