@@ -306,8 +306,16 @@ class Scene:
         self.ctx.check(self.L.mpt_film_srgb8(self.h, out.ctypes.data))
         return out
 
-    def save(self, path: str):
-        self.ctx.check(self.L.mpt_film_save(self.h, path.encode()))
+    def save(self, path: str | None = None):
+        """Film::save by extension; None: the film block's output-filename."""
+        self.ctx.check(self.L.mpt_film_save(self.h, path.encode() if path is not None else None))
+
+    def output_filename(self) -> str:
+        n = C.c_size_t(0)
+        self.ctx.check(self.L.mpt_scene_output_filename(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self.ctx.check(self.L.mpt_scene_output_filename(self.h, buf, n.value, None))
+        return buf.value.decode()
 
 
 def batch_opts(width: int = 4, stack_lds: int = 24, block: int = 128, capacity_log2: int = 21,
@@ -373,6 +381,21 @@ class Batch:
         self.ctx.check(self.L.mpt_batch_autotune(self.h, sd.ctypes.data if sd is not None else None, trial_spp,
                                                  C.byref(best), trials, max_trials, C.byref(nt)))
         return best.as_dict(), [(trials[i].opts.as_dict(), trials[i].ms) for i in range(min(nt.value, max_trials))]
+
+
+def srgb8_device(ctx: Context, film_ptr: int, npix: int, out_ptr: int, stream: int = 0):
+    """mpt_film_srgb8_device on device buffers (npix float4 in, npix x 3 bytes out)."""
+    ctx.check(ctx.L.mpt_film_srgb8_device(ctx.h, film_ptr, npix, out_ptr, stream or None))
+
+
+def image_write(path: str, rgb: np.ndarray):
+    """mpt_image_write: 8-bit RGB (H x W x 3) to .png / .jpg / .ppm (host only)."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    assert rgb.ndim == 3 and rgb.shape[2] == 3
+    L = native.load_library()
+    rc = L.mpt_image_write(path.encode(), rgb.ctypes.data, rgb.shape[1], rgb.shape[0])
+    if rc != 0:
+        raise MptError(rc, f"mpt_image_write({path}) failed")
 
 
 def film_mean(film: np.ndarray) -> np.ndarray:

@@ -312,6 +312,7 @@ int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
         S.exec = nullptr;
     }
     const size_t n = static_cast<size_t>(cap);
+    S.capacity_alloc = 0;   // until every buffer below has its new size
     HIP_TRY(ctx, S.path.alloc(sizeof(PathRec) * n));
     HIP_TRY(ctx, S.free_ids.alloc(sizeof(int32_t) * n));
     const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
@@ -456,6 +457,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     if (job.adjoint) {
         if (job.use_graph) return fail(ctx, MPT_ERR_INVALID, "the adjoint runs without a recorded graph");
         if (cap > S.adj_alloc) {
+            S.adj_alloc = 0;   // until every buffer below has its new size
             const size_t sizes[6] = {8, 16, 16, 16, 16, 16};
             for (int q = 0; q < 2; ++q)
                 for (int k = 0; k < 6; ++k) HIP_TRY(ctx, S.adjq[q][k].alloc(sizes[k] * static_cast<size_t>(cap)));

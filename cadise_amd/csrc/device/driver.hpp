@@ -85,11 +85,18 @@ struct DevBuf {
         p = nullptr;
         bytes = 0;
     }
+    // Replaces the buffer by one of n bytes. On failure the buffer is empty
+    // (p null, bytes 0), so a size check against `bytes` never passes on a
+    // buffer that does not exist.
     hipError_t alloc(size_t n) {
         release();
         if (n == 0) n = 16;
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, n);
+        if (e != hipSuccess) return e;
+        p = q;
         bytes = n;
-        return hipMalloc(&p, n);
+        return hipSuccess;
     }
     template <typename T>
     T* as() const { return static_cast<T*>(p); }

@@ -104,7 +104,6 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                 r.o = v3f4(po);
                 r.d = v3f4(pd);
                 r.inv = safe_inv(r.d);
-                r.oi = v3(r.o.x * r.inv.x, r.o.y * r.inv.y, r.o.z * r.inv.z);
                 r.tmin = po.w;
                 r.tmax = pd.w;
                 cl.ray[2 * lane] = lds_f4{po.x, po.y, po.z, po.w};
@@ -926,17 +925,28 @@ __global__ void __launch_bounds__(256) k_film(PassArgs a) {
 }
 
 // Film::save's resolve (film.cpp:159-196), math::forward_gamma_correction
-// (math.cpp:92-99) and ImageUtils::hdrToLdr (imageUtils.cpp:23-36).
+// (math.cpp:92-99) and ImageUtils::hdrToLdr (imageUtils.cpp:23-36). The power
+// is pow in double rounded once to float: the correctly rounded x^(1/2.4f). Over
+// every float in [0, 1] that gives the same byte as the reference's glibc powf
+// (which differs from the correctly rounded value in its last bit for some
+// inputs, none of them at a byte boundary); the device's single-precision powf
+// is not used because its last-bit differences are not so placed.
+__device__ __forceinline__ uint8_t srgb8_of(float lin) {
+    const float s = lin <= 0.0031308f
+                        ? 12.92f * lin
+                        : 1.055f * static_cast<float>(pow(static_cast<double>(lin), static_cast<double>(1.0f / 2.4f))) -
+                              0.055f;
+    return static_cast<uint8_t>(mclamp(s * 255.0f + 0.5f, 0.0f, 255.0f));
+}
+
 __global__ void __launch_bounds__(256) k_srgb8(const float4* film, int64_t npix, uint8_t* out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
-    const float4 p = film[i];
-    const float invw = (p.w == 0.0f) ? 0.0f : 1.0f / p.w;
-    const float v[3] = {p.x * invw, p.y * invw, p.z * invw};
-    for (int c = 0; c < 3; ++c) {
-        const float lin = v[c] + 0.0f;
-        const float s = lin <= 0.0031308f ? 12.92f * lin : 1.055f * powf(lin, 1.0f / 2.4f) - 0.055f;
-        out[3 * i + c] = static_cast<uint8_t>(mclamp(s * 255.0f + 0.5f, 0.0f, 255.0f));
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < npix; i += stride) {
+        const float4 p = film[i];
+        const float invw = (p.w == 0.0f) ? 0.0f : 1.0f / p.w;
+        out[3 * i + 0] = srgb8_of(p.x * invw + 0.0f);
+        out[3 * i + 1] = srgb8_of(p.y * invw + 0.0f);
+        out[3 * i + 2] = srgb8_of(p.z * invw + 0.0f);
     }
 }
 

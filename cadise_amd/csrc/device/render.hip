@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "driver.hpp"
+#include "../host/picture.hpp"
 
 namespace {
 
@@ -311,6 +312,7 @@ int ensure_spill(mpt_scene* S, int blocks) {
     if (extra <= 0) return MPT_OK;
     const int64_t lanes = static_cast<int64_t>(blocks) * S->cfg.block;
     if (lanes <= S->spill_lanes) return MPT_OK;
+    S->spill_lanes = 0;
     HIP_TRY(S->ctx, S->spill.alloc(sizeof(int32_t) * static_cast<size_t>(lanes * extra)));
     S->spill_lanes = lanes;
     return MPT_OK;
@@ -383,6 +385,7 @@ int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_h
 int ensure_pass_buffers(mpt_scene* S, int64_t npix, int32_t max_depth) {
     mpt_ctx* ctx = S->ctx;
     if (npix != S->npaths) {
+        S->npaths = 0;   // until every buffer below has its new size
         HIP_TRY(ctx, S->path.alloc(sizeof(PathRec) * npix));
         const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
         for (int q = 0; q < 2; ++q)
@@ -390,6 +393,7 @@ int ensure_pass_buffers(mpt_scene* S, int64_t npix, int32_t max_depth) {
         S->npaths = npix;
     }
     if (max_depth != S->max_depth_alloc) {
+        S->max_depth_alloc = -1;
         HIP_TRY(ctx, S->qcount.alloc(sizeof(int32_t) * (max_depth + 2)));
         HIP_TRY(ctx, S->work.alloc(sizeof(int32_t) * 3 * (max_depth + 2)));
         S->max_depth_alloc = max_depth;
@@ -684,6 +688,7 @@ int do_adjoint_passes(mpt_scene* S, const mpt_render_params* p, const float* d_i
     const mpt_stats fwd = S->rstats;
     if ((rc = ensure_pass_buffers(S, npix, max_depth)) != MPT_OK) return rc;
     if (S->adj_npaths != npix) {
+        S->adj_npaths = 0;   // until every buffer below has its new size
         const size_t sizes[6] = {8, 16, 16, 16, 16, 16};
         for (int q = 0; q < 2; ++q)
             for (int k = 0; k < 6; ++k) HIP_TRY(ctx, S->adjq[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
@@ -864,8 +869,6 @@ int finish_scene(mpt_ctx* ctx, std::unique_ptr<mpt_scene>& S, const mpt_bvh_opts
     *out = S.release();
     return MPT_OK;
 }
-
-void write_png(const std::string& path, const uint8_t* rgb, int w, int h, std::string* err);
 
 }  // namespace
 
@@ -1145,19 +1148,19 @@ int mpt_scene_binary_bvh(mpt_scene* s, int64_t* count, float* bounds, int32_t* i
 
 int mpt_trace_device(mpt_scene* s, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev, void* stream) {
     if (!s) return MPT_ERR_INVALID;
-    REQUIRE_DEVICE(s);
+    if (n > MPT_TRACE_MAX_RAYS) return fail(s->ctx, MPT_ERR_INVALID, "at most MPT_TRACE_MAX_RAYS rays per call");
     if (n > 0 && (!rays_dev || !hits_dev)) return fail(s->ctx, MPT_ERR_INVALID, "null ray / hit buffer");
-    if (n >= (1ull << 31)) return fail(s->ctx, MPT_ERR_INVALID, "at most 2^31 - 1 rays per call");
+    REQUIRE_DEVICE(s);
     HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
     return do_trace(s, rays_dev, n, any_hit, hits_dev, static_cast<hipStream_t>(stream));
 }
 
 int mpt_trace(mpt_scene* s, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits) {
     if (!s) return MPT_ERR_INVALID;
+    if (n > MPT_TRACE_MAX_RAYS) return fail(s->ctx, MPT_ERR_INVALID, "at most MPT_TRACE_MAX_RAYS rays per call");
+    if (n > 0 && (!rays || !hits)) return fail(s->ctx, MPT_ERR_INVALID, "null ray / hit buffer");
     REQUIRE_DEVICE(s);
     if (n == 0) return MPT_OK;
-    if (!rays || !hits) return fail(s->ctx, MPT_ERR_INVALID, "null ray / hit buffer");
-    if (n >= (1ull << 31)) return fail(s->ctx, MPT_ERR_INVALID, "at most 2^31 - 1 rays per call");
     HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
     DevBuf r, h;
     HIP_TRY(s->ctx, r.alloc(n * sizeof(mpt_ray)));
@@ -1201,47 +1204,52 @@ int mpt_film_srgb8(mpt_scene* s, uint8_t* out) {
     return MPT_OK;
 }
 
+int mpt_film_srgb8_device(mpt_ctx* ctx, const float* film_dev, int64_t npix, uint8_t* out_dev, void* stream) {
+    if (!ctx) return MPT_ERR_INVALID;
+    if (ctx->device < 0) return fail(ctx, MPT_ERR_DEVICE, "host-only context has no device");
+    if (npix < 0) return fail(ctx, MPT_ERR_INVALID, "negative pixel count");
+    if (npix == 0) return MPT_OK;
+    if (!film_dev || !out_dev) return fail(ctx, MPT_ERR_INVALID, "null film / byte buffer");
+    if (npix > (static_cast<int64_t>(1) << 40)) return fail(ctx, MPT_ERR_INVALID, "at most 2^40 pixels per call");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int64_t blocks = (npix + 255) / 256;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(blocks, 1 << 20));
+    hipLaunchKernelGGL(k_srgb8, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const float4*>(film_dev), npix, out_dev);
+    HIP_TRY(ctx, hipGetLastError());
+    return MPT_OK;
+}
+
 int mpt_film_save(mpt_scene* s, const char* path) {
-    if (!s || !path) return MPT_ERR_INVALID;
+    if (!s) return MPT_ERR_INVALID;
     REQUIRE_DEVICE(s);
-    const std::string p(path);
+    // NULL: the film block's output-filename (filmInstantiator.cpp:24), as Film::save
+    const std::string p = path ? std::string(path) : s->host.output;
     const int w = s->host.width, h = s->host.height;
-    auto ends = [&](const char* e) {
-        const size_t n = std::strlen(e);
-        return p.size() >= n && p.compare(p.size() - n, n, e) == 0;
-    };
-    if (ends(".pfm")) {
+    const int kind = picture_kind(p);
+    if (kind == PICTURE_NONE)
+        return fail(s->ctx, MPT_ERR_UNSUPPORTED, "output format of " + p + " (png, jpg, ppm; pfm for linear films)");
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    std::string err;
+    if (kind == PICTURE_PFM) {
         std::vector<float> film(static_cast<size_t>(4) * w * h);
         HIP_TRY(s->ctx, hipMemcpy(film.data(), s->film.p, film.size() * sizeof(float), hipMemcpyDeviceToHost));
-        std::ofstream f(p, std::ios::binary);
-        if (!f) return fail(s->ctx, MPT_ERR_IO, "cannot write " + p);
-        f << "PF\n" << w << " " << h << "\n-1.0\n";
-        // PFM rows run bottom to top
-        for (int y = h - 1; y >= 0; --y)
-            for (int x = 0; x < w; ++x) {
-                const float* px = &film[4 * (static_cast<size_t>(y) * w + x)];
-                const float inv = px[3] == 0.f ? 0.f : 1.f / px[3];
-                const float rgb[3] = {px[0] * inv, px[1] * inv, px[2] * inv};
-                f.write(reinterpret_cast<const char*>(rgb), sizeof(rgb));
-            }
-        return f ? MPT_OK : fail(s->ctx, MPT_ERR_IO, "short write " + p);
+        return write_pfm(p, film.data(), w, h, &err) ? MPT_OK : fail(s->ctx, MPT_ERR_IO, err);
     }
     std::vector<uint8_t> rgb(static_cast<size_t>(3) * w * h);
     const int rc = mpt_film_srgb8(s, rgb.data());
     if (rc != MPT_OK) return rc;
-    if (ends(".ppm")) {
-        std::ofstream f(p, std::ios::binary);
-        if (!f) return fail(s->ctx, MPT_ERR_IO, "cannot write " + p);
-        f << "P6\n" << w << " " << h << "\n255\n";
-        f.write(reinterpret_cast<const char*>(rgb.data()), static_cast<std::streamsize>(rgb.size()));
-        return f ? MPT_OK : fail(s->ctx, MPT_ERR_IO, "short write " + p);
-    }
-    if (ends(".png")) {
-        std::string err;
-        write_png(p, rgb.data(), w, h, &err);
-        return err.empty() ? MPT_OK : fail(s->ctx, MPT_ERR_IO, err);
-    }
-    return fail(s->ctx, MPT_ERR_UNSUPPORTED, "output format of " + p + " (png, ppm, pfm)");
+    return write_picture8(p, rgb.data(), w, h, &err) ? MPT_OK : fail(s->ctx, MPT_ERR_IO, err);
+}
+
+int mpt_scene_output_filename(const mpt_scene* s, char* buf, size_t n, size_t* needed) {
+    if (!s) return MPT_ERR_INVALID;
+    const std::string& o = s->host.output;
+    if (needed) *needed = o.size() + 1;
+    if (!buf) return MPT_OK;
+    if (n < o.size() + 1) return fail(s->ctx, MPT_ERR_INVALID, "buffer shorter than the output file name");
+    std::memcpy(buf, o.c_str(), o.size() + 1);
+    return MPT_OK;
 }
 
 int mpt_scene_texture_info(const mpt_scene* s, int32_t index, int64_t* width, int64_t* height, int64_t* channels,
@@ -1347,80 +1355,3 @@ int mpt_stats_get(const mpt_scene* s, mpt_stats* out) {
 }
 
 }  // extern "C"
-
-// ---- PNG writer (stored deflate blocks, no compression library) ---------------------
-
-namespace {
-
-uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n) {
-    static uint32_t table[256];
-    static bool init = false;
-    if (!init) {
-        for (uint32_t i = 0; i < 256; ++i) {
-            uint32_t k = i;
-            for (int j = 0; j < 8; ++j) k = (k & 1) ? 0xEDB88320u ^ (k >> 1) : k >> 1;
-            table[i] = k;
-        }
-        init = true;
-    }
-    c ^= 0xFFFFFFFFu;
-    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
-    return c ^ 0xFFFFFFFFu;
-}
-
-void put_be32(std::vector<uint8_t>& v, uint32_t x) {
-    v.push_back(static_cast<uint8_t>(x >> 24));
-    v.push_back(static_cast<uint8_t>(x >> 16));
-    v.push_back(static_cast<uint8_t>(x >> 8));
-    v.push_back(static_cast<uint8_t>(x));
-}
-
-void chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
-    put_be32(out, static_cast<uint32_t>(data.size()));
-    std::vector<uint8_t> td(type, type + 4);
-    td.insert(td.end(), data.begin(), data.end());
-    out.insert(out.end(), td.begin(), td.end());
-    put_be32(out, crc32_update(0, td.data(), td.size()));
-}
-
-void write_png(const std::string& path, const uint8_t* rgb, int w, int h, std::string* err) {
-    std::vector<uint8_t> raw;
-    raw.reserve(static_cast<size_t>(h) * (1 + 3 * w));
-    for (int y = 0; y < h; ++y) {
-        raw.push_back(0);
-        raw.insert(raw.end(), rgb + static_cast<size_t>(y) * 3 * w, rgb + static_cast<size_t>(y + 1) * 3 * w);
-    }
-    std::vector<uint8_t> z = {0x78, 0x01};
-    uint32_t a = 1, b = 0;
-    for (uint8_t c : raw) {
-        a = (a + c) % 65521;
-        b = (b + a) % 65521;
-    }
-    size_t pos = 0;
-    do {
-        const size_t n = std::min<size_t>(65535, raw.size() - pos);
-        const bool last = pos + n == raw.size();
-        z.push_back(last ? 1 : 0);
-        z.push_back(static_cast<uint8_t>(n));
-        z.push_back(static_cast<uint8_t>(n >> 8));
-        z.push_back(static_cast<uint8_t>(~n));
-        z.push_back(static_cast<uint8_t>(~n >> 8));
-        z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + n);
-        pos += n;
-    } while (pos < raw.size());
-    put_be32(z, (b << 16) | a);
-    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
-    std::vector<uint8_t> ihdr;
-    put_be32(ihdr, static_cast<uint32_t>(w));
-    put_be32(ihdr, static_cast<uint32_t>(h));
-    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});
-    chunk(png, "IHDR", ihdr);
-    chunk(png, "IDAT", z);
-    chunk(png, "IEND", {});
-    std::ofstream f(path, std::ios::binary);
-    if (!f) { *err = "cannot write " + path; return; }
-    f.write(reinterpret_cast<const char*>(png.data()), static_cast<std::streamsize>(png.size()));
-    if (!f) *err = "short write " + path;
-}
-
-}  // namespace

@@ -7,11 +7,13 @@
 //   Triangle::isIntersecting   (triangle.cpp:54-89)
 //   Rectangle::isIntersecting  (rectangle.cpp:49-75)
 //   Sphere::isIntersecting     (sphere.cpp:42-78, 146-171)
-// Box culling differs from the reference's slab test (tAabb.ipp:68-127): it is
-// the FMA form (lo * inv - o * inv) with a 4-ulp widening of the far distance,
-// over 8-bit quantized boxes that contain the tight boxes (which contain each
-// primitive's 1e-4-padded bound), so it never culls a primitive the reference
-// would hit.
+// Box culling follows the reference's slab test (tAabb.ipp:68-127) in form,
+// (plane - o) * invDir per plane, with plane - o formed as q * step + (node
+// origin - o) so that its rounding scales with the distances near the node, as
+// the reference's (min - o) does, and not with the size of the coordinates; the
+// far distance is widened by 4 ulps; the boxes are 8-bit quantized boxes that
+// contain the tight boxes (which contain each primitive's 1e-4-padded bound),
+// so it does not cull a primitive the reference would hit.
 #pragma once
 
 #include "dmath.hpp"
@@ -122,7 +124,7 @@ struct LaneStack {
 };
 
 // Direction reciprocal for culling: zero components become +-1e-30 so that
-// lo * inv - o * inv stays finite.
+// (lo - o) * inv stays finite (0 when the ray lies in the plane).
 __device__ __forceinline__ V3 safe_inv(V3 d) {
     auto one = [](float x) {
         const float ax = fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x;
@@ -134,7 +136,7 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
 // Traversal state of one lane: the ray, its running closest distance and
 // hit, the node to visit next (-1: pop the stack) and the stack depth.
 struct LaneRay {
-    V3      o, d, inv, oi;
+    V3      o, d, inv;
     float   tmin, tmax;
     int32_t hit;
     int32_t node;
@@ -246,7 +248,8 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             wd[4 * k + 2] = v.z;
             wd[4 * k + 3] = v.w;
         }
-        const float ox = __uint_as_float(wd[0]), oy = __uint_as_float(wd[1]), oz = __uint_as_float(wd[2]);
+        const float dox = __uint_as_float(wd[0]) - r.o.x, doy = __uint_as_float(wd[1]) - r.o.y,
+                    doz = __uint_as_float(wd[2]) - r.o.z;
         const float sx = __uint_as_float((wd[3] & 0xFFu) << 23), sy = __uint_as_float(((wd[3] >> 8) & 0xFFu) << 23),
                     sz = __uint_as_float(((wd[3] >> 16) & 0xFFu) << 23);
         auto qb = [&](int plane, int c) {
@@ -258,12 +261,15 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         for (int c = 0; c < W; ++c) {
             child[c] = static_cast<int32_t>(wd[4 + c]);
             count[c] = static_cast<int32_t>((wd[CW + c / 2] >> (16 * (c & 1))) & 0xFFFFu);
-            const float lox = __builtin_fmaf(qb(0, c), sx, ox), loy = __builtin_fmaf(qb(1, c), sy, oy),
-                        loz = __builtin_fmaf(qb(2, c), sz, oz), hix = __builtin_fmaf(qb(3, c), sx, ox),
-                        hiy = __builtin_fmaf(qb(4, c), sy, oy), hiz = __builtin_fmaf(qb(5, c), sz, oz);
-            const float tx0 = __builtin_fmaf(lox, r.inv.x, -r.oi.x), tx1 = __builtin_fmaf(hix, r.inv.x, -r.oi.x);
-            const float ty0 = __builtin_fmaf(loy, r.inv.y, -r.oi.y), ty1 = __builtin_fmaf(hiy, r.inv.y, -r.oi.y);
-            const float tz0 = __builtin_fmaf(loz, r.inv.z, -r.oi.z), tz1 = __builtin_fmaf(hiz, r.inv.z, -r.oi.z);
+            // plane - origin as q * step + (node origin - ray origin): the rounding is
+            // relative to the distances near the node, as in the reference's
+            // (min - o) * invDir, not to the size of the coordinates
+            const float lox = __builtin_fmaf(qb(0, c), sx, dox), loy = __builtin_fmaf(qb(1, c), sy, doy),
+                        loz = __builtin_fmaf(qb(2, c), sz, doz), hix = __builtin_fmaf(qb(3, c), sx, dox),
+                        hiy = __builtin_fmaf(qb(4, c), sy, doy), hiz = __builtin_fmaf(qb(5, c), sz, doz);
+            const float tx0 = lox * r.inv.x, tx1 = hix * r.inv.x;
+            const float ty0 = loy * r.inv.y, ty1 = hiy * r.inv.y;
+            const float tz0 = loz * r.inv.z, tz1 = hiz * r.inv.z;
             const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
             const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
             hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));

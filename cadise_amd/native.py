@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpt.so")
 
 MPT_OK = 0
+MPT_ERR_INVALID, MPT_ERR_PARSE, MPT_ERR_UNSUPPORTED, MPT_ERR_DEVICE, MPT_ERR_IO, MPT_ERR_NOMEM = -1, -2, -3, -4, -5, -6
 ERRORS = {-1: "invalid argument", -2: "parse error", -3: "unsupported", -4: "device error", -5: "I/O error",
           -6: "out of memory"}
 
@@ -102,7 +103,8 @@ class BatchTrial(C.Structure):
 EXPORTS = ["mpt_create", "mpt_destroy", "mpt_last_error", "mpt_version", "mpt_scene_load_crsd", "mpt_scene_parse_crsd",
            "mpt_scene_create", "mpt_scene_destroy", "mpt_scene_stats_get", "mpt_scene_binary_bvh", "mpt_scene_camera",
            "mpt_scene_light_radiance", "mpt_scene_light_pick_pdf", "mpt_trace",
-           "mpt_trace_device", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_save", "mpt_stats_get",
+           "mpt_trace_device", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_srgb8_device", "mpt_film_save",
+           "mpt_scene_output_filename", "mpt_image_write", "mpt_stats_get",
            "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_get_texture", "mpt_scene_set_light_radiance",
            "mpt_render_adjoint",
            "mpt_batch_create", "mpt_batch_destroy", "mpt_batch_render", "mpt_batch_film", "mpt_batch_autotune",
@@ -140,6 +142,9 @@ def load_library():
     L.mpt_film_device.argtypes = [vp, C.POINTER(vp)]
     L.mpt_film_srgb8.argtypes = [vp, vp]
     L.mpt_film_save.argtypes = [vp, C.c_char_p]
+    L.mpt_film_srgb8_device.argtypes = [vp, vp, i64, vp, vp]
+    L.mpt_scene_output_filename.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.mpt_image_write.argtypes = [C.c_char_p, vp, i32, i32]
     L.mpt_stats_get.argtypes = [vp, C.POINTER(Stats)]
     L.mpt_scene_texture_info.argtypes = [vp, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
     L.mpt_scene_set_texture.argtypes = [vp, i32, vp]

@@ -99,6 +99,9 @@ class SceneSpec:
     filter: str = "box"
     filter_width: float | None = None
     filter_height: float | None = None
+    # other filter keys of the film block ("sigma-x", "sigma-y", "amplitude",
+    # "b", "c"), absent = the reference's defaults (filterInstantiator.cpp:16-57)
+    filter_params: dict = field(default_factory=dict)
     tile: int = 16
     cam_pos: tuple = (0.0, 0.0, 5.0)
     cam_dir: tuple = (0.0, 0.0, -1.0)
@@ -172,6 +175,8 @@ class SceneSpec:
             flt.append(f'[filter-width real "{_f(self.filter_width)}"]')
         if self.filter_height is not None:
             flt.append(f'[filter-height real "{_f(self.filter_height)}"]')
+        for k, v in self.filter_params.items():
+            flt.append(f'[{k} real "{_f(v)}"]')
         out.append("film {\n"
                    f'    [image-width int32 "{self.width}"]\n    [image-height int32 "{self.height}"]\n'
                    f'    [tile-size-x int32 "{self.tile}"]\n    [tile-size-y int32 "{self.tile}"]\n'

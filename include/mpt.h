@@ -226,7 +226,11 @@ int  mpt_scene_light_radiance(const mpt_scene* scene, float* out);
  * EnvironmentLight::approximateFlux (environmentLight.cpp:125-139). */
 int  mpt_scene_light_pick_pdf(const mpt_scene* scene, float* out);
 
-/* Batch ray queries; host buffers (copied) or device buffers on a stream. */
+/* Batch ray queries; host buffers (copied) or device buffers on a stream.
+ * At most MPT_TRACE_MAX_RAYS rays per call (2^31 - 2^24: the traversal's 32-bit
+ * work counter, which each wave may advance past n by one reservation, stays
+ * below 2^31); more returns MPT_ERR_INVALID before any device work. */
+#define MPT_TRACE_MAX_RAYS ((size_t)2147483648u - (size_t)16777216u)
 int  mpt_trace(mpt_scene* scene, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits);
 int  mpt_trace_device(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
                       void* stream);
@@ -236,10 +240,27 @@ int  mpt_trace_device(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int a
 int  mpt_render(mpt_scene* scene, const mpt_render_params* params, float* film_host);
 /* Device pointer of the scene's linear film (W x H x float4). */
 int  mpt_film_device(mpt_scene* scene, void** film_dev);
-/* Film::save: sum w*rgb / sum w, sRGB OETF, 8-bit (W x H x 3). */
+/* Film::save's resolve (film.cpp:159-196, imageUtils.cpp:23-36): sum w*rgb / sum w,
+ * sRGB OETF (math.cpp:92-99), 8-bit (W x H x 3, top row first). */
 int  mpt_film_srgb8(mpt_scene* scene, uint8_t* out);
-/* Writes the film as 8-bit PNG / PPM or linear PFM, by the file extension. */
+/* The same resolve of a film already in device memory (npix float4: sum w*rgb,
+ * sum w) into npix x 3 device bytes, queued on stream (NULL: the default stream);
+ * for callers that keep their films on the GPU. */
+int  mpt_film_srgb8_device(mpt_ctx* ctx, const float* film_dev, int64_t npix, uint8_t* out_dev, void* stream);
+/* Film::save (film.cpp:159-196) + PictureSaver::save (pictureSaver.cpp:12-82):
+ * writes the film by the file extension: .png, .jpg / .jpeg (baseline JPEG,
+ * quality 92), .ppm (the reference's "P6 w h 255" header), and .pfm (linear,
+ * not in the reference). path NULL: the film block's output-filename (default
+ * "cadise.jpg", filmInstantiator.cpp:24). Any other extension:
+ * MPT_ERR_UNSUPPORTED (the reference then writes nothing). */
 int  mpt_film_save(mpt_scene* scene, const char* path);
+/* The film block's output-filename. *needed (if non-NULL) receives its length + 1;
+ * buf (if non-NULL) receives it NUL-terminated when n >= that length. */
+int  mpt_scene_output_filename(const mpt_scene* scene, char* buf, size_t n, size_t* needed);
+/* PictureSaver::saveLdrImage (pictureSaver.cpp:30-84) of 8-bit RGB (height rows of
+ * width x 3 bytes, top row first) by extension (.png, .jpg / .jpeg, .ppm). Host
+ * only: needs no context or device. */
+int  mpt_image_write(const char* path, const uint8_t* rgb, int32_t width, int32_t height);
 
 int  mpt_stats_get(const mpt_scene* scene, mpt_stats* out);
 

@@ -91,6 +91,8 @@ def oracle_lib():
         L.or_time_render.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_uint64, _i64p, _i64p]
         L.or_philox.argtypes = [np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")] * 3
         L.or_film_to_srgb8.argtypes = [_f32p, C.c_int64, _u8p]
+        L.or_srgb8_thresholds.restype = C.c_int
+        L.or_srgb8_thresholds.argtypes = [np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS"), C.c_int]
         L.or_adjoint.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int, _f32p, _f32p,
                                  C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_set_light_radiance.argtypes = [C.c_void_p, C.c_int, _f32p]
@@ -191,9 +193,11 @@ class OracleScene:
         d.filter = _FILTERS[spec.filter]
         d.filter_w = spec.filter_width if spec.filter_width is not None else _FILTER_W[spec.filter]
         d.filter_h = spec.filter_height if spec.filter_height is not None else _FILTER_W[spec.filter]
-        d.sigma_x = d.sigma_y = 0.5
-        d.amplitude = 1.0
-        d.mitchell_b = d.mitchell_c = float(np.float32(1.0) / np.float32(3.0))
+        fp = spec.filter_params
+        third = float(np.float32(1.0) / np.float32(3.0))
+        d.sigma_x, d.sigma_y = fp.get("sigma-x", 0.5), fp.get("sigma-y", 0.5)
+        d.amplitude = fp.get("amplitude", 1.0)
+        d.mitchell_b, d.mitchell_c = fp.get("b", third), fp.get("c", third)
         d.sampler = 1 if spec.sampler == "stratified" else 0
         d.spp, d.max_depth = spec.spp, spec.max_depth
         d.split_sah = 0 if spec.split_mode == "equal" else 1
@@ -364,6 +368,17 @@ def film_to_srgb8(film):
     return out
 
 
+def srgb8_thresholds(threads=8):
+    """The byte function of a resolved linear value v as a table: entry k is the
+    bit pattern of the smallest float v >= +0 whose 8-bit sRGB value is >= k
+    (entry 0 is 0). The oracle scans every float in [+0, 1] and raises if the
+    byte is not monotone in v there; below +0 the byte is 0 and above 1 it is 255."""
+    t = np.zeros(256, np.uint32)
+    if oracle_lib().or_srgb8_thresholds(t, threads) != 0:
+        raise RuntimeError("8-bit sRGB byte is not monotone on [0, 1]")
+    return t
+
+
 # -- the compiled reference ------------------------------------------------------------
 
 _ref = None
@@ -414,6 +429,7 @@ def ref_lib():
         L.ref_light_sample.argtypes = [C.c_void_p, C.c_int, _f32p, C.c_int64, C.c_int, _f32p]
         L.ref_light_hit.argtypes = [C.c_void_p, _f32p, C.c_int64, _f32p]
         L.ref_light_flux.argtypes = [C.c_void_p, _f32p]
+        L.ref_film_save.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_char_p]
         _ref = L
     return _ref
 
@@ -522,6 +538,14 @@ class RefScene:
         out = np.zeros((max(1, self.nlights), 2), np.float32)
         self.L.ref_light_flux(self.h, out)
         return out[: self.nlights]
+
+    def film_save(self, film, spp, path):
+        """Film::save of a linear film (H x W x 4) to path; the file's bytes."""
+        film = np.ascontiguousarray(film, np.float32)
+        assert film.shape == (self.height, self.width, 4)
+        self.L.ref_film_save(self.h, film, int(spp), path.encode())
+        with open(path, "rb") as f:
+            return f.read()
 
 
 def seq_key(seed: int, a: int, b: int) -> int:

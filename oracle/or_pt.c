@@ -1648,6 +1648,69 @@ void or_film_to_srgb8(const float* film, int64_t npix, uint8_t* out) {
     }
 }
 
+/* The byte of one resolved linear value, as or_film_to_srgb8 forms it. */
+static uint8_t srgb8_of(float lin) {
+    return (uint8_t)m_clamp(gamma_fwd(lin) * 255.0f + 0.5f, 0.0f, 255.0f);
+}
+
+/* Exhaustive scan of the byte function over every float bit pattern in [+0, 1]
+ * (0x00000000 .. 0x3F800000), split into nthreads contiguous ranges. Each range
+ * records its first byte, its last byte, whether the byte ever decreases inside
+ * it, and the first pattern at which each byte value is reached. */
+typedef struct {
+    uint32_t a, b;
+    uint32_t first[256];
+    int      seen[256];
+    int      monotone;
+    uint8_t  b0, b1;
+} srgb_scan;
+
+static void* srgb_scan_run(void* p) {
+    srgb_scan* J = (srgb_scan*)p;
+    J->monotone = 1;
+    uint8_t prev = 0;
+    for (uint32_t u = J->a; u < J->b; ++u) {
+        float v;
+        memcpy(&v, &u, 4);
+        const uint8_t by = srgb8_of(v);
+        if (u == J->a) J->b0 = by;
+        else if (by < prev) J->monotone = 0;
+        if (!J->seen[by]) { J->seen[by] = 1; J->first[by] = u; }
+        prev = by;
+    }
+    J->b1 = prev;
+    return NULL;
+}
+
+int or_srgb8_thresholds(uint32_t* out, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    const uint64_t lo = 0, hi = 0x3F800001ull;   /* [+0, 1] inclusive */
+    srgb_scan* J = (srgb_scan*)calloc((size_t)nthreads, sizeof(srgb_scan));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    for (int i = 0; i < nthreads; ++i) {
+        J[i].a = (uint32_t)(lo + (hi - lo) * (uint64_t)i / (uint64_t)nthreads);
+        J[i].b = (uint32_t)(lo + (hi - lo) * (uint64_t)(i + 1) / (uint64_t)nthreads);
+        pthread_create(&th[i], NULL, srgb_scan_run, &J[i]);
+    }
+    int ok = 1;
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    for (int k = 0; k < 256; ++k) out[k] = 0xFFFFFFFFu;
+    for (int i = 0; i < nthreads; ++i) {
+        ok &= J[i].monotone;
+        if (i > 0 && J[i].b0 < J[i - 1].b1) ok = 0;
+        for (int k = 0; k < 256; ++k)
+            if (J[i].seen[k] && J[i].first[k] < out[k]) out[k] = J[i].first[k];
+    }
+    /* a byte value the scan never produced (skipped by a jump of two) starts
+     * where the next one does */
+    for (int k = 254; k >= 0; --k)
+        if (out[k] == 0xFFFFFFFFu) out[k] = out[k + 1];
+    out[0] = 0;
+    free(J); free(th);
+    return ok ? 0 : -1;
+}
+
 /* ---- path-replay gradient, FP64 host reference -------------------------------------------
  * Not a reference function: the reference has no derivatives. This is the host check of
  * the GPU adjoint. Each path of the Philox layout is re-run exactly as estimate() runs it

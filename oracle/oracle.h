@@ -181,6 +181,11 @@ void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 
 /* Film resolve to 8-bit sRGB as Film::save + ImageUtils::hdrToLdr. */
 void or_film_to_srgb8(const float* film, int64_t npix, uint8_t* out);
+/* The 8-bit byte of a resolved linear value v as a threshold table: out[k] = bit
+ * pattern of the smallest float in [+0, 1] whose byte is >= k (out[0] = 0), from an
+ * exhaustive scan of [+0, 1] over nthreads threads; returns -1 if the byte is not
+ * monotone there. */
+int or_srgb8_thresholds(uint32_t* out, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -620,6 +620,27 @@ void ref_load_rgb_image(const char* path, int32_t* width, int32_t* height, float
     for (size_t i = 0; i < img.dataSize(); ++i) out[i] = d[i];
 }
 
+// Film::save (film.cpp:159-196) of a given linear film (w*h*4: sum of w*rgb, sum
+// of w per pixel) to path, through the scene's own film settings and filter: the
+// sensors are loaded into one film tile spanning the image and merged into a
+// fresh Film (mergeWithFilmTile adds them to zero sensors), which then writes
+// the picture by its extension (PictureSaver::save, pictureSaver.cpp:12-82).
+void ref_film_save(void* h, const float* film, int64_t spp, const char* path) {
+    auto* s = static_cast<RefScene*>(h);
+    Film out(Vector2I(s->width, s->height), Vector2I(s->tileX, s->tileY), Path(std::string(path)), s->filter);
+    auto tile = std::make_unique<FilmTile>(AABB2I(Vector2I(0, 0), Vector2I(s->width, s->height)), s->filter.get());
+    for (int iy = 0; iy < s->height; ++iy) {
+        for (int ix = 0; ix < s->width; ++ix) {
+            const float* p = film + 4 * (static_cast<size_t>(iy) * s->width + ix);
+            auto& sensor = const_cast<RgbRadianceSensor&>(tile->getSensor(ix, iy));
+            sensor.addValue(p[0], p[1], p[2]);
+            sensor.addWeight(p[3]);
+        }
+    }
+    out.mergeWithFilmTile(std::move(tile));
+    out.save(static_cast<std::size_t>(spp));
+}
+
 // Light::approximateFlux and LightCluster::evaluatePickLightPdf of every light.
 // out: nlights x 2 (flux, pick pdf).
 void ref_light_flux(void* h, float* out) {

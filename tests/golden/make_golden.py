@@ -72,6 +72,74 @@ def scene_set():
             ("zoo_sky", zoo_sky), ("checker", checker)]
 
 
+def filter_scenes():
+    """Film-only golden scenes, one per filter kind and key set of the film block
+    (filterInstantiator.cpp:16-57): defaults and explicit keys, anisotropic
+    footprints, and 8 x 8 tiles (the footprint is clamped to the tile)."""
+    def cb(filt, **kw):
+        s = scenes.cornell_box(width=24, height=20, spp=4, max_depth=5, filter=filt)
+        for k, v in kw.items():
+            setattr(s, k, v)
+        return s
+    return [
+        ("mitchell", cb("mitchell")),
+        ("mitchell_keys", cb("mitchell", filter_width=3.2, filter_height=2.6, filter_params={"b": 0.25, "c": 0.375})),
+        ("mitchell_tile8", cb("mitchell", tile=8, filter_params={"b": 1.0, "c": 0.0})),
+        ("cone", cb("cone")),
+        ("cone_keys", cb("cone", filter_width=3.0, filter_height=2.2, tile=8)),
+        ("gaussian_keys", cb("gaussian", filter_width=3.0, filter_height=5.0,
+                             filter_params={"sigma-x": 0.7, "sigma-y": 0.4, "amplitude": 1.5})),
+        ("box_wide", cb("box", filter_width=2.5, filter_height=1.5)),
+    ]
+
+
+def srgb8_film(width=64, height=48, seed=17):
+    """A linear film (H x W x 4) for the 8-bit save: for every byte boundary k the
+    four floats around the oracle's threshold (v = T[k] - 2 .. T[k] + 1 ulp, weight
+    1), then random sums and weights (zero, negative and large weights, values
+    beyond 1 and below 0, subnormals)."""
+    from oracle.binding import srgb8_thresholds
+    rng = np.random.default_rng(seed)
+    n = width * height
+    t = srgb8_thresholds().astype(np.int64)
+    near = np.concatenate([t[1:] + d for d in (-2, -1, 0, 1)]).astype(np.uint32).view(np.float32)
+    film = np.zeros((n, 4), np.float32)
+    w = rng.uniform(0.25, 8.0, n).astype(np.float32)
+    w[rng.random(n) < 0.05] = 0.0
+    w[rng.random(n) < 0.05] = -1.5
+    w[rng.random(n) < 0.02] = 3.0e4
+    film[:, 3] = w
+    film[:, :3] = (rng.uniform(-0.2, 1.6, (n, 3)) * w[:, None]).astype(np.float32)
+    film[rng.random(n) < 0.02, :3] = np.float32(1.0e-41)
+    vals = film[:, :3].reshape(-1)
+    vals[: len(near)] = near
+    film[: (len(near) + 2) // 3, 3] = 1.0
+    return film.reshape(height, width, 4)
+
+
+def write_film_goldens(out_dir):
+    """filters.npz (linear films of the filter scenes) and srgb8.npz (Film::save's
+    PPM, PNG and JPG files of a crafted film and of a rendered one)."""
+    out = {}
+    for name, spec in filter_scenes():
+        R = RefScene(spec.to_crsd())
+        out[name] = R.render_linear(seed=31, threads=4)
+    np.savez_compressed(os.path.join(out_dir, "filters.npz"), **out)
+    spec = scenes.cornell_box(width=64, height=48, spp=16, max_depth=6, filter="gaussian")
+    R = RefScene(spec.to_crsd())
+    crafted = srgb8_film(spec.width, spec.height)
+    render = R.render_linear(seed=5, threads=4)
+    g = {"crafted": crafted, "render": render}
+    side = tempfile.mkdtemp()
+    for key, film in (("crafted", crafted), ("render", render)):
+        for ext in ("ppm", "png", "jpg"):
+            data = R.film_save(film, spec.spp, os.path.join(side, f"{key}.{ext}"))
+            g[f"{key}_{ext}"] = np.frombuffer(data, np.uint8)
+    np.savez_compressed(os.path.join(out_dir, "srgb8.npz"), **g)
+    for f in ("filters.npz", "srgb8.npz"):
+        print(os.path.join(out_dir, f), os.path.getsize(os.path.join(out_dir, f)), "bytes")
+
+
 SHIPPED = "/root/reference/scene/cornell-box.crsd"
 
 
@@ -101,6 +169,8 @@ def main():
     if not ref_available():
         raise SystemExit("oracle/_ref/libcadise_ref.so is not built (make -C oracle ref)")
     only = set(sys.argv[1:])
+    if not only or "films" in only:
+        write_film_goldens(HERE)
     for name, spec in scene_set():
         if only and name not in only:
             continue

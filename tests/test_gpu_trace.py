@@ -1,7 +1,7 @@
 """GPU parity of the traversal (Accelerator::isIntersecting / isOccluded,
 reference bvhAccelerator.cpp:40-156) through the C ABI against the oracle,
 whose binary traversal reproduces the compiled reference bit for bit
-(tests/test_oracle_vs_ref.py).
+(tests/test_oracle_golden.py).
 
 Bar: closest-hit primitive id identical and t bit-identical on every ray whose
 closest hit is unique; rays with an exact-t tie (several primitives at the
@@ -103,6 +103,25 @@ def test_trace_edge_cases(ctx):
     _check_any(sc, orc, below, "maxT < t any")
     prim, t0 = sc.trace(np.zeros((0, 8), np.float32))
     assert len(prim) == 0
+
+
+@pytest.mark.parametrize("offset", [1.0e4, 1.0e5])
+def test_trace_far_from_origin(ctx, tmp_path, offset):
+    """The sphere scene translated to (offset, offset, offset), rays starting
+    there: the slab test's rounding must scale with the distances near the
+    boxes, not with the coordinates (at 1e5 the float spacing is 2^-7, far above
+    the primitives' 1e-4 padding, which then rounds away)."""
+    from cadise_amd.api import Scene, bvh_opts
+    spec = small_spheres(n_spheres=8, level=3, offset=(offset, offset, offset))
+    orc = OracleScene(spec)
+    lo, hi = (offset - 18, offset, offset - 18), (offset + 18, offset + 8, offset + 18)
+    rays = np.concatenate([random_rays(60000, lo, hi, seed=15), camera_rays(spec, 20000, seed=16)])
+    for width in (4, 8):
+        sc = Scene.from_crsd(ctx, _write(tmp_path, spec), bvh_opts(width=width))
+        _check_closest(sc, orc, rays, f"offset {offset:g} bvh{width}")
+        short = rays.copy()
+        short[:, 7] = np.random.default_rng(17).uniform(0.1, 20.0, len(rays)).astype(np.float32)
+        _check_any(sc, orc, short, f"offset {offset:g} bvh{width} any")
 
 
 def _write(tmp_path, spec):

@@ -229,3 +229,23 @@ def test_wide_collapse_keeps_oversized_binary_leaves(ctx):
         for leaf_max in (1, 2):
             st = Scene.from_spec(ctx, spec, bvh_opts(width=width, leaf_max=leaf_max)).stats()
             assert st["wide_nodes"] > 70000 // (2 * width), (width, leaf_max, st["wide_nodes"])
+
+
+def test_trace_ray_count_cap(ctx):
+    """mpt_trace / mpt_trace_device refuse more than MPT_TRACE_MAX_RAYS = 2^31 - 2^24
+    rays (the traversal's 32-bit work counter stays below 2^31 even when every
+    wave over-reserves once) before any device work; at the cap the host-only
+    context gets as far as its device check. The buffers are never read."""
+    import ctypes as C
+    header = open(os.path.join(ROOT, "include", "mpt.h")).read()
+    assert "#define MPT_TRACE_MAX_RAYS ((size_t)2147483648u - (size_t)16777216u)" in header
+    cap = (1 << 31) - (1 << 24)
+    sc = Scene.from_spec(ctx, scenes.cornell_box(width=8, height=8, spp=1))
+    L = native.load_library()
+    dummy = np.zeros(16, np.float32)
+    for fn, args in ((L.mpt_trace, lambda n: (sc.h, dummy.ctypes.data, n, 0, dummy.ctypes.data)),
+                     (L.mpt_trace_device, lambda n: (sc.h, dummy.ctypes.data, n, 0, dummy.ctypes.data, None))):
+        assert fn(*args(cap + 1)) == native.MPT_ERR_INVALID
+        assert fn(*args(1 << 40)) == native.MPT_ERR_INVALID
+        assert fn(*args(cap)) == native.MPT_ERR_DEVICE
+    assert L.mpt_trace(sc.h, None, 5, 0, None) == native.MPT_ERR_INVALID
