@@ -160,13 +160,17 @@ class Scene:
         d.filter = _FILTERS[spec.filter]
         d.filter_w = spec.filter_width if spec.filter_width is not None else _FILTER_W[spec.filter]
         d.filter_h = spec.filter_height if spec.filter_height is not None else _FILTER_W[spec.filter]
-        d.sigma_x = d.sigma_y = 0.5
-        d.amplitude = 1.0
-        d.mitchell_b = d.mitchell_c = float(np.float32(1.0) / np.float32(3.0))
+        fp = spec.filter_params
+        third = float(np.float32(1.0) / np.float32(3.0))
+        d.sigma_x, d.sigma_y = fp.get("sigma-x", 0.5), fp.get("sigma-y", 0.5)
+        d.amplitude = fp.get("amplitude", 1.0)
+        d.mitchell_b, d.mitchell_c = fp.get("b", third), fp.get("c", third)
         d.sampler = 1 if spec.sampler == "stratified" else 0
         d.spp, d.max_depth = spec.spp, spec.max_depth
         d.split_sah = 0 if spec.split_mode == "equal" else 1
         d.estimator = 1 if spec.estimator == "vpt" else 0
+        out_name = spec.output.encode()
+        d.output_filename = out_name
         h = C.c_void_p()
         ctx.check(ctx.L.mpt_scene_create(ctx.h, C.byref(d), C.byref(opts) if opts else None, C.byref(h)))
         del keep

@@ -1074,6 +1074,7 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
     h.spp = d->spp;
     h.max_depth = d->max_depth;
     h.split_sah = d->split_sah;
+    if (d->output_filename) h.output = d->output_filename;
     return finish_scene(ctx, S, opts, out);
 }
 

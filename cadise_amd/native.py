@@ -49,7 +49,7 @@ class SceneDesc(C.Structure):
                 ("filter", C.c_int32), ("filter_w", C.c_float), ("filter_h", C.c_float), ("sigma_x", C.c_float),
                 ("sigma_y", C.c_float), ("amplitude", C.c_float), ("mitchell_b", C.c_float),
                 ("mitchell_c", C.c_float), ("sampler", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
-                ("split_sah", C.c_int32), ("estimator", C.c_int32)]
+                ("split_sah", C.c_int32), ("estimator", C.c_int32), ("output_filename", C.c_char_p)]
 
 
 class BvhOpts(C.Structure):

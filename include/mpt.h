@@ -131,6 +131,7 @@ typedef struct {
     int32_t spp, max_depth;
     int32_t split_sah;         /* 1 sah, 0 equal */
     int32_t estimator;         /* MPT_ESTIMATOR_*: pt (NEE + MIS) or vpt (BSDF sampling only) */
+    const char* output_filename; /* the film's output-filename; NULL: "cadise.jpg" */
 } mpt_scene_desc;
 
 /* Accelerator options (the reference's accelerator block plus the wide-BVH

@@ -96,3 +96,22 @@ def test_batch_autotune(ctx, batch_set):
     assert len(trials) >= 6 and b.opts == best
     bst = b.render(seeds)
     _check(b, batch_set, bst, "autotuned", best["width"])
+
+
+def test_batch_films_vs_oracle(ctx, batch_set):
+    """Each film of a batched render against the oracle's render of that scene
+    with the GPU's random layout (Philox per pixel / sample / bounce / slot): per
+    pixel within 1e-4 relative (+1e-6) on >= 98% of pixels, image mean within 1e-3."""
+    from cadise_amd.api import Batch, film_mean
+    from oracle.binding import OR_RNG_PHILOX, OracleScene
+    specs, seeds, out = batch_set
+    b = Batch(ctx, out[4][0])
+    b.render(seeds)
+    for i, (spec, sd) in enumerate(zip(specs, seeds)):
+        g = film_mean(b.film(i))
+        o = film_mean(OracleScene(spec).render_linear(sd, rng_mode=OR_RNG_PHILOX))
+        close = np.all(np.abs(g - o) <= 1e-4 * np.abs(o) + 1e-6, axis=-1)
+        rel = abs(g.mean() - o.mean()) / max(o.mean(), 1e-12)
+        print(f"[batch vs oracle] scene {i} ({spec.width}x{spec.height}, {spec.spp} spp, depth {spec.max_depth}): "
+              f"close={close.mean():.5f} rel={rel:.2e}", flush=True)
+        assert close.mean() >= 0.98 and rel < 1e-3, i

@@ -22,6 +22,7 @@ import pytest
 from cadise_amd import scenes
 from oracle.binding import OR_RNG_PHILOX, OracleScene, film_to_srgb8, srgb8_thresholds
 from tests.golden.make_golden import filter_scenes
+from tests.hipmem import DeviceBuffer, synchronize
 from tests.pictures import decode_jpeg, jpeg_tables, read_png_rgb8, read_ppm
 
 pytestmark = pytest.mark.gpu
@@ -50,43 +51,46 @@ def test_filter_film_vs_oracle(ctx, name):
         np.testing.assert_allclose(film[..., 3], orf[..., 3], rtol=1e-5, atol=1e-6)
 
 
-def _srgb8_gpu(ctx, film_t):
-    import torch
+def _srgb8_gpu(ctx, film):
+    """mpt_film_srgb8_device of a host film (n x 4 float32) -> n x 3 bytes."""
     from cadise_amd.api import srgb8_device
-    n = film_t.shape[0]
-    out = torch.empty(3 * n, dtype=torch.uint8, device=film_t.device)
-    srgb8_device(ctx, film_t.data_ptr(), n, out.data_ptr())
-    torch.cuda.synchronize()
+    film = np.ascontiguousarray(film, np.float32).reshape(-1, 4)
+    n = len(film)
+    out = np.zeros((n, 3), np.uint8)
+    with DeviceBuffer(film.nbytes) as f, DeviceBuffer(out.nbytes) as o:
+        f.upload(film)
+        srgb8_device(ctx, f.ptr.value, n, o.ptr.value)
+        synchronize()
+        o.download(out)
     return out
 
 
 def test_srgb8_every_float_in_unit_interval(ctx):
-    import torch
-    t = torch.from_numpy(srgb8_thresholds().astype(np.int64)).cuda()
+    t = srgb8_thresholds().astype(np.int64)
     top = 0x3F800000   # 1.0f
     chunk = 3 * (1 << 25)
     mism = 0
     for start in range(0, top + 1, chunk):
         stop = min(top + 1, start + chunk)
-        n = stop - start
-        npix = (n + 2) // 3
-        bits = torch.arange(start, start + 3 * npix, dtype=torch.int64, device="cuda").clamp_(max=top)
-        film = torch.ones((npix, 4), dtype=torch.float32, device="cuda")
-        film[:, :3] = bits.to(torch.int32).view(torch.float32).view(npix, 3)
-        got = _srgb8_gpu(ctx, film.view(-1, 4)).to(torch.int64)
-        want = torch.searchsorted(t, bits, right=True) - 1
-        bad = int((got != want).sum())
-        if bad:
-            i = int(torch.nonzero(got != want)[0])
+        npix = (stop - start + 2) // 3
+        bits = np.minimum(np.arange(start, start + 3 * npix, dtype=np.int64), top)
+        film = np.ones((npix, 4), np.float32)
+        film[:, :3] = bits.astype(np.uint32).view(np.float32).reshape(npix, 3)
+        got = _srgb8_gpu(ctx, film).reshape(-1)
+        # expected byte of every bit pattern: runs between consecutive thresholds
+        edges = np.clip(np.concatenate([t, [top + 1]]), start, start + 3 * npix)
+        want = np.repeat(np.arange(256, dtype=np.uint8), np.diff(edges))
+        want = np.concatenate([want, np.full(3 * npix - len(want), 255, np.uint8)])   # the clamped tail is 1.0f
+        bad = np.flatnonzero(got != want)
+        if len(bad):
+            i = bad[0]
             print(f"first mismatch at bits {int(bits[i]):#x}: gpu {int(got[i])} oracle {int(want[i])}", flush=True)
-        mism += bad
-        del bits, film, got, want
+        mism += len(bad)
     print(f"[srgb8] floats in [0, 1]: {top + 1}, mismatches {mism}", flush=True)
     assert mism == 0
 
 
 def test_srgb8_outside_unit_interval_and_weights(ctx):
-    import torch
     rng = np.random.default_rng(3)
     n = 1 << 20
     film = np.zeros((n, 4), np.float32)
@@ -101,18 +105,16 @@ def test_srgb8_outside_unit_interval_and_weights(ctx):
     w[::7] = 0.0
     film[m:, 3] = w
     film[m:, :3] = (rng.uniform(-0.5, 2.0, (m, 3)) * w[:, None]).astype(np.float32)
-    got = _srgb8_gpu(ctx, torch.from_numpy(film).cuda()).cpu().numpy().reshape(n, 3)
+    got = _srgb8_gpu(ctx, film)
     want = film_to_srgb8(film)
     assert np.array_equal(got, want), int((got != want).sum())
 
 
 def test_srgb8_reference_crafted_film(ctx):
-    import torch
     g = np.load(os.path.join(GOLDEN, "srgb8.npz"))
     for key in ("crafted", "render"):
         _, px = read_ppm(g[key + "_ppm"].tobytes())
-        film = torch.from_numpy(np.ascontiguousarray(g[key]).reshape(-1, 4)).cuda()
-        got = _srgb8_gpu(ctx, film).cpu().numpy().reshape(px.shape)
+        got = _srgb8_gpu(ctx, g[key]).reshape(px.shape)
         assert np.array_equal(got, px), (key, int((got != px).sum()))
 
 

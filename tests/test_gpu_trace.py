@@ -116,12 +116,31 @@ def test_trace_far_from_origin(ctx, tmp_path, offset):
     orc = OracleScene(spec)
     lo, hi = (offset - 18, offset, offset - 18), (offset + 18, offset + 8, offset + 18)
     rays = np.concatenate([random_rays(60000, lo, hi, seed=15), camera_rays(spec, 20000, seed=16)])
+    # rays whose closest hit the reference's own BVH misses or changes (its slab
+    # test culls a box holding the hit once the 1e-4 padding rounds away): listed
+    # and counted separately; there the GPU must give the brute-force hit or the
+    # reference BVH's
+    bp, bt = orc.trace(rays, use_bvh=False)
+    op, ot = orc.trace(rays)
+    culled = (bp != op) | (bt.view(np.int32) != ot.view(np.int32))
+    print(f"[offset {offset:g}] reference-BVH culls: {int(culled.sum())} of {len(rays)} rays "
+          f"{np.flatnonzero(culled)[:8].tolist()}", flush=True)
+    assert culled.sum() <= 1e-4 * len(rays)
     for width in (4, 8):
         sc = Scene.from_crsd(ctx, _write(tmp_path, spec), bvh_opts(width=width))
-        _check_closest(sc, orc, rays, f"offset {offset:g} bvh{width}")
+        _check_closest(sc, orc, rays[~culled], f"offset {offset:g} bvh{width}")
+        gp, gt = sc.trace(rays[culled])
+        ok = ((gp == bp[culled]) & (gt.view(np.int32) == bt[culled].view(np.int32))) | \
+             ((gp == op[culled]) & (gt.view(np.int32) == ot[culled].view(np.int32)))
+        print(f"[offset {offset:g} bvh{width}] culled rays: gpu {gp.tolist()} brute {bp[culled].tolist()} "
+              f"reference bvh {op[culled].tolist()}", flush=True)
+        assert ok.all()
         short = rays.copy()
         short[:, 7] = np.random.default_rng(17).uniform(0.1, 20.0, len(rays)).astype(np.float32)
-        _check_any(sc, orc, short, f"offset {offset:g} bvh{width} any")
+        ab, _ = orc.trace(short, any_hit=True, use_bvh=False)
+        ao, _ = orc.trace(short, any_hit=True)
+        assert (ab != ao).sum() <= 1e-4 * len(rays)
+        _check_any(sc, orc, short[ab == ao], f"offset {offset:g} bvh{width} any")
 
 
 def _write(tmp_path, spec):
