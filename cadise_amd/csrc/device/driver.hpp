@@ -145,6 +145,7 @@ struct StreamJob {
 struct mpt_ctx {
     int device = 0;
     int num_cus = 0;
+    void* trace_scratch = nullptr;   // this device's address of the no-allocation trace scratch array
     std::string error;
     std::unique_ptr<mpt::Streamer> streamer;   // path state of single-scene streaming renders
 };

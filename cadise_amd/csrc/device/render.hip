@@ -16,6 +16,7 @@
 #include <fstream>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -870,6 +871,37 @@ int finish_scene(mpt_ctx* ctx, std::unique_ptr<mpt_scene>& S, const mpt_bvh_opts
     return MPT_OK;
 }
 
+// ---- fixed scratch of mpt_trace_device_noalloc ------------------------------------
+//
+// One array per device, held until the process ends: the traversal's work
+// counter (word 0), its statistics (bytes 256..), and from byte 4096 the
+// per-lane spill slices of traversal stacks deeper than the LDS part. Host code
+// uses it only through the device address hipGetSymbolAddress returns (kept
+// per device in kScratchPtr and in each context); the array's name is not a
+// device address on the host.
+constexpr size_t kScratchBytes = MPT_TRACE_SCRATCH_BYTES;
+constexpr size_t kScratchSpillOffset = 4096;
+static_assert(256 + sizeof(unsigned long long) * ST_COUNT <= kScratchSpillOffset, "scratch header");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: this device's address of the scratch array (looked up once).
+int scratch_address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return MPT_ERR_DEVICE;
+    std::lock_guard<std::mutex> g(g_scratch_lock);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MPT_ERR_DEVICE;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return MPT_OK;
+}
+
 }  // namespace
 
 // ---- C ABI --------------------------------------------------------------------------
@@ -897,6 +929,11 @@ int mpt_create(int device, mpt_ctx** out) {
     auto* ctx = new mpt_ctx();
     ctx->device = device;
     ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    // the no-allocation trace's scratch: only that entry point fails without it
+    if (scratch_address(device, &ctx->trace_scratch) != MPT_OK) {
+        (void)hipGetLastError();
+        ctx->trace_scratch = nullptr;
+    }
     *out = ctx;
     return MPT_OK;
 }
@@ -1170,6 +1207,49 @@ int mpt_trace(mpt_scene* s, const mpt_ray* rays, size_t n, int any_hit, mpt_hit*
     const int rc = do_trace(s, r.as<mpt_ray>(), n, any_hit, h.as<mpt_hit>(), nullptr);
     if (rc != MPT_OK) return rc;
     HIP_TRY(s->ctx, hipMemcpy(hits, h.p, n * sizeof(mpt_hit), hipMemcpyDeviceToHost));
+    return MPT_OK;
+}
+
+int mpt_trace_device_noalloc(mpt_scene* s, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
+                             void* stream) {
+    if (!s) return MPT_ERR_INVALID;
+    if (n > MPT_TRACE_MAX_RAYS) return fail(s->ctx, MPT_ERR_INVALID, "at most MPT_TRACE_MAX_RAYS rays per call");
+    if (n > 0 && (!rays_dev || !hits_dev)) return fail(s->ctx, MPT_ERR_INVALID, "null ray / hit buffer");
+    REQUIRE_DEVICE(s);
+    if (n == 0) return MPT_OK;
+    mpt_ctx* ctx = s->ctx;
+    if (!ctx->trace_scratch) return fail(ctx, MPT_ERR_DEVICE, "no scratch array on this device");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    unsigned char* base = static_cast<unsigned char*>(ctx->trace_scratch);
+    int blocks = trace_blocks(s, static_cast<int64_t>(n));
+    const int64_t extra = s->sc.max_stack - s->cfg.stack;
+    if (extra > 0) {
+        const int64_t entries = static_cast<int64_t>((kScratchBytes - kScratchSpillOffset) / sizeof(int32_t));
+        const int64_t fit = entries / (extra * s->cfg.block);   // whole blocks whose lanes' spill slices fit
+        if (fit < 1)
+            return fail(ctx, MPT_ERR_INVALID, "the tree's traversal stack (" + std::to_string(s->sc.max_stack) +
+                                                  " entries) needs more spill than MPT_TRACE_SCRATCH_BYTES holds");
+        blocks = static_cast<int>(std::min<int64_t>(blocks, fit));
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // counter and statistics to zero with a fill kernel (a caller may capture this)
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(base),
+                       static_cast<int>(kScratchSpillOffset / sizeof(unsigned long long)));
+    HIP_TRY(ctx, hipGetLastError());
+    TraceArgs a{};
+    a.sc = s->sc;
+    a.org = reinterpret_cast<const float4*>(rays_dev);
+    a.dir = reinterpret_cast<const float4*>(rays_dev) + 1;
+    a.stride = 2;
+    a.n_host = static_cast<int64_t>(n);
+    a.hit = reinterpret_cast<int2*>(hits_dev);
+    a.want_prim = 1;
+    a.work = reinterpret_cast<int32_t*>(base);
+    a.stats = reinterpret_cast<unsigned long long*>(base + 256);
+    a.stat_slot = any_hit ? ST_SHADOW_RAYS : ST_CLOSEST_RAYS;
+    a.spill = extra > 0 ? reinterpret_cast<int32_t*>(base + kScratchSpillOffset) : nullptr;
+    launch_trace_cfg(s->cfg, a, any_hit != 0 ? TRACE_ANY : TRACE_CLOSEST, blocks, st);
+    HIP_TRY(ctx, hipGetLastError());
     return MPT_OK;
 }
 

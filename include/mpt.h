@@ -235,6 +235,19 @@ int  mpt_scene_light_pick_pdf(const mpt_scene* scene, float* out);
 int  mpt_trace(mpt_scene* scene, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits);
 int  mpt_trace_device(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
                       void* stream);
+/* mpt_trace_device that allocates nothing. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same traversal and
+ * results, queued on stream (NULL: the default stream) without waiting; its
+ * work counter, statistics and traversal-stack spill live in one fixed array of
+ * MPT_TRACE_SCRATCH_BYTES per device, held until the process ends (statistics
+ * are not read back: mpt_stats_get is not updated). Limits: n <= MPT_TRACE_MAX_RAYS;
+ * a tree whose stack spill for one block exceeds the array: MPT_ERR_INVALID (a
+ * deep tree runs with fewer persistent blocks). Every call on one GPU shares its
+ * array, so calls that overlap on one GPU (any streams) race; serialize them. */
+#define MPT_TRACE_SCRATCH_BYTES ((size_t)32u << 20)
+int  mpt_trace_device_noalloc(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
+                              void* stream);
 
 /* Sampling renderer + pt estimator. film_host (W x H x 4: sum w*rgb, sum w)
  * receives the linear film when non-NULL. */
