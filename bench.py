@@ -280,7 +280,7 @@ def main():
     log(f"procedural: {args.steps} passes in {dt:.3f} s: {rays} rays ({mrays:.1f} Mrays/s), closest "
         f"{st['ms_closest']:.1f} ms (closest + shadow) shade {st['ms_shade']:.1f} ms regen "
         f"{st['ms_camera']:.1f} ms over {launches} steps")
-    kernel = f"k_trace<{st_scene['width']},24,mixed>"
+    kernel = f"k_trace<{st_scene['width']},{st_scene['stack_lds']},mixed>"
     # the same frame pass by pass (every pass's paths bounce in lock step)
     dt_pw = None
     if not args.no_passwise:

@@ -322,8 +322,9 @@ class Scene:
         return buf.value.decode()
 
 
-def batch_opts(width: int = 4, stack_lds: int = 24, block: int = 128, capacity_log2: int = 21,
-               graph_steps: int = 16, blocks_per_cu: int = 6) -> BatchOpts:
+def batch_opts(width: int = 4, stack_lds: int = 0, block: int = 128, capacity_log2: int = 21,
+               graph_steps: int = 16, blocks_per_cu: int = 0) -> BatchOpts:
+    """Zero stack_lds / blocks_per_cu: the library's default for the width."""
     return BatchOpts(width, stack_lds, block, capacity_log2, graph_steps, blocks_per_cu)
 
 

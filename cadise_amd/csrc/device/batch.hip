@@ -85,11 +85,12 @@ using namespace mpt;
 
 void normalize_opts(mpt_batch_opts& o) {
     if (o.width != 8) o.width = 4;
-    if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
+    if (o.stack_lds != 16 && o.stack_lds != 20 && o.stack_lds != 24 && o.stack_lds != 32)
+        o.stack_lds = o.width == 4 ? 20 : 24;
     if (o.block != 256) o.block = 128;
     if (o.capacity_log2 < 16 || o.capacity_log2 > 23) o.capacity_log2 = 21;
     if (o.graph_steps < 2 || o.graph_steps > 64) o.graph_steps = 16;
-    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = 6;
+    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 6;
     o.graph_steps &= ~1;
 }
 
@@ -718,11 +719,11 @@ int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, m
             if (dim == 1)
                 for (int w : {4, 8}) { mpt_batch_opts o = cur; o.width = w; cands.push_back(o); }
             if (dim == 2)
-                for (int s : {16, 24, 32}) { mpt_batch_opts o = cur; o.stack_lds = s; cands.push_back(o); }
+                for (int s : {16, 20, 24, 32}) { mpt_batch_opts o = cur; o.stack_lds = s; cands.push_back(o); }
             if (dim == 3)
                 for (int k : {128, 256}) { mpt_batch_opts o = cur; o.block = k; cands.push_back(o); }
             if (dim == 4)
-                for (int k : {4, 6, 8, 12}) { mpt_batch_opts o = cur; o.blocks_per_cu = k; cands.push_back(o); }
+                for (int k : {4, 6, 7, 8, 12}) { mpt_batch_opts o = cur; o.blocks_per_cu = k; cands.push_back(o); }
             for (const mpt_batch_opts& o : cands) {
                 if (std::memcmp(&o, &cur, sizeof(o)) == 0) continue;
                 double ms = 0.0;

@@ -53,6 +53,8 @@ inline void launch_trace_cfg(const TraceCfg& c, const TraceArgs& a, int mode, in
     const int key = c.width * 100 + c.stack;
     switch (key) {
         case 416: launch_ws<4, 16>(mode, blocks, c.block, lds, st, a); break;
+        case 420: launch_ws<4, 20>(mode, blocks, c.block, lds, st, a); break;
+        case 820: launch_ws<8, 20>(mode, blocks, c.block, lds, st, a); break;
         case 432: launch_ws<4, 32>(mode, blocks, c.block, lds, st, a); break;
         case 816: launch_ws<8, 16>(mode, blocks, c.block, lds, st, a); break;
         case 824: launch_ws<8, 24>(mode, blocks, c.block, lds, st, a); break;
@@ -65,9 +67,14 @@ inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
     if (o.width != 8) o.width = 4;
     if (o.leaf_max <= 0) o.leaf_max = 2;
     if (o.leaf_max > 64) o.leaf_max = 64;
-    if (o.stack_lds != 16 && o.stack_lds != 32) o.stack_lds = 24;
+    // defaults per width (measured on the 1.31M-primitive 1080p frame): BVH4's
+    // traversal fits 125 VGPRs, so 8 blocks of 128 per CU (4 waves per SIMD) with
+    // 20 LDS stack entries per lane (the LDS of 8 blocks then fits the CU's 160 KB);
+    // BVH8's needs 129 VGPRs (3 waves per SIMD): 6 blocks, 24 entries
+    if (o.stack_lds != 16 && o.stack_lds != 20 && o.stack_lds != 24 && o.stack_lds != 32)
+        o.stack_lds = o.width == 4 ? 20 : 24;
     if (o.block != 256) o.block = 128;
-    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = 6;
+    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 6;
     TraceCfg c;
     c.width = o.width;
     c.stack = o.stack_lds;

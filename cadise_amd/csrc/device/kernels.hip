@@ -70,9 +70,11 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
     const int64_t n = n0 * (1 + a.dual + (MODE == TRACE_MIXED ? a.shadow : 0));
     TravCounters cnt{0, 0, 0, 0};
-    unsigned long long rays = 0, rays2 = 0, rays3 = 0, errors = 0;
-    // ray i: set i / n0 (0: dir -> hit, 1: dir2 -> hit2, 2: dir3 -> hit3, any hit), slot i % n0
-    auto set_of = [&](int64_t i) { return i >= 2 * n0 ? 2 : (i >= n0 ? 1 : 0); };
+    uint32_t rays = 0, rays2 = 0, rays3 = 0, errors = 0;
+    // ray i: set i / n0 (0: dir -> hit, 1: dir2 -> hit2, 2: dir3 -> hit3, any hit), slot i % n0.
+    // Ray indices and the work counter are 32-bit: the host caps n at 2^31 - 2^24, so
+    // the counter (n plus at most one reservation of 64 per wave) stays below 2^31.
+    auto set_of = [&](int32_t i) { return i >= 2 * n0 ? 2 : (i >= n0 ? 1 : 0); };
     auto is_any = [&](int set) { return MODE == TRACE_ANY || (MODE == TRACE_MIXED && set == 2); };
     auto put = [&](int set, int64_t q, int2 out) {
         if (set == 0) a.hit[q] = out;
@@ -81,15 +83,15 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     };
     const bool no_tree = a.sc.nnodes == 0;
     LaneRay r;
-    int64_t rid = -1;            // ray of this lane, -1 = idle
-    int64_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
+    int32_t rid = -1;            // ray of this lane, -1 = idle
+    int32_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
     float4 po = make_float4(0.f, 0.f, 0.f, 0.f), pd = po;
     int32_t proot = 0;           // its BVH root
     bool exhausted = false;      // wave-uniform: the work counter has run out
     // a finished lane takes its prefetched ray at once; invalid rays are answered here
     auto install = [&]() {
         while (rid < 0 && pid >= 0) {
-            const int64_t i = pid;
+            const int32_t i = pid;
             pid = -1;
             const int set = set_of(i);
             const int64_t q = i - set * n0;
@@ -101,10 +103,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                 rays3 += set == 2 ? 1 : 0;
                 rid = i;
                 r.any = is_any(set);
-                r.o = v3f4(po);
-                r.d = v3f4(pd);
-                r.inv = safe_inv(r.d);
-                r.tmin = po.w;
+                r.inv = safe_inv(v3f4(pd));
                 r.tmax = pd.w;
                 cl.ray[2 * lane] = lds_f4{po.x, po.y, po.z, po.w};
                 cl.ray[2 * lane + 1] = lds_f4{pd.x, pd.y, pd.z, pd.w};
@@ -125,7 +124,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
             base = __shfl(base, leader, WAVE);
             if (base + nneed >= n) exhausted = true;
             if (pid < 0) {
-                const int64_t i = base + __popcll(need & ((1ull << lane) - 1ull));
+                const int32_t i = base + static_cast<int32_t>(__popcll(need & ((1ull << lane) - 1ull)));
                 if (i < n) {
                     const int set = set_of(i);
                     const int64_t q = i - set * n0;
@@ -195,6 +194,12 @@ template __global__ void k_trace<4, 16, TRACE_MIXED>(TraceArgs);
 template __global__ void k_trace<8, 16, TRACE_CLOSEST>(TraceArgs);
 template __global__ void k_trace<8, 16, TRACE_ANY>(TraceArgs);
 template __global__ void k_trace<8, 16, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<4, 20, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<4, 20, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<4, 20, TRACE_MIXED>(TraceArgs);
+template __global__ void k_trace<8, 20, TRACE_CLOSEST>(TraceArgs);
+template __global__ void k_trace<8, 20, TRACE_ANY>(TraceArgs);
+template __global__ void k_trace<8, 20, TRACE_MIXED>(TraceArgs);
 template __global__ void k_trace<4, 32, TRACE_CLOSEST>(TraceArgs);
 template __global__ void k_trace<4, 32, TRACE_ANY>(TraceArgs);
 template __global__ void k_trace<4, 32, TRACE_MIXED>(TraceArgs);

@@ -59,6 +59,9 @@ int build_and_upload(mpt_scene* S) {
     S->sstats.wide_depth = S->wide.depth;
     S->sstats.width = S->wide.width;
     S->sstats.leaf_max = o.leaf_max;
+    S->sstats.stack_lds = S->cfg.stack;
+    S->sstats.trace_block = S->cfg.block;
+    S->sstats.blocks_per_cu = S->cfg.bpc;
     S->sstats.image_width = h.width;
     S->sstats.image_height = h.height;
     S->sstats.spp = h.spp;

@@ -133,11 +133,12 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
     return v3(one(d.x), one(d.y), one(d.z));
 }
 
-// Traversal state of one lane: the ray, its running closest distance and
-// hit, the node to visit next (-1: pop the stack) and the stack depth.
+// Traversal state of one lane: the ray's direction reciprocal, its running
+// closest distance and hit, the node to visit next (-1: pop the stack). Its
+// origin, tmin and direction stay in the wave's LDS ray slots (CoopLds::ray).
 struct LaneRay {
-    V3      o, d, inv;
-    float   tmin, tmax;
+    V3      inv;
+    float   tmax;
     int32_t hit;
     int32_t node;
     bool    any;   // any hit (occlusion) query
@@ -248,8 +249,9 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             wd[4 * k + 2] = v.z;
             wd[4 * k + 3] = v.w;
         }
-        const float dox = __uint_as_float(wd[0]) - r.o.x, doy = __uint_as_float(wd[1]) - r.o.y,
-                    doz = __uint_as_float(wd[2]) - r.o.z;
+        const lds_f4 ro = cl.ray[2 * lane];   // the lane's origin and tmin (kept in LDS, not registers)
+        const float dox = __uint_as_float(wd[0]) - ro.x, doy = __uint_as_float(wd[1]) - ro.y,
+                    doz = __uint_as_float(wd[2]) - ro.z;
         const float sx = __uint_as_float((wd[3] & 0xFFu) << 23), sy = __uint_as_float(((wd[3] >> 8) & 0xFFu) << 23),
                     sz = __uint_as_float(((wd[3] >> 16) & 0xFFu) << 23);
         auto qb = [&](int plane, int c) {
@@ -270,7 +272,7 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             const float tx0 = lox * r.inv.x, tx1 = hix * r.inv.x;
             const float ty0 = loy * r.inv.y, ty1 = hiy * r.inv.y;
             const float tz0 = loz * r.inv.z, tz1 = hiz * r.inv.z;
-            const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
+            const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), ro.w));
             const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
             hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
             tn[c] = n;

@@ -63,7 +63,8 @@ class SceneStats(C.Structure):
                 ("width", C.c_int32), ("leaf_max", C.c_int32), ("build_ms", C.c_double), ("collapse_ms", C.c_double),
                 ("device_bytes", C.c_int64), ("image_width", C.c_int32), ("image_height", C.c_int32),
                 ("spp", C.c_int32), ("max_depth", C.c_int32), ("ntextures", C.c_int64),
-                ("tex_elements", C.c_int64)]
+                ("tex_elements", C.c_int64), ("stack_lds", C.c_int32), ("trace_block", C.c_int32),
+                ("blocks_per_cu", C.c_int32)]
 
 
 class RenderParams(C.Structure):

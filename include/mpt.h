@@ -140,8 +140,10 @@ typedef struct {
     int32_t width;             /* children per node: 4 or 8 (default 4) */
     int32_t leaf_max;          /* largest subtree folded into one leaf (default 2) */
     int32_t block;             /* traversal threads per block: 128 (default) or 256 */
-    int32_t stack_lds;         /* per-lane stack entries kept in LDS (default 24) */
-    int32_t blocks_per_cu;     /* resident traversal blocks per CU of the persistent grid (default 6) */
+    int32_t stack_lds;         /* per-lane stack entries kept in LDS: 16, 20, 24 or 32 (default 20
+                                  for width 4, 24 for width 8) */
+    int32_t blocks_per_cu;     /* resident traversal blocks per CU of the persistent grid (default 8
+                                  for width 4, 6 for width 8) */
 } mpt_bvh_opts;
 
 typedef struct {
@@ -155,6 +157,7 @@ typedef struct {
     int32_t image_width, image_height, spp, max_depth;
     int64_t ntextures;         /* image and constant textures, in creation order */
     int64_t tex_elements;      /* floats of all textures together (the texture-gradient length) */
+    int32_t stack_lds, trace_block, blocks_per_cu;   /* the traversal launch shape in use */
 } mpt_scene_stats;
 
 typedef struct {
@@ -317,7 +320,7 @@ typedef struct mpt_batch mpt_batch;
 
 typedef struct {
     int32_t width;             /* BVH width, 4 or 8 */
-    int32_t stack_lds;         /* per-lane LDS stack entries: 16, 24 or 32 */
+    int32_t stack_lds;         /* per-lane LDS stack entries: 16, 20, 24 or 32 */
     int32_t block;             /* traversal threads per block: 128 or 256 */
     int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..23) */
     int32_t graph_steps;       /* wavefront steps per graph launch (even, 2..64) */
