@@ -39,6 +39,10 @@ class Context:
         if rc != 0:
             raise MptError(rc, self.L.mpt_last_error(self.h).decode())
 
+    def release_stream_buffers(self):
+        """Frees the streaming renderer's path slots and queues (reallocated on demand)."""
+        self.check(self.L.mpt_release_stream_buffers(self.h))
+
     def close(self):
         if getattr(self, "h", None):
             self.L.mpt_destroy(self.h)

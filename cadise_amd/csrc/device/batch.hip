@@ -572,6 +572,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     r.ms_shade = ms[SK_SHADE];
     r.ms_gpu = gms;
     r.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    r.path_slots = cap;
     if (out) *out = r;
     if (st[ST_ERRORS]) return fail(ctx, MPT_ERR_DEVICE, "traversal stack overrun");
     return MPT_OK;

@@ -1432,6 +1432,14 @@ int mpt_render_adjoint(mpt_scene* s, const mpt_render_params* p, const float* d_
     return do_adjoint(s, p, d_image, d_tex, d_light);
 }
 
+int mpt_release_stream_buffers(mpt_ctx* ctx) {
+    if (!ctx) return MPT_ERR_INVALID;
+    if (ctx->device >= 0) HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (ctx->streamer) HIP_TRY(ctx, hipDeviceSynchronize());
+    ctx->streamer.reset();
+    return MPT_OK;
+}
+
 int mpt_stats_get(const mpt_scene* s, mpt_stats* out) {
     if (!s || !out) return MPT_ERR_INVALID;
     *out = s->rstats;

@@ -83,7 +83,7 @@ class Stats(C.Structure):
                 ("anyhit_launches", C.c_int64),
                 ("ms_total", C.c_double), ("ms_closest", C.c_double), ("ms_anyhit", C.c_double),
                 ("ms_shade", C.c_double), ("ms_camera", C.c_double), ("ms_film", C.c_double),
-                ("ms_adjoint", C.c_double), ("ms_gpu", C.c_double)]
+                ("ms_adjoint", C.c_double), ("ms_gpu", C.c_double), ("path_slots", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,7 +105,7 @@ EXPORTS = ["mpt_create", "mpt_destroy", "mpt_last_error", "mpt_version", "mpt_sc
            "mpt_scene_create", "mpt_scene_destroy", "mpt_scene_stats_get", "mpt_scene_binary_bvh", "mpt_scene_camera",
            "mpt_scene_light_radiance", "mpt_scene_light_pick_pdf", "mpt_trace",
            "mpt_trace_device", "mpt_trace_device_noalloc", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_srgb8_device", "mpt_film_save",
-           "mpt_scene_output_filename", "mpt_image_write", "mpt_stats_get",
+           "mpt_scene_output_filename", "mpt_image_write", "mpt_stats_get", "mpt_release_stream_buffers",
            "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_get_texture", "mpt_scene_set_light_radiance",
            "mpt_render_adjoint",
            "mpt_batch_create", "mpt_batch_destroy", "mpt_batch_render", "mpt_batch_film", "mpt_batch_autotune",
@@ -148,6 +148,7 @@ def load_library():
     L.mpt_scene_output_filename.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.mpt_image_write.argtypes = [C.c_char_p, vp, i32, i32]
     L.mpt_stats_get.argtypes = [vp, C.POINTER(Stats)]
+    L.mpt_release_stream_buffers.argtypes = [vp]
     L.mpt_scene_texture_info.argtypes = [vp, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
     L.mpt_scene_set_texture.argtypes = [vp, i32, vp]
     L.mpt_scene_set_light_radiance.argtypes = [vp, i32, vp]

@@ -190,6 +190,7 @@ typedef struct {
                                      fields then describe the forward render it replays */
     double  ms_gpu;               /* GPU time of the call's kernels (events; mpt_batch_render: the
                                      streaming loop) */
+    int64_t path_slots;           /* streaming renderer: path slots in flight (0 pass by pass) */
 } mpt_stats;
 
 /* device >= 0: a GPU context. device == -1: a host-only context, in which
@@ -280,6 +281,11 @@ int  mpt_scene_output_filename(const mpt_scene* scene, char* buf, size_t n, size
 int  mpt_image_write(const char* path, const uint8_t* rgb, int32_t width, int32_t height);
 
 int  mpt_stats_get(const mpt_scene* scene, mpt_stats* out);
+/* Frees the context's streaming path state (path slots and vertex queues: up to
+ * 2^24 slots, ~6 GB, for single-scene renders of more than 2^23 samples; the
+ * adjoint keeps at most 2^22, a batch its own 2^16..2^23). The next streaming
+ * render allocates it again. */
+int  mpt_release_stream_buffers(mpt_ctx* ctx);
 
 /* ---- differentiable rendering (no counterpart in the reference) -------------------
  * Parameters: the texels of every texture (albedo of matte-lambertian, roughness

@@ -12,6 +12,7 @@ from __future__ import annotations
 import os
 import sys
 import tempfile
+import time
 
 import numpy as np
 
@@ -140,6 +141,23 @@ def write_film_goldens(out_dir):
         print(os.path.join(out_dir, f), os.path.getsize(os.path.join(out_dir, f)), "bytes")
 
 
+def write_cornell_reference(out_dir, spp=16384, seed=0xC1):
+    """cornell_r16k.npz: the reference renderer's image of the Cornell box
+    workload (36 triangles, 256 x 256, depth 8, stratified, box filter) at
+    16384 spp, the R of its time-to-relMSE (H x W x 3 float32, sum w rgb / sum w)."""
+    spec = scenes.cornell_box(width=256, height=256, spp=spp, max_depth=8)
+    R = RefScene(spec.to_crsd())
+    t0 = time.time()
+    film = R.render_linear(seed=seed, threads=os.cpu_count() or 8)
+    secs = time.time() - t0
+    w = film[..., 3:]
+    img = (film[..., :3] / np.where(w == 0, 1, w)).astype(np.float32)
+    path = os.path.join(out_dir, "cornell_r16k.npz")
+    np.savez_compressed(path, image=img, spp=np.int64(spp), seed=np.uint64(seed), seconds=np.float64(secs),
+                        threads=np.int64(os.cpu_count() or 8))
+    print(path, os.path.getsize(path), "bytes,", f"{secs:.0f} s")
+
+
 SHIPPED = "/root/reference/scene/cornell-box.crsd"
 
 
@@ -171,6 +189,8 @@ def main():
     only = set(sys.argv[1:])
     if not only or "films" in only:
         write_film_goldens(HERE)
+    if "cornell_r16k" in only:   # ~10 minutes on 8 cores: only on request
+        write_cornell_reference(HERE)
     for name, spec in scene_set():
         if only and name not in only:
             continue

@@ -136,3 +136,25 @@ def test_output_filename_and_host_save(tmp_path):
     assert Scene.from_crsd_text(ctx, text).output_filename() == "cadise.jpg"
     with pytest.raises(MptError):
         sc.save()
+
+
+def test_cornell_reference_image_fixture():
+    """tests/golden/cornell_r16k.npz is the reference renderer's 16384-spp image of
+    the Cornell box workload (the R of its time-to-relMSE): the oracle's own
+    64-spp render of that workload lies within Monte Carlo noise of it (relMSE
+    ~3e-3 expected at 64 spp), and a render of another scene does not."""
+    from cadise_amd import scenes
+    g = np.load(os.path.join(GOLDEN, "cornell_r16k.npz"))
+    R = g["image"].astype(np.float64)
+    assert R.shape == (256, 256, 3) and int(g["spp"]) == 16384 and np.all(np.isfinite(R))
+
+    def rel(spec):
+        f = OracleScene(spec).render_linear(9, threads=8)
+        img = f[..., :3] / f[..., 3:]
+        return float(np.mean((img - R) ** 2 / (R ** 2 + 0.01)))
+
+    e = rel(scenes.cornell_box(width=256, height=256, spp=64, max_depth=8))
+    other = scenes.cornell_box(width=256, height=256, spp=64, max_depth=2)
+    e2 = rel(other)
+    print(f"relMSE of a 64-spp oracle render vs R: {e:.2e}; depth-2 render: {e2:.2e}")
+    assert e < 1e-2 and e2 > 3 * e
