@@ -1,5 +1,6 @@
 #include "hdr.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -55,6 +56,16 @@ bool read_hdr(const std::string& path, std::vector<float>* rgb, int* width, int*
     int w = 0, h = 0;
     if (std::sscanf(line().c_str(), "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0) {
         *err = path + ": unsupported resolution line (-Y H +X W expected)";
+        return false;
+    }
+    // the smallest encoding of a scanline: 4 bytes a pixel flat, or (widths 8 to
+    // 32767) a 4-byte header and per channel runs of up to 127 pixels in 2 bytes;
+    // a header whose scanlines cannot fit in the rest of the file is rejected
+    // before anything is allocated
+    const size_t wz = static_cast<size_t>(w);
+    const size_t min_line = (w >= 8 && w < 32768) ? std::min(4 * wz, 4 + 8 * ((wz + 126) / 127)) : 4 * wz;
+    if (static_cast<uint64_t>(h) * min_line > data.size() - pos || wz * static_cast<size_t>(h) > (size_t{1} << 30)) {
+        *err = path + ": resolution larger than the file holds";
         return false;
     }
     rgb->assign(static_cast<size_t>(w) * h * 3, 0.0f);

@@ -267,16 +267,26 @@ bool decode_png(const std::vector<uint8_t>& f, Picture8* out, std::string* err) 
 
     const int img_n = color == 2 ? 3 : color == 4 ? 2 : color == 6 ? 4 : 1;   // samples per pixel in the file
     const int color_n = (color == 2 || color == 3 || color == 6) ? 3 : 1;   // colour channels kept
+    const int bpp_bits = img_n * depth;
+    static const int ax0[7] = {0, 4, 0, 2, 0, 1, 0}, ay0[7] = {0, 0, 4, 0, 2, 0, 1}, adx[7] = {8, 8, 4, 4, 2, 2, 1},
+                     ady[7] = {8, 8, 8, 4, 4, 2, 2};
+    {   // the filtered rows of every pass must be in the inflated data before the image is allocated
+        size_t total = 0;
+        for (int pass = 0; pass < (interlace ? 7 : 1); ++pass) {
+            const uint32_t x0 = interlace ? ax0[pass] : 0, y0 = interlace ? ay0[pass] : 0;
+            const uint32_t dx = interlace ? adx[pass] : 1, dy = interlace ? ady[pass] : 1;
+            const uint64_t pw = w > x0 ? (w - x0 + dx - 1) / dx : 0, ph = h > y0 ? (h - y0 + dy - 1) / dy : 0;
+            if (pw && ph) total += static_cast<size_t>(((pw * bpp_bits + 7) / 8 + 1) * ph);
+        }
+        if (raw.size() < total) return bad("not enough pixel data");
+    }
     out->width = static_cast<int32_t>(w);
     out->height = static_cast<int32_t>(h);
     out->channels = color_n;
     out->has_alpha = color == 4 || color == 6 || trns;
     out->data.assign(static_cast<size_t>(w) * h * color_n, 0);
-    const int bpp_bits = img_n * depth;
     const int fbytes = std::max(1, bpp_bits / 8);   // filter distance
     const int gray_scale = (color == 0 || color == 4) && depth < 8 ? (depth == 1 ? 0xFF : depth == 2 ? 0x55 : 0x11) : 1;
-    static const int ax0[7] = {0, 4, 0, 2, 0, 1, 0}, ay0[7] = {0, 0, 4, 0, 2, 0, 1}, adx[7] = {8, 8, 4, 4, 2, 2, 1},
-                     ady[7] = {8, 8, 8, 4, 4, 2, 2};
     size_t off = 0;
     const int npass = interlace ? 7 : 1;
     for (int pass = 0; pass < npass; ++pass) {
@@ -349,6 +359,11 @@ bool decode_tga(const std::vector<uint8_t>& f, Picture8* out, std::string* err) 
     if (gray ? (bpp != 8 && bpp != 16) : (bpp != 24 && bpp != 32)) return bad("unsupported bits per pixel");
     if (w == 0 || h == 0) return bad("empty image");
     const int bytes = bpp / 8;
+    {   // the smallest encoding of w x h pixels must fit in the file before anything is allocated
+        const size_t npx = static_cast<size_t>(w) * h, avail = f.size() - std::min(f.size(), 18 + size_t(idlen));
+        const size_t least = rle ? (npx + 127) / 128 * (1 + static_cast<size_t>(bytes)) : npx * bytes;
+        if (least > avail) return bad("image larger than the file holds");
+    }
     out->width = w;
     out->height = h;
     out->channels = gray ? 1 : 3;

@@ -9,7 +9,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# MPT_ORACLE_SO: another build of the same sources (the sanitizer build of
+# tests/sanitize/Makefile)
+ORACLE_SO = os.environ.get("MPT_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcadise_ref.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
