@@ -7,6 +7,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -83,6 +84,27 @@ inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
     return c;
 }
 
+// Allocation-failure injection for tests: with MPT_FAIL_ALLOC_AFTER=k in the
+// environment, the allocation k + 1 allocations after the variable was set (or
+// changed) fails once with hipErrorOutOfMemory. Unset: no effect.
+inline bool alloc_fault_injected() {
+    static std::string seen;
+    static long remaining = -1;
+    const char* e = std::getenv("MPT_FAIL_ALLOC_AFTER");
+    if (!e) {
+        seen.clear();
+        remaining = -1;
+        return false;
+    }
+    if (seen != e) {
+        seen = e;
+        remaining = std::atol(e);
+    }
+    if (remaining < 0) return false;
+    if (remaining-- == 0) return true;
+    return false;
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -99,6 +121,7 @@ struct DevBuf {
         release();
         if (n == 0) n = 16;
         void* q = nullptr;
+        if (alloc_fault_injected()) return hipErrorOutOfMemory;
         const hipError_t e = hipMalloc(&q, n);
         if (e != hipSuccess) return e;
         p = q;

@@ -43,9 +43,11 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 template <int W, int S, int MODE>
 __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     constexpr int STEPS = 8;
-    // reserve new prefetch slots once this many lanes lack one (measured: BVH4 best at
-    // 48 of 64, 3.5% over 16; BVH8 flat from 16 to 32, worse from 48)
-    constexpr int REFILL = W == 4 ? 48 : 16;
+    // reserve new prefetch slots once this many lanes lack one (measured on the 1080p
+    // frame at 4 waves per SIMD, ms per launch: BVH4 32 / 48 / 56 / 60 / 64 lanes ->
+    // 3.68 / 3.45 / 3.42 / 3.49 / 3.76; BVH8 (3 waves per SIMD) flat from 8 to 24;
+    // blocks of 6, 8 or 12 steps between reservations within 0.7%)
+    constexpr int REFILL = W == 4 ? 56 : 16;
     extern __shared__ int32_t lds_stack[];
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
