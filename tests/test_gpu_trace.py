@@ -147,3 +147,36 @@ def _write(tmp_path, spec):
     path = tmp_path / "scene.crsd"
     path.write_text(spec.to_crsd(sidecar_dir=str(tmp_path)))
     return str(path)
+
+
+def test_trace_degenerate_rays(ctx):
+    """Rays the reference's tests never make: tmin > tmax, negative and zero tmax
+    (an absent ray on the GPU returns no hit and t = tmax, as the reference's
+    untouched maxT), infinite tmax, an infinite origin, tmin = -inf (hits behind
+    the origin count, as in the reference), a zero direction. Ids and t as the
+    oracle's. Rays with NaN components must come back (no hang, ids in range),
+    but their ids are not compared: with NaN every comparison of the reference's
+    tests passes, so its answer is whichever primitive it tests last, an
+    artefact of traversal order."""
+    from cadise_amd.api import Scene
+    spec = scenes.cornell_box()
+    sc = Scene.from_crsd_text(ctx, spec.to_crsd())
+    orc = OracleScene(spec)
+    F = np.finfo(np.float32).max
+    rays = np.array([[0.5, 0.5, 0.5, 2.0, 0, 1, 0, 1.0], [0.5, 0.5, 0.5, 1e-3, 0, 1, 0, -1.0],
+                     [0.5, 0.5, 0.5, 1e-3, 0, -1, 0, 0.0], [0.5, 0.5, 0.5, 1e-3, 0, 1, 0, np.inf],
+                     [np.inf, 0.5, 0.5, 1e-3, -1, 0, 0, F], [0.5, 0.5, 0.5, -np.inf, 0, 1, 0, F],
+                     [0.5, 0.5, 0.5, 1e-3, 0, 0, 0, F], [0.3, 0.2, 0.9, 0.0, 0, 0, -1, F]], np.float32)
+    for any_hit in (False, True):
+        prim, t = sc.trace(rays, any_hit)
+        oprim, ot = orc.trace(rays, any_hit=any_hit)
+        assert np.array_equal(prim, oprim), (any_hit, prim, oprim)
+        if not any_hit:
+            assert np.array_equal(t.view(np.int32), ot.view(np.int32)), (t, ot)
+    rng = np.random.default_rng(31)
+    mix = random_rays(20000, -0.1, 1.1, seed=32)
+    bad = rng.random((len(mix), 8)) < 0.05
+    mix[bad] = rng.choice(np.array([np.nan, np.inf, -np.inf, 0.0], np.float32), int(bad.sum()))
+    for any_hit in (False, True):
+        prim, _ = sc.trace(mix, any_hit)
+        assert np.all((prim >= -1) & (prim < spec.num_prims())) if not any_hit else np.all((prim == 0) | (prim == 1))
