@@ -53,7 +53,8 @@ class Context:
 
 
 def bvh_opts(width=4, leaf_max=2, block=0, stack_lds=0, blocks_per_cu=0):
-    """Zero fields take the library defaults (block 128, 24 LDS stack entries, 6 blocks per CU)."""
+    """Zero fields take the library defaults for the width (block 128; BVH4: 20 LDS stack entries,
+    8 blocks per CU; BVH8: 16 entries, 7 blocks)."""
     return BvhOpts(width, leaf_max, block, stack_lds, blocks_per_cu)
 
 

@@ -86,11 +86,11 @@ using namespace mpt;
 void normalize_opts(mpt_batch_opts& o) {
     if (o.width != 8) o.width = 4;
     if (o.stack_lds != 16 && o.stack_lds != 20 && o.stack_lds != 24 && o.stack_lds != 32)
-        o.stack_lds = o.width == 4 ? 20 : 24;
+        o.stack_lds = o.width == 4 ? 20 : 16;
     if (o.block != 256) o.block = 128;
     if (o.capacity_log2 < 16 || o.capacity_log2 > 23) o.capacity_log2 = 21;
     if (o.graph_steps < 2 || o.graph_steps > 64) o.graph_steps = 16;
-    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 6;
+    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 7;
     o.graph_steps &= ~1;
 }
 

@@ -69,13 +69,15 @@ inline TraceCfg normalize_cfg(mpt_bvh_opts& o) {
     if (o.leaf_max <= 0) o.leaf_max = 2;
     if (o.leaf_max > 64) o.leaf_max = 64;
     // defaults per width (measured on the 1.31M-primitive 1080p frame): BVH4's
-    // traversal fits 125 VGPRs, so 8 blocks of 128 per CU (4 waves per SIMD) with
-    // 20 LDS stack entries per lane (the LDS of 8 blocks then fits the CU's 160 KB);
-    // BVH8's needs 129 VGPRs (3 waves per SIMD): 6 blocks, 24 entries
+    // traversal fits 117 VGPRs, so 8 blocks of 128 per CU (4 waves per SIMD) with
+    // 20 LDS stack entries per lane (the LDS of 8 blocks then fits the CU's 160 KB;
+    // 16, 20 or 24 entries and 7 or 8 blocks within 0.2%); BVH8's fits 125 VGPRs
+    // but its leaf queue (8 entries per lane) leaves LDS for 7 blocks with 16
+    // entries (3.39 ms per launch against 3.49 for 6 blocks with 24)
     if (o.stack_lds != 16 && o.stack_lds != 20 && o.stack_lds != 24 && o.stack_lds != 32)
-        o.stack_lds = o.width == 4 ? 20 : 24;
+        o.stack_lds = o.width == 4 ? 20 : 16;
     if (o.block != 256) o.block = 128;
-    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 6;
+    if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 7;
     TraceCfg c;
     c.width = o.width;
     c.stack = o.stack_lds;

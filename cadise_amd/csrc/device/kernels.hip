@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     st.spill = a.spill ? a.spill + gslot * (a.sc.max_stack - S) : nullptr;
     st.sp = 0;
     // after the stacks: per thread two float4 (ray) and a key, then per wave a
-    // queue of 64 * W leaves
+    // queue of 64 * W + LEAF_DEFER leaves
     CoopLds cl;
     {
         const int wv = tid / WAVE, nt = static_cast<int>(blockDim.x);
@@ -67,7 +67,8 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         cl.ray = (__attribute__((address_space(3))) lds_f4*)(reinterpret_cast<lds_f4*>(base) + 2 * WAVE * wv);
         cl.key = (__attribute__((address_space(3))) unsigned long long*)(
                      reinterpret_cast<unsigned long long*>(base + 32 * nt) + WAVE * wv);
-        cl.queue = (__attribute__((address_space(3))) lds_i2*)(reinterpret_cast<lds_i2*>(base + 40 * nt) + WAVE * W * wv);
+        cl.queue = (__attribute__((address_space(3))) lds_i2*)(reinterpret_cast<lds_i2*>(base + 40 * nt) +
+                                                               (WAVE * W + LEAF_DEFER) * wv);
     }
     const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
     const int64_t n = n0 * (1 + a.dual + (MODE == TRACE_MIXED ? a.shadow : 0));
@@ -85,6 +86,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     };
     const bool no_tree = a.sc.nnodes == 0;
     LaneRay r;
+    r.pend = false;
+    r.parked = false;
+    int qn = 0;                  // wave-uniform: leaves in the wave's queue
     int32_t rid = -1;            // ray of this lane, -1 = idle
     int32_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
     float4 po = make_float4(0.f, 0.f, 0.f, 0.f), pd = po;
@@ -109,6 +113,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                 r.tmax = pd.w;
                 cl.ray[2 * lane] = lds_f4{po.x, po.y, po.z, po.w};
                 cl.ray[2 * lane + 1] = lds_f4{pd.x, pd.y, pd.z, pd.w};
+                cl.key[lane] = (static_cast<unsigned long long>(ord_dist(pd.w)) << 32) | 0xFFFFFFFFull;
+                r.pend = false;
+                r.parked = false;
                 r.hit = -1;
                 r.node = proot;
                 st.sp = 0;
@@ -144,7 +151,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         }
         // -- traversal steps
         for (int s = 0; s < STEPS; ++s) {
-            const bool fin = visit<W, S, MODE>(a.sc, st, r, cnt, rid >= 0, cl, lane);
+            const bool fin = visit<W, S, MODE>(a.sc, st, r, cnt, rid >= 0, cl, lane, qn);
             if (rid >= 0 && fin) {
                 if (r.hit == -2) {
                     errors += 1;

@@ -23,9 +23,16 @@ enum StatSlot {
     ST_COUNT
 };
 
+// The traversal tests queued leaves once LEAF_DEFER are queued in a wave, or
+// LEAF_PARK of its lanes have ended traversal and wait on queued leaves, or no
+// lane is left traversing (a multiple of 8: the queue's extra entries are
+// spread over the wave's 64 threads).
+constexpr int LEAF_DEFER = 32;
+constexpr int LEAF_PARK = 8;
+
 // LDS bytes per traversal thread beyond its stack entries: its ray (two
-// float4), its best-hit key and W leaf-queue entries (int2).
-__host__ __device__ constexpr int coop_lds_bytes(int W) { return 32 + 8 + 8 * W; }
+// float4), its best-hit key and W + LEAF_DEFER / 64 leaf-queue entries (int2).
+__host__ __device__ constexpr int coop_lds_bytes(int W) { return 32 + 8 + 8 * W + LEAF_DEFER / 8; }
 
 // A batch of rays for the persistent traversal kernel. Ray i is
 // (org[i * stride], dir[i * stride]) for i < n0 and (org[(i - n0) * stride],

@@ -141,7 +141,9 @@ struct LaneRay {
     float   tmax;
     int32_t hit;
     int32_t node;
-    bool    any;   // any hit (occlusion) query
+    bool    any;      // any hit (occlusion) query
+    bool    pend;     // leaves of this ray wait in the wave's leaf queue
+    bool    parked;   // traversal has ended; waiting for the queued leaves' round
 };
 
 // Compare-exchange of a bitonic sorting network on (distance, child) pairs.
@@ -200,25 +202,40 @@ __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
     return excl;
 }
 
-// One traversal step of a wave. Every lane with a ray (act) pops its next
-// node, slab-tests the W children and lists its hit leaves; the primitives of
-// all lanes' hit leaves are then tested by the whole wave, one leaf per lane
-// (so a step costs one round of primitive tests however the leaves are spread
-// over the lanes), the per-ray minimum taken by LDS atomics on hit keys; each
-// lane with a ray then sorts its hit interior children by entry distance,
-// continues with the nearest and pushes the rest. Returns true when the
-// lane's ray has finished (any hit found for an any-hit query, or the stack
-// is empty). Must be called by all lanes of the wave together. All register
-// arrays are indexed with compile-time indices.
+// One traversal step of a wave. Every lane that is still traversing pops its
+// next node, slab-tests the W children, appends its hit leaves to the wave's
+// leaf queue in LDS, sorts its hit interior children by entry distance,
+// continues with the nearest and pushes the rest. The queued leaves are tested
+// by the whole wave, one leaf per lane, in rounds of 64 (the per-ray minimum
+// taken by LDS atomics on hit keys), once at least LEAF_DEFER leaves are
+// queued, LEAF_PARK lanes wait on them or no lane is left traversing: a round
+// then serves the leaves of several steps instead of the ~10 of one. A lane
+// whose traversal has ended while leaves of its ray are still queued waits
+// (parked) until that round; interior children are culled by the distance of
+// the last round, which can only be larger than the current best. Returns
+// true when the lane's ray has finished (any hit found for an any-hit query,
+// or traversal ended and every leaf tested). qn (wave-uniform) carries the
+// number of queued leaves from step to step. Must be called by all lanes of
+// the wave together. All register arrays are indexed with compile-time
+// indices.
 template <int W, int S, int MODE>
 __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, LaneRay& r, TravCounters& cnt, bool act,
-                                      const CoopLds& cl, int lane) {
+                                      const CoopLds& cl, int lane, int& qn) {
     const bool any = MODE == TRACE_ANY || (MODE == TRACE_MIXED && r.any);
-    bool done = !act;
+    bool trav = act && !r.parked;
+    bool ended = false;   // traversal of the lane's ray ends with this step
+    if (trav && r.node < 0) {
+        if (st.sp == 0) {
+            ended = true;
+            trav = false;
+        } else {
+            r.node = st.pop();
+        }
+    }
+    int nl = 0;
     int32_t child[W], count[W], lf[W], lc[W];
     float tn[W];
     bool hitc[W];
-    int nl = 0;
 #pragma unroll
     for (int c = 0; c < W; ++c) {
         child[c] = -1;
@@ -228,13 +245,7 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         tn[c] = INFINITY;
         hitc[c] = false;
     }
-    if (act) {
-        if (r.node < 0) {
-            if (st.sp == 0) done = true;
-            else r.node = st.pop();
-        }
-    }
-    if (!done) {
+    if (trav) {
         // QNode<W>: origin + exponents, W children, 6 x W box bytes, W 16-bit counts
         constexpr int NWORD = W == 4 ? 16 : 28;
         constexpr int QW = 4 + W;                 // first word of the box bytes
@@ -291,91 +302,108 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             nl += isl ? 1 : 0;
         }
     }
-    // -- the wave's leaves: queue them, then one leaf per lane
+    // -- queue this step's leaves behind the pending ones (at most LEAF_DEFER - 1
+    // are pending here, so 64 * W + LEAF_DEFER entries hold them)
     int ntot;
     const int excl = wave_excl_scan<W == 4 ? 3 : 4>(nl, ntot);
-    if (ntot > 0) {
-        if (nl > 0) {
+    if (nl > 0) {
 #pragma unroll
-            for (int k = 0; k < W; ++k) {
-                if (k < nl) cl.queue[excl + k] = lds_i2{lf[k], lc[k] | (any ? 0x8000 : 0) | (lane << 16)};
-            }
-            cl.ray[2 * lane + 1].w = r.tmax;
-            cl.key[lane] = (static_cast<unsigned long long>(ord_dist(r.tmax)) << 32) | 0xFFFFFFFFull;
+        for (int k = 0; k < W; ++k) {
+            if (k < nl) cl.queue[qn + excl + k] = lds_i2{lf[k], lc[k] | (any ? 0x8000 : 0) | (lane << 16)};
         }
-        __builtin_amdgcn_wave_barrier();
-        for (int base = 0; base < ntot; base += 64) {
-            const int e = base + lane;
-            if (e < ntot) {
-                const lds_i2 q = cl.queue[e];
-                const int own = q.y >> 16, n = q.y & 0x7FFF;
-                const bool qany = MODE == TRACE_ANY || (MODE == TRACE_MIXED && (q.y & 0x8000) != 0);
-                const lds_f4 ro = cl.ray[2 * own], rd = cl.ray[2 * own + 1];
-                const V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
-                float tmax = rd.w;
-                int32_t best = -1;
-                // two records per round: both loads in flight together
-                for (int j = 0; j < n; j += 2) {
-                    const int32_t la = q.x + j;
-                    const bool two = j + 1 < n;
-                    const int32_t lb = two ? la + 1 : la;
-                    const PrimRec A = prim_load(sc.tris, la);
-                    const PrimRec B = prim_load(sc.tris, lb);
-                    if (qany) cnt.prims_any += two ? 2 : 1;
-                    else cnt.prims += two ? 2 : 1;
-                    if (prim_hit(A, o, d, ro.w, tmax)) best = la;
-                    if (two && prim_hit(B, o, d, ro.w, tmax)) best = lb;
+        r.pend = true;
+    }
+    qn += ntot;
+    // -- interior children, nearest first
+    if (trav) {
+        float key[W];
+        int32_t kid[W];
+        int nk = 0;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            const bool inner = hitc[c] && count[c] == 0 && tn[c] <= r.tmax;
+            key[c] = inner ? tn[c] : INFINITY;
+            kid[c] = child[c];
+            nk += inner ? 1 : 0;
+        }
+        // bitonic network, ascending distance
+#pragma unroll
+        for (int k = 2; k <= W; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    const int l = i ^ j;
+                    if (l > i) cmpx(key[i], kid[i], key[l], kid[l], (i & k) == 0);
                 }
-                if (best >= 0)
-                    __hip_atomic_fetch_min(cl.key + own, hit_key(tmax, best), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        if (nl > 0) {
-            const unsigned long long k = cl.key[lane];
-            const uint32_t lo = static_cast<uint32_t>(k);
-            if (lo != 0xFFFFFFFFu) {
-                r.hit = 0x7FFFFFFF - static_cast<int32_t>(lo);
-                r.tmax = dist_of(static_cast<uint32_t>(k >> 32));
-                if (any) done = true;
+        if (st.sp + nk - 1 > sc.max_stack) {   // bounded by the tree depth; never taken for a built tree
+            r.hit = -2;
+            ended = true;
+        } else {
+#pragma unroll
+            for (int k = W - 1; k >= 1; --k) {
+                if (k < nk) st.push(kid[k]);
             }
+            r.node = nk > 0 ? kid[0] : -1;
+            if (nk == 0 && st.sp == 0) ended = true;
         }
     }
-    if (done) return true;
-    float key[W];
-    int32_t kid[W];
-    int nk = 0;
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
-        const bool inner = hitc[c] && count[c] == 0 && tn[c] <= r.tmax;
-        key[c] = inner ? tn[c] : INFINITY;
-        kid[c] = child[c];
-        nk += inner ? 1 : 0;
-    }
-    // bitonic network, ascending distance
-#pragma unroll
-    for (int k = 2; k <= W; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-            for (int i = 0; i < W; ++i) {
-                const int l = i ^ j;
-                if (l > i) cmpx(key[i], kid[i], key[l], kid[l], (i & k) == 0);
+    if (ended) r.parked = true;
+    bool done = r.parked && !r.pend;   // ended with no leaf of its ray queued
+    // -- the queued leaves: one leaf per lane, rounds of 64
+    if (qn > 0) {
+        const int waiting = __popcll(__ballot(r.parked && r.pend));
+        if (qn >= LEAF_DEFER || waiting >= LEAF_PARK || __ballot(act && !r.parked) == 0ull) {
+            __builtin_amdgcn_wave_barrier();
+            for (int base = 0; base < qn; base += 64) {
+                const int e = base + lane;
+                if (e < qn) {
+                    const lds_i2 q = cl.queue[e];
+                    const int own = q.y >> 16, n = q.y & 0x7FFF;
+                    const bool qany = MODE == TRACE_ANY || (MODE == TRACE_MIXED && (q.y & 0x8000) != 0);
+                    const lds_f4 ro = cl.ray[2 * own], rd = cl.ray[2 * own + 1];
+                    const V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
+                    float tmax = rd.w;
+                    int32_t best = -1;
+                    // two records per round: both loads in flight together
+                    for (int j = 0; j < n; j += 2) {
+                        const int32_t la = q.x + j;
+                        const bool two = j + 1 < n;
+                        const int32_t lb = two ? la + 1 : la;
+                        const PrimRec A = prim_load(sc.tris, la);
+                        const PrimRec B = prim_load(sc.tris, lb);
+                        const uint32_t np = two ? 2u : 1u;
+                        cnt.prims_any += qany ? np : 0u;
+                        cnt.prims += qany ? 0u : np;
+                        if (prim_hit(A, o, d, ro.w, tmax)) best = la;
+                        if (two && prim_hit(B, o, d, ro.w, tmax)) best = lb;
+                    }
+                    if (best >= 0)
+                        __hip_atomic_fetch_min(cl.key + own, hit_key(tmax, best), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                }
             }
+            __builtin_amdgcn_wave_barrier();
+            if (r.pend) {
+                const unsigned long long k = cl.key[lane];
+                const uint32_t lo = static_cast<uint32_t>(k);
+                if (lo != 0xFFFFFFFFu && r.hit != -2) {
+                    r.hit = 0x7FFFFFFF - static_cast<int32_t>(lo);
+                    r.tmax = dist_of(static_cast<uint32_t>(k >> 32));
+                    cl.ray[2 * lane + 1].w = r.tmax;
+                    cl.key[lane] = (static_cast<unsigned long long>(ord_dist(r.tmax)) << 32) | 0xFFFFFFFFull;
+                    if (any) done = true;
+                }
+                r.pend = false;
+            }
+            if (r.parked) done = true;
+            qn = 0;
         }
     }
-    if (st.sp + nk - 1 > sc.max_stack) {   // bounded by the tree depth; never taken for a built tree
-        r.hit = -2;
-        return true;
-    }
-#pragma unroll
-    for (int k = W - 1; k >= 1; --k) {
-        if (k < nk) st.push(kid[k]);
-    }
-    r.node = nk > 0 ? kid[0] : -1;
-    if (nk == 0 && st.sp == 0) return true;
-    return false;
+    if (done) r.parked = false;
+    return done;
 }
 
 }  // namespace mpt

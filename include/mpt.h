@@ -141,9 +141,9 @@ typedef struct {
     int32_t leaf_max;          /* largest subtree folded into one leaf (default 2) */
     int32_t block;             /* traversal threads per block: 128 (default) or 256 */
     int32_t stack_lds;         /* per-lane stack entries kept in LDS: 16, 20, 24 or 32 (default 20
-                                  for width 4, 24 for width 8) */
+                                  for width 4, 16 for width 8) */
     int32_t blocks_per_cu;     /* resident traversal blocks per CU of the persistent grid (default 8
-                                  for width 4, 6 for width 8) */
+                                  for width 4, 7 for width 8) */
 } mpt_bvh_opts;
 
 typedef struct {
