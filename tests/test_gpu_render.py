@@ -92,9 +92,10 @@ def test_render_progressive_accumulate(ctx, passwise):
 def test_streaming_matches_passwise(ctx, case):
     """The streaming wavefront (default) and the pass-by-pass wavefront draw the
     same paths: ray and sample counts equal exactly, films up to float summation
-    order. Slab and primitive test counts agree to 0.5%: a ray's traversal work
-    depends on when its wave's deferred leaf round lowers its distance bound,
-    which depends on the other rays of the wave."""
+    order. Slab and primitive test counts agree to 2%: a ray's traversal work
+    depends on when its wave's deferred leaf round lowers its distance bound (or,
+    for a shadow ray, ends it at its first hit), which depends on the other rays
+    of the wave."""
     from cadise_amd.api import Scene
     spec = {"spheres": lambda: small_spheres(n_spheres=6, level=3, width=96, height=64, spp=5, max_depth=7),
             "cornell_gaussian": lambda: scenes.cornell_box(width=48, height=40, spp=6, max_depth=9, filter="gaussian"),
@@ -113,7 +114,7 @@ def test_streaming_matches_passwise(ctx, case):
     for k in ("camera_rays", "continuation_rays", "mis_rays", "shadow_rays", "samples"):
         assert sa[k] == sb[k], (k, sa[k], sb[k])
     for k in ("box_tests", "tri_tests", "box_tests_any", "tri_tests_any"):
-        assert abs(sa[k] - sb[k]) <= 0.005 * max(sa[k], sb[k]), (k, sa[k], sb[k])
+        assert abs(sa[k] - sb[k]) <= 0.02 * max(sa[k], sb[k]), (k, sa[k], sb[k])
     np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6)
 
 
