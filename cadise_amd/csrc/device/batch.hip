@@ -316,9 +316,9 @@ int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
     S.capacity_alloc = 0;   // until every buffer below has its new size
     HIP_TRY(ctx, S.path.alloc(sizeof(PathRec) * n));
     HIP_TRY(ctx, S.free_ids.alloc(sizeof(int32_t) * n));
-    const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
+    const size_t sizes[10] = {4, 16, 16, 16, 16, 16, 16, 8, 8, 8};
     for (int q = 0; q < 2; ++q) {
-        for (int k = 0; k < 12; ++k) HIP_TRY(ctx, S.qbuf[q][k].alloc(sizes[k] * n));
+        for (int k = 0; k < 10; ++k) HIP_TRY(ctx, S.qbuf[q][k].alloc(sizes[k] * n));
         HIP_TRY(ctx, S.rootq[q].alloc(sizeof(int32_t) * n));
     }
     if (!S.ctr.p) {
@@ -360,6 +360,7 @@ int enqueue_step(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StepShape&
     t.hit = Q.hit;
     t.hit2 = Q.mis_hit;
     t.want_prim = 0;
+    t.mis_emitters = 1;
     t.root = S.rootq[qc].as<int32_t>();
     t.work = bt.ctr + 2;
     t.spill = a.sc.max_stack > sh.cfg.stack ? S.spill.as<int32_t>() : nullptr;
@@ -431,11 +432,9 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
         Q.shd_dir = S.qbuf[q][4].as<float4>();
         Q.lt = S.qbuf[q][5].as<float4>();
         Q.fac = S.qbuf[q][6].as<float4>();
-        Q.bv = S.qbuf[q][7].as<float4>();
-        Q.light = S.qbuf[q][8].as<int32_t>();
-        Q.hit = S.qbuf[q][9].as<int2>();
-        Q.mis_hit = S.qbuf[q][10].as<int2>();
-        Q.occ = S.qbuf[q][11].as<int2>();
+        Q.hit = S.qbuf[q][7].as<int2>();
+        Q.mis_hit = S.qbuf[q][8].as<int2>();
+        Q.occ = S.qbuf[q][9].as<int2>();
     }
     a.qcount = S.qcount.as<int32_t>();
     a.npaths = cap;

@@ -138,7 +138,7 @@ struct DevBuf {
 // through a free stack, two queues, counters, and the recorded HIP graph.
 struct Streamer {
     DevBuf path, free_ids, ctr, work_next, qcount, stats, spill, views;
-    DevBuf qbuf[2][12], rootq[2];
+    DevBuf qbuf[2][10], rootq[2];
     DevBuf adjq[2][6], adjA;          // adjoint jobs
     int32_t capacity_alloc = 0, adj_alloc = 0;
     int64_t spill_lanes = 0;
@@ -195,7 +195,7 @@ struct mpt_scene {
     mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, env_cdf;
     // film + pass state
     mpt::DevBuf film, path, qcount, work, stats, spill, srgb;
-    mpt::DevBuf qbuf[2][12];
+    mpt::DevBuf qbuf[2][10];
     int64_t npaths = 0;
     // parameters that can change after the upload (differentiable rendering)
     std::vector<mpt::DevMaterial> h_mats;

@@ -166,6 +166,11 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                     int32_t idx = r.hit;
                     if (r.hit >= 0 && a.want_prim)
                         idx = static_cast<int32_t>(__float_as_uint(a.sc.tris[3 * r.hit].w) & 0x7FFFFFFFu);
+                    // a MIS-BSDF ray only counts when it reaches a light: its record was
+                    // just read by the leaf round, so the emitter check costs a cache hit
+                    // here instead of a gather in shading
+                    if (set == 1 && a.mis_emitters && r.hit >= 0 && __float_as_int(a.sc.tris[3 * r.hit + 2].w) < 0)
+                        idx = -2;
                     out = make_int2(r.hit >= 0 ? idx : -1, __float_as_int(r.tmax));
                 }
                 put(set, q, out);
@@ -350,7 +355,6 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     q.dir[p] = make_float4(dir.x, dir.y, dir.z, a.max_depth > 0 ? FLT_MAX : -1.0f);
     q.mis_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
     q.shd_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
-    q.bv[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (ADJ) {
         const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
         V3 A = v3(0.f, 0.f, 0.f);
@@ -442,6 +446,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    __shared__ int32_t s_wcount[SHADE_WAVES], s_wfree[SHADE_WAVES], s_qbase, s_fbase;
     __shared__ int   s_tkeys[ADJ ? SHADE_WAVES * TEX_SLOTS : 1];
     __shared__ float s_tvals[ADJ ? SHADE_WAVES * TEX_SLOTS * 3 : 1];
     __shared__ int   s_lkeys[ADJ ? SHADE_WAVES * LIGHT_SLOTS : 1];
@@ -459,8 +464,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         const bool active = s < n;
         bool pending = false, finished = false;
         int32_t p = 0;
-        float4 org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org,
-               bv4 = org;
+        float4 org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org;
         int32_t pick_light = 0;
         float4 beta4 = org;
         int32_t sid = 0, root_next = 0;
@@ -510,37 +514,39 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             bool count_next = false;
             V3 A = v3(0.f, 0.f, 0.f);
             if constexpr (ADJ) A = v3f4(g.A[p]);
-            // -- resolve the previous vertex's NEE
-            const float4 pv = cur.bv[s];
-            if (pb > 0 && pv.w > 0.0f) {
+            // -- resolve the previous vertex's NEE: its terms come pre-scaled by the
+            // path throughput over the light-pick pdf (L += beta * (Ld / pick),
+            // ptEstimator.cpp:94-99, distributed over Ld's two terms)
+            const float4 lt4r = pb > 0 ? cur.lt[s] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            const int32_t lw = __float_as_int(lt4r.w);
+            if (lw != -1) {
                 V3 Ld = v3(0.f, 0.f, 0.f);
-                const float4 sd = cur.shd_dir[s];
-                const int32_t li = cur.light[s];
-                if (sd.w >= 0.0f && cur.occ[s].x == 0) {
-                    Ld = Ld + v3f4(cur.lt[s]);
+                const int32_t li = lw >= 0 ? lw : -2 - lw;   // lw < -1: the shadow ray was absent
+                if (lw >= 0 && cur.occ[s].x == 0) {
+                    Ld = Ld + v3f4(lt4r);
                     if constexpr (ADJ) {
                         tval = tval + A * v3f4(g.q[qc].sL[s]);
                         lval = lval + A * v3f4(g.q[qc].eL[s]);
                     }
                 }
-                const float4 md4 = cur.mis_dir[s];
-                if (md4.w >= 0.0f) {
-                    const int2 mh = cur.mis_hit[s];
-                    if (mh.x < 0 && li == envl) {
+                const float4 f4 = cur.fac[s];
+                if (f4.w != -1.0f) {   // the MIS-BSDF ray was traced (a sample pdf is never -1)
+                    const int2 mh = cur.mis_hit[s];   // x: -1 background, -2 a primitive that emits nothing
+                    if (mh.x == -1 && li == envl) {
                         // the MIS-BSDF ray reaches the background, which the picked light lights
+                        const float4 md4 = cur.mis_dir[s];
                         const DevLight& lg = sc.lights[li];
                         const V3 Le = env_emittance(sc, lg, v3f4(md4));
-                        const float4 f4 = cur.fac[s];
                         const float lp = env_pdfW(sc, lg, v3f4(md4));
                         const float w = mis_power(f4.w, lp);
                         Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
                     }
                     if (mh.x >= 0) {
                         if (__float_as_int(sc.tris[3 * mh.x + 2].w) == li) {   // the record's area light
+                            const float4 md4 = cur.mis_dir[s];
                             const SurfaceHit h = surface_detail(sc, mh.x, P0, v3f4(md4), __int_as_float(mh.y));
                             const DevLight& lg = sc.lights[li];
                             const V3 Le = light_emittance(lg, h);
-                            const float4 f4 = cur.fac[s];
                             const float lp = light_pdfW(lg, h, P0);
                             const float w = mis_power(f4.w, lp);
                             Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
@@ -553,7 +559,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                         }
                     }
                 }
-                L = L + v3f4(pv) * divs(Ld, pv.w);
+                L = L + Ld;
                 if constexpr (ADJ) {
                     const float4 sc4 = g.q[qc].sC[s];
                     if (sc4.w > 0.0f) {
@@ -674,7 +680,20 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                         }
                     }
                 }
-                bv4 = make_float4(beta.x, beta.y, beta.z, pick);
+                // the NEE terms pre-scaled by beta / pick; lt4.w: the picked light (-2 - it
+                // when the shadow ray is absent, -1 without NEE); fac4.w: the MIS-BSDF
+                // sample's pdf (-1: no MIS-BSDF ray)
+                if (nee && pick > 0.0f) {
+                    const float ip = 1.0f / pick;
+                    const V3 lts = beta * (v3f4(lt4) * ip), fcs = beta * (v3f4(fac4) * ip);
+                    const bool shadow = sdir.w >= 0.0f;
+                    lt4 = make_float4(shadow ? lts.x : 0.f, shadow ? lts.y : 0.f, shadow ? lts.z : 0.f,
+                                      __int_as_float(shadow ? pick_light : -2 - pick_light));
+                    fac4 = make_float4(fcs.x, fcs.y, fcs.z, mdir.w >= 0.0f ? fac4.w : -1.0f);
+                } else {
+                    lt4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                    fac4 = make_float4(0.f, 0.f, 0.f, -1.0f);
+                }
                 // continuation
                 cdir = make_float4(0.f, 0.f, 0.f, -1.f);
                 {
@@ -755,19 +774,37 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             }
             __syncthreads();
         }
+        // finished paths give their slots back and pending ones go to the next queue:
+        // wave ballots, then one atomic per block for each (the block's waves' counts
+        // prefix-summed in LDS), so that the two global counters see a quarter of the
+        // requests a per-wave atomic would send
+        const unsigned long long fmask = BATCH ? __ballot(finished) : 0ull;
+        const unsigned long long mask = __ballot(pending);
+        if (lane == 0) {
+            s_wcount[wave] = __popcll(mask);
+            s_wfree[wave] = __popcll(fmask);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t tq = 0, tf = 0;
+            for (int w = 0; w < SHADE_WAVES; ++w) {
+                const int32_t cq = s_wcount[w], cf = s_wfree[w];
+                s_wcount[w] = tq;
+                s_wfree[w] = tf;
+                tq += cq;
+                tf += cf;
+            }
+            s_qbase = tq ? atomicAdd(ncount, tq) : 0;
+            if constexpr (BATCH) s_fbase = tf ? atomicAdd(&bt.ctr[0], tf) : 0;
+        }
+        __syncthreads();
         if constexpr (BATCH) {
-            // finished paths give their slots back (one atomic per wave)
-            const unsigned long long fmask = __ballot(finished);
-            int32_t fbase = 0;
-            if (lane == 0 && fmask) fbase = atomicAdd(&bt.ctr[0], __popcll(fmask));
-            fbase = __shfl(fbase, 0, WAVE);
+            const int32_t fbase = s_fbase + s_wfree[wave];
             if (finished) bt.free_ids[fbase + static_cast<int32_t>(__popcll(fmask & ((1ull << lane) - 1ull)))] = p;
         }
-        // wave-ballot compaction into the next queue
-        const unsigned long long mask = __ballot(pending);
-        int32_t base = 0;
-        if (lane == 0 && mask) base = atomicAdd(ncount, __popcll(mask));
-        base = __shfl(base, 0, WAVE);
+        // (the next iteration rewrites the LDS counts only after its first barrier,
+        // which every wave reaches after these reads)
+        const int32_t base = s_qbase + s_wcount[wave];
         if (pending) {
             const int32_t slot = base + static_cast<int32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
             nxt.path[slot] = p;
@@ -777,8 +814,6 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             nxt.shd_dir[slot] = sdir;
             nxt.lt[slot] = lt4;
             nxt.fac[slot] = fac4;
-            nxt.bv[slot] = bv4;
-            nxt.light[slot] = pick_light;
             if constexpr (BATCH) bt.root[qc ^ 1][slot] = root_next;
             if constexpr (ADJ) {
                 const AdjQueue& an = g.q[qc ^ 1];
@@ -858,7 +893,6 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArg
     q.dir[slot] = make_float4(dir.x, dir.y, dir.z, v.max_depth > 0 ? FLT_MAX : -1.0f);
     q.mis_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
     q.shd_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
-    q.bv[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     bt.root[qc][slot] = v.root;
     if constexpr (ADJ) {
         const int64_t si = static_cast<int64_t>(sample - v.sample_base) * npx + pix;

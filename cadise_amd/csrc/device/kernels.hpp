@@ -54,6 +54,7 @@ struct TraceArgs {
     int2*          hit3;
     int32_t        shadow;       // mixed launch: 1 when the third set is present
     int32_t        want_prim;    // closest: write the creation index instead of the leaf position
+    int32_t        mis_emitters; // second set: report a hit on a primitive without an area light as -2
     const int32_t* root;         // per-ray root node (index q of the first set), null = node 0
     int32_t*       work;         // atomic work counter, zero on entry
     int32_t*       spill;        // per-lane stack spill, (max_stack - S) entries per lane
@@ -69,10 +70,9 @@ struct Queue {
     float4*  dir;      // continuation ray (w = tmax, < 0 absent)
     float4*  mis_dir;  // MIS-BSDF ray
     float4*  shd_dir;  // shadow ray
-    float4*  lt;       // light-sample term of DirectLightEvaluator, unscaled
-    float4*  fac;      // MIS-BSDF factor f |L.N| (xyz), pdf (w)
-    float4*  bv;       // throughput at the vertex (xyz), light-pick pdf (w)
-    int32_t* light;    // picked light
+    float4*  lt;       // light-sample term of DirectLightEvaluator x throughput / pick pdf (xyz);
+                       // w (int bits): the picked light, -2 - it when the shadow ray is absent, -1 no NEE
+    float4*  fac;      // MIS-BSDF factor f |L.N| x throughput / pick pdf (xyz), sample pdf (w; -1: no ray)
     int2*    hit;      // continuation hit
     int2*    mis_hit;
     int2*    occ;      // shadow: x = occluded

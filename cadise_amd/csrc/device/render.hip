@@ -391,9 +391,9 @@ int ensure_pass_buffers(mpt_scene* S, int64_t npix, int32_t max_depth) {
     if (npix != S->npaths) {
         S->npaths = 0;   // until every buffer below has its new size
         HIP_TRY(ctx, S->path.alloc(sizeof(PathRec) * npix));
-        const size_t sizes[12] = {4, 16, 16, 16, 16, 16, 16, 16, 4, 8, 8, 8};
+        const size_t sizes[10] = {4, 16, 16, 16, 16, 16, 16, 8, 8, 8};
         for (int q = 0; q < 2; ++q)
-            for (int k = 0; k < 12; ++k) HIP_TRY(ctx, S->qbuf[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
+            for (int k = 0; k < 10; ++k) HIP_TRY(ctx, S->qbuf[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
         S->npaths = npix;
     }
     if (max_depth != S->max_depth_alloc) {
@@ -421,11 +421,9 @@ PassArgs pass_args(mpt_scene* S, uint64_t seed, int32_t max_depth) {
         Q.shd_dir = S->qbuf[q][4].as<float4>();
         Q.lt = S->qbuf[q][5].as<float4>();
         Q.fac = S->qbuf[q][6].as<float4>();
-        Q.bv = S->qbuf[q][7].as<float4>();
-        Q.light = S->qbuf[q][8].as<int32_t>();
-        Q.hit = S->qbuf[q][9].as<int2>();
-        Q.mis_hit = S->qbuf[q][10].as<int2>();
-        Q.occ = S->qbuf[q][11].as<int2>();
+        Q.hit = S->qbuf[q][7].as<int2>();
+        Q.mis_hit = S->qbuf[q][8].as<int2>();
+        Q.occ = S->qbuf[q][9].as<int2>();
     }
     a.qcount = S->qcount.as<int32_t>();
     a.work = S->work.as<int32_t>();
@@ -503,6 +501,7 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
             t.hit = Q.hit;
             t.hit2 = Q.mis_hit;
             t.want_prim = 0;
+            t.mis_emitters = 1;
             t.work = a.work + 3 * b;
             t.stats = S->stats.as<unsigned long long>();
             t.stat_slot = ST_CLOSEST_RAYS;
@@ -756,6 +755,7 @@ int do_adjoint_passes(mpt_scene* S, const mpt_render_params* p, const float* d_i
             t.hit = Q.hit;
             t.hit2 = Q.mis_hit;
             t.want_prim = 0;
+            t.mis_emitters = 1;
             t.work = a.work + 3 * b;
             t.stats = S->stats.as<unsigned long long>();
             t.stat_slot = ST_CLOSEST_RAYS;
