@@ -41,13 +41,11 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // sets and the shadow (any-hit) set from one work counter, so one ramp and one
 // tail of stragglers serve both kinds of query.
 template <int W, int S, int MODE>
-__global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
-    constexpr int STEPS = 8;
-    // reserve new prefetch slots once this many lanes lack one (measured on the 1080p
-    // frame at 4 waves per SIMD, ms per launch: BVH4 32 / 48 / 56 / 60 / 64 lanes ->
-    // 3.68 / 3.45 / 3.42 / 3.49 / 3.76; BVH8 (3 waves per SIMD) flat from 8 to 24;
-    // blocks of 6, 8 or 12 steps between reservations within 0.7%)
-    constexpr int REFILL = W == 4 ? 56 : 16;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_trace(TraceArgs a) {
+    // node visits between hand-outs of new rays (1080p frame, ms per launch, steps
+    // 2 / 3 / 4 / 6 / 8: BVH4 2.04 / 1.97 / 1.95 / 1.93 / 1.96, BVH8 2.25 / 2.21 /
+    // 2.22 / 2.31 / 2.44)
+    constexpr int STEPS = W == 4 ? 6 : 3;
     extern __shared__ int32_t lds_stack[];
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -75,8 +73,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     TravCounters cnt{0, 0, 0, 0};
     uint32_t rays = 0, rays2 = 0, rays3 = 0, errors = 0;
     // ray i: set i / n0 (0: dir -> hit, 1: dir2 -> hit2, 2: dir3 -> hit3, any hit), slot i % n0.
-    // Ray indices and the work counter are 32-bit: the host caps n at 2^31 - 2^24, so
-    // the counter (n plus at most one reservation of 64 per wave) stays below 2^31.
+    // Ray indices are 32-bit and the work counter unsigned 32-bit: the host caps n at
+    // 2^31 - 2^24, so the counter (n plus at most one chunk of TRACE_CHUNK <= 512 per
+    // wave, at most 2^15 waves) stays below 2^32.
     auto set_of = [&](int32_t i) { return i >= 2 * n0 ? 2 : (i >= n0 ? 1 : 0); };
     auto is_any = [&](int set) { return MODE == TRACE_ANY || (MODE == TRACE_MIXED && set == 2); };
     auto put = [&](int set, int64_t q, int2 out) {
@@ -93,7 +92,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
     int32_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
     float4 po = make_float4(0.f, 0.f, 0.f, 0.f), pd = po;
     int32_t proot = 0;           // its BVH root
-    bool exhausted = false;      // wave-uniform: the work counter has run out
+    bool exhausted = false;      // wave-uniform: the work counter has run out and the pool is empty
+    bool counter_done = false;   // wave-uniform: the work counter has run out
+    int32_t pool = 0, pool_end = 0;   // wave-uniform: rays reserved by this wave and not yet handed out
     // a finished lane takes its prefetched ray at once; invalid rays are answered here
     auto install = [&]() {
         while (rid < 0 && pid >= 0) {
@@ -123,18 +124,31 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
         }
     };
     while (true) {
-        // -- reserve and start loading rays for lanes without a prefetched one
-        const unsigned long long need = __ballot(pid < 0);
-        const int nneed = __popcll(need);
-        if (!exhausted && (nneed >= REFILL || __ballot(rid >= 0) == 0ull)) {
-            const int leader = __ffsll(static_cast<long long>(need)) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(a.work, nneed);
-            base = __shfl(base, leader, WAVE);
-            if (base + nneed >= n) exhausted = true;
-            if (pid < 0) {
-                const int32_t i = base + static_cast<int32_t>(__popcll(need & ((1ull << lane) - 1ull)));
-                if (i < n) {
+        // -- hand out rays to lanes without a prefetched one and start loading them:
+        // from the wave's pool, refilled TRACE_CHUNK at a time by one atomic (one
+        // atomic per refill of a few lanes serializes every wave of the GPU on the
+        // counter: 3.29 -> 1.93 ms per launch of the 1080p frame)
+        if (!exhausted && __ballot(pid < 0) != 0ull) {
+          for (int pass = 0; pass < 2; ++pass) {
+            if (pool == pool_end && !counter_done) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(reinterpret_cast<uint32_t*>(a.work), static_cast<uint32_t>(TRACE_CHUNK));
+                b = __shfl(b, 0, WAVE);
+                if (static_cast<int64_t>(b) >= n) {
+                    counter_done = true;
+                } else {
+                    pool = static_cast<int32_t>(b);
+                    pool_end = static_cast<int32_t>(min(static_cast<int64_t>(b) + TRACE_CHUNK, n));
+                }
+            }
+            const unsigned long long need = __ballot(pid < 0);
+            const int take = min(__popcll(need), pool_end - pool);
+            const int rank = static_cast<int>(__popcll(need & ((1ull << lane) - 1ull)));
+            const int32_t base = pool;
+            pool += take;
+            if (pid < 0 && rank < take) {
+                const int32_t i = base + rank;
+                {
                     const int set = set_of(i);
                     const int64_t q = i - set * n0;
                     pid = i;
@@ -143,6 +157,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceArgs a) {
                     proot = a.root ? a.root[q] : 0;
                 }
             }
+            if (__ballot(pid < 0) == 0ull || (counter_done && pool == pool_end)) break;
+          }
+          if (counter_done && pool == pool_end) exhausted = true;
         }
         install();
         if (__ballot(rid >= 0) == 0ull) {

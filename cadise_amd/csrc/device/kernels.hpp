@@ -28,6 +28,10 @@ enum StatSlot {
 // lane is left traversing (a multiple of 8: the queue's extra entries are
 // spread over the wave's 64 threads).
 constexpr int LEAF_DEFER = 32;
+// Rays a traversal wave reserves with one atomic on the launch's work counter
+// (1080p frame, ms per launch: 64 / 128 / 256 / 512 -> 3.19 / 2.40 / 2.14 / 2.07
+// when a wave refilled once 56 lanes lacked a ray; 1.93 refilling any lane).
+constexpr int TRACE_CHUNK = 512;
 constexpr int LEAF_PARK = 8;
 
 // LDS bytes per traversal thread beyond its stack entries: its ray (two

@@ -232,9 +232,10 @@ int  mpt_scene_light_radiance(const mpt_scene* scene, float* out);
 int  mpt_scene_light_pick_pdf(const mpt_scene* scene, float* out);
 
 /* Batch ray queries; host buffers (copied) or device buffers on a stream.
- * At most MPT_TRACE_MAX_RAYS rays per call (2^31 - 2^24: the traversal's 32-bit
- * work counter, which each wave may advance past n by one reservation, stays
- * below 2^31); more returns MPT_ERR_INVALID before any device work. */
+ * At most MPT_TRACE_MAX_RAYS rays per call (2^31 - 2^24: ray indices stay below
+ * 2^31 and the traversal's unsigned 32-bit work counter, which each wave may
+ * advance past n by one chunk of 512 rays, below 2^32); more returns
+ * MPT_ERR_INVALID before any device work. */
 #define MPT_TRACE_MAX_RAYS ((size_t)2147483648u - (size_t)16777216u)
 int  mpt_trace(mpt_scene* scene, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits);
 int  mpt_trace_device(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
