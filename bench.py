@@ -35,8 +35,9 @@ Reported (one JSON line):
                  average launch time (HIP events on the render stream), against
                  the 8 TB/s HBM peak; beside it the same launches' VALU work (16
                  lane-ops per slab test, 32 per primitive test) against 39.3 T
-                 lane-ops/s, and their HBM traffic measured by rocprofv3 at this
-                 exact launch shape (profiles/traffic.json) when it was profiled
+                 lane-ops/s, and their HBM traffic and issued VALU instructions
+                 measured by rocprofv3 at this exact launch shape and device
+                 source digest (profiles/traffic.json) when it was profiled
   cpu_baseline   the compiled reference (oracle/_ref, built from the reference's
                  sources) on this host's cores, on a bounded sample of the same
                  frame at 1 spp; the oracle port when the reference build is absent
@@ -44,6 +45,7 @@ Reported (one JSON line):
 from __future__ import annotations
 
 import argparse
+import hashlib
 import concurrent.futures as cf
 import ctypes as C
 import json
@@ -355,6 +357,18 @@ def bench_adjoint(ctx, adam_steps):
     return out
 
 
+def device_source_digest():
+    """First 12 hex digits of the SHA-256 over the device sources (sorted file
+    names and contents): a traffic record applies only to the kernels it measured."""
+    d = os.path.join(ROOT, "cadise_amd", "csrc", "device")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        h.update(f.encode())
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:12]
+
+
 def measured_traffic(kernel, shape):
     """rocprofv3's HBM bytes per launch for `kernel` at this launch shape
     (profiles/traffic.json), or None when that shape was not profiled."""
@@ -454,7 +468,8 @@ def main():
         f"{st['ms_camera']:.1f} ms over {launches} steps, {slots} path slots")
     kernel = f"k_trace<{st_scene['width']},{st_scene['stack_lds']},mixed>"
     shape = {"steps": args.steps, "warmup": args.warmup, "path_slots": slots, "leaf_max": st_scene["leaf_max"],
-             "block": st_scene["trace_block"], "blocks_per_cu": st_scene["blocks_per_cu"]}
+             "block": st_scene["trace_block"], "blocks_per_cu": st_scene["blocks_per_cu"],
+             "device_src": device_source_digest()}
     # the same frame pass by pass (every pass's paths bounce in lock step)
     dt_pw = None
     if not args.no_passwise:
@@ -483,7 +498,7 @@ def main():
         roof["hbm_measured_gbs"] = round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1)
         roof["hbm_measured_frac"] = round(traffic / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         roof["traffic_per_algorithmic_byte"] = round(traffic / per_launch_bytes, 3)
-        for k in ("l2_hit_rate", "valu_busy_pct", "avg_duration_us_rocprof"):
+        for k in ("l2_hit_rate", "valu_busy_pct", "valu_issued_frac", "avg_duration_us_rocprof"):
             if k in meas:
                 roof[k] = meas[k]
     else:

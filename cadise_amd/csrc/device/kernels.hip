@@ -76,6 +76,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     // Ray indices are 32-bit and the work counter unsigned 32-bit: the host caps n at
     // 2^31 - 2^24, so the counter (n plus at most one chunk of TRACE_CHUNK <= 512 per
     // wave, at most 2^15 waves) stays below 2^32.
+    // rays per reservation: TRACE_CHUNK for large launches; a smaller launch is cut
+    // so that every wave of the grid gets about TRACE_SPREAD reservations (at least
+    // TRACE_CHUNK_MIN rays each), else a few waves would take all of it
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / WAVE;
+    const int32_t chunk = static_cast<int32_t>(
+        min<int64_t>(TRACE_CHUNK, max<int64_t>(TRACE_CHUNK_MIN, n / (TRACE_SPREAD * nwaves))));
     auto set_of = [&](int32_t i) { return i >= 2 * n0 ? 2 : (i >= n0 ? 1 : 0); };
     auto is_any = [&](int set) { return MODE == TRACE_ANY || (MODE == TRACE_MIXED && set == 2); };
     auto put = [&](int set, int64_t q, int2 out) {
@@ -125,20 +131,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     };
     while (true) {
         // -- hand out rays to lanes without a prefetched one and start loading them:
-        // from the wave's pool, refilled TRACE_CHUNK at a time by one atomic (one
+        // from the wave's pool, refilled `chunk` rays at a time by one atomic (one
         // atomic per refill of a few lanes serializes every wave of the GPU on the
         // counter: 3.29 -> 1.93 ms per launch of the 1080p frame)
         if (!exhausted && __ballot(pid < 0) != 0ull) {
           for (int pass = 0; pass < 2; ++pass) {
             if (pool == pool_end && !counter_done) {
                 uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(reinterpret_cast<uint32_t*>(a.work), static_cast<uint32_t>(TRACE_CHUNK));
+                if (lane == 0) b = atomicAdd(reinterpret_cast<uint32_t*>(a.work), static_cast<uint32_t>(chunk));
                 b = __shfl(b, 0, WAVE);
                 if (static_cast<int64_t>(b) >= n) {
                     counter_done = true;
                 } else {
                     pool = static_cast<int32_t>(b);
-                    pool_end = static_cast<int32_t>(min(static_cast<int64_t>(b) + TRACE_CHUNK, n));
+                    pool_end = static_cast<int32_t>(min(static_cast<int64_t>(b) + chunk, n));
                 }
             }
             const unsigned long long need = __ballot(pid < 0);

@@ -32,6 +32,15 @@ constexpr int LEAF_DEFER = 32;
 // (1080p frame, ms per launch: 64 / 128 / 256 / 512 -> 3.19 / 2.40 / 2.14 / 2.07
 // when a wave refilled once 56 lanes lacked a ray; 1.93 refilling any lane).
 constexpr int TRACE_CHUNK = 512;
+// A launch of fewer than TRACE_SPREAD * TRACE_CHUNK rays per wave of its grid
+// reserves n / (TRACE_SPREAD * waves) rays at a time, but no fewer than
+// TRACE_CHUNK_MIN, so that small launches still spread over many waves
+// (sequential pass-by-pass renders of the 64-scene batch, s: a fixed 512 6.03;
+// n / (SPREAD * waves) with SPREAD 4 / 2 / 1 and at least 16 rays 5.09 / 4.68 /
+// 4.35; SPREAD 1 and at least 4 / 32 / 64 rays 4.61 / 4.27 / 4.17; the 1080p
+// frame's launches within 1%).
+constexpr int TRACE_SPREAD = 1;
+constexpr int TRACE_CHUNK_MIN = 64;
 constexpr int LEAF_PARK = 8;
 
 // LDS bytes per traversal thread beyond its stack entries: its ray (two
