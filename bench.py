@@ -26,8 +26,13 @@ Workloads (seeded, synthetic; no files except the committed Cornell-box image R)
                events), then 200 Adam steps of texture / radiance recovery.
 
 Reported (one JSON line):
-  value          all ray queries (camera, continuation, MIS-BSDF, shadow) / s of the
-                 procedural render, whole job, inputs resident in HBM
+  value          all ray queries of the reference's algorithm (camera, continuation,
+                 MIS-BSDF, shadow) answered per second on the procedural render,
+                 whole job, inputs resident in HBM. A MIS-BSDF query whose ray
+                 misses the picked area light's own primitive is answered in
+                 shading by that one primitive test (its closest hit cannot be the
+                 light; the reference traces it and finds the same zero term): it
+                 counts as a query, and traced_mrays_s counts traced rays only
   roofline       the traversal kernel (one mixed launch per wavefront step: the
                  continuation and MIS-BSDF rays closest hit, the shadow rays any
                  hit): algorithmic bytes per query (32 B ray + 16 B hit + 24 B per
@@ -125,7 +130,9 @@ def cpu_baseline(spec, seconds_target=15.0, seed=7):
 
 
 def all_rays(st):
-    return st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["shadow_rays"]
+    """Ray queries of the reference's algorithm: traced ones and the MIS-BSDF
+    queries answered by the picked light's own record."""
+    return st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["mis_resolved"] + st["shadow_rays"]
 
 
 def relmse(img, ref):
@@ -202,7 +209,7 @@ def bench_batch(ctx, relmse_target=None):
 def batch_time_to_relmse(ctx, specs, scs, seeds, best, target):
     """Per scene, the spp that reaches relMSE <= target, then the scenes at those
     spp rendered one after another with the default streaming renderer and as
-    one batch (the autotuned launch shape).
+    one batch (the autotuned launch shape), and one after another pass by pass.
 
     relMSE against the unknown converged image is estimated without a
     reference render from two independent renders A, B of equal spp (the
@@ -249,6 +256,16 @@ def batch_time_to_relmse(ctx, specs, scs, seeds, best, target):
             log(f"batch relMSE: {k + 1} sequential renders ({time.perf_counter() - t0:.1f} s)")
     sync()
     plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    prays = 0
+    for k, (sc, sd) in enumerate(zip(scs2, seeds2)):
+        sc.render(seed=sd, readback=False, passwise=True)
+        prays += all_rays(sc.last_stats())
+        if k % 16 == 15:
+            sync()
+            log(f"batch relMSE: {k + 1} sequential pass-by-pass renders ({time.perf_counter() - t0:.1f} s)")
+    sync()
+    passwise_s = time.perf_counter() - t0
     bb = Batch(ctx, scs2, batch_opts(**best))
     bb.render(seeds)        # records the graph for this work
     log("batch relMSE: graph recorded")
@@ -257,7 +274,8 @@ def batch_time_to_relmse(ctx, specs, scs, seeds, best, target):
     batch_s = time.perf_counter() - t0
     errs = [est(film_mean(bb.film(i)).astype(np.float64), film_mean(sc.render(seed=sd)).astype(np.float64))
             for i, (sc, sd) in enumerate(zip(scs2, seeds2))]
-    log(f"batch relMSE: sequential streaming {plain_s:.2f} s, batched {batch_s:.2f} s ({plain_s / batch_s:.2f}x); "
+    log(f"batch relMSE: sequential pass-by-pass {passwise_s:.2f} s, sequential streaming {plain_s:.2f} s, "
+        f"batched {batch_s:.2f} s ({passwise_s / batch_s:.2f}x, {plain_s / batch_s:.2f}x); "
         f"estimated relMSE at those spp: max {max(errs):.2e}, mean {np.mean(errs):.2e}")
     del bb, scs2
     ctx.release_stream_buffers()
@@ -265,9 +283,11 @@ def batch_time_to_relmse(ctx, specs, scs, seeds, best, target):
             "estimate": "relMSE to the converged image from two independent renders of equal spp: "
                         "mean((A - B)^2 / 2 / (((A + B) / 2)^2 + 0.01))",
             "spp_min": int(min(need)), "spp_max": int(max(need)), "samples": samples,
-            "plain_streaming_s": round(plain_s, 3), "batched_s": round(batch_s, 3),
+            "plain_passwise_s": round(passwise_s, 3), "plain_streaming_s": round(plain_s, 3),
+            "batched_s": round(batch_s, 3), "speedup_vs_passwise": round(passwise_s / batch_s, 2),
             "speedup_vs_streaming": round(plain_s / batch_s, 2), "relmse_at_spp_max": max(errs),
             "relmse_at_spp_mean": float(np.mean(errs)), "batched_rays": all_rays(stats), "plain_rays": rays,
+            "plain_passwise_rays": prays,
             "relmse_at_configured_spp_mean": float(np.mean(base))}
 
 
@@ -449,13 +469,13 @@ def main():
     sync()
     dt = time.perf_counter() - t0
     st = scene.last_stats()
-    closest = st["camera_rays"] + st["continuation_rays"] + st["mis_rays"]
-    rays = closest + st["shadow_rays"]
+    rays = all_rays(st)
+    traced = rays - st["mis_resolved"]
     mrays = rays / dt / 1e6
-    # roofline of the traversal kernel: one mixed launch per step traces every query
+    # roofline of the traversal kernel: one mixed launch per step traces every traced query
     boxes = st["box_tests"] + st["box_tests_any"]
     prims = st["tri_tests"] + st["tri_tests_any"]
-    bytes_trace = 48.0 * rays + 24.0 * boxes + 36.0 * prims
+    bytes_trace = 48.0 * traced + 24.0 * boxes + 36.0 * prims
     ops_trace = 16.0 * boxes + 32.0 * prims
     launches = max(1, st["closest_launches"])
     per_launch_bytes = bytes_trace / launches
@@ -485,8 +505,8 @@ def main():
             "kernel": f"{kernel} (closest hit + shadow any hit, one launch per step)",
             "launches": launches, "per_launch_bytes": round(per_launch_bytes),
             "avg_launch_ms": round(avg_launch_ms, 4),
-            "box_tests_per_ray": round(boxes / max(rays, 1), 2),
-            "prim_tests_per_ray": round(prims / max(rays, 1), 2),
+            "box_tests_per_ray": round(boxes / max(traced, 1), 2),
+            "prim_tests_per_ray": round(prims / max(traced, 1), 2),
             "valu_ops_per_launch": round(ops_trace / launches),
             "valu_achieved_tops": round(valu_tops, 3), "valu_peak_tops": VALU_PEAK_TOPS,
             "valu_frac": round(valu_tops / VALU_PEAK_TOPS, 4)}
@@ -534,9 +554,10 @@ def main():
                    "trace_launch": f"{st_scene['blocks_per_cu']} blocks of {st_scene['trace_block']} per CU, "
                                    f"{st_scene['stack_lds']} LDS stack entries per lane"},
         "samples_per_s": round(st["samples"] / dt, 1),
+        "traced_mrays_s": round(traced / dt / 1e6, 2),
         "passwise_mrays_s": round(rays / dt_pw / 1e6, 1) if dt_pw else None,
         "rays": {"camera": st["camera_rays"], "continuation": st["continuation_rays"], "mis_bsdf": st["mis_rays"],
-                 "shadow": st["shadow_rays"]},
+                 "mis_bsdf_resolved_by_light_record": st["mis_resolved"], "shadow": st["shadow_rays"]},
         "kernel_ms": {"trace": round(st["ms_closest"], 3),
                       "shade": round(st["ms_shade"], 3), "regenerate": round(st["ms_camera"], 3),
                       "gpu_total": round(st["ms_gpu"], 3)},

@@ -195,6 +195,7 @@ int merge_tables(mpt_batch* B) {
                 l.env_cdf += env0;
             } else {
                 l.prim += static_cast<int32_t>(np);
+                l.rec += static_cast<int32_t>(np);   // the scene's records start at np in the merged array
             }
             lights.push_back(l);
         }
@@ -451,6 +452,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     bt.ctr = S.ctr.as<int32_t>();
     bt.work_next = S.work_next.as<unsigned long long>();
     bt.stats = S.stats.as<unsigned long long>();
+    a.stats = bt.stats;
     const int steps = job.graph_steps;
     AdjArgs g = job.adj;
     const AdjArgs* gp = nullptr;
@@ -558,6 +560,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
             r.camera_rays += static_cast<int64_t>(job.views[i].img_w) * job.views[i].img_h * job.views[i].nsamples;
     r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
     r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
+    r.mis_resolved = static_cast<int64_t>(st[ST_MIS_RESOLVED]);
     r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
     r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
     r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);

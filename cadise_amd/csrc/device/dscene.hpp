@@ -66,6 +66,7 @@ struct DevLight {
     // environment light (type 2): radiance texture and sampling tables at
     // env_cdf: marginal CDF (env_h + 1), then env_h conditional CDFs (env_w + 1)
     int32_t env_tex, env_w, env_h, env_cdf;
+    int32_t rec;         // leaf position of the primitive's record in DevScene::tris (-1: environment)
 };
 
 constexpr int32_t LIGHT_TYPE_ENV = 2;

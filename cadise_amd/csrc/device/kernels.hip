@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     LaneStack<S> st;
     st.lds = (__attribute__((address_space(3))) int32_t*)lds_stack;
     st.lane = tid;
-    st.stride = blockDim.x;
+    st.shift = 31 - __builtin_clz(blockDim.x);   // blocks of 128 or 256 threads
     st.spill = a.spill ? a.spill + gslot * (a.sc.max_stack - S) : nullptr;
     st.sp = 0;
     // after the stacks: per thread two float4 (ray) and a key, then per wave a
@@ -481,6 +481,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         ltab.clear_lane(lane);
         __syncthreads();
     }
+    uint32_t mis_resolved = 0;   // MIS-BSDF queries answered without tracing
     // block-uniform trip count: every lane reaches the ballot and the barriers
     for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
         const int64_t s = bbase + threadIdx.x;
@@ -690,7 +691,17 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                         const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1), uc_mis);
                         if (bs.valid) {
                             const V3 mdv = normalize(bs.L);
-                            mdir = make_float4(mdv.x, mdv.y, mdv.z, FLT_MAX);
+                            // the term counts only when the ray's closest hit is the picked
+                            // area light's primitive, so the ray must hit that primitive: one
+                            // that misses it (the test the traversal would run on its record,
+                            // same origin, tmin and direction) is answered here, untraced
+                            bool traced = true;
+                            if (lg.rec >= 0) {
+                                float tl = FLT_MAX;
+                                traced = prim_hit(prim_load(sc.tris, lg.rec), h.P, mdv, K_RAY_EPS, tl);
+                                mis_resolved += traced ? 0u : 1u;
+                            }
+                            mdir = make_float4(mdv.x, mdv.y, mdv.z, traced ? FLT_MAX : -1.0f);
                             const V3 fc = bs.f * absdot(bs.L, h.Ns);
                             fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
                             if constexpr (ADJ) {
@@ -848,6 +859,16 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 an.sC[slot] = sC4;
             }
         }
+    }
+    // one atomic per block for the queries answered here (the waves' sums in LDS)
+    const unsigned long long wres = wave_sum(mis_resolved);
+    __syncthreads();   // every wave has read the last iteration's LDS counts
+    if (lane == 0) s_wcount[wave] = static_cast<int32_t>(wres);
+    __syncthreads();
+    if (threadIdx.x == 0 && a.stats) {
+        unsigned long long t = 0;
+        for (int w = 0; w < SHADE_WAVES; ++w) t += static_cast<unsigned long long>(s_wcount[w]);
+        if (t) atomicAdd(&a.stats[ST_MIS_RESOLVED], t);
     }
 }
 

@@ -20,6 +20,7 @@ enum StatSlot {
     ST_BOXES_ANY,         // slab tests of any-hit queries
     ST_PRIMS_ANY,         // primitive tests of any-hit queries
     ST_ERRORS,            // traversal stack overruns (must stay 0)
+    ST_MIS_RESOLVED,      // MIS-BSDF queries answered by the picked light's own record (not traced)
     ST_COUNT
 };
 
@@ -176,6 +177,7 @@ struct PassArgs {
     int32_t* work;         // traversal work counters, 3 per bounce
     int32_t  nwork;
     float4*  film;         // sum w*rgb, sum w
+    unsigned long long* stats;   // StatSlot counters (shading adds ST_MIS_RESOLVED)
 };
 
 }  // namespace mpt

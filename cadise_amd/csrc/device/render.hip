@@ -165,6 +165,9 @@ int build_and_upload(mpt_scene* S) {
         texs[i] = d;
     }
     const std::vector<F3> rad = light_radiance(h);
+    // leaf position of each primitive's record (the records follow the binary BVH's order)
+    std::vector<int32_t> leaf_pos(S->bin.ordered.size());
+    for (size_t j = 0; j < S->bin.ordered.size(); ++j) leaf_pos[static_cast<size_t>(S->bin.ordered[j])] = static_cast<int32_t>(j);
     std::vector<DevLight> lights(h.lights.size());
     std::vector<float> envcdf;
     int32_t env_light = -1;
@@ -173,6 +176,7 @@ int build_and_upload(mpt_scene* S) {
         if (h.lights[l].type == LIGHT_ENVIRONMENT) {
             const Texture& t = h.textures[h.lights[l].env_tex];
             d.prim = -1;
+            d.rec = -1;
             d.type = LIGHT_TYPE_ENV;
             d.env_tex = h.lights[l].env_tex;
             d.env_w = t.width;
@@ -187,6 +191,7 @@ int build_and_upload(mpt_scene* S) {
         const int32_t p = h.lights[l].prim;
         const float* v = &h.prim_v[9 * p];
         d.prim = p;
+        d.rec = leaf_pos[static_cast<size_t>(p)];
         d.type = h.prim_type[p];
         d.back_face = h.lights[l].back_face;
         d.radiance[0] = rad[l].x; d.radiance[1] = rad[l].y; d.radiance[2] = rad[l].z;
@@ -429,6 +434,7 @@ PassArgs pass_args(mpt_scene* S, uint64_t seed, int32_t max_depth) {
     a.work = S->work.as<int32_t>();
     a.nwork = 3 * (max_depth + 2);
     a.film = S->film.as<float4>();
+    a.stats = S->stats.as<unsigned long long>();
     return a;
 }
 
@@ -550,6 +556,7 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
     r.camera_rays = npix * (max_depth > 0 ? passes : 0);
     r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
     r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
+    r.mis_resolved = static_cast<int64_t>(st[ST_MIS_RESOLVED]);
     r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
     r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
     r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);

@@ -106,10 +106,10 @@ struct LaneStack {
     __attribute__((address_space(3))) int32_t* lds;   // this block's LDS stack (LDS-typed: ds_read / ds_write)
     int32_t* spill;    // this lane's spill slice (may be null when max <= S)
     int      lane;     // thread index in block
-    int      stride;   // threads per block
+    int      shift;    // log2(threads per block): entry e of this lane is lds[(e << shift) + lane]
     int      sp;
     __device__ __forceinline__ void push(int32_t v) {
-        if (sp < S) lds[sp * stride + lane] = v;
+        if (sp < S) lds[(sp << shift) + lane] = v;
         else spill[sp - S] = v;
         ++sp;
     }
@@ -117,7 +117,7 @@ struct LaneStack {
         --sp;
         // the LDS read always (clamped index), the spill read only past S: two
         // loads of their own address spaces instead of one generic (flat) load
-        int32_t v = lds[min(sp, S - 1) * stride + lane];
+        int32_t v = lds[(min(sp, S - 1) << shift) + lane];
         if (sp >= S) v = spill[sp - S];
         return v;
     }

@@ -191,6 +191,9 @@ typedef struct {
     double  ms_gpu;               /* GPU time of the call's kernels (events; mpt_batch_render: the
                                      streaming loop) */
     int64_t path_slots;           /* streaming renderer: path slots in flight (0 pass by pass) */
+    int64_t mis_resolved;         /* MIS-BSDF queries answered without traversal: the ray misses the
+                                     picked area light's own primitive, so its closest hit cannot be
+                                     that light (mis_rays counts the traced ones) */
 } mpt_stats;
 
 /* device >= 0: a GPU context. device == -1: a host-only context, in which

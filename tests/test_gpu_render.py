@@ -71,6 +71,38 @@ def test_render_spheres_ggx_vs_oracle(ctx, tmp_path):
     _compare(film, orf, "spheres ggx", frac_bar=0.98)
 
 
+@pytest.mark.parametrize("case", ["cornell", "spheres", "sky", "zoo"])
+@pytest.mark.parametrize("passwise", [False, True])
+def test_mis_queries_accounted_against_oracle(ctx, case, passwise):
+    """A MIS-BSDF query whose ray misses the picked area light's own primitive
+    is answered in shading without traversal (its closest hit cannot be that
+    light). The oracle traces every query of the same paths (Philox layout), so
+    the film agrees with it and every query is accounted for: camera +
+    continuation + traced MIS-BSDF + resolved MIS-BSDF queries = the oracle's
+    closest-hit queries, shadow rays = its any-hit queries (within 1e-3: a rare
+    last-ulp flip of a discrete decision changes a path's length)."""
+    from cadise_amd.api import Scene
+    spec = {"cornell": lambda: scenes.cornell_box(width=48, height=48, spp=4, max_depth=8),
+            "spheres": lambda: small_spheres(n_spheres=6, level=3, width=64, height=48, spp=4, max_depth=7),
+            "sky": lambda: scenes.sky_spheres(n_spheres=5, level=2, width=48, height=32, spp=4, max_depth=6,
+                                              env_size=(128, 64), sun_diameter_deg=4.0),
+            "zoo": lambda: scenes.material_zoo(width=48, height=36, spp=4, max_depth=12)}[case]()
+    sc = Scene.from_spec(ctx, spec)
+    film = sc.render(seed=77, passwise=passwise)
+    st = sc.last_stats()
+    orc = OracleScene(spec)
+    _compare(film, orc.render_linear(77, rng_mode=OR_RNG_PHILOX), f"mis {case}")
+    _, rays, samples = orc.time_render(0, spec.width * spec.height, spec.spp, 8, seed=77)
+    closest = st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["mis_resolved"]
+    print(f"[mis {case}] traced MIS {st['mis_rays']} resolved {st['mis_resolved']}; closest {closest} "
+          f"oracle {rays[0]}; shadow {st['shadow_rays']} oracle {rays[1]}", flush=True)
+    assert samples == st["samples"]
+    assert abs(closest - rays[0]) <= 1e-3 * rays[0]
+    assert abs(st["shadow_rays"] - rays[1]) <= 1e-3 * rays[1]
+    if case != "sky":
+        assert st["mis_resolved"] > 0
+
+
 @pytest.mark.parametrize("passwise", [False, True])
 def test_render_progressive_accumulate(ctx, passwise):
     """Passes [0, 2) + [2, 4) accumulated equal one render of [0, 4): bit for
@@ -111,7 +143,7 @@ def test_streaming_matches_passwise(ctx, case):
     sa = sc.last_stats()
     b = sc.render(seed=21, passwise=True)
     sb = sc.last_stats()
-    for k in ("camera_rays", "continuation_rays", "mis_rays", "shadow_rays", "samples"):
+    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "samples"):
         assert sa[k] == sb[k], (k, sa[k], sb[k])
     for k in ("box_tests", "tri_tests", "box_tests_any", "tri_tests_any"):
         assert abs(sa[k] - sb[k]) <= 0.02 * max(sa[k], sb[k]), (k, sa[k], sb[k])
