@@ -50,7 +50,7 @@ __global__ void k_batch_init(int32_t* free_ids, int32_t capacity, int32_t* ctr, 
         ctr[0] = capacity;
         ctr[1] = 0;
         ctr[2] = ctr[3] = 0;
-        qcount[0] = qcount[1] = 0;
+        qcount[0] = qcount[1] = qcount[2] = qcount[3] = 0;
         *work_next = 0ull;
     }
     if (i < ST_COUNT) stats[i] = 0ull;
@@ -317,15 +317,14 @@ int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
     S.capacity_alloc = 0;   // until every buffer below has its new size
     HIP_TRY(ctx, S.path.alloc(sizeof(PathRec) * n));
     HIP_TRY(ctx, S.free_ids.alloc(sizeof(int32_t) * n));
-    const size_t sizes[10] = {4, 16, 16, 16, 16, 16, 16, 8, 8, 8};
     for (int q = 0; q < 2; ++q) {
-        for (int k = 0; k < 10; ++k) HIP_TRY(ctx, S.qbuf[q][k].alloc(sizes[k] * n));
+        for (int k = 0; k < QBUFS; ++k) HIP_TRY(ctx, S.qbuf[q][k].alloc(kQueueBytes[k] * n));
         HIP_TRY(ctx, S.rootq[q].alloc(sizeof(int32_t) * n));
     }
     if (!S.ctr.p) {
         HIP_TRY(ctx, S.ctr.alloc(sizeof(int32_t) * 4));
         HIP_TRY(ctx, S.work_next.alloc(sizeof(unsigned long long)));
-        HIP_TRY(ctx, S.qcount.alloc(sizeof(int32_t) * 2));
+        HIP_TRY(ctx, S.qcount.alloc(sizeof(int32_t) * 4));
         HIP_TRY(ctx, S.stats.alloc(sizeof(unsigned long long) * ST_COUNT));
     }
     S.capacity_alloc = cap;
@@ -349,28 +348,13 @@ int enqueue_step(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StepShape&
     else hipLaunchKernelGGL(k_bregen<false>, dim3(sh.rblocks), dim3(256), 0, stream, a, bt, AdjArgs{}, qc);
     hipLaunchKernelGGL(k_bcommit, dim3(1), dim3(64), 0, stream, a, bt, qc);
     HIP_TRY(ctx, rec(SK_REGEN, 1));
-    const Queue& Q = a.q[qc];
-    TraceArgs t{};
-    t.sc = a.sc;
-    t.org = Q.org;
-    t.dir = Q.dir;
-    t.dir2 = Q.mis_dir;
-    t.stride = 1;
-    t.count = a.qcount + qc;
-    t.dual = 1;
-    t.hit = Q.hit;
-    t.hit2 = Q.mis_hit;
-    t.want_prim = 0;
-    t.mis_emitters = 1;
+    // continuation rays, then the list's shadow rays (any-hit lanes beside
+    // closest-hit lanes) and traced MIS-BSDF rays, in one launch
+    TraceArgs t = queue_trace_args(a.sc, a.q[qc], a.qcount + 2 * qc, true);
     t.root = S.rootq[qc].as<int32_t>();
     t.work = bt.ctr + 2;
     t.spill = a.sc.max_stack > sh.cfg.stack ? S.spill.as<int32_t>() : nullptr;
     t.stats = bt.stats;
-    t.stat_slot = ST_CLOSEST_RAYS;
-    // the shadow rays join the same launch (any-hit lanes beside closest-hit lanes)
-    t.dir3 = Q.shd_dir;
-    t.hit3 = Q.occ;
-    t.shadow = 1;
     HIP_TRY(ctx, rec(SK_CLOSEST, 0));
     launch_trace_cfg(sh.cfg, t, TRACE_MIXED, sh.tblocks, stream);
     HIP_TRY(ctx, rec(SK_CLOSEST, 1));
@@ -424,19 +408,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     PassArgs a{};
     a.sc = job.sc;
     a.path = S.path.as<PathRec>();
-    for (int q = 0; q < 2; ++q) {
-        Queue& Q = a.q[q];
-        Q.path = S.qbuf[q][0].as<int32_t>();
-        Q.org = S.qbuf[q][1].as<float4>();
-        Q.dir = S.qbuf[q][2].as<float4>();
-        Q.mis_dir = S.qbuf[q][3].as<float4>();
-        Q.shd_dir = S.qbuf[q][4].as<float4>();
-        Q.lt = S.qbuf[q][5].as<float4>();
-        Q.fac = S.qbuf[q][6].as<float4>();
-        Q.hit = S.qbuf[q][7].as<int2>();
-        Q.mis_hit = S.qbuf[q][8].as<int2>();
-        Q.occ = S.qbuf[q][9].as<int2>();
-    }
+    for (int q = 0; q < 2; ++q) a.q[q] = queue_view(S.qbuf[q]);
     a.qcount = S.qcount.as<int32_t>();
     a.npaths = cap;
     BatchArgs bt{};

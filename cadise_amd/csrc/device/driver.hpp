@@ -134,11 +134,59 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// A queue's arrays, bytes per path slot: path, org, dir, mis_dir, lt, fac, hit,
+// mis_hit, occ, then its ray list of up to two rays per slot: lorg, ldir, lq, lroot.
+constexpr int QBUFS = 13;
+constexpr size_t kQueueBytes[QBUFS] = {4, 16, 16, 16, 16, 16, 8, 8, 8, 32, 32, 8, 8};
+
+inline Queue queue_view(const DevBuf* b) {
+    Queue Q;
+    Q.path = b[0].as<int32_t>();
+    Q.org = b[1].as<float4>();
+    Q.dir = b[2].as<float4>();
+    Q.mis_dir = b[3].as<float4>();
+    Q.lt = b[4].as<float4>();
+    Q.fac = b[5].as<float4>();
+    Q.hit = b[6].as<int2>();
+    Q.mis_hit = b[7].as<int2>();
+    Q.occ = b[8].as<int2>();
+    Q.lorg = b[9].as<float4>();
+    Q.ldir = b[10].as<float4>();
+    Q.lq = b[11].as<int32_t>();
+    Q.lroot = b[12].as<int32_t>();
+    return Q;
+}
+
+// The rays of a renderer's traversal launch on queue Q: its entries'
+// continuation rays, then its ray list (shadow and MIS-BSDF rays); counts at
+// qc2[0] (entries) and qc2[1] (list rays). Launched TRACE_MIXED (TRACE_CLOSEST
+// only while the list is empty: a list's shadow rays need the any-hit lanes).
+inline TraceArgs queue_trace_args(const DevScene& sc, const Queue& Q, const int32_t* qc2, bool roots) {
+    TraceArgs t{};
+    t.sc = sc;
+    t.org = Q.org;
+    t.dir = Q.dir;
+    t.stride = 1;
+    t.count = qc2;
+    t.hit = Q.hit;
+    t.lcount = qc2 + 1;
+    t.lorg = Q.lorg;
+    t.ldir = Q.ldir;
+    t.lq = Q.lq;
+    t.lroot = roots ? Q.lroot : nullptr;
+    t.hit2 = Q.mis_hit;
+    t.hit3 = Q.occ;
+    t.want_prim = 0;
+    t.mis_emitters = 1;
+    t.stat_slot = ST_CLOSEST_RAYS;
+    return t;
+}
+
 // Path state of the streaming wavefront (batch.hip): path slots recycled
 // through a free stack, two queues, counters, and the recorded HIP graph.
 struct Streamer {
     DevBuf path, free_ids, ctr, work_next, qcount, stats, spill, views;
-    DevBuf qbuf[2][10], rootq[2];
+    DevBuf qbuf[2][QBUFS], rootq[2];
     DevBuf adjq[2][6], adjA;          // adjoint jobs
     int32_t capacity_alloc = 0, adj_alloc = 0;
     int64_t spill_lanes = 0;
@@ -195,7 +243,7 @@ struct mpt_scene {
     mpt::DevBuf nodes, tris, prim_info, prim_n, prim_uvw, mats, texs, tex_data, lights, power_cdf, env_cdf;
     // film + pass state
     mpt::DevBuf film, path, qcount, work, stats, spill, srgb;
-    mpt::DevBuf qbuf[2][10];
+    mpt::DevBuf qbuf[2][mpt::QBUFS];
     int64_t npaths = 0;
     // parameters that can change after the upload (differentiable rendering)
     std::vector<mpt::DevMaterial> h_mats;

@@ -69,10 +69,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                                                                (WAVE * W + LEAF_DEFER) * wv);
     }
     const int64_t n0 = a.count ? static_cast<int64_t>(*a.count) : a.n_host;
-    const int64_t n = n0 * (1 + a.dual + (MODE == TRACE_MIXED ? a.shadow : 0));
+    const int64_t n = n0 + (a.lcount ? static_cast<int64_t>(*a.lcount) : 0);
     TravCounters cnt{0, 0, 0, 0};
     uint32_t rays = 0, rays2 = 0, rays3 = 0, errors = 0;
-    // ray i: set i / n0 (0: dir -> hit, 1: dir2 -> hit2, 2: dir3 -> hit3, any hit), slot i % n0.
+    // ray i: for i < n0 the first set's ray i (set 0: dir -> hit), then list ray i - n0,
+    // whose lq entry names its set (1: MIS-BSDF -> hit2, 2: shadow -> hit3) and slot.
     // Ray indices are 32-bit and the work counter unsigned 32-bit: the host caps n at
     // 2^31 - 2^24, so the counter (n plus at most one chunk of TRACE_CHUNK <= 512 per
     // wave, at most 2^15 waves) stays below 2^32.
@@ -82,9 +83,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / WAVE;
     const int32_t chunk = static_cast<int32_t>(
         min<int64_t>(TRACE_CHUNK, max<int64_t>(TRACE_CHUNK_MIN, n / (TRACE_SPREAD * nwaves))));
-    auto set_of = [&](int32_t i) { return i >= 2 * n0 ? 2 : (i >= n0 ? 1 : 0); };
+    // a ray's set from its index and its destination word (i for the first set, else lq)
+    auto set_of = [&](int32_t i, int32_t dst) { return i < n0 ? 0 : (dst < 0 ? 1 : 2); };
     auto is_any = [&](int set) { return MODE == TRACE_ANY || (MODE == TRACE_MIXED && set == 2); };
-    auto put = [&](int set, int64_t q, int2 out) {
+    auto put = [&](int set, int32_t q, int2 out) {
         if (set == 0) a.hit[q] = out;
         else if (set == 1) a.hit2[q] = out;
         else a.hit3[q] = out;
@@ -96,6 +98,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     int qn = 0;                  // wave-uniform: leaves in the wave's queue
     int32_t rid = -1;            // ray of this lane, -1 = idle
     int32_t pid = -1;            // prefetched ray (loads in flight or landed), -1 = none
+    int32_t rdst = 0, pdst = 0;  // their destination words: i, or the list ray's lq entry
     float4 po = make_float4(0.f, 0.f, 0.f, 0.f), pd = po;
     int32_t proot = 0;           // its BVH root
     bool exhausted = false;      // wave-uniform: the work counter has run out and the pool is empty
@@ -106,15 +109,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
         while (rid < 0 && pid >= 0) {
             const int32_t i = pid;
             pid = -1;
-            const int set = set_of(i);
-            const int64_t q = i - set * n0;
+            const int set = set_of(i, pdst);
             if (pd.w < 0.0f || no_tree) {
-                put(set, q, make_int2(is_any(set) ? 0 : -1, __float_as_int(pd.w)));
+                put(set, pdst & 0x7FFFFFFF, make_int2(is_any(set) ? 0 : -1, __float_as_int(pd.w)));
             } else {
                 rays += set == 0 ? 1 : 0;
                 rays2 += set == 1 ? 1 : 0;
                 rays3 += set == 2 ? 1 : 0;
                 rid = i;
+                rdst = pdst;
                 r.any = is_any(set);
                 r.inv = safe_inv(v3f4(pd));
                 r.tmax = pd.w;
@@ -154,13 +157,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
             pool += take;
             if (pid < 0 && rank < take) {
                 const int32_t i = base + rank;
-                {
-                    const int set = set_of(i);
-                    const int64_t q = i - set * n0;
-                    pid = i;
-                    po = a.org[q * a.stride];
-                    pd = (set == 0 ? a.dir : set == 1 ? a.dir2 : a.dir3)[q * a.stride];
-                    proot = a.root ? a.root[q] : 0;
+                pid = i;
+                if (i < n0) {
+                    const int64_t o = static_cast<int64_t>(i) * a.stride;
+                    po = a.org[o];
+                    pd = a.dir[o];
+                    proot = a.root ? a.root[i] : 0;
+                    pdst = i;
+                } else {   // the list's loads do not depend on each other
+                    const int32_t k = i - static_cast<int32_t>(n0);
+                    po = a.lorg[k];
+                    pd = a.ldir[k];
+                    proot = a.lroot ? a.lroot[k] : 0;
+                    pdst = a.lq[k];
                 }
             }
             if (__ballot(pid < 0) == 0ull || (counter_done && pool == pool_end)) break;
@@ -180,8 +189,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                     errors += 1;
                     r.hit = -1;
                 }
-                const int set = set_of(rid);
-                const int64_t q = rid - set * n0;
+                const int set = set_of(rid, rdst);
+                const int32_t q = rdst & 0x7FFFFFFF;
                 int2 out;
                 if (is_any(set)) {
                     out = make_int2(r.hit >= 0 ? 1 : 0, 0);
@@ -362,7 +371,10 @@ template <bool ADJ>
 __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (gid < a.nwork) a.work[gid] = 0;
-    if (gid < a.max_depth + 2) a.qcount[gid] = gid == 0 ? a.npaths : 0;
+    if (gid < a.max_depth + 2) {   // queue entries and list rays per bounce
+        a.qcount[2 * gid] = gid == 0 ? a.npaths : 0;
+        a.qcount[2 * gid + 1] = 0;
+    }
     if (gid >= a.npaths) return;
     const int32_t p = static_cast<int32_t>(gid);
     const DevScene& sc = a.sc;
@@ -376,8 +388,6 @@ __global__ void __launch_bounds__(256) k_camera(PassArgs a, AdjArgs g) {
     q.path[p] = p;
     q.org[p] = make_float4(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2], K_RAY_EPS);
     q.dir[p] = make_float4(dir.x, dir.y, dir.z, a.max_depth > 0 ? FLT_MAX : -1.0f);
-    q.mis_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
-    q.shd_dir[p] = make_float4(0.f, 0.f, 0.f, -1.f);
     if constexpr (ADJ) {
         const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
         V3 A = v3(0.f, 0.f, 0.f);
@@ -464,12 +474,13 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
     const int qc = step & 1;
     const Queue& cur = a.q[qc];
     const Queue& nxt = a.q[qc ^ 1];
-    const int32_t n = BATCH ? a.qcount[qc] : a.qcount[step];
-    int32_t* ncount = BATCH ? &a.qcount[qc ^ 1] : &a.qcount[step + 1];
+    const int32_t n = BATCH ? a.qcount[2 * qc] : a.qcount[2 * step];
+    // the next queue's entry count and list length (one 64-bit word)
+    int32_t* ncount = BATCH ? &a.qcount[2 * (qc ^ 1)] : &a.qcount[2 * (step + 1)];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    __shared__ int32_t s_wcount[SHADE_WAVES], s_wfree[SHADE_WAVES], s_qbase, s_fbase;
+    __shared__ int32_t s_wcount[SHADE_WAVES], s_wfree[SHADE_WAVES], s_wlist[SHADE_WAVES], s_qbase, s_fbase, s_lbase;
     __shared__ int   s_tkeys[ADJ ? SHADE_WAVES * TEX_SLOTS : 1];
     __shared__ float s_tvals[ADJ ? SHADE_WAVES * TEX_SLOTS * 3 : 1];
     __shared__ int   s_lkeys[ADJ ? SHADE_WAVES * LIGHT_SLOTS : 1];
@@ -814,21 +825,35 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         // requests a per-wave atomic would send
         const unsigned long long fmask = BATCH ? __ballot(finished) : 0ull;
         const unsigned long long mask = __ballot(pending);
+        // the rays of the next queue's list: the vertex's shadow ray and its traced
+        // MIS-BSDF ray (absent ones are not listed, so no traversal lane draws them)
+        const bool lshd = pending && sdir.w >= 0.0f, lmis = pending && mdir.w >= 0.0f;
+        int ltot;
+        const int lexcl = wave_excl_scan<2>((lshd ? 1 : 0) + (lmis ? 1 : 0), ltot);
         if (lane == 0) {
             s_wcount[wave] = __popcll(mask);
             s_wfree[wave] = __popcll(fmask);
+            s_wlist[wave] = ltot;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            int32_t tq = 0, tf = 0;
+            int32_t tq = 0, tf = 0, tl = 0;
             for (int w = 0; w < SHADE_WAVES; ++w) {
-                const int32_t cq = s_wcount[w], cf = s_wfree[w];
+                const int32_t cq = s_wcount[w], cf = s_wfree[w], cl = s_wlist[w];
                 s_wcount[w] = tq;
                 s_wfree[w] = tf;
+                s_wlist[w] = tl;
                 tq += cq;
                 tf += cf;
+                tl += cl;
             }
-            s_qbase = tq ? atomicAdd(ncount, tq) : 0;
+            // entries and list rays reserved together: (list << 32) | entries
+            unsigned long long old = 0ull;
+            if (tq | tl)
+                old = atomicAdd(reinterpret_cast<unsigned long long*>(ncount),
+                                static_cast<unsigned long long>(tq) | (static_cast<unsigned long long>(tl) << 32));
+            s_qbase = static_cast<int32_t>(old & 0xFFFFFFFFull);
+            s_lbase = static_cast<int32_t>(old >> 32);
             if constexpr (BATCH) s_fbase = tf ? atomicAdd(&bt.ctr[0], tf) : 0;
         }
         __syncthreads();
@@ -844,11 +869,24 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             nxt.path[slot] = p;
             nxt.org[slot] = org;
             nxt.dir[slot] = cdir;
-            nxt.mis_dir[slot] = mdir;
-            nxt.shd_dir[slot] = sdir;
+            if (lmis) nxt.mis_dir[slot] = mdir;   // read back only when its ray was traced
             nxt.lt[slot] = lt4;
             nxt.fac[slot] = fac4;
             if constexpr (BATCH) bt.root[qc ^ 1][slot] = root_next;
+            int32_t li = s_lbase + s_wlist[wave] + lexcl;
+            if (lshd) {
+                nxt.lorg[li] = org;
+                nxt.ldir[li] = sdir;
+                nxt.lq[li] = slot;
+                if constexpr (BATCH) nxt.lroot[li] = root_next;
+                ++li;
+            }
+            if (lmis) {
+                nxt.lorg[li] = org;
+                nxt.ldir[li] = mdir;
+                nxt.lq[li] = static_cast<int32_t>(static_cast<uint32_t>(slot) | 0x80000000u);
+                if constexpr (BATCH) nxt.lroot[li] = root_next;
+            }
             if constexpr (ADJ) {
                 const AdjQueue& an = g.q[qc ^ 1];
                 an.par[slot] = par;
@@ -906,7 +944,7 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArg
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int32_t nfree = bt.ctr[0];
     const int64_t next = static_cast<int64_t>(*bt.work_next);
-    const int32_t qn = a.qcount[qc];
+    const int32_t qn = a.qcount[2 * qc];
     const int64_t n_new = min(min(static_cast<int64_t>(nfree), bt.total_work - next), static_cast<int64_t>(bt.capacity - qn));
     if (gid >= n_new) return;
     const int64_t w = next + gid;
@@ -935,8 +973,6 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArg
     q.path[slot] = p;
     q.org[slot] = make_float4(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2], K_RAY_EPS);
     q.dir[slot] = make_float4(dir.x, dir.y, dir.z, v.max_depth > 0 ? FLT_MAX : -1.0f);
-    q.mis_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
-    q.shd_dir[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
     bt.root[qc][slot] = v.root;
     if constexpr (ADJ) {
         const int64_t si = static_cast<int64_t>(sample - v.sample_base) * npx + pix;
@@ -954,12 +990,13 @@ __global__ void k_bcommit(PassArgs a, BatchArgs bt, int32_t qc) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int32_t nfree = bt.ctr[0];
     const int64_t next = static_cast<int64_t>(*bt.work_next);
-    const int32_t qn = a.qcount[qc];
+    const int32_t qn = a.qcount[2 * qc];
     const int64_t n_new = min(min(static_cast<int64_t>(nfree), bt.total_work - next), static_cast<int64_t>(bt.capacity - qn));
     bt.ctr[0] = nfree - static_cast<int32_t>(n_new);
     *bt.work_next = static_cast<unsigned long long>(next + n_new);
-    a.qcount[qc] = qn + static_cast<int32_t>(n_new);
-    a.qcount[qc ^ 1] = 0;
+    a.qcount[2 * qc] = qn + static_cast<int32_t>(n_new);
+    a.qcount[2 * (qc ^ 1)] = 0;       // the queue this step's shading fills, and its list
+    a.qcount[2 * (qc ^ 1) + 1] = 0;
     bt.ctr[2] = 0;
     bt.ctr[3] = 0;
     bt.ctr[1] = (next + n_new >= bt.total_work && qn + n_new == 0) ? 1 : 0;

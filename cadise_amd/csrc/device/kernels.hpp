@@ -48,28 +48,32 @@ constexpr int LEAF_PARK = 8;
 // float4), its best-hit key and W + LEAF_DEFER / 64 leaf-queue entries (int2).
 __host__ __device__ constexpr int coop_lds_bytes(int W) { return 32 + 8 + 8 * W + LEAF_DEFER / 8; }
 
-// A batch of rays for the persistent traversal kernel. Ray i is
-// (org[i * stride], dir[i * stride]) for i < n0 and (org[(i - n0) * stride],
-// dir2[(i - n0) * stride]) for n0 <= i < n0 + n1 (the second set shares
-// origins: the MIS-BSDF rays leave from the continuation rays' origins).
-// A ray with dir.w (tmax) < 0 is absent and reports a miss.
+// A batch of rays for the persistent traversal kernel. Ray i < n0 is
+// (org[i * stride], dir[i * stride]) -> hit[i] (closest hit, or any hit in a
+// TRACE_ANY launch). A renderer launch (TRACE_MIXED) then traces the *lcount
+// rays of its queue's ray list: ray n0 + k is (lorg[k], ldir[k]), answered into
+// the queue slot lq[k] names: lq >= 0 a shadow ray, any hit -> hit3[lq]; lq < 0
+// a MIS-BSDF ray, closest hit -> hit2[lq & 0x7FFFFFFF]. The list holds only rays
+// that are present, so no lane draws an absent one. A ray with dir.w (tmax) < 0
+// is absent and reports a miss.
 struct TraceArgs {
     DevScene       sc;
     const float4*  org;
     const float4*  dir;
-    const float4*  dir2;
     int32_t        stride;
     const int32_t* count;        // n0 = *count (device) when non-null, else n0 = n_host
     int64_t        n_host;
-    int32_t        dual;         // 1: n1 = n0 rays from dir2
     int2*          hit;          // closest: (leaf or prim, t bits); any: x = 0/1
-    int2*          hit2;
-    const float4*  dir3;         // mixed launch: a third set of n0 any-hit rays (org, dir3) -> hit3
-    int2*          hit3;
-    int32_t        shadow;       // mixed launch: 1 when the third set is present
+    const int32_t* lcount;       // rays in the list (device; null: no list)
+    const float4*  lorg;
+    const float4*  ldir;
+    const int32_t* lq;
+    const int32_t* lroot;        // per list ray root node, null = node 0
+    int2*          hit2;         // MIS-BSDF rays' closest hits
+    int2*          hit3;         // shadow rays' any hits
     int32_t        want_prim;    // closest: write the creation index instead of the leaf position
-    int32_t        mis_emitters; // second set: report a hit on a primitive without an area light as -2
-    const int32_t* root;         // per-ray root node (index q of the first set), null = node 0
+    int32_t        mis_emitters; // MIS-BSDF rays: report a hit on a primitive without an area light as -2
+    const int32_t* root;         // per-ray root node of the first n0, null = node 0
     int32_t*       work;         // atomic work counter, zero on entry
     int32_t*       spill;        // per-lane stack spill, (max_stack - S) entries per lane
     unsigned long long* stats;
@@ -77,19 +81,26 @@ struct TraceArgs {
 };
 
 // Path state per path (one path per pixel in a sample pass) and the ping-pong
-// queues of pending path vertices.
+// queues of pending path vertices. The queue's entry count and its ray list's
+// length sit side by side in one 64-bit word (qcount[2k], qcount[2k + 1]), so
+// that shading reserves both with one atomic.
 struct Queue {
     int32_t* path;     // path id
     float4*  org;      // vertex position, tmin
     float4*  dir;      // continuation ray (w = tmax, < 0 absent)
-    float4*  mis_dir;  // MIS-BSDF ray
-    float4*  shd_dir;  // shadow ray
+    float4*  mis_dir;  // MIS-BSDF ray (written only when it is traced)
     float4*  lt;       // light-sample term of DirectLightEvaluator x throughput / pick pdf (xyz);
                        // w (int bits): the picked light, -2 - it when the shadow ray is absent, -1 no NEE
     float4*  fac;      // MIS-BSDF factor f |L.N| x throughput / pick pdf (xyz), sample pdf (w; -1: no ray)
     int2*    hit;      // continuation hit
     int2*    mis_hit;
     int2*    occ;      // shadow: x = occluded
+    // the ray list: the entries' shadow rays and traced MIS-BSDF rays, compacted
+    // (up to two per entry; TraceArgs)
+    float4*  lorg;
+    float4*  ldir;
+    int32_t* lq;
+    int32_t* lroot;    // batches: the ray's BVH root
 };
 
 // Per-vertex factors of the path-replay adjoint, stored beside Queue when a
@@ -173,7 +184,7 @@ struct PassArgs {
     int32_t  npaths;
     PathRec* path;         // per path slot (pass by pass: per pixel)
     Queue    q[2];
-    int32_t* qcount;       // [max_depth + 2]
+    int32_t* qcount;       // [2 (max_depth + 2)]: entries and list rays of each queue (Queue)
     int32_t* work;         // traversal work counters, 3 per bounce
     int32_t  nwork;
     float4*  film;         // sum w*rgb, sum w

@@ -362,13 +362,10 @@ int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_h
         a.sc = S->sc;
         a.org = reinterpret_cast<const float4*>(rays_dev);
         a.dir = reinterpret_cast<const float4*>(rays_dev) + 1;
-        a.dir2 = nullptr;
         a.stride = 2;
         a.count = nullptr;
         a.n_host = static_cast<int64_t>(n);
-        a.dual = 0;
         a.hit = reinterpret_cast<int2*>(hits_dev);
-        a.hit2 = nullptr;
         a.want_prim = 1;
         a.work = work.as<int32_t>();
         a.stats = S->stats.as<unsigned long long>();
@@ -396,14 +393,13 @@ int ensure_pass_buffers(mpt_scene* S, int64_t npix, int32_t max_depth) {
     if (npix != S->npaths) {
         S->npaths = 0;   // until every buffer below has its new size
         HIP_TRY(ctx, S->path.alloc(sizeof(PathRec) * npix));
-        const size_t sizes[10] = {4, 16, 16, 16, 16, 16, 16, 8, 8, 8};
         for (int q = 0; q < 2; ++q)
-            for (int k = 0; k < 10; ++k) HIP_TRY(ctx, S->qbuf[q][k].alloc(sizes[k] * static_cast<size_t>(npix)));
+            for (int k = 0; k < QBUFS; ++k) HIP_TRY(ctx, S->qbuf[q][k].alloc(kQueueBytes[k] * static_cast<size_t>(npix)));
         S->npaths = npix;
     }
     if (max_depth != S->max_depth_alloc) {
         S->max_depth_alloc = -1;
-        HIP_TRY(ctx, S->qcount.alloc(sizeof(int32_t) * (max_depth + 2)));
+        HIP_TRY(ctx, S->qcount.alloc(sizeof(int32_t) * 2 * (max_depth + 2)));
         HIP_TRY(ctx, S->work.alloc(sizeof(int32_t) * 3 * (max_depth + 2)));
         S->max_depth_alloc = max_depth;
     }
@@ -417,19 +413,7 @@ PassArgs pass_args(mpt_scene* S, uint64_t seed, int32_t max_depth) {
     a.max_depth = max_depth;
     a.npaths = static_cast<int32_t>(S->npaths);
     a.path = S->path.as<PathRec>();
-    for (int q = 0; q < 2; ++q) {
-        Queue& Q = a.q[q];
-        Q.path = S->qbuf[q][0].as<int32_t>();
-        Q.org = S->qbuf[q][1].as<float4>();
-        Q.dir = S->qbuf[q][2].as<float4>();
-        Q.mis_dir = S->qbuf[q][3].as<float4>();
-        Q.shd_dir = S->qbuf[q][4].as<float4>();
-        Q.lt = S->qbuf[q][5].as<float4>();
-        Q.fac = S->qbuf[q][6].as<float4>();
-        Q.hit = S->qbuf[q][7].as<int2>();
-        Q.mis_hit = S->qbuf[q][8].as<int2>();
-        Q.occ = S->qbuf[q][9].as<int2>();
-    }
+    for (int q = 0; q < 2; ++q) a.q[q] = queue_view(S->qbuf[q]);
     a.qcount = S->qcount.as<int32_t>();
     a.work = S->work.as<int32_t>();
     a.nwork = 3 * (max_depth + 2);
@@ -495,28 +479,13 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, stop());
         for (int32_t b = 0; b <= max_depth; ++b) {
-            const Queue& Q = a.q[b & 1];
-            TraceArgs t{};
-            t.sc = S->sc;
-            t.org = Q.org;
-            t.dir = Q.dir;
-            t.dir2 = Q.mis_dir;
-            t.stride = 1;
-            t.count = a.qcount + b;
-            t.dual = 1;
-            t.hit = Q.hit;
-            t.hit2 = Q.mis_hit;
-            t.want_prim = 0;
-            t.mis_emitters = 1;
+            // continuation rays, then (from bounce 1) the previous vertices' shadow
+            // rays and traced MIS-BSDF rays from the queue's list
+            TraceArgs t = queue_trace_args(S->sc, a.q[b & 1], a.qcount + 2 * b, false);
             t.work = a.work + 3 * b;
             t.stats = S->stats.as<unsigned long long>();
-            t.stat_slot = ST_CLOSEST_RAYS;
-            if (b > 0) {   // the previous vertices' shadow rays join the launch
-                t.dir3 = Q.shd_dir;
-                t.hit3 = Q.occ;
-                t.shadow = 1;
-            }
             HIP_TRY(ctx, mark(EV_CLOSEST));
+            // bounce 0's list is empty: camera rays only, closest hit
             if ((rc = launch_trace(S, t, b > 0 ? TRACE_MIXED : TRACE_CLOSEST, tblocks)) != MPT_OK) return rc;
             HIP_TRY(ctx, stop());
             HIP_TRY(ctx, mark(EV_SHADE));
@@ -527,7 +496,7 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
             // bounces, stop once no path is left for the next one
             if ((b & 15) == 15 && b < max_depth) {
                 int32_t left = 0;
-                HIP_TRY(ctx, hipMemcpyAsync(&left, a.qcount + b + 1, sizeof(int32_t), hipMemcpyDeviceToHost, S->stream));
+                HIP_TRY(ctx, hipMemcpyAsync(&left, a.qcount + 2 * (b + 1), sizeof(int32_t), hipMemcpyDeviceToHost, S->stream));
                 HIP_TRY(ctx, hipStreamSynchronize(S->stream));
                 if (left == 0) break;
             }
@@ -750,27 +719,9 @@ int do_adjoint_passes(mpt_scene* S, const mpt_render_params* p, const float* d_i
         hipLaunchKernelGGL(k_camera<true>, dim3((cam_threads + 255) / 256), dim3(256), 0, S->stream, a, g);
         HIP_TRY(ctx, hipGetLastError());
         for (int32_t b = 0; b <= max_depth; ++b) {
-            const Queue& Q = a.q[b & 1];
-            TraceArgs t{};
-            t.sc = S->sc;
-            t.org = Q.org;
-            t.dir = Q.dir;
-            t.dir2 = Q.mis_dir;
-            t.stride = 1;
-            t.count = a.qcount + b;
-            t.dual = 1;
-            t.hit = Q.hit;
-            t.hit2 = Q.mis_hit;
-            t.want_prim = 0;
-            t.mis_emitters = 1;
+            TraceArgs t = queue_trace_args(S->sc, a.q[b & 1], a.qcount + 2 * b, false);
             t.work = a.work + 3 * b;
             t.stats = S->stats.as<unsigned long long>();
-            t.stat_slot = ST_CLOSEST_RAYS;
-            if (b > 0) {
-                t.dir3 = Q.shd_dir;
-                t.hit3 = Q.occ;
-                t.shadow = 1;
-            }
             if ((rc = launch_trace(S, t, b > 0 ? TRACE_MIXED : TRACE_CLOSEST, tblocks)) != MPT_OK) return rc;
             hipLaunchKernelGGL(k_shade<SHADE_ADJ>, dim3(sblocks), dim3(256), 0, S->stream, a, g, BatchArgs{}, b);
             HIP_TRY(ctx, hipGetLastError());
