@@ -31,8 +31,10 @@ Reported (one JSON line):
                  whole job, inputs resident in HBM. A MIS-BSDF query whose ray
                  misses the picked area light's own primitive is answered in
                  shading by that one primitive test (its closest hit cannot be the
-                 light; the reference traces it and finds the same zero term): it
-                 counts as a query, and traced_mrays_s counts traced rays only
+                 light; the reference traces it and finds the same zero term), and
+                 so is a shadow query whose NEE term is exactly zero (occluded or
+                 not, it adds +0): both count as queries; traced_mrays_s counts
+                 traced rays only
   roofline       the traversal kernel (one mixed launch per wavefront step: the
                  continuation and MIS-BSDF rays closest hit, the shadow rays any
                  hit): algorithmic bytes per query (32 B ray + 16 B hit + 24 B per
@@ -130,9 +132,11 @@ def cpu_baseline(spec, seconds_target=15.0, seed=7):
 
 
 def all_rays(st):
-    """Ray queries of the reference's algorithm: traced ones and the MIS-BSDF
-    queries answered by the picked light's own record."""
-    return st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["mis_resolved"] + st["shadow_rays"]
+    """Ray queries of the reference's algorithm: traced ones, the MIS-BSDF
+    queries answered by the picked light's own record and the shadow queries
+    whose NEE term is exactly zero."""
+    return (st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["mis_resolved"] + st["shadow_rays"]
+            + st["shadow_resolved"])
 
 
 def relmse(img, ref):
@@ -267,6 +271,10 @@ def batch_time_to_relmse(ctx, specs, scs, seeds, best, target):
     sync()
     passwise_s = time.perf_counter() - t0
     bb = Batch(ctx, scs2, batch_opts(**best))
+    # the launch shape for this work: the autotuner on trial renders of 8 spp per scene
+    t0 = time.perf_counter()
+    best2, trials2 = bb.autotune(seeds, trial_spp=8)
+    log(f"batch relMSE: autotune {time.perf_counter() - t0:.1f} s, {len(trials2)} trials -> {best2}")
     bb.render(seeds)        # records the graph for this work
     log("batch relMSE: graph recorded")
     t0 = time.perf_counter()
@@ -283,7 +291,7 @@ def batch_time_to_relmse(ctx, specs, scs, seeds, best, target):
             "estimate": "relMSE to the converged image from two independent renders of equal spp: "
                         "mean((A - B)^2 / 2 / (((A + B) / 2)^2 + 0.01))",
             "spp_min": int(min(need)), "spp_max": int(max(need)), "samples": samples,
-            "plain_passwise_s": round(passwise_s, 3), "plain_streaming_s": round(plain_s, 3),
+            "autotuned": best2, "plain_passwise_s": round(passwise_s, 3), "plain_streaming_s": round(plain_s, 3),
             "batched_s": round(batch_s, 3), "speedup_vs_passwise": round(passwise_s / batch_s, 2),
             "speedup_vs_streaming": round(plain_s / batch_s, 2), "relmse_at_spp_max": max(errs),
             "relmse_at_spp_mean": float(np.mean(errs)), "batched_rays": all_rays(stats), "plain_rays": rays,
@@ -470,7 +478,7 @@ def main():
     dt = time.perf_counter() - t0
     st = scene.last_stats()
     rays = all_rays(st)
-    traced = rays - st["mis_resolved"]
+    traced = rays - st["mis_resolved"] - st["shadow_resolved"]
     mrays = rays / dt / 1e6
     # roofline of the traversal kernel: one mixed launch per step traces every traced query
     boxes = st["box_tests"] + st["box_tests_any"]
@@ -557,7 +565,8 @@ def main():
         "traced_mrays_s": round(traced / dt / 1e6, 2),
         "passwise_mrays_s": round(rays / dt_pw / 1e6, 1) if dt_pw else None,
         "rays": {"camera": st["camera_rays"], "continuation": st["continuation_rays"], "mis_bsdf": st["mis_rays"],
-                 "mis_bsdf_resolved_by_light_record": st["mis_resolved"], "shadow": st["shadow_rays"]},
+                 "mis_bsdf_resolved_by_light_record": st["mis_resolved"], "shadow": st["shadow_rays"],
+                 "shadow_resolved_zero_term": st["shadow_resolved"]},
         "kernel_ms": {"trace": round(st["ms_closest"], 3),
                       "shade": round(st["ms_shade"], 3), "regenerate": round(st["ms_camera"], 3),
                       "gpu_total": round(st["ms_gpu"], 3)},

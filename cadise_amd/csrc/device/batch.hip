@@ -88,7 +88,7 @@ void normalize_opts(mpt_batch_opts& o) {
     if (o.stack_lds != 16 && o.stack_lds != 20 && o.stack_lds != 24 && o.stack_lds != 32)
         o.stack_lds = o.width == 4 ? 20 : 16;
     if (o.block != 256) o.block = 128;
-    if (o.capacity_log2 < 16 || o.capacity_log2 > 23) o.capacity_log2 = 21;
+    if (o.capacity_log2 < 16 || o.capacity_log2 > 24) o.capacity_log2 = 21;
     if (o.graph_steps < 2 || o.graph_steps > 64) o.graph_steps = 16;
     if (o.blocks_per_cu < 2 || o.blocks_per_cu > 32) o.blocks_per_cu = o.width == 4 ? 8 : 7;
     o.graph_steps &= ~1;
@@ -533,6 +533,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
     r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
     r.mis_resolved = static_cast<int64_t>(st[ST_MIS_RESOLVED]);
+    r.shadow_resolved = static_cast<int64_t>(st[ST_SHADOW_RESOLVED]);
     r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
     r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
     r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);
@@ -690,7 +691,7 @@ int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, m
         for (int dim = 0; dim < 5; ++dim) {
             std::vector<mpt_batch_opts> cands;
             if (dim == 0)
-                for (int c : {19, 20, 21, 22, 23}) { mpt_batch_opts o = cur; o.capacity_log2 = c; cands.push_back(o); }
+                for (int c : {19, 20, 21, 22, 23, 24}) { mpt_batch_opts o = cur; o.capacity_log2 = c; cands.push_back(o); }
             if (dim == 1)
                 for (int w : {4, 8}) { mpt_batch_opts o = cur; o.width = w; cands.push_back(o); }
             if (dim == 2)

@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         ltab.clear_lane(lane);
         __syncthreads();
     }
-    uint32_t mis_resolved = 0;   // MIS-BSDF queries answered without tracing
+    uint32_t mis_resolved = 0, shadow_resolved = 0;   // queries answered without tracing
     // block-uniform trip count: every lane reaches the ballot and the barriers
     for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
         const int64_t s = bbase + threadIdx.x;
@@ -731,13 +731,25 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 if (nee && pick > 0.0f) {
                     const float ip = 1.0f / pick;
                     const V3 lts = beta * (v3f4(lt4) * ip), fcs = beta * (v3f4(fac4) * ip);
-                    const bool shadow = sdir.w >= 0.0f;
+                    bool shadow = sdir.w >= 0.0f;
+                    // a shadow ray whose term is exactly zero adds +0 to the radiance
+                    // whether or not it is occluded: it is answered here (the adjoint
+                    // still traces it, its gradient factors need not be zero)
+                    if (!ADJ && shadow && is_zero(lts)) {
+                        shadow = false;
+                        sdir.w = -1.0f;
+                        ++shadow_resolved;
+                    }
                     lt4 = make_float4(shadow ? lts.x : 0.f, shadow ? lts.y : 0.f, shadow ? lts.z : 0.f,
                                       __int_as_float(shadow ? pick_light : -2 - pick_light));
                     fac4 = make_float4(fcs.x, fcs.y, fcs.z, mdir.w >= 0.0f ? fac4.w : -1.0f);
                 } else {
                     lt4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
                     fac4 = make_float4(0.f, 0.f, 0.f, -1.0f);
+                    if (!ADJ && sdir.w >= 0.0f) {   // no NEE term reads its result
+                        sdir.w = -1.0f;
+                        ++shadow_resolved;
+                    }
                 }
                 // continuation
                 cdir = make_float4(0.f, 0.f, 0.f, -1.f);
@@ -898,15 +910,23 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             }
         }
     }
-    // one atomic per block for the queries answered here (the waves' sums in LDS)
-    const unsigned long long wres = wave_sum(mis_resolved);
+    // one atomic per block for each count of queries answered here (the waves'
+    // sums in LDS)
+    const unsigned long long wres = wave_sum(mis_resolved), wsres = wave_sum(shadow_resolved);
     __syncthreads();   // every wave has read the last iteration's LDS counts
-    if (lane == 0) s_wcount[wave] = static_cast<int32_t>(wres);
+    if (lane == 0) {
+        s_wcount[wave] = static_cast<int32_t>(wres);
+        s_wfree[wave] = static_cast<int32_t>(wsres);
+    }
     __syncthreads();
     if (threadIdx.x == 0 && a.stats) {
-        unsigned long long t = 0;
-        for (int w = 0; w < SHADE_WAVES; ++w) t += static_cast<unsigned long long>(s_wcount[w]);
+        unsigned long long t = 0, ts = 0;
+        for (int w = 0; w < SHADE_WAVES; ++w) {
+            t += static_cast<unsigned long long>(s_wcount[w]);
+            ts += static_cast<unsigned long long>(s_wfree[w]);
+        }
         if (t) atomicAdd(&a.stats[ST_MIS_RESOLVED], t);
+        if (ts) atomicAdd(&a.stats[ST_SHADOW_RESOLVED], ts);
     }
 }
 

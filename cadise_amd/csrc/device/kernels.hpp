@@ -21,6 +21,7 @@ enum StatSlot {
     ST_PRIMS_ANY,         // primitive tests of any-hit queries
     ST_ERRORS,            // traversal stack overruns (must stay 0)
     ST_MIS_RESOLVED,      // MIS-BSDF queries answered by the picked light's own record (not traced)
+    ST_SHADOW_RESOLVED,   // shadow queries answered untraced: their term is exactly zero
     ST_COUNT
 };
 

@@ -526,6 +526,7 @@ int do_render_passes(mpt_scene* S, const mpt_render_params* p) {
     r.continuation_rays = static_cast<int64_t>(st[ST_CLOSEST_RAYS]) - r.camera_rays;
     r.mis_rays = static_cast<int64_t>(st[ST_MIS_RAYS]);
     r.mis_resolved = static_cast<int64_t>(st[ST_MIS_RESOLVED]);
+    r.shadow_resolved = static_cast<int64_t>(st[ST_SHADOW_RESOLVED]);
     r.shadow_rays = static_cast<int64_t>(st[ST_SHADOW_RAYS]);
     r.box_tests = static_cast<int64_t>(st[ST_BOXES]);
     r.tri_tests = static_cast<int64_t>(st[ST_PRIMS]);

@@ -84,7 +84,7 @@ class Stats(C.Structure):
                 ("ms_total", C.c_double), ("ms_closest", C.c_double), ("ms_anyhit", C.c_double),
                 ("ms_shade", C.c_double), ("ms_camera", C.c_double), ("ms_film", C.c_double),
                 ("ms_adjoint", C.c_double), ("ms_gpu", C.c_double), ("path_slots", C.c_int64),
-                ("mis_resolved", C.c_int64)]
+                ("mis_resolved", C.c_int64), ("shadow_resolved", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -194,6 +194,8 @@ typedef struct {
     int64_t mis_resolved;         /* MIS-BSDF queries answered without traversal: the ray misses the
                                      picked area light's own primitive, so its closest hit cannot be
                                      that light (mis_rays counts the traced ones) */
+    int64_t shadow_resolved;      /* shadow queries answered without traversal: the NEE term they
+                                     would gate is exactly zero (shadow_rays counts the traced ones) */
 } mpt_stats;
 
 /* device >= 0: a GPU context. device == -1: a host-only context, in which
@@ -332,7 +334,7 @@ typedef struct {
     int32_t width;             /* BVH width, 4 or 8 */
     int32_t stack_lds;         /* per-lane LDS stack entries: 16, 20, 24 or 32 */
     int32_t block;             /* traversal threads per block: 128 or 256 */
-    int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..23) */
+    int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..24) */
     int32_t graph_steps;       /* wavefront steps per graph launch (even, 2..64) */
     int32_t blocks_per_cu;     /* resident traversal blocks per CU (2..32) */
 } mpt_batch_opts;

@@ -61,7 +61,8 @@ def _check(batch, batch_set, bst, label, width):
     for i in range(len(scs)):
         f = batch.film(i)
         np.testing.assert_allclose(f, plain[i], rtol=2e-5, atol=2e-6, err_msg=f"{label}: scene {i}")
-    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "samples"):
+    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "shadow_resolved",
+              "samples"):
         assert bst[k] == sum(s[k] for s in stats), (label, k, bst[k], sum(s[k] for s in stats))
     print(f"[batch {label}] {bst['samples']} samples, {bst['ms_gpu']:.2f} ms GPU, films match", flush=True)
 

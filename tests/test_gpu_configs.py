@@ -104,7 +104,8 @@ def test_batch_64_scenes_matches_plain(ctx):
     bst = b.render(seeds)
     print(f"[batch 64] {sum(s.num_prims() for s in specs)} prims, {bst['samples']} samples; built in {t1 - t0:.1f} s; "
           f"plain {sum(s['ms_gpu'] for s in stats):.0f} ms GPU, batched {bst['ms_gpu']:.0f} ms GPU", flush=True)
-    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "samples"):
+    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "shadow_resolved",
+              "samples"):
         assert bst[k] == sum(s[k] for s in stats), (k, bst[k], sum(s[k] for s in stats))
     for i in range(len(scs)):
         np.testing.assert_allclose(b.film(i), plain[i], rtol=2e-5, atol=2e-6, err_msg=f"scene {i}")

@@ -76,11 +76,12 @@ def test_render_spheres_ggx_vs_oracle(ctx, tmp_path):
 def test_mis_queries_accounted_against_oracle(ctx, case, passwise):
     """A MIS-BSDF query whose ray misses the picked area light's own primitive
     is answered in shading without traversal (its closest hit cannot be that
-    light). The oracle traces every query of the same paths (Philox layout), so
-    the film agrees with it and every query is accounted for: camera +
-    continuation + traced MIS-BSDF + resolved MIS-BSDF queries = the oracle's
-    closest-hit queries, shadow rays = its any-hit queries (within 1e-3: a rare
-    last-ulp flip of a discrete decision changes a path's length)."""
+    light), and so is a shadow query whose NEE term is exactly zero. The oracle
+    traces every query of the same paths (Philox layout), so the film agrees
+    with it and every query is accounted for: camera + continuation + traced +
+    answered MIS-BSDF queries = the oracle's closest-hit queries, traced +
+    answered shadow queries = its any-hit queries (within 1e-3: a rare last-ulp
+    flip of a discrete decision changes a path's length)."""
     from cadise_amd.api import Scene
     spec = {"cornell": lambda: scenes.cornell_box(width=48, height=48, spp=4, max_depth=8),
             "spheres": lambda: small_spheres(n_spheres=6, level=3, width=64, height=48, spp=4, max_depth=7),
@@ -94,11 +95,12 @@ def test_mis_queries_accounted_against_oracle(ctx, case, passwise):
     _compare(film, orc.render_linear(77, rng_mode=OR_RNG_PHILOX), f"mis {case}")
     _, rays, samples = orc.time_render(0, spec.width * spec.height, spec.spp, 8, seed=77)
     closest = st["camera_rays"] + st["continuation_rays"] + st["mis_rays"] + st["mis_resolved"]
+    shadow = st["shadow_rays"] + st["shadow_resolved"]
     print(f"[mis {case}] traced MIS {st['mis_rays']} resolved {st['mis_resolved']}; closest {closest} "
-          f"oracle {rays[0]}; shadow {st['shadow_rays']} oracle {rays[1]}", flush=True)
+          f"oracle {rays[0]}; shadow {st['shadow_rays']} + {st['shadow_resolved']} oracle {rays[1]}", flush=True)
     assert samples == st["samples"]
     assert abs(closest - rays[0]) <= 1e-3 * rays[0]
-    assert abs(st["shadow_rays"] - rays[1]) <= 1e-3 * rays[1]
+    assert abs(shadow - rays[1]) <= 1e-3 * rays[1]
     if case != "sky":
         assert st["mis_resolved"] > 0
 
@@ -143,7 +145,8 @@ def test_streaming_matches_passwise(ctx, case):
     sa = sc.last_stats()
     b = sc.render(seed=21, passwise=True)
     sb = sc.last_stats()
-    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "samples"):
+    for k in ("camera_rays", "continuation_rays", "mis_rays", "mis_resolved", "shadow_rays", "shadow_resolved",
+              "samples"):
         assert sa[k] == sb[k], (k, sa[k], sb[k])
     for k in ("box_tests", "tri_tests", "box_tests_any", "tri_tests_any"):
         assert abs(sa[k] - sb[k]) <= 0.02 * max(sa[k], sb[k]), (k, sa[k], sb[k])
