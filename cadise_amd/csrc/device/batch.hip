@@ -42,12 +42,10 @@ __global__ void k_fix_info(int4* info, int64_t n, int32_t mat_base, int32_t ligh
     info[j] = v;
 }
 
-__global__ void k_batch_init(int32_t* free_ids, int32_t capacity, int32_t* ctr, int32_t* qcount,
-                             unsigned long long* work_next, unsigned long long* stats) {
+__global__ void k_batch_init(int32_t* ctr, int32_t* qcount, unsigned long long* work_next, unsigned long long* stats) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < capacity) free_ids[i] = static_cast<int32_t>(i);
     if (i == 0) {
-        ctr[0] = capacity;
+        ctr[0] = 0;
         ctr[1] = 0;
         ctr[2] = ctr[3] = 0;
         qcount[0] = qcount[1] = qcount[2] = qcount[3] = 0;
@@ -315,8 +313,6 @@ int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
     }
     const size_t n = static_cast<size_t>(cap);
     S.capacity_alloc = 0;   // until every buffer below has its new size
-    HIP_TRY(ctx, S.path.alloc(sizeof(PathRec) * n));
-    HIP_TRY(ctx, S.free_ids.alloc(sizeof(int32_t) * n));
     for (int q = 0; q < 2; ++q) {
         for (int k = 0; k < QBUFS; ++k) HIP_TRY(ctx, S.qbuf[q][k].alloc(kQueueBytes[k] * n));
         HIP_TRY(ctx, S.rootq[q].alloc(sizeof(int32_t) * n));
@@ -407,7 +403,7 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     }
     PassArgs a{};
     a.sc = job.sc;
-    a.path = S.path.as<PathRec>();
+    a.path = nullptr;   // the path state travels in the queues
     for (int q = 0; q < 2; ++q) a.q[q] = queue_view(S.qbuf[q]);
     a.qcount = S.qcount.as<int32_t>();
     a.npaths = cap;
@@ -420,7 +416,6 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
     bt.root[1] = S.rootq[1].as<int32_t>();
     bt.film = job.film;
     bt.keep = job.keep;
-    bt.free_ids = S.free_ids.as<int32_t>();
     bt.ctr = S.ctr.as<int32_t>();
     bt.work_next = S.work_next.as<unsigned long long>();
     bt.stats = S.stats.as<unsigned long long>();
@@ -432,10 +427,9 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
         if (job.use_graph) return fail(ctx, MPT_ERR_INVALID, "the adjoint runs without a recorded graph");
         if (cap > S.adj_alloc) {
             S.adj_alloc = 0;   // until every buffer below has its new size
-            const size_t sizes[6] = {8, 16, 16, 16, 16, 16};
+            const size_t sizes[7] = {8, 16, 16, 16, 16, 16, 16};
             for (int q = 0; q < 2; ++q)
-                for (int k = 0; k < 6; ++k) HIP_TRY(ctx, S.adjq[q][k].alloc(sizes[k] * static_cast<size_t>(cap)));
-            HIP_TRY(ctx, S.adjA.alloc(sizeof(float4) * static_cast<size_t>(cap)));
+                for (int k = 0; k < 7; ++k) HIP_TRY(ctx, S.adjq[q][k].alloc(sizes[k] * static_cast<size_t>(cap)));
             S.adj_alloc = cap;
         }
         for (int q = 0; q < 2; ++q) {
@@ -445,8 +439,9 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
             g.q[q].sB = S.adjq[q][3].as<float4>();
             g.q[q].eB = S.adjq[q][4].as<float4>();
             g.q[q].sC = S.adjq[q][5].as<float4>();
+            g.q[q].A = S.adjq[q][6].as<float4>();
         }
-        g.A = S.adjA.as<float4>();
+        g.A = nullptr;
         bt.film = nullptr;
         bt.keep = nullptr;
         gp = &g;
@@ -482,9 +477,8 @@ int stream_render(mpt_ctx* ctx, hipStream_t stream, Streamer& S, const StreamJob
         HIP_TRY(ctx, hipEventCreate(&e));
         S.events.push_back(e);
     }
-    // reset: free stack, counters, film
-    hipLaunchKernelGGL(k_batch_init, dim3((cap + 255) / 256), dim3(256), 0, stream, bt.free_ids, cap, bt.ctr, a.qcount,
-                       bt.work_next, bt.stats);
+    // reset: counters, film
+    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, stream, bt.ctr, a.qcount, bt.work_next, bt.stats);
     if (job.clear_film && !job.adjoint)
         hipLaunchKernelGGL(k_zero_f4, dim3((job.film_px + 255) / 256), dim3(256), 0, stream, job.film, job.film_px);
     HIP_TRY(ctx, hipGetLastError());

@@ -134,10 +134,11 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-// A queue's arrays, bytes per path slot: path, org, dir, mis_dir, lt, fac, hit,
-// mis_hit, occ, then its ray list of up to two rays per slot: lorg, ldir, lq, lroot.
-constexpr int QBUFS = 13;
-constexpr size_t kQueueBytes[QBUFS] = {4, 16, 16, 16, 16, 16, 8, 8, 8, 32, 32, 8, 8};
+// A queue's arrays, bytes per entry: path, org, dir, mis_dir, lt, fac, hit,
+// mis_hit, occ, then its ray list of up to two rays per entry (lorg, ldir, lq,
+// lroot), then the streaming path state (prad, pbeta, pmeta).
+constexpr int QBUFS = 16;
+constexpr size_t kQueueBytes[QBUFS] = {4, 16, 16, 16, 16, 16, 8, 8, 8, 32, 32, 8, 8, 16, 16, 16};
 
 inline Queue queue_view(const DevBuf* b) {
     Queue Q;
@@ -154,6 +155,9 @@ inline Queue queue_view(const DevBuf* b) {
     Q.ldir = b[10].as<float4>();
     Q.lq = b[11].as<int32_t>();
     Q.lroot = b[12].as<int32_t>();
+    Q.prad = b[13].as<float4>();
+    Q.pbeta = b[14].as<float4>();
+    Q.pmeta = b[15].as<int4>();
     return Q;
 }
 
@@ -185,9 +189,9 @@ inline TraceArgs queue_trace_args(const DevScene& sc, const Queue& Q, const int3
 // Path state of the streaming wavefront (batch.hip): path slots recycled
 // through a free stack, two queues, counters, and the recorded HIP graph.
 struct Streamer {
-    DevBuf path, free_ids, ctr, work_next, qcount, stats, spill, views;
+    DevBuf ctr, work_next, qcount, stats, spill, views;
     DevBuf qbuf[2][QBUFS], rootq[2];
-    DevBuf adjq[2][6], adjA;          // adjoint jobs
+    DevBuf adjq[2][7];                // adjoint jobs
     int32_t capacity_alloc = 0, adj_alloc = 0;
     int64_t spill_lanes = 0;
     hipGraphExec_t exec = nullptr;

@@ -331,12 +331,11 @@ __device__ __forceinline__ int4 footprint(const F& sc, float fx, float fy, int t
 
 __device__ __forceinline__ bool finite3(float4 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 
-// The film position and primary ray of sample `sample` of pixel p: the
-// stratified (or random) film jitter of StratifiedSampler::requestSample2D
-// (stratifiedSampler.cpp:38-73) and PerspectivePinholeCamera::spawnPrimaryRay
-// (perspectivePinholeCamera.cpp:52-73). F is DevScene or BatchView.
+// The film position of sample `sample` of pixel p: the stratified (or random)
+// film jitter of StratifiedSampler::requestSample2D (stratifiedSampler.cpp:38-73).
+// F is DevScene or BatchView.
 template <class F>
-__device__ __forceinline__ V3 camera_ray(const F& sc, int32_t p, int32_t sample, uint2 key, float& fx, float& fy) {
+__device__ __forceinline__ void film_position(const F& sc, int32_t p, int32_t sample, uint2 key, float& fx, float& fy) {
     const int32_t ix = p % sc.img_w, iy = p / sc.img_w;
     const uint4 r = philox(make_uint4(static_cast<uint32_t>(p), static_cast<uint32_t>(sample), 0xFFFFFFFFu, 0u), key);
     const float u0 = unit24(r.x), u1 = unit24(r.y);
@@ -353,6 +352,13 @@ __device__ __forceinline__ V3 camera_ray(const F& sc, int32_t p, int32_t sample,
     }
     fx = static_cast<float>(ix) + jx;
     fy = static_cast<float>(iy) + jy;
+}
+
+// The film position and primary ray of sample `sample` of pixel p
+// (PerspectivePinholeCamera::spawnPrimaryRay, perspectivePinholeCamera.cpp:52-73).
+template <class F>
+__device__ __forceinline__ V3 camera_ray(const F& sc, int32_t p, int32_t sample, uint2 key, float& fx, float& fy) {
+    film_position(sc, p, sample, key, fx, fy);
     // film -> camera point, camera -> world vector (tMatrix4.ipp:180-195)
     const float* m = sc.f2c;
     const float z = 0.0f;
@@ -480,7 +486,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    __shared__ int32_t s_wcount[SHADE_WAVES], s_wfree[SHADE_WAVES], s_wlist[SHADE_WAVES], s_qbase, s_fbase, s_lbase;
+    __shared__ int32_t s_wcount[SHADE_WAVES], s_wfree[SHADE_WAVES], s_wlist[SHADE_WAVES], s_qbase, s_lbase;
     __shared__ int   s_tkeys[ADJ ? SHADE_WAVES * TEX_SLOTS : 1];
     __shared__ float s_tvals[ADJ ? SHADE_WAVES * TEX_SLOTS * 3 : 1];
     __shared__ int   s_lkeys[ADJ ? SHADE_WAVES * LIGHT_SLOTS : 1];
@@ -497,7 +503,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
     for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
         const int64_t s = bbase + threadIdx.x;
         const bool active = s < n;
-        bool pending = false, finished = false;
+        bool pending = false;
         int32_t p = 0;
         float4 org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org;
         int32_t pick_light = 0;
@@ -509,8 +515,11 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         V3 tval = v3(0.f, 0.f, 0.f), lval = v3(0.f, 0.f, 0.f);
         int2 par = make_int2(-1, 0);
         float4 sL4 = org, eL4 = org, sB4 = org, eB4 = org, sC4 = org;
+        // streaming: the path state that goes to the next queue with its vertex
+        float4 nrad = org, nbeta = org, A4 = org;
+        int4 nmeta = make_int4(0, 0, 0, 0);
         if (active) {
-            p = cur.path[s];
+            if constexpr (!BATCH) p = cur.path[s];   // pass by pass: the path of pixel p
             // the path's step, sample, pixel and scene settings
             int32_t pb = step, smp = a.sample, pix = p, md = a.max_depth;
             uint2 key = a.key;
@@ -518,7 +527,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             int32_t est = sc.estimator;
             int4 meta = make_int4(0, 0, 0, 0);
             if constexpr (BATCH) {
-                meta = a.path[p].meta;
+                meta = cur.pmeta[s];
                 sid = meta.x;
                 pix = meta.y;
                 smp = meta.z;
@@ -539,16 +548,19 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             }
             org = cur.org[s];
             const V3 P0 = v3f4(org);
-            float4 rad4 = a.path[p].rad;
+            float4 rad4 = BATCH ? cur.prad[s] : a.path[p].rad;
             V3 L = v3f4(rad4);
-            beta4 = a.path[p].beta;
+            beta4 = BATCH ? cur.pbeta[s] : a.path[p].beta;
             V3 beta = v3f4(beta4);
             // emission counts at a hit that follows the camera or a specular vertex (pt),
             // at every hit (vpt) (ptEstimator.cpp:51-56, 80-82; vptEstimator.cpp:48-51)
             const bool count_emit = beta4.w != 0.0f;
             bool count_next = false;
             V3 A = v3(0.f, 0.f, 0.f);
-            if constexpr (ADJ) A = v3f4(g.A[p]);
+            if constexpr (ADJ) {
+                A4 = BATCH ? g.q[qc].A[s] : g.A[p];
+                A = v3f4(A4);
+            }
             // -- resolve the previous vertex's NEE: its terms come pre-scaled by the
             // path throughput over the light-pick pdf (L += beta * (Ld / pick),
             // ptEstimator.cpp:94-99, distributed over Ld's two terms)
@@ -782,21 +794,24 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 org = make_float4(h.P.x, h.P.y, h.P.z, K_RAY_EPS);
                 pending = true;
             }
-            a.path[p].rad = make_float4(L.x, L.y, L.z, 0.f);
-            a.path[p].beta = make_float4(beta.x, beta.y, beta.z, count_next ? 1.0f : 0.0f);
+            nrad = make_float4(L.x, L.y, L.z, 0.f);
+            nbeta = make_float4(beta.x, beta.y, beta.z, count_next ? 1.0f : 0.0f);
             if constexpr (BATCH) {
                 if (pending) {
-                    a.path[p].meta = make_int4(meta.x, meta.y, meta.z, pb + 1);
-                } else if (ADJ) {
-                    finished = true;
-                } else {
-                    finished = true;
+                    nmeta = make_int4(meta.x, meta.y, meta.z, pb + 1);
+                } else if (!ADJ) {   // the finished sample splats into its scene's film
                     const BatchView& v = bt.views[sid];
-                    batch_splat(bt, v, pix, a.path[p].fpos, L);
+                    // the sample's film position, drawn again from its Philox stream
+                    float fx, fy;
+                    film_position(v, pix, smp, key, fx, fy);
+                    batch_splat(bt, v, pix, make_float2(fx, fy), L);
                     if (bt.keep)
                         bt.keep[static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix] =
                             make_float4(L.x, L.y, L.z, 0.f);
                 }
+            } else {
+                a.path[p].rad = nrad;
+                a.path[p].beta = nbeta;
             }
         }
         if constexpr (ADJ) {
@@ -831,11 +846,9 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             }
             __syncthreads();
         }
-        // finished paths give their slots back and pending ones go to the next queue:
-        // wave ballots, then one atomic per block for each (the block's waves' counts
-        // prefix-summed in LDS), so that the two global counters see a quarter of the
-        // requests a per-wave atomic would send
-        const unsigned long long fmask = BATCH ? __ballot(finished) : 0ull;
+        // pending paths go to the next queue: wave ballots, then one atomic per block
+        // (the block's waves' counts prefix-summed in LDS), so that the global counter
+        // sees a quarter of the requests a per-wave atomic would send
         const unsigned long long mask = __ballot(pending);
         // the rays of the next queue's list: the vertex's shadow ray and its traced
         // MIS-BSDF ray (absent ones are not listed, so no traversal lane draws them)
@@ -844,19 +857,16 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
         const int lexcl = wave_excl_scan<2>((lshd ? 1 : 0) + (lmis ? 1 : 0), ltot);
         if (lane == 0) {
             s_wcount[wave] = __popcll(mask);
-            s_wfree[wave] = __popcll(fmask);
             s_wlist[wave] = ltot;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            int32_t tq = 0, tf = 0, tl = 0;
+            int32_t tq = 0, tl = 0;
             for (int w = 0; w < SHADE_WAVES; ++w) {
-                const int32_t cq = s_wcount[w], cf = s_wfree[w], cl = s_wlist[w];
+                const int32_t cq = s_wcount[w], cl = s_wlist[w];
                 s_wcount[w] = tq;
-                s_wfree[w] = tf;
                 s_wlist[w] = tl;
                 tq += cq;
-                tf += cf;
                 tl += cl;
             }
             // entries and list rays reserved together: (list << 32) | entries
@@ -866,19 +876,20 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                                 static_cast<unsigned long long>(tq) | (static_cast<unsigned long long>(tl) << 32));
             s_qbase = static_cast<int32_t>(old & 0xFFFFFFFFull);
             s_lbase = static_cast<int32_t>(old >> 32);
-            if constexpr (BATCH) s_fbase = tf ? atomicAdd(&bt.ctr[0], tf) : 0;
         }
         __syncthreads();
-        if constexpr (BATCH) {
-            const int32_t fbase = s_fbase + s_wfree[wave];
-            if (finished) bt.free_ids[fbase + static_cast<int32_t>(__popcll(fmask & ((1ull << lane) - 1ull)))] = p;
-        }
         // (the next iteration rewrites the LDS counts only after its first barrier,
         // which every wave reaches after these reads)
         const int32_t base = s_qbase + s_wcount[wave];
         if (pending) {
             const int32_t slot = base + static_cast<int32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-            nxt.path[slot] = p;
+            if constexpr (BATCH) {
+                nxt.prad[slot] = nrad;
+                nxt.pbeta[slot] = nbeta;
+                nxt.pmeta[slot] = nmeta;
+            } else {
+                nxt.path[slot] = p;
+            }
             nxt.org[slot] = org;
             nxt.dir[slot] = cdir;
             if (lmis) nxt.mis_dir[slot] = mdir;   // read back only when its ray was traced
@@ -907,6 +918,7 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 an.sB[slot] = sB4;
                 an.eB[slot] = eB4;
                 an.sC[slot] = sC4;
+                if constexpr (BATCH) an.A[slot] = A4;
             }
         }
     }
@@ -954,18 +966,18 @@ __device__ __forceinline__ V3 adjoint_weight(const F& sc, const AdjArgs& g, int6
     return A;
 }
 
-// New camera paths for the free path slots of a batch, appended to queue qc:
-// n_new = min(free slots, work left, queue room), all threads reading the same
-// counters (k_bcommit applies them afterwards). Work item w of scene s (w in
+// New camera paths of a batch, appended to queue qc with their state (a path
+// lives in the queues, one entry per path in flight): n_new = min(work left,
+// queue room), all threads reading the same counters (k_bcommit applies them
+// afterwards). Work item w of scene s (w in
 // [work_base_s, work_base_s+1)) is sample (w - base) / pixels of pixel
 // (w - base) % pixels, so consecutive lanes trace neighbouring pixels.
 template <bool ADJ>
 __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArgs g, int32_t qc) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int32_t nfree = bt.ctr[0];
     const int64_t next = static_cast<int64_t>(*bt.work_next);
     const int32_t qn = a.qcount[2 * qc];
-    const int64_t n_new = min(min(static_cast<int64_t>(nfree), bt.total_work - next), static_cast<int64_t>(bt.capacity - qn));
+    const int64_t n_new = min(bt.total_work - next, static_cast<int64_t>(bt.capacity - qn));
     if (gid >= n_new) return;
     const int64_t w = next + gid;
     int lo = 0, hi = bt.nviews - 1;    // last view with work_base <= w
@@ -978,41 +990,33 @@ __global__ void __launch_bounds__(256) k_bregen(PassArgs a, BatchArgs bt, AdjArg
     const int64_t local = w - v.work_base;
     const int64_t npx = static_cast<int64_t>(v.img_w) * v.img_h;
     const int32_t sample = v.sample_base + static_cast<int32_t>(local / npx), pix = static_cast<int32_t>(local % npx);
-    const int32_t p = bt.free_ids[nfree - 1 - gid];
     float fx, fy;
     const V3 dir = camera_ray(v, pix, sample, make_uint2(v.key_lo, v.key_hi), fx, fy);
-    PathRec pr;
-    pr.rad = make_float4(0.f, 0.f, 0.f, 0.f);
-    pr.beta = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
-    pr.meta = make_int4(lo, pix, sample, 0);
-    pr.fpos = make_float2(fx, fy);
-    pr.pad = make_float2(0.f, 0.f);
-    a.path[p] = pr;
     const int32_t slot = qn + static_cast<int32_t>(gid);
     const Queue& q = a.q[qc];
-    q.path[slot] = p;
+    q.prad[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    q.pbeta[slot] = make_float4(1.f, 1.f, 1.f, 1.f);   // w: emission counts at the next hit
+    q.pmeta[slot] = make_int4(lo, pix, sample, 0);
     q.org[slot] = make_float4(v.cam_pos[0], v.cam_pos[1], v.cam_pos[2], K_RAY_EPS);
     q.dir[slot] = make_float4(dir.x, dir.y, dir.z, v.max_depth > 0 ? FLT_MAX : -1.0f);
     bt.root[qc][slot] = v.root;
     if constexpr (ADJ) {
         const int64_t si = static_cast<int64_t>(sample - v.sample_base) * npx + pix;
         const V3 A = adjoint_weight(v, g, v.film_base, pix, fx, fy, g.samples[si]);
-        g.A[p] = make_float4(A.x, A.y, A.z, 0.f);
+        g.q[qc].A[slot] = make_float4(A.x, A.y, A.z, 0.f);
         g.q[qc].par[slot] = make_int2(-1, 0);
         g.q[qc].sC[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
-// Applies k_bregen's counts (one thread): slots taken, work consumed, queue qc
-// grown; clears the other queue's count and this step's traversal work
-// counters; raises the done flag once no work is left and qc is empty.
+// Applies k_bregen's counts (one thread): work consumed, queue qc grown;
+// clears the other queue's count and this step's traversal work counters;
+// raises the done flag once no work is left and qc is empty.
 __global__ void k_bcommit(PassArgs a, BatchArgs bt, int32_t qc) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int32_t nfree = bt.ctr[0];
     const int64_t next = static_cast<int64_t>(*bt.work_next);
     const int32_t qn = a.qcount[2 * qc];
-    const int64_t n_new = min(min(static_cast<int64_t>(nfree), bt.total_work - next), static_cast<int64_t>(bt.capacity - qn));
-    bt.ctr[0] = nfree - static_cast<int32_t>(n_new);
+    const int64_t n_new = min(bt.total_work - next, static_cast<int64_t>(bt.capacity - qn));
     *bt.work_next = static_cast<unsigned long long>(next + n_new);
     a.qcount[2 * qc] = qn + static_cast<int32_t>(n_new);
     a.qcount[2 * (qc ^ 1)] = 0;       // the queue this step's shading fills, and its list

@@ -102,6 +102,11 @@ struct Queue {
     float4*  ldir;
     int32_t* lq;
     int32_t* lroot;    // batches: the ray's BVH root
+    // streaming renders: the path's state travels with its pending vertex (read
+    // and written in queue order; pass by pass it sits in PathRec per pixel)
+    float4*  prad;     // radiance so far (xyz)
+    float4*  pbeta;    // throughput (xyz); w: emission counts at the next hit
+    int4*    pmeta;    // scene, pixel, sample, bounce
 };
 
 // Per-vertex factors of the path-replay adjoint, stored beside Queue when a
@@ -116,13 +121,14 @@ struct AdjQueue {
     float4* sB;    // MIS-BSDF term: dLterm / dtexel = sB * Le * w (w, Le known at resolution)
     float4* eB;    // MIS-BSDF term: dLterm / dLe = eB * w
     float4* sC;    // continuation: d ln f / dtexel (xyz), w = 1 when the continuation ray was traced
+    float4* A;     // streaming: the path's adjoint weight (pass by pass: AdjArgs::A per pixel)
 };
 
 struct AdjArgs {
     AdjQueue      q[2];
     const float4* samples;  // the forward pass's radiance per sample: [pixel] of this pass (pass by
                             // pass) or [sample - base][pixel] (streaming)
-    float4*       A;        // per-path adjoint weight dLoss / dL_sample
+    float4*       A;        // pass by pass: per-pixel adjoint weight dLoss / dL_sample
     const float*  dimg;     // dLoss / dI, npix x 3
     const float4* film;     // the forward film (sum w per pixel)
     float*        gtex;     // dLoss / d(texture element)
@@ -152,13 +158,12 @@ struct BatchView {
 struct BatchArgs {
     const BatchView* views;
     int32_t  nviews;
-    int32_t  capacity;           // path slots (= queue entries)
+    int32_t  capacity;           // paths in flight (= queue entries)
     int64_t  total_work;
     int32_t* root[2];            // per queue entry: BVH root of the entry's scene
     float4*  film;               // merged linear film (sum w*rgb, sum w)
     float4*  keep;               // one scene: each sample's radiance [sample - base][pixel], or null
-    int32_t* free_ids;           // stack of free path slots
-    int32_t* ctr;                // [0] free count, [1] done flag, [2..3] trace work counters
+    int32_t* ctr;                // [1] done flag, [2..3] trace work counters
     unsigned long long* work_next;
     unsigned long long* stats;
 };
@@ -167,12 +172,14 @@ struct BatchArgs {
 // wavefront of mixed-bounce paths; else one sample pass of one scene).
 enum { SHADE_FWD = 0, SHADE_ADJ = 1, SHADE_BATCH = 2, SHADE_BATCH_ADJ = 3 };
 
-// The state of one path, one 64-byte record per path slot so that a shading
-// step reads and writes one line per path instead of gathering four arrays.
+// The state of one path of a pass-by-pass render, one 64-byte record per pixel
+// so that a shading step reads and writes one line per path and the film
+// gather finds each pixel's radiance (streaming renders carry the path state
+// in the queues instead).
 struct __align__(64) PathRec {
     float4 rad;    // radiance so far (xyz)
     float4 beta;   // throughput (xyz); w: emission counts at the next hit
-    int4   meta;   // streaming: scene, pixel, sample, bounce
+    int4   meta;   // unused (pads the record to 64 bytes)
     float2 fpos;   // film position of the path's sample
     float2 pad;
 };
@@ -183,7 +190,7 @@ struct PassArgs {
     int32_t  sample;       // sample index of this pass
     int32_t  max_depth;
     int32_t  npaths;
-    PathRec* path;         // per path slot (pass by pass: per pixel)
+    PathRec* path;         // pass by pass: per pixel (streaming: null, the state is in the queues)
     Queue    q[2];
     int32_t* qcount;       // [2 (max_depth + 2)]: entries and list rays of each queue (Queue)
     int32_t* work;         // traversal work counters, 3 per bounce

@@ -38,9 +38,9 @@ def _fail_once(k, call):
 
 def test_failed_stream_growth_then_renders_recover(ctx):
     """The context's streaming path state: 2^16 slots for the small render, then
-    2^21 for the large one, whose growth reallocates 30 buffers (path records,
-    free stack, 2 x 13 queue arrays, 2 root queues); it fails at the 1st, 2nd,
-    16th and 30th."""
+    2^21 for the large one, whose growth reallocates 34 buffers (2 x 16 queue
+    arrays with the paths' state, 2 root queues); it fails at the 1st, 2nd,
+    18th and 34th."""
     from cadise_amd.api import Context, Scene
     small = scenes.cornell_box(width=32, height=32, spp=1, max_depth=5)
     big = scenes.cornell_box(width=256, height=256, spp=32, max_depth=5)
@@ -50,7 +50,7 @@ def test_failed_stream_growth_then_renders_recover(ctx):
     del fresh
     a, b = Scene.from_spec(ctx, small), Scene.from_spec(ctx, big)
     tol = dict(rtol=2e-5, atol=1e-6)
-    for k in (0, 1, 15, 29):
+    for k in (0, 1, 17, 33):
         ctx.release_stream_buffers()
         np.testing.assert_allclose(a.render(seed=5), want_small, **tol)
         msg = _fail_once(k, lambda: b.render(seed=6))
@@ -60,16 +60,16 @@ def test_failed_stream_growth_then_renders_recover(ctx):
 
 
 def test_failed_pass_buffers_then_render_recovers(ctx):
-    """A scene's pass-by-pass buffers (path records, 2 x 13 queue arrays, queue
-    counts, work counters: 29 allocations on its first render): the first render
-    of a new scene fails at the 1st, 2nd, 16th and 29th; the next render of the
+    """A scene's pass-by-pass buffers (path records, 2 x 16 queue arrays, queue
+    counts, work counters: 35 allocations on its first render): the first render
+    of a new scene fails at the 1st, 2nd, 18th and 35th; the next render of the
     same scene succeeds with the undisturbed film."""
     from cadise_amd.api import Context, Scene
     big = scenes.cornell_box(width=256, height=256, spp=2, max_depth=5)
     fresh = Context(0)
     want = Scene.from_spec(fresh, big).render(seed=6, passwise=True)
     del fresh
-    for k in (0, 1, 15, 28):
+    for k in (0, 1, 17, 34):
         b = Scene.from_spec(ctx, big)
         msg = _fail_once(k, lambda: b.render(seed=6, passwise=True))
         print(f"[alloc failure, pass by pass, k={k}] {msg}", flush=True)
