@@ -113,6 +113,11 @@ struct LaneStack {
         else spill[sp - S] = v;
         ++sp;
     }
+    // a push the caller knows to fit in the LDS entries (sp < S)
+    __device__ __forceinline__ void push_lds(int32_t v) {
+        lds[(sp << shift) + lane] = v;
+        ++sp;
+    }
     __device__ __forceinline__ int32_t pop() {
         --sp;
         // the LDS read always (clamped index), the spill read only past S: two
@@ -232,25 +237,19 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             r.node = st.pop();
         }
     }
-    int nl = 0;
-    int32_t child[W], count[W], lf[W], lc[W];
+    int32_t child[W], count[W];
     float tn[W];
     bool hitc[W];
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
-        child[c] = -1;
-        count[c] = 0;
-        lf[c] = 0;
-        lc[c] = 0;
-        tn[c] = INFINITY;
-        hitc[c] = false;
-    }
-    if (trav) {
+    // every lane of a wave with a traversing lane reads a node (the others read
+    // node 0, which is cached, and discard it), so the slab tests need no
+    // per-lane defaults and no divergent branch
+    if (__ballot(trav) != 0ull) {
         // QNode<W>: origin + exponents, W children, 6 x W box bytes, W 16-bit counts
         constexpr int NWORD = W == 4 ? 16 : 28;
         constexpr int QW = 4 + W;                 // first word of the box bytes
         constexpr int CW = 4 + W + 6 * W / 4;     // first word of the counts
-        const uint4* p = reinterpret_cast<const uint4*>(sc.nodes) + static_cast<int64_t>(r.node) * (W == 4 ? 4 : 8);
+        const int32_t nidx = trav ? r.node : 0;
+        const uint4* p = reinterpret_cast<const uint4*>(sc.nodes) + static_cast<int64_t>(nidx) * (W == 4 ? 4 : 8);
         uint32_t wd[NWORD];
 #pragma unroll
         for (int k = 0; k < NWORD / 4; ++k) {
@@ -268,8 +267,8 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
         auto qb = [&](int plane, int c) {
             return static_cast<float>((wd[QW + (plane * W + c) / 4] >> (8 * (c & 3))) & 0xFFu);
         };
-        if (any) cnt.boxes_any += W;
-        else cnt.boxes += W;
+        cnt.boxes_any += (trav && any) ? W : 0;
+        cnt.boxes += (trav && !any) ? W : 0;
 #pragma unroll
         for (int c = 0; c < W; ++c) {
             child[c] = static_cast<int32_t>(wd[4 + c]);
@@ -285,33 +284,38 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             const float tz0 = loz * r.inv.z, tz1 = hiz * r.inv.z;
             const float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), ro.w));
             const float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-            hitc[c] = (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
+            hitc[c] = trav & (child[c] >= 0) & (n <= fminf(f * 1.0000005f, r.tmax));
             tn[c] = n;
         }
-        // the hit leaves, compacted in child order
+    } else {
 #pragma unroll
         for (int c = 0; c < W; ++c) {
-            const bool isl = hitc[c] && count[c] > 0;
-#pragma unroll
-            for (int k = 0; k <= c; ++k) {
-                if (isl && nl == k) {
-                    lf[k] = child[c];
-                    lc[k] = count[c];
-                }
-            }
-            nl += isl ? 1 : 0;
+            child[c] = -1;
+            count[c] = 0;
+            tn[c] = INFINITY;
+            hitc[c] = false;
         }
     }
-    // -- queue this step's leaves behind the pending ones (at most LEAF_DEFER - 1
-    // are pending here, so 64 * W + LEAF_DEFER entries hold them)
+    // -- queue this step's hit leaves, in child order, behind the pending ones (at
+    // most LEAF_DEFER - 1 are pending here, so 64 * W + LEAF_DEFER entries hold them)
+    bool isl[W];
+    int nl = 0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        isl[c] = hitc[c] && count[c] > 0;
+        nl += isl[c] ? 1 : 0;
+    }
     int ntot;
     const int excl = wave_excl_scan<W == 4 ? 3 : 4>(nl, ntot);
-    if (nl > 0) {
+    {
+        const int32_t tag = (any ? 0x8000 : 0) | (lane << 16);
+        int k = qn + excl;
 #pragma unroll
-        for (int k = 0; k < W; ++k) {
-            if (k < nl) cl.queue[qn + excl + k] = lds_i2{lf[k], lc[k] | (any ? 0x8000 : 0) | (lane << 16)};
+        for (int c = 0; c < W; ++c) {
+            if (isl[c]) cl.queue[k] = lds_i2{child[c], count[c] | tag};
+            k += isl[c] ? 1 : 0;
         }
-        r.pend = true;
+        if (nl > 0) r.pend = true;
     }
     qn += ntot;
     // -- interior children, nearest first
@@ -326,25 +330,32 @@ __device__ __forceinline__ bool visit(const DevScene& sc, LaneStack<S>& st, Lane
             kid[c] = child[c];
             nk += inner ? 1 : 0;
         }
-        // bitonic network, ascending distance
+        // ascending distance: the least comparator networks for 4 and 8 keys (5 and
+        // 19 compare-exchanges; Knuth, TAOCP vol. 3, 5.3.4)
+        if constexpr (W == 4) {
+            constexpr int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
 #pragma unroll
-        for (int k = 2; k <= W; k <<= 1) {
+            for (int e = 0; e < 5; ++e) cmpx(key[net[e][0]], kid[net[e][0]], key[net[e][1]], kid[net[e][1]], true);
+        } else {
+            constexpr int net[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 1}, {2, 3},
+                                        {4, 5}, {6, 7}, {2, 4}, {3, 5}, {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
 #pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-                for (int i = 0; i < W; ++i) {
-                    const int l = i ^ j;
-                    if (l > i) cmpx(key[i], kid[i], key[l], kid[l], (i & k) == 0);
-                }
-            }
+            for (int e = 0; e < 19; ++e) cmpx(key[net[e][0]], kid[net[e][0]], key[net[e][1]], kid[net[e][1]], true);
         }
         if (st.sp + nk - 1 > sc.max_stack) {   // bounded by the tree depth; never taken for a built tree
             r.hit = -2;
             ended = true;
         } else {
+            if (st.sp + W - 1 <= S) {   // every push fits in the LDS entries: no spill test per push
 #pragma unroll
-            for (int k = W - 1; k >= 1; --k) {
-                if (k < nk) st.push(kid[k]);
+                for (int k = W - 1; k >= 1; --k) {
+                    if (k < nk) st.push_lds(kid[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = W - 1; k >= 1; --k) {
+                    if (k < nk) st.push(kid[k]);
+                }
             }
             r.node = nk > 0 ? kid[0] : -1;
             if (nk == 0 && st.sp == 0) ended = true;
