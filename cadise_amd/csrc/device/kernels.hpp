@@ -28,8 +28,10 @@ enum StatSlot {
 // The traversal tests queued leaves once LEAF_DEFER are queued in a wave, or
 // LEAF_PARK of its lanes have ended traversal and wait on queued leaves, or no
 // lane is left traversing (a multiple of 8: the queue's extra entries are
-// spread over the wave's 64 threads).
-constexpr int LEAF_DEFER = 32;
+// spread over the wave's 64 threads). 1080p frame, ms per launch: LEAF_DEFER
+// 24 / 32 / 40 / 48 / 56 / 64 -> 1.22 / 1.21 / 1.20 / 1.18-1.19 / 1.20 / 1.21;
+// LEAF_PARK 4 / 8 / 12 at 48 -> 1.25 / 1.18 / 1.19.
+constexpr int LEAF_DEFER = 48;
 // Rays a traversal wave reserves with one atomic on the launch's work counter
 // (1080p frame, ms per launch: 64 / 128 / 256 / 512 -> 3.19 / 2.40 / 2.14 / 2.07
 // when a wave refilled once 56 lanes lacked a ray; 1.93 refilling any lane).
