@@ -3,11 +3,13 @@
 // The scenes' device tables are merged into one DevScene (node child indices,
 // leaf positions, primitive ids, material / texture / light indices offset),
 // so the traversal kernel is the single-scene one with a per-ray root node.
-// Path slots are recycled: each step regenerates camera paths into the free
-// slots from one global work counter over (scene, sample, pixel) (k_bregen /
-// k_bcommit), traces the queue's continuation + MIS-BSDF rays and shadow rays,
-// and shades (k_shade<SHADE_BATCH>), where finished paths splat into their
-// scene's film and free their slot. Steps are recorded once into a HIP graph
+// A path lives in the vertex queues (one pending vertex per path in flight,
+// carrying the path's state): each step appends new camera paths up to the
+// queue's capacity from one global work counter over (scene, sample, pixel)
+// (k_bregen / k_bcommit), traces the queue's continuation rays and its ray
+// list (shadow and MIS-BSDF rays), and shades (k_shade<SHADE_BATCH>), where
+// finished paths splat into their scene's film and leave the queue, and
+// pending ones move to the next queue. Steps are recorded once into a HIP graph
 // and the graph is relaunched until the device's done flag is raised, so the
 // GPU stays full across scene boundaries and path-length tails instead of
 // draining at the end of every sample pass of every scene.
@@ -300,7 +302,7 @@ int merge_tables(mpt_batch* B) {
     return MPT_OK;
 }
 
-// Streaming buffers for `cap` path slots.
+// Streaming buffers for `cap` paths in flight.
 int ensure_stream_buffers(mpt_ctx* ctx, Streamer& S, int32_t cap) {
     if (!S.done_host) {
         HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&S.done_host), sizeof(int32_t), hipHostMallocDefault));
@@ -679,7 +681,7 @@ int mpt_batch_autotune(mpt_batch* b, const uint64_t* seeds, int32_t trial_spp, m
     };
     int rc;
     if ((rc = trial(cur, &best_ms)) != MPT_OK) return rc;
-    // coordinate descent: path slots, BVH width, LDS stack entries, block size
+    // coordinate descent: paths in flight, BVH width, LDS stack entries, block size
     for (int pass = 0; pass < 2; ++pass) {
         bool improved = false;
         for (int dim = 0; dim < 5; ++dim) {

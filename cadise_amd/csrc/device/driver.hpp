@@ -186,8 +186,8 @@ inline TraceArgs queue_trace_args(const DevScene& sc, const Queue& Q, const int3
     return t;
 }
 
-// Path state of the streaming wavefront (batch.hip): path slots recycled
-// through a free stack, two queues, counters, and the recorded HIP graph.
+// Buffers of the streaming wavefront (batch.hip): two vertex queues that carry
+// the paths in flight, counters, and the recorded HIP graph.
 struct Streamer {
     DevBuf ctr, work_next, qcount, stats, spill, views;
     DevBuf qbuf[2][QBUFS], rootq[2];

@@ -469,9 +469,9 @@ __device__ __forceinline__ void batch_splat(const BatchArgs& bt, const BatchView
 // distinct key.
 //
 // SHADE_BATCH: paths of many scenes at mixed bounces (each path carries its
-// scene, pixel, sample and bounce); queue `step` (0 / 1) is consumed; a path
-// that does not go on splats its radiance into its scene's film and returns its
-// slot to the free stack.
+// scene, pixel, sample and bounce in its queue entry); queue `step` (0 / 1) is
+// consumed; a path that goes on moves with its state to the other queue, one
+// that does not splats its radiance into its scene's film.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step) {
     constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;

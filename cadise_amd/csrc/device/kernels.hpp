@@ -155,8 +155,8 @@ struct BatchView {
     float   f2c[12], c2w[12], cam_pos[3];
 };
 
-// Streaming state of the batched renderer: path slots are recycled through a
-// free stack; new camera paths are drawn from one global work counter.
+// Streaming state of the batched renderer: paths live in the vertex queues;
+// new camera paths are drawn from one global work counter.
 struct BatchArgs {
     const BatchView* views;
     int32_t  nviews;

@@ -595,7 +595,7 @@ BatchView scene_view(const mpt_scene* S, uint64_t seed, int32_t spp_begin, int32
 }
 
 // The default render: the scene's sample passes [begin, end) as one streaming
-// wavefront (batch.hip) over the context's path slots.
+// wavefront (batch.hip) over the context's streaming buffers.
 int do_render(mpt_scene* S, const mpt_render_params* p) {
     if (p && (p->flags & MPT_RENDER_PASSWISE)) return do_render_passes(S, p);
     mpt_ctx* ctx = S->ctx;
@@ -625,9 +625,9 @@ int do_render(mpt_scene* S, const mpt_render_params* p) {
     job.keep = keep ? S->samples.as<float4>() : nullptr;
     job.cfg = S->cfg;
     int lg = 16;
-    // up to 16M slots (~6 GB): larger traversal launches amortize each launch's
-    // ramp and straggler tail (1080p, 32 passes: 99.8 / 94.6 / 91.3 ms of
-    // traversal with 4M / 8M / 16M slots)
+    // up to 16M paths in flight (~8 GB of queues): larger traversal launches
+    // amortize each launch's ramp and straggler tail (1080p, 32 passes: 99.8 /
+    // 94.6 / 91.3 ms of traversal with 4M / 8M / 16M paths)
     while (lg < 24 && (int64_t{1} << lg) < job.total_work) ++lg;
     job.capacity_log2 = lg;
     job.graph_steps = 16;
@@ -749,7 +749,7 @@ int do_adjoint_passes(mpt_scene* S, const mpt_render_params* p, const float* d_i
 }
 
 // Path-replay adjoint on the streaming wavefront: the forward render's paths
-// (same seed, passes, depth) regenerated into path slots, each carrying its
+// (same seed, passes, depth) regenerated into the queues, each carrying its
 // adjoint weight, shaded by k_shade<SHADE_BATCH_ADJ>.
 int do_adjoint(mpt_scene* S, const mpt_render_params* p, const float* d_image, float* d_tex, float* d_light) {
     if (p && (p->flags & MPT_RENDER_PASSWISE)) return do_adjoint_passes(S, p, d_image, d_tex, d_light);

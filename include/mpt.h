@@ -170,7 +170,7 @@ typedef struct {
 
 /* MPT_RENDER_PASSWISE: render sample pass by sample pass (each pass's paths
  * bounce in lock step; the film is gathered per tile) instead of the default
- * streaming wavefront (path slots refilled as paths finish, samples splatted
+ * streaming wavefront (new paths start as paths finish, samples splatted
  * as they finish). Both draw the same paths; the films differ only in the order
  * of float additions. */
 enum { MPT_RENDER_KEEP_SAMPLES = 1, MPT_RENDER_PASSWISE = 2 };
@@ -190,7 +190,7 @@ typedef struct {
                                      fields then describe the forward render it replays */
     double  ms_gpu;               /* GPU time of the call's kernels (events; mpt_batch_render: the
                                      streaming loop) */
-    int64_t path_slots;           /* streaming renderer: path slots in flight (0 pass by pass) */
+    int64_t path_slots;           /* streaming renderer: capacity for paths in flight (0 pass by pass) */
     int64_t mis_resolved;         /* MIS-BSDF queries answered without traversal: the ray misses the
                                      picked area light's own primitive, so its closest hit cannot be
                                      that light (mis_rays counts the traced ones) */
@@ -287,10 +287,10 @@ int  mpt_scene_output_filename(const mpt_scene* scene, char* buf, size_t n, size
 int  mpt_image_write(const char* path, const uint8_t* rgb, int32_t width, int32_t height);
 
 int  mpt_stats_get(const mpt_scene* scene, mpt_stats* out);
-/* Frees the context's streaming path state (path slots and vertex queues: up to
- * 2^24 slots, ~6 GB, for single-scene renders of more than 2^23 samples; the
- * adjoint keeps at most 2^22, a batch its own 2^16..2^23). The next streaming
- * render allocates it again. */
+/* Frees the context's streaming path state (the vertex queues that carry the
+ * paths in flight: up to 2^24 paths, ~8 GB, for single-scene renders of more
+ * than 2^23 samples; the adjoint keeps at most 2^22, a batch its own
+ * 2^16..2^24). The next streaming render allocates it again. */
 int  mpt_release_stream_buffers(mpt_ctx* ctx);
 
 /* ---- differentiable rendering (no counterpart in the reference) -------------------
@@ -323,7 +323,7 @@ int  mpt_render_adjoint(mpt_scene* scene, const mpt_render_params* params, const
 /* ---- batched rendering (no counterpart in the reference) -----------------------------
  * Renders many scenes (each with its own camera, film, spp and max-depth) in
  * one streaming wavefront: the scenes' BVHs and tables are merged into one
- * device scene, path slots are refilled from a global work counter over
+ * device scene, new paths are drawn from a global work counter over
  * (scene, sample, pixel), and the bounce loop runs as a captured HIP graph.
  * Each sample is the same path that mpt_render draws for that scene with the
  * same seed; films differ from mpt_render's only by floating-point summation
@@ -334,7 +334,7 @@ typedef struct {
     int32_t width;             /* BVH width, 4 or 8 */
     int32_t stack_lds;         /* per-lane LDS stack entries: 16, 20, 24 or 32 */
     int32_t block;             /* traversal threads per block: 128 or 256 */
-    int32_t capacity_log2;     /* path slots = 2^capacity_log2 (16..24) */
+    int32_t capacity_log2;     /* paths in flight = 2^capacity_log2 (16..24) */
     int32_t graph_steps;       /* wavefront steps per graph launch (even, 2..64) */
     int32_t blocks_per_cu;     /* resident traversal blocks per CU (2..32) */
 } mpt_batch_opts;
