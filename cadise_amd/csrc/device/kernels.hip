@@ -447,20 +447,379 @@ __device__ __forceinline__ void batch_splat(const BatchArgs& bt, const BatchView
         }
 }
 
+// The state of one queue entry in the shading kernel: its path's settings and
+// state, and what shading the vertex produces for the next queue.
+struct ShadeVtx {
+    int64_t s = 0;           // entry of the current queue
+    int32_t p = 0;           // pass by pass: the path (its pixel)
+    int32_t pb = 0, smp = 0, pix = 0, md = 0;
+    int32_t nl = 0, cluster = 0, lbase = 0, cdf0 = 0, envl = -1, est = 0, sid = 0, root_next = 0;
+    uint2   key = make_uint2(0u, 0u);
+    int4    meta = make_int4(0, 0, 0, 0);
+    int64_t sample_idx = 0;  // the forward pass's sample record of this path (adjoint)
+    V3      P0 = {0.f, 0.f, 0.f}, L = {0.f, 0.f, 0.f}, beta = {0.f, 0.f, 0.f}, A = {0.f, 0.f, 0.f};
+    float4  A4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool    count_emit = false, count_next = false, pending = false;
+    int2    ch = make_int2(-1, 0);
+    float4  org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org;
+    int32_t pick_light = 0;
+    // adjoint: this lane's contributions and the factors of the vertex shaded now
+    int32_t tkey = -1, lkey = -1;
+    V3      tval = {0.f, 0.f, 0.f}, lval = {0.f, 0.f, 0.f};
+    int2    par = make_int2(-1, 0);
+    float4  sL4 = org, eL4 = org, sB4 = org, eB4 = org, sC4 = org;
+};
+
+// Loads entry s of queue step & 1: the path's step, sample, pixel and scene
+// settings, its throughput and (with rad) radiance, the vertex and its
+// continuation ray and hit.
+template <int MODE>
+__device__ __forceinline__ void shade_load(ShadeVtx& v, const PassArgs& a, const AdjArgs& g, const BatchArgs& bt,
+                                           int32_t step, int64_t s, bool rad) {
+    constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
+    constexpr bool BATCH = (MODE & SHADE_BATCH) != 0;
+    const DevScene& sc = a.sc;
+    const int qc = step & 1;
+    const Queue& cur = a.q[qc];
+    v.s = s;
+    if constexpr (!BATCH) v.p = cur.path[s];   // pass by pass: the path of pixel p
+    v.pb = step;
+    v.smp = a.sample;
+    v.pix = v.p;
+    v.md = a.max_depth;
+    v.key = a.key;
+    v.nl = sc.nlights;
+    v.cluster = sc.cluster;
+    v.envl = sc.env_light;
+    v.est = sc.estimator;
+    if constexpr (BATCH) {
+        v.meta = cur.pmeta[s];
+        v.sid = v.meta.x;
+        v.pix = v.meta.y;
+        v.smp = v.meta.z;
+        v.pb = v.meta.w;
+        const BatchView& bv = bt.views[v.sid];
+        v.md = bv.max_depth;
+        v.key = make_uint2(bv.key_lo, bv.key_hi);
+        v.nl = bv.nlights;
+        v.cluster = bv.cluster;
+        v.lbase = bv.light_base;
+        v.cdf0 = bv.cdf_base;
+        v.envl = bv.env_light;
+        v.est = bv.estimator;
+        v.root_next = bv.root;
+        v.sample_idx = static_cast<int64_t>(v.smp - bv.sample_base) * bv.img_w * bv.img_h + v.pix;
+    } else {
+        v.sample_idx = v.p;
+    }
+    v.org = cur.org[s];
+    v.P0 = v3f4(v.org);
+    if (rad) v.L = v3f4(BATCH ? cur.prad[s] : a.path[v.p].rad);
+    const float4 beta4 = BATCH ? cur.pbeta[s] : a.path[v.p].beta;
+    v.beta = v3f4(beta4);
+    // emission counts at a hit that follows the camera or a specular vertex (pt),
+    // at every hit (vpt) (ptEstimator.cpp:51-56, 80-82; vptEstimator.cpp:48-51)
+    v.count_emit = beta4.w != 0.0f;
+    if constexpr (ADJ) {
+        v.A4 = BATCH ? g.q[qc].A[s] : g.A[v.p];
+        v.A = v3f4(v.A4);
+    }
+    v.cdir = cur.dir[s];
+    v.ch = cur.hit[s];
+}
+
+// Resolves the previous vertex's NEE (its terms come pre-scaled by the path
+// throughput over the light-pick pdf: L += beta * (Ld / pick),
+// ptEstimator.cpp:94-99, distributed over Ld's two terms) and adds the
+// background's emission when the continuation ray left the scene.
+template <int MODE>
+__device__ __forceinline__ void shade_resolve(ShadeVtx& v, const PassArgs& a, const AdjArgs& g, int32_t step) {
+    constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
+    const DevScene& sc = a.sc;
+    const int qc = step & 1;
+    const Queue& cur = a.q[qc];
+    const int64_t s = v.s;
+    const float4 lt4r = v.pb > 0 ? cur.lt[s] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    const int32_t lw = __float_as_int(lt4r.w);
+    if (lw != -1) {
+        V3 Ld = v3(0.f, 0.f, 0.f);
+        const int32_t li = lw >= 0 ? lw : -2 - lw;   // lw < -1: the shadow ray was absent
+        if (lw >= 0 && cur.occ[s].x == 0) {
+            Ld = Ld + v3f4(lt4r);
+            if constexpr (ADJ) {
+                v.tval = v.tval + v.A * v3f4(g.q[qc].sL[s]);
+                v.lval = v.lval + v.A * v3f4(g.q[qc].eL[s]);
+            }
+        }
+        const float4 f4 = cur.fac[s];
+        if (f4.w != -1.0f) {   // the MIS-BSDF ray was traced (a sample pdf is never -1)
+            const int2 mh = cur.mis_hit[s];   // x: -1 background, -2 a primitive that emits nothing
+            if (mh.x == -1 && li == v.envl) {
+                // the MIS-BSDF ray reaches the background, which the picked light lights
+                const float4 md4 = cur.mis_dir[s];
+                const DevLight& lg = sc.lights[li];
+                const V3 Le = env_emittance(sc, lg, v3f4(md4));
+                const float lp = env_pdfW(sc, lg, v3f4(md4));
+                const float w = mis_power(f4.w, lp);
+                Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
+            }
+            if (mh.x >= 0) {
+                if (__float_as_int(sc.tris[3 * mh.x + 2].w) == li) {   // the record's area light
+                    const float4 md4 = cur.mis_dir[s];
+                    const SurfaceHit h = surface_detail(sc, mh.x, v.P0, v3f4(md4), __int_as_float(mh.y));
+                    const DevLight& lg = sc.lights[li];
+                    const V3 Le = light_emittance(lg, h);
+                    const float lp = light_pdfW(lg, h, v.P0);
+                    const float w = mis_power(f4.w, lp);
+                    Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
+                    if constexpr (ADJ) {
+                        if (can_emit(lg, h.wi, h.Ns)) {
+                            v.tval = v.tval + (v.A * v3f4(g.q[qc].sB[s])) * (load3(lg.radiance) * w);
+                            v.lval = v.lval + v.A * (v3f4(g.q[qc].eB[s]) * w);
+                        }
+                    }
+                }
+            }
+        }
+        v.L = v.L + Ld;
+        if constexpr (ADJ) {
+            const float4 sc4 = g.q[qc].sC[s];
+            if (sc4.w > 0.0f) {
+                const V3 R = v3f4(g.samples[v.sample_idx]) - v.L;
+                v.tval = v.tval + (v.A * R) * v3f4(sc4);
+            }
+            const int2 pp = g.q[qc].par[s];
+            if (pp.x >= 0) v.tkey = 2 * pp.x + (pp.y == 3 ? 1 : 0);
+            v.lkey = li;
+        }
+    }
+    if (v.count_emit && v.pb < v.md && v.cdir.w >= 0.0f && v.ch.x < 0 && v.envl >= 0) {
+        // an unbounded ray reaches the background: its emission, and the path
+        // ends there (the background absorbs)
+        v.L = v.L + v.beta * env_emittance(sc, sc.lights[v.envl], v3f4(v.cdir));
+    }
+}
+
+// Whether the entry's continuation ray hit a surface to shade.
+__device__ __forceinline__ bool shade_has_hit(const ShadeVtx& v) { return v.pb < v.md && v.cdir.w >= 0.0f && v.ch.x >= 0; }
+
+// Shades the continuation hit (one iteration of PtEstimator::estimate's loop
+// body): emission, light pick (uniformLightCluster.cpp:15-24 /
+// powerLightCluster.cpp:27-34), light sample + shadow ray, MIS-BSDF ray,
+// continuation sample, Russian roulette (russianRoulette.cpp:9-28). The path
+// goes on (pending) with the next vertex at the hit.
+template <int MODE>
+__device__ __forceinline__ void shade_hit(ShadeVtx& v, const PassArgs& a, uint32_t& mis_resolved,
+                                          uint32_t& shadow_resolved) {
+    constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
+    const DevScene& sc = a.sc;
+    const int32_t pb = v.pb;
+    const V3 beta0 = v.beta;
+    const uint32_t upix = static_cast<uint32_t>(v.pix);
+    const uint4 rA = philox(make_uint4(upix, static_cast<uint32_t>(v.smp), 2u * pb, 0u), v.key);
+    const uint4 rB = philox(make_uint4(upix, static_cast<uint32_t>(v.smp), 2u * pb + 1u, 0u), v.key);
+    const SurfaceHit h = surface_detail(sc, v.ch.x, v.P0, v3f4(v.cdir), __int_as_float(v.ch.y));
+    const int4 info = sc.prim_info[h.prim];
+    const DevMaterial mat = sc.mats[info.x];
+    if (v.count_emit && info.y >= 0) {
+        v.L = v.L + beta0 * light_emittance(sc.lights[info.y], h);
+        if constexpr (ADJ) {
+            if (can_emit(sc.lights[info.y], h.wi, h.Ns)) {
+                v.lkey = info.y;
+                v.lval = v.A * beta0;
+            }
+        }
+    }
+    V3 alb = v3(0.f, 0.f, 0.f);
+    if constexpr (ADJ) {
+        v.par = vertex_param(sc, mat, h.uvw);
+        if (mat.type == 0) alb = mat_albedo(sc, mat, h.uvw);
+    }
+    // next-event estimation at non-specular vertices of pt; the absorber's
+    // DirectLightEvaluator returns before sampling (directLightEvaluator.cpp:33-35)
+    const bool nee = v.est == 0 && !mat_specular(mat.type) && mat.type != DM_ABSORB;
+    v.count_next = v.est != 0 || mat_specular(mat.type);
+    // lobe choices of the plastic and glass samples: counter (pixel, sample, 2b, 1)
+    float uc_mis = 0.0f, uc_cont = 0.0f;
+    if (mat_needs_choice(mat.type)) {
+        const uint4 rC = philox(make_uint4(upix, static_cast<uint32_t>(v.smp), 2u * pb, 1u), v.key);
+        uc_mis = u01(rC, SL_MISC);
+        uc_cont = u01(rC, SL_CONTC);
+    }
+    v.sdir = make_float4(0.f, 0.f, 0.f, -1.f);
+    v.lt4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    v.mdir = make_float4(0.f, 0.f, 0.f, -1.f);
+    v.fac4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float pick = 0.0f;
+    if (nee) {
+        // light pick
+        {
+            const float u = u01(rA, SL_PICK);
+            int32_t idx = 0;
+            if (v.cluster == 1) {
+                const float* cdf = sc.power_cdf + v.cdf0;
+                for (int32_t i = v.nl - 1; i >= 0; --i)
+                    if (cdf[i] <= u) { idx = i; break; }
+                pick = cdf[idx + 1] - cdf[idx];
+            } else {
+                const uint32_t raw = static_cast<uint32_t>(0.0f + u * static_cast<float>(v.nl));
+                idx = raw < static_cast<uint32_t>(v.nl - 1) ? static_cast<int32_t>(raw) : v.nl - 1;
+                pick = 1.0f / static_cast<float>(v.nl);
+            }
+            v.pick_light = v.lbase + idx;
+        }
+        // light sample + shadow ray
+        const DevLight& lg = sc.lights[v.pick_light];
+        const LightSample ls = any_light_sample(sc, lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
+        if (ls.valid) {
+            const V3 LV = ls.pos - h.P;
+            const float dist = len(LV);
+            const V3 Ldir = divs(LV, dist);
+            const V3 sd = normalize(Ldir);
+            v.sdir = make_float4(sd.x, sd.y, sd.z, dist - K_RAY_EPS);
+            V3 f;
+            float bpdf;
+            bsdf_eval_pdf(sc, mat, h, Ldir, f, bpdf);
+            const V3 factor = f * absdot(Ldir, h.Ns);
+            const float w = mis_power(ls.pdfW, bpdf);
+            const V3 lt = divs((ls.radiance * w) * factor, ls.pdfW);
+            v.lt4 = make_float4(lt.x, lt.y, lt.z, 0.f);
+            if constexpr (ADJ) {
+                const float k = w * absdot(Ldir, h.Ns) / ls.pdfW / pick;
+                const V3 e = (beta0 * f) * k;
+                v.eL4 = make_float4(e.x, e.y, e.z, 0.f);
+                V3 sl = v3(0.f, 0.f, 0.f);
+                if (mat.type == 0) {
+                    if (dot(h.wi, h.Ns) * dot(Ldir, h.Ns) > 0.0f) sl = (beta0 * ls.radiance) * (k * K_INV_PI);
+                } else if (!is_zero(f)) {
+                    sl = (e * ls.radiance) * ggx_dlnf_dr(sc, mat, h, Ldir);
+                }
+                v.sL4 = make_float4(sl.x, sl.y, sl.z, 0.f);
+            }
+        }
+        // MIS-BSDF ray
+        {
+            const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1), uc_mis);
+            if (bs.valid) {
+                const V3 mdv = normalize(bs.L);
+                // the term counts only when the ray's closest hit is the picked
+                // area light's primitive, so the ray must hit that primitive: one
+                // that misses it (the test the traversal would run on its record,
+                // same origin, tmin and direction) is answered here, untraced
+                bool traced = true;
+                if (lg.rec >= 0) {
+                    float tl = FLT_MAX;
+                    traced = prim_hit(prim_load(sc.tris, lg.rec), h.P, mdv, K_RAY_EPS, tl);
+                    mis_resolved += traced ? 0u : 1u;
+                }
+                v.mdir = make_float4(mdv.x, mdv.y, mdv.z, traced ? FLT_MAX : -1.0f);
+                const V3 fc = bs.f * absdot(bs.L, h.Ns);
+                v.fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
+                if constexpr (ADJ) {
+                    const float k = absdot(bs.L, h.Ns) / bs.pdf / pick;
+                    const V3 e = (beta0 * bs.f) * k;
+                    v.eB4 = make_float4(e.x, e.y, e.z, 0.f);
+                    const V3 sb = mat.type == 0 ? beta0 * (k * K_INV_PI) : e * ggx_dlnf_dr(sc, mat, h, bs.L);
+                    v.sB4 = make_float4(sb.x, sb.y, sb.z, 0.f);
+                }
+            }
+        }
+    }
+    // the NEE terms pre-scaled by beta / pick; lt4.w: the picked light (-2 - it
+    // when the shadow ray is absent, -1 without NEE); fac4.w: the MIS-BSDF
+    // sample's pdf (-1: no MIS-BSDF ray)
+    if (nee && pick > 0.0f) {
+        const float ip = 1.0f / pick;
+        const V3 lts = beta0 * (v3f4(v.lt4) * ip), fcs = beta0 * (v3f4(v.fac4) * ip);
+        bool shadow = v.sdir.w >= 0.0f;
+        // a shadow ray whose term is exactly zero adds +0 to the radiance
+        // whether or not it is occluded: it is answered here (the adjoint
+        // still traces it, its gradient factors need not be zero)
+        if (!ADJ && shadow && is_zero(lts)) {
+            shadow = false;
+            v.sdir.w = -1.0f;
+            ++shadow_resolved;
+        }
+        v.lt4 = make_float4(shadow ? lts.x : 0.f, shadow ? lts.y : 0.f, shadow ? lts.z : 0.f,
+                            __int_as_float(shadow ? v.pick_light : -2 - v.pick_light));
+        v.fac4 = make_float4(fcs.x, fcs.y, fcs.z, v.mdir.w >= 0.0f ? v.fac4.w : -1.0f);
+    } else {
+        v.lt4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+        v.fac4 = make_float4(0.f, 0.f, 0.f, -1.0f);
+        if (!ADJ && v.sdir.w >= 0.0f) {   // no NEE term reads its result
+            v.sdir.w = -1.0f;
+            ++shadow_resolved;
+        }
+    }
+    // continuation
+    v.cdir = make_float4(0.f, 0.f, 0.f, -1.f);
+    {
+        const BsdfSample cs = bsdf_sample(sc, mat, h, u01(rB, SL_CONT0), u01(rB, SL_CONT1), uc_cont);
+        if (cs.valid) {
+            V3 beta = beta0 * divs(cs.f * absdot(cs.L, h.Ns), cs.pdf);
+            bool alive = true;
+            if (pb > 2) {
+                const float qq = mclamp(1.0f - max_comp(beta), 0.05f, 1.0f);
+                if (u01(rA, SL_RR) > qq) beta = beta * (1.0f / (1.0f - qq));
+                else alive = false;
+            }
+            v.beta = beta;
+            if (alive && is_zero(beta)) alive = false;
+            if (alive && pb + 1 < v.md) {
+                const V3 nd = normalize(cs.L);
+                v.cdir = make_float4(nd.x, nd.y, nd.z, FLT_MAX);
+                if constexpr (ADJ) {
+                    if (mat.type == 0) {
+                        v.sC4 = make_float4(alb.x > 0.f ? 1.0f / alb.x : 0.f, alb.y > 0.f ? 1.0f / alb.y : 0.f,
+                                            alb.z > 0.f ? 1.0f / alb.z : 0.f, 1.f);
+                    } else {
+                        const float d = ggx_dlnf_dr(sc, mat, h, cs.L);
+                        v.sC4 = make_float4(d, d, d, 1.f);
+                    }
+                }
+            }
+        }
+    }
+    v.org = make_float4(h.P.x, h.P.y, h.P.z, K_RAY_EPS);
+    v.pending = true;
+}
+
+// A path that does not go on: a streaming sample splats its radiance into its
+// scene's film (and the kept samples); pass by pass, the pixel's record takes
+// the radiance for the film gather. (The adjoint splats nothing.)
+template <int MODE>
+__device__ __forceinline__ void shade_finish(const ShadeVtx& v, const PassArgs& a, const BatchArgs& bt) {
+    constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
+    constexpr bool BATCH = (MODE & SHADE_BATCH) != 0;
+    if constexpr (BATCH) {
+        if constexpr (!ADJ) {
+            const BatchView& bv = bt.views[v.sid];
+            // the sample's film position, drawn again from its Philox stream
+            float fx, fy;
+            film_position(bv, v.pix, v.smp, v.key, fx, fy);
+            batch_splat(bt, bv, v.pix, make_float2(fx, fy), v.L);
+            if (bt.keep)
+                bt.keep[static_cast<int64_t>(v.smp - bv.sample_base) * bv.img_w * bv.img_h + v.pix] =
+                    make_float4(v.L.x, v.L.y, v.L.z, 0.f);
+        }
+    } else {
+        a.path[v.p].rad = make_float4(v.L.x, v.L.y, v.L.z, 0.f);
+        a.path[v.p].beta = make_float4(v.beta.x, v.beta.y, v.beta.z, v.count_next ? 1.0f : 0.0f);
+    }
+}
+
 // One step of every queued path: first the deferred half of
 // DirectLightEvaluator::evaluate (directLightEvaluator.cpp:18-109) for the
 // vertex shaded one step earlier, then, for the continuation hit, one
-// iteration of PtEstimator::estimate's loop body: emission, light pick
-// (uniformLightCluster.cpp:15-24 / powerLightCluster.cpp:27-34), light sample +
-// shadow ray, MIS-BSDF ray, continuation sample, Russian roulette
-// (russianRoulette.cpp:9-28). A path at bounce == max-depth only resolves.
+// iteration of PtEstimator::estimate's loop body (shade_hit). A path at
+// bounce == max-depth only resolves.
 //
 // SHADE_FWD: one sample pass of one scene; every queued path is at bounce
 // `step`, queues alternate by step.
 //
 // SHADE_ADJ replays the same steps with the same numbers (the radiance sum L
-// below is bit-identical to the forward pass's) and adds the path-replay
-// gradient: when vertex b-1's NEE resolves, its light and MIS-BSDF terms give
+// is bit-identical to the forward pass's) and adds the path-replay gradient:
+// when vertex b-1's NEE resolves, its light and MIS-BSDF terms give
 // dLoss/dtexel and dLoss/dLe directly; its continuation gives
 // A * (L_sample - L) * d ln f/dtexel, (L_sample - L) being all radiance the path
 // gathers from vertex b on. Sampling pdfs, MIS weights and the roulette factor
@@ -472,13 +831,18 @@ __device__ __forceinline__ void batch_splat(const BatchArgs& bt, const BatchView
 // scene, pixel, sample and bounce in its queue entry); queue `step` (0 / 1) is
 // consumed; a path that goes on moves with its state to the other queue, one
 // that does not splats its radiance into its scene's film.
+//
+// The forward modes shade in two phases: every entry of a tile of 256 resolves
+// its NEE, and the entries whose continuation hit a surface (about half) join
+// a block-wide backlog in LDS, which is shaded 256 at a time, so that the
+// shading code runs with full waves instead of waves half of whose lanes have
+// nothing to shade. The adjoint shades each tile in one pass.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step) {
     constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
     constexpr bool BATCH = (MODE & SHADE_BATCH) != 0;
-    const DevScene& sc = a.sc;
+    constexpr int TILE = 256;
     const int qc = step & 1;
-    const Queue& cur = a.q[qc];
     const Queue& nxt = a.q[qc ^ 1];
     const int32_t n = BATCH ? a.qcount[2 * qc] : a.qcount[2 * step];
     // the next queue's entry count and list length (one 64-bit word)
@@ -491,333 +855,114 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
     __shared__ float s_tvals[ADJ ? SHADE_WAVES * TEX_SLOTS * 3 : 1];
     __shared__ int   s_lkeys[ADJ ? SHADE_WAVES * LIGHT_SLOTS : 1];
     __shared__ float s_lvals[ADJ ? SHADE_WAVES * LIGHT_SLOTS * 3 : 1];
+    // forward: the backlog of entries to shade (entry, radiance after the NEE)
+    __shared__ int64_t s_bl_s[ADJ ? 1 : 2 * TILE];
+    __shared__ float   s_bl_L[ADJ ? 1 : 2 * TILE * 3];
+    __shared__ int32_t s_bl_n;
     WaveTable<TEX_SLOTS> ttab{s_tkeys + (ADJ ? wave * TEX_SLOTS : 0), s_tvals + (ADJ ? wave * TEX_SLOTS * 3 : 0)};
     WaveTable<LIGHT_SLOTS> ltab{s_lkeys + (ADJ ? wave * LIGHT_SLOTS : 0), s_lvals + (ADJ ? wave * LIGHT_SLOTS * 3 : 0)};
     if constexpr (ADJ) {
         ttab.clear_lane(lane);
         ltab.clear_lane(lane);
-        __syncthreads();
     }
+    if (threadIdx.x == 0) s_bl_n = 0;
+    __syncthreads();
     uint32_t mis_resolved = 0, shadow_resolved = 0;   // queries answered without tracing
-    // block-uniform trip count: every lane reaches the ballot and the barriers
-    for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
-        const int64_t s = bbase + threadIdx.x;
-        const bool active = s < n;
-        bool pending = false;
-        int32_t p = 0;
-        float4 org = make_float4(0.f, 0.f, 0.f, 0.f), cdir = org, mdir = org, sdir = org, lt4 = org, fac4 = org;
-        int32_t pick_light = 0;
-        float4 beta4 = org;
-        int32_t sid = 0, root_next = 0;
-        int64_t sample_idx = 0;   // the forward pass's sample record of this path (adjoint)
-        // adjoint: this lane's contributions and the factors of the vertex shaded now
-        int32_t tkey = -1, lkey = -1;
-        V3 tval = v3(0.f, 0.f, 0.f), lval = v3(0.f, 0.f, 0.f);
-        int2 par = make_int2(-1, 0);
-        float4 sL4 = org, eL4 = org, sB4 = org, eB4 = org, sC4 = org;
-        // streaming: the path state that goes to the next queue with its vertex
-        float4 nrad = org, nbeta = org, A4 = org;
-        int4 nmeta = make_int4(0, 0, 0, 0);
-        if (active) {
-            if constexpr (!BATCH) p = cur.path[s];   // pass by pass: the path of pixel p
-            // the path's step, sample, pixel and scene settings
-            int32_t pb = step, smp = a.sample, pix = p, md = a.max_depth;
-            uint2 key = a.key;
-            int32_t nl = sc.nlights, cluster = sc.cluster, lbase = 0, cdf0 = 0, envl = sc.env_light;
-            int32_t est = sc.estimator;
-            int4 meta = make_int4(0, 0, 0, 0);
-            if constexpr (BATCH) {
-                meta = cur.pmeta[s];
-                sid = meta.x;
-                pix = meta.y;
-                smp = meta.z;
-                pb = meta.w;
-                const BatchView& v = bt.views[sid];
-                md = v.max_depth;
-                key = make_uint2(v.key_lo, v.key_hi);
-                nl = v.nlights;
-                cluster = v.cluster;
-                lbase = v.light_base;
-                cdf0 = v.cdf_base;
-                envl = v.env_light;
-                est = v.estimator;
-                root_next = v.root;
-                sample_idx = static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix;
-            } else {
-                sample_idx = p;
+
+    // pending paths go to the next queue: wave ballots, then one atomic per block
+    // (the block's waves' counts prefix-summed in LDS), so that the global counter
+    // sees a quarter of the requests a per-wave atomic would send. Called by every
+    // thread of the block.
+    auto emit = [&](const ShadeVtx& v) __attribute__((always_inline)) {
+        const bool pending = v.pending;
+        const unsigned long long mask = __ballot(pending);
+        // the rays of the next queue's list: the vertex's shadow ray and its traced
+        // MIS-BSDF ray (absent ones are not listed, so no traversal lane draws them)
+        const bool lshd = pending && v.sdir.w >= 0.0f, lmis = pending && v.mdir.w >= 0.0f;
+        int ltot;
+        const int lexcl = wave_excl_scan<2>((lshd ? 1 : 0) + (lmis ? 1 : 0), ltot);
+        if (lane == 0) {
+            s_wcount[wave] = __popcll(mask);
+            s_wlist[wave] = ltot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t tq = 0, tl = 0;
+            for (int w = 0; w < SHADE_WAVES; ++w) {
+                const int32_t cq = s_wcount[w], cl = s_wlist[w];
+                s_wcount[w] = tq;
+                s_wlist[w] = tl;
+                tq += cq;
+                tl += cl;
             }
-            org = cur.org[s];
-            const V3 P0 = v3f4(org);
-            float4 rad4 = BATCH ? cur.prad[s] : a.path[p].rad;
-            V3 L = v3f4(rad4);
-            beta4 = BATCH ? cur.pbeta[s] : a.path[p].beta;
-            V3 beta = v3f4(beta4);
-            // emission counts at a hit that follows the camera or a specular vertex (pt),
-            // at every hit (vpt) (ptEstimator.cpp:51-56, 80-82; vptEstimator.cpp:48-51)
-            const bool count_emit = beta4.w != 0.0f;
-            bool count_next = false;
-            V3 A = v3(0.f, 0.f, 0.f);
+            // entries and list rays reserved together: (list << 32) | entries
+            unsigned long long old = 0ull;
+            if (tq | tl)
+                old = atomicAdd(reinterpret_cast<unsigned long long*>(ncount),
+                                static_cast<unsigned long long>(tq) | (static_cast<unsigned long long>(tl) << 32));
+            s_qbase = static_cast<int32_t>(old & 0xFFFFFFFFull);
+            s_lbase = static_cast<int32_t>(old >> 32);
+        }
+        __syncthreads();
+        if (pending) {
+            const int32_t slot =
+                s_qbase + s_wcount[wave] + static_cast<int32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+            if constexpr (BATCH) {
+                nxt.prad[slot] = make_float4(v.L.x, v.L.y, v.L.z, 0.f);
+                nxt.pbeta[slot] = make_float4(v.beta.x, v.beta.y, v.beta.z, v.count_next ? 1.0f : 0.0f);
+                nxt.pmeta[slot] = make_int4(v.meta.x, v.meta.y, v.meta.z, v.pb + 1);
+            } else {
+                nxt.path[slot] = v.p;
+            }
+            nxt.org[slot] = v.org;
+            nxt.dir[slot] = v.cdir;
+            if (lmis) nxt.mis_dir[slot] = v.mdir;   // read back only when its ray was traced
+            nxt.lt[slot] = v.lt4;
+            nxt.fac[slot] = v.fac4;
+            if constexpr (BATCH) bt.root[qc ^ 1][slot] = v.root_next;
+            int32_t li = s_lbase + s_wlist[wave] + lexcl;
+            if (lshd) {
+                nxt.lorg[li] = v.org;
+                nxt.ldir[li] = v.sdir;
+                nxt.lq[li] = slot;
+                if constexpr (BATCH) nxt.lroot[li] = v.root_next;
+                ++li;
+            }
+            if (lmis) {
+                nxt.lorg[li] = v.org;
+                nxt.ldir[li] = v.mdir;
+                nxt.lq[li] = static_cast<int32_t>(static_cast<uint32_t>(slot) | 0x80000000u);
+                if constexpr (BATCH) nxt.lroot[li] = v.root_next;
+            }
             if constexpr (ADJ) {
-                A4 = BATCH ? g.q[qc].A[s] : g.A[p];
-                A = v3f4(A4);
-            }
-            // -- resolve the previous vertex's NEE: its terms come pre-scaled by the
-            // path throughput over the light-pick pdf (L += beta * (Ld / pick),
-            // ptEstimator.cpp:94-99, distributed over Ld's two terms)
-            const float4 lt4r = pb > 0 ? cur.lt[s] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-            const int32_t lw = __float_as_int(lt4r.w);
-            if (lw != -1) {
-                V3 Ld = v3(0.f, 0.f, 0.f);
-                const int32_t li = lw >= 0 ? lw : -2 - lw;   // lw < -1: the shadow ray was absent
-                if (lw >= 0 && cur.occ[s].x == 0) {
-                    Ld = Ld + v3f4(lt4r);
-                    if constexpr (ADJ) {
-                        tval = tval + A * v3f4(g.q[qc].sL[s]);
-                        lval = lval + A * v3f4(g.q[qc].eL[s]);
-                    }
-                }
-                const float4 f4 = cur.fac[s];
-                if (f4.w != -1.0f) {   // the MIS-BSDF ray was traced (a sample pdf is never -1)
-                    const int2 mh = cur.mis_hit[s];   // x: -1 background, -2 a primitive that emits nothing
-                    if (mh.x == -1 && li == envl) {
-                        // the MIS-BSDF ray reaches the background, which the picked light lights
-                        const float4 md4 = cur.mis_dir[s];
-                        const DevLight& lg = sc.lights[li];
-                        const V3 Le = env_emittance(sc, lg, v3f4(md4));
-                        const float lp = env_pdfW(sc, lg, v3f4(md4));
-                        const float w = mis_power(f4.w, lp);
-                        Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
-                    }
-                    if (mh.x >= 0) {
-                        if (__float_as_int(sc.tris[3 * mh.x + 2].w) == li) {   // the record's area light
-                            const float4 md4 = cur.mis_dir[s];
-                            const SurfaceHit h = surface_detail(sc, mh.x, P0, v3f4(md4), __int_as_float(mh.y));
-                            const DevLight& lg = sc.lights[li];
-                            const V3 Le = light_emittance(lg, h);
-                            const float lp = light_pdfW(lg, h, P0);
-                            const float w = mis_power(f4.w, lp);
-                            Ld = Ld + divs((Le * w) * v3f4(f4), f4.w);
-                            if constexpr (ADJ) {
-                                if (can_emit(lg, h.wi, h.Ns)) {
-                                    tval = tval + (A * v3f4(g.q[qc].sB[s])) * (load3(lg.radiance) * w);
-                                    lval = lval + A * (v3f4(g.q[qc].eB[s]) * w);
-                                }
-                            }
-                        }
-                    }
-                }
-                L = L + Ld;
-                if constexpr (ADJ) {
-                    const float4 sc4 = g.q[qc].sC[s];
-                    if (sc4.w > 0.0f) {
-                        const V3 R = v3f4(g.samples[sample_idx]) - L;
-                        tval = tval + (A * R) * v3f4(sc4);
-                    }
-                    const int2 pp = g.q[qc].par[s];
-                    if (pp.x >= 0) tkey = 2 * pp.x + (pp.y == 3 ? 1 : 0);
-                    lkey = li;
-                }
-            }
-            // -- shade the continuation hit
-            cdir = cur.dir[s];
-            const int2 ch = cur.hit[s];
-            if (count_emit && pb < md && cdir.w >= 0.0f && ch.x < 0 && envl >= 0) {
-                // an unbounded ray reaches the background: its emission, and the path
-                // ends there (the background absorbs)
-                L = L + beta * env_emittance(sc, sc.lights[envl], v3f4(cdir));
-            }
-            if (pb < md && cdir.w >= 0.0f && ch.x >= 0) {
-                const uint32_t upix = static_cast<uint32_t>(pix);
-                const uint4 rA = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb, 0u), key);
-                const uint4 rB = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb + 1u, 0u), key);
-                const SurfaceHit h = surface_detail(sc, ch.x, P0, v3f4(cdir), __int_as_float(ch.y));
-                const int4 info = sc.prim_info[h.prim];
-                const DevMaterial mat = sc.mats[info.x];
-                if (count_emit && info.y >= 0) {
-                    L = L + beta * light_emittance(sc.lights[info.y], h);
-                    if constexpr (ADJ) {
-                        if (can_emit(sc.lights[info.y], h.wi, h.Ns)) {
-                            lkey = info.y;
-                            lval = A * beta;
-                        }
-                    }
-                }
-                V3 alb = v3(0.f, 0.f, 0.f);
-                if constexpr (ADJ) {
-                    par = vertex_param(sc, mat, h.uvw);
-                    if (mat.type == 0) alb = mat_albedo(sc, mat, h.uvw);
-                }
-                // next-event estimation at non-specular vertices of pt; the absorber's
-                // DirectLightEvaluator returns before sampling (directLightEvaluator.cpp:33-35)
-                const bool nee = est == 0 && !mat_specular(mat.type) && mat.type != DM_ABSORB;
-                count_next = est != 0 || mat_specular(mat.type);
-                // lobe choices of the plastic and glass samples: counter (pixel, sample, 2b, 1)
-                float uc_mis = 0.0f, uc_cont = 0.0f;
-                if (mat_needs_choice(mat.type)) {
-                    const uint4 rC = philox(make_uint4(upix, static_cast<uint32_t>(smp), 2u * pb, 1u), key);
-                    uc_mis = u01(rC, SL_MISC);
-                    uc_cont = u01(rC, SL_CONTC);
-                }
-                sdir = make_float4(0.f, 0.f, 0.f, -1.f);
-                lt4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                mdir = make_float4(0.f, 0.f, 0.f, -1.f);
-                fac4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                float pick = 0.0f;
-                if (nee) {
-                    // light pick
-                    {
-                        const float u = u01(rA, SL_PICK);
-                        int32_t idx = 0;
-                        if (cluster == 1) {
-                            const float* cdf = sc.power_cdf + cdf0;
-                            for (int32_t i = nl - 1; i >= 0; --i)
-                                if (cdf[i] <= u) { idx = i; break; }
-                            pick = cdf[idx + 1] - cdf[idx];
-                        } else {
-                            const uint32_t raw = static_cast<uint32_t>(0.0f + u * static_cast<float>(nl));
-                            idx = raw < static_cast<uint32_t>(nl - 1) ? static_cast<int32_t>(raw) : nl - 1;
-                            pick = 1.0f / static_cast<float>(nl);
-                        }
-                        pick_light = lbase + idx;
-                    }
-                    // light sample + shadow ray
-                    const DevLight& lg = sc.lights[pick_light];
-                    const LightSample ls = any_light_sample(sc, lg, h.P, u01(rA, SL_LPOS0), u01(rA, SL_LPOS1));
-                    if (ls.valid) {
-                        const V3 LV = ls.pos - h.P;
-                        const float dist = len(LV);
-                        const V3 Ldir = divs(LV, dist);
-                        const V3 sd = normalize(Ldir);
-                        sdir = make_float4(sd.x, sd.y, sd.z, dist - K_RAY_EPS);
-                        V3 f;
-                        float bpdf;
-                        bsdf_eval_pdf(sc, mat, h, Ldir, f, bpdf);
-                        const V3 factor = f * absdot(Ldir, h.Ns);
-                        const float w = mis_power(ls.pdfW, bpdf);
-                        const V3 lt = divs((ls.radiance * w) * factor, ls.pdfW);
-                        lt4 = make_float4(lt.x, lt.y, lt.z, 0.f);
-                        if constexpr (ADJ) {
-                            const float k = w * absdot(Ldir, h.Ns) / ls.pdfW / pick;
-                            const V3 e = (beta * f) * k;
-                            eL4 = make_float4(e.x, e.y, e.z, 0.f);
-                            V3 sl = v3(0.f, 0.f, 0.f);
-                            if (mat.type == 0) {
-                                if (dot(h.wi, h.Ns) * dot(Ldir, h.Ns) > 0.0f) sl = (beta * ls.radiance) * (k * K_INV_PI);
-                            } else if (!is_zero(f)) {
-                                sl = (e * ls.radiance) * ggx_dlnf_dr(sc, mat, h, Ldir);
-                            }
-                            sL4 = make_float4(sl.x, sl.y, sl.z, 0.f);
-                        }
-                    }
-                    // MIS-BSDF ray
-                    {
-                        const BsdfSample bs = bsdf_sample(sc, mat, h, u01(rB, SL_MIS0), u01(rB, SL_MIS1), uc_mis);
-                        if (bs.valid) {
-                            const V3 mdv = normalize(bs.L);
-                            // the term counts only when the ray's closest hit is the picked
-                            // area light's primitive, so the ray must hit that primitive: one
-                            // that misses it (the test the traversal would run on its record,
-                            // same origin, tmin and direction) is answered here, untraced
-                            bool traced = true;
-                            if (lg.rec >= 0) {
-                                float tl = FLT_MAX;
-                                traced = prim_hit(prim_load(sc.tris, lg.rec), h.P, mdv, K_RAY_EPS, tl);
-                                mis_resolved += traced ? 0u : 1u;
-                            }
-                            mdir = make_float4(mdv.x, mdv.y, mdv.z, traced ? FLT_MAX : -1.0f);
-                            const V3 fc = bs.f * absdot(bs.L, h.Ns);
-                            fac4 = make_float4(fc.x, fc.y, fc.z, bs.pdf);
-                            if constexpr (ADJ) {
-                                const float k = absdot(bs.L, h.Ns) / bs.pdf / pick;
-                                const V3 e = (beta * bs.f) * k;
-                                eB4 = make_float4(e.x, e.y, e.z, 0.f);
-                                const V3 sb = mat.type == 0 ? beta * (k * K_INV_PI) : e * ggx_dlnf_dr(sc, mat, h, bs.L);
-                                sB4 = make_float4(sb.x, sb.y, sb.z, 0.f);
-                            }
-                        }
-                    }
-                }
-                // the NEE terms pre-scaled by beta / pick; lt4.w: the picked light (-2 - it
-                // when the shadow ray is absent, -1 without NEE); fac4.w: the MIS-BSDF
-                // sample's pdf (-1: no MIS-BSDF ray)
-                if (nee && pick > 0.0f) {
-                    const float ip = 1.0f / pick;
-                    const V3 lts = beta * (v3f4(lt4) * ip), fcs = beta * (v3f4(fac4) * ip);
-                    bool shadow = sdir.w >= 0.0f;
-                    // a shadow ray whose term is exactly zero adds +0 to the radiance
-                    // whether or not it is occluded: it is answered here (the adjoint
-                    // still traces it, its gradient factors need not be zero)
-                    if (!ADJ && shadow && is_zero(lts)) {
-                        shadow = false;
-                        sdir.w = -1.0f;
-                        ++shadow_resolved;
-                    }
-                    lt4 = make_float4(shadow ? lts.x : 0.f, shadow ? lts.y : 0.f, shadow ? lts.z : 0.f,
-                                      __int_as_float(shadow ? pick_light : -2 - pick_light));
-                    fac4 = make_float4(fcs.x, fcs.y, fcs.z, mdir.w >= 0.0f ? fac4.w : -1.0f);
-                } else {
-                    lt4 = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-                    fac4 = make_float4(0.f, 0.f, 0.f, -1.0f);
-                    if (!ADJ && sdir.w >= 0.0f) {   // no NEE term reads its result
-                        sdir.w = -1.0f;
-                        ++shadow_resolved;
-                    }
-                }
-                // continuation
-                cdir = make_float4(0.f, 0.f, 0.f, -1.f);
-                {
-                    const BsdfSample cs = bsdf_sample(sc, mat, h, u01(rB, SL_CONT0), u01(rB, SL_CONT1), uc_cont);
-                    if (cs.valid) {
-                        beta = beta * divs(cs.f * absdot(cs.L, h.Ns), cs.pdf);
-                        bool alive = true;
-                        if (pb > 2) {
-                            const float qq = mclamp(1.0f - max_comp(beta), 0.05f, 1.0f);
-                            if (u01(rA, SL_RR) > qq) beta = beta * (1.0f / (1.0f - qq));
-                            else alive = false;
-                        }
-                        if (alive && is_zero(beta)) alive = false;
-                        if (alive && pb + 1 < md) {
-                            const V3 nd = normalize(cs.L);
-                            cdir = make_float4(nd.x, nd.y, nd.z, FLT_MAX);
-                            if constexpr (ADJ) {
-                                if (mat.type == 0) {
-                                    sC4 = make_float4(alb.x > 0.f ? 1.0f / alb.x : 0.f, alb.y > 0.f ? 1.0f / alb.y : 0.f,
-                                                      alb.z > 0.f ? 1.0f / alb.z : 0.f, 1.f);
-                                } else {
-                                    const float d = ggx_dlnf_dr(sc, mat, h, cs.L);
-                                    sC4 = make_float4(d, d, d, 1.f);
-                                }
-                            }
-                        }
-                    }
-                }
-                org = make_float4(h.P.x, h.P.y, h.P.z, K_RAY_EPS);
-                pending = true;
-            }
-            nrad = make_float4(L.x, L.y, L.z, 0.f);
-            nbeta = make_float4(beta.x, beta.y, beta.z, count_next ? 1.0f : 0.0f);
-            if constexpr (BATCH) {
-                if (pending) {
-                    nmeta = make_int4(meta.x, meta.y, meta.z, pb + 1);
-                } else if (!ADJ) {   // the finished sample splats into its scene's film
-                    const BatchView& v = bt.views[sid];
-                    // the sample's film position, drawn again from its Philox stream
-                    float fx, fy;
-                    film_position(v, pix, smp, key, fx, fy);
-                    batch_splat(bt, v, pix, make_float2(fx, fy), L);
-                    if (bt.keep)
-                        bt.keep[static_cast<int64_t>(smp - v.sample_base) * v.img_w * v.img_h + pix] =
-                            make_float4(L.x, L.y, L.z, 0.f);
-                }
-            } else {
-                a.path[p].rad = nrad;
-                a.path[p].beta = nbeta;
+                const AdjQueue& an = g.q[qc ^ 1];
+                an.par[slot] = v.par;
+                an.sL[slot] = v.sL4;
+                an.eL[slot] = v.eL4;
+                an.sB[slot] = v.sB4;
+                an.eB[slot] = v.eB4;
+                an.sC[slot] = v.sC4;
+                if constexpr (BATCH) an.A[slot] = v.A4;
             }
         }
-        if constexpr (ADJ) {
+        __syncthreads();   // the LDS counts are read before anyone writes them again
+    };
+
+    if constexpr (ADJ) {
+        // block-uniform trip count: every lane reaches the ballot and the barriers
+        for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
+            const int64_t s = bbase + threadIdx.x;
+            ShadeVtx v;
+            if (s < n) {
+                shade_load<MODE>(v, a, g, bt, step, s, true);
+                shade_resolve<MODE>(v, a, g, step);
+                if (shade_has_hit(v)) shade_hit<MODE>(v, a, mis_resolved, shadow_resolved);
+                if (!v.pending) shade_finish<MODE>(v, a, bt);
+                else if constexpr (!BATCH) shade_finish<MODE>(v, a, bt);   // pass by pass: every step
+            }
             // per-wave pre-reduction, then one atomic per distinct texel / light
-            if (tkey >= 0 && !is_zero(tval)) ttab.insert(tkey, tval);
-            if (lkey >= 0 && !is_zero(lval)) ltab.insert(lkey, lval);
+            if (v.tkey >= 0 && !is_zero(v.tval)) ttab.insert(v.tkey, v.tval);
+            if (v.lkey >= 0 && !is_zero(v.lval)) ltab.insert(v.lkey, v.lval);
             __syncthreads();
             for (int i = lane; i < TEX_SLOTS; i += WAVE) {
                 const int k = ttab.keys[i];
@@ -845,82 +990,59 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
                 ltab.vals[3 * i] = ltab.vals[3 * i + 1] = ltab.vals[3 * i + 2] = 0.f;
             }
             __syncthreads();
+            emit(v);
         }
-        // pending paths go to the next queue: wave ballots, then one atomic per block
-        // (the block's waves' counts prefix-summed in LDS), so that the global counter
-        // sees a quarter of the requests a per-wave atomic would send
-        const unsigned long long mask = __ballot(pending);
-        // the rays of the next queue's list: the vertex's shadow ray and its traced
-        // MIS-BSDF ray (absent ones are not listed, so no traversal lane draws them)
-        const bool lshd = pending && sdir.w >= 0.0f, lmis = pending && mdir.w >= 0.0f;
-        int ltot;
-        const int lexcl = wave_excl_scan<2>((lshd ? 1 : 0) + (lmis ? 1 : 0), ltot);
-        if (lane == 0) {
-            s_wcount[wave] = __popcll(mask);
-            s_wlist[wave] = ltot;
+    } else {
+        // shade the top `count` entries of the backlog (count <= TILE; block-uniform)
+        auto shade_backlog = [&](int count) __attribute__((always_inline)) {
+            const int top = s_bl_n;
+            ShadeVtx v;
+            if (static_cast<int>(threadIdx.x) < count) {
+                const int i = top - count + static_cast<int>(threadIdx.x);
+                shade_load<MODE>(v, a, g, bt, step, s_bl_s[i], false);
+                v.L = v3(s_bl_L[3 * i], s_bl_L[3 * i + 1], s_bl_L[3 * i + 2]);
+                shade_hit<MODE>(v, a, mis_resolved, shadow_resolved);
+                if (!v.pending) shade_finish<MODE>(v, a, bt);
+                else if constexpr (!BATCH) shade_finish<MODE>(v, a, bt);   // pass by pass: every step
+            }
+            __syncthreads();   // every read of the backlog is done
+            if (threadIdx.x == 0) s_bl_n = top - count;
+            emit(v);           // (its barriers publish s_bl_n)
+        };
+        // block-uniform trip count: every lane reaches the ballot and the barriers
+        for (int64_t bbase = static_cast<int64_t>(blockIdx.x) * blockDim.x; bbase < n; bbase += stride) {
+            const int64_t s = bbase + threadIdx.x;
+            bool hit = false;
+            ShadeVtx v;
+            if (s < n) {
+                shade_load<MODE>(v, a, g, bt, step, s, true);
+                shade_resolve<MODE>(v, a, g, step);
+                hit = shade_has_hit(v);
+                if (!hit) shade_finish<MODE>(v, a, bt);
+            }
+            // append this tile's hits to the backlog (below TILE entries are waiting)
+            const unsigned long long hm = __ballot(hit);
+            if (lane == 0) s_wcount[wave] = __popcll(hm);
+            __syncthreads();
+            int base = s_bl_n;
+            for (int w = 0; w < wave; ++w) base += s_wcount[w];
+            if (hit) {
+                const int i = base + static_cast<int>(__popcll(hm & ((1ull << lane) - 1ull)));
+                s_bl_s[i] = s;
+                s_bl_L[3 * i] = v.L.x;
+                s_bl_L[3 * i + 1] = v.L.y;
+                s_bl_L[3 * i + 2] = v.L.z;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int t = 0;
+                for (int w = 0; w < SHADE_WAVES; ++w) t += s_wcount[w];
+                s_bl_n += t;
+            }
+            __syncthreads();
+            while (s_bl_n >= TILE) shade_backlog(TILE);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int32_t tq = 0, tl = 0;
-            for (int w = 0; w < SHADE_WAVES; ++w) {
-                const int32_t cq = s_wcount[w], cl = s_wlist[w];
-                s_wcount[w] = tq;
-                s_wlist[w] = tl;
-                tq += cq;
-                tl += cl;
-            }
-            // entries and list rays reserved together: (list << 32) | entries
-            unsigned long long old = 0ull;
-            if (tq | tl)
-                old = atomicAdd(reinterpret_cast<unsigned long long*>(ncount),
-                                static_cast<unsigned long long>(tq) | (static_cast<unsigned long long>(tl) << 32));
-            s_qbase = static_cast<int32_t>(old & 0xFFFFFFFFull);
-            s_lbase = static_cast<int32_t>(old >> 32);
-        }
-        __syncthreads();
-        // (the next iteration rewrites the LDS counts only after its first barrier,
-        // which every wave reaches after these reads)
-        const int32_t base = s_qbase + s_wcount[wave];
-        if (pending) {
-            const int32_t slot = base + static_cast<int32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-            if constexpr (BATCH) {
-                nxt.prad[slot] = nrad;
-                nxt.pbeta[slot] = nbeta;
-                nxt.pmeta[slot] = nmeta;
-            } else {
-                nxt.path[slot] = p;
-            }
-            nxt.org[slot] = org;
-            nxt.dir[slot] = cdir;
-            if (lmis) nxt.mis_dir[slot] = mdir;   // read back only when its ray was traced
-            nxt.lt[slot] = lt4;
-            nxt.fac[slot] = fac4;
-            if constexpr (BATCH) bt.root[qc ^ 1][slot] = root_next;
-            int32_t li = s_lbase + s_wlist[wave] + lexcl;
-            if (lshd) {
-                nxt.lorg[li] = org;
-                nxt.ldir[li] = sdir;
-                nxt.lq[li] = slot;
-                if constexpr (BATCH) nxt.lroot[li] = root_next;
-                ++li;
-            }
-            if (lmis) {
-                nxt.lorg[li] = org;
-                nxt.ldir[li] = mdir;
-                nxt.lq[li] = static_cast<int32_t>(static_cast<uint32_t>(slot) | 0x80000000u);
-                if constexpr (BATCH) nxt.lroot[li] = root_next;
-            }
-            if constexpr (ADJ) {
-                const AdjQueue& an = g.q[qc ^ 1];
-                an.par[slot] = par;
-                an.sL[slot] = sL4;
-                an.eL[slot] = eL4;
-                an.sB[slot] = sB4;
-                an.eB[slot] = eB4;
-                an.sC[slot] = sC4;
-                if constexpr (BATCH) an.A[slot] = A4;
-            }
-        }
+        while (s_bl_n > 0) shade_backlog(min(s_bl_n, TILE));
     }
     // one atomic per block for each count of queries answered here (the waves'
     // sums in LDS)
