@@ -470,6 +470,27 @@ struct ShadeVtx {
     float4  sL4 = org, eL4 = org, sB4 = org, eB4 = org, sC4 = org;
 };
 
+// A streaming entry's path settings from its (scene, pixel, sample, bounce)
+// word: the scene's view (depth, random key, lights, estimator, BVH root).
+__device__ __forceinline__ void shade_view(ShadeVtx& v, const BatchArgs& bt, int4 meta) {
+    v.meta = meta;
+    v.sid = meta.x;
+    v.pix = meta.y;
+    v.smp = meta.z;
+    v.pb = meta.w;
+    const BatchView& bv = bt.views[v.sid];
+    v.md = bv.max_depth;
+    v.key = make_uint2(bv.key_lo, bv.key_hi);
+    v.nl = bv.nlights;
+    v.cluster = bv.cluster;
+    v.lbase = bv.light_base;
+    v.cdf0 = bv.cdf_base;
+    v.envl = bv.env_light;
+    v.est = bv.estimator;
+    v.root_next = bv.root;
+    v.sample_idx = static_cast<int64_t>(v.smp - bv.sample_base) * bv.img_w * bv.img_h + v.pix;
+}
+
 // Loads entry s of queue step & 1: the path's step, sample, pixel and scene
 // settings, its throughput and (with rad) radiance, the vertex and its
 // continuation ray and hit.
@@ -493,22 +514,7 @@ __device__ __forceinline__ void shade_load(ShadeVtx& v, const PassArgs& a, const
     v.envl = sc.env_light;
     v.est = sc.estimator;
     if constexpr (BATCH) {
-        v.meta = cur.pmeta[s];
-        v.sid = v.meta.x;
-        v.pix = v.meta.y;
-        v.smp = v.meta.z;
-        v.pb = v.meta.w;
-        const BatchView& bv = bt.views[v.sid];
-        v.md = bv.max_depth;
-        v.key = make_uint2(bv.key_lo, bv.key_hi);
-        v.nl = bv.nlights;
-        v.cluster = bv.cluster;
-        v.lbase = bv.light_base;
-        v.cdf0 = bv.cdf_base;
-        v.envl = bv.env_light;
-        v.est = bv.estimator;
-        v.root_next = bv.root;
-        v.sample_idx = static_cast<int64_t>(v.smp - bv.sample_base) * bv.img_w * bv.img_h + v.pix;
+        shade_view(v, bt, cur.pmeta[s]);
     } else {
         v.sample_idx = v.p;
     }
@@ -855,9 +861,18 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
     __shared__ float s_tvals[ADJ ? SHADE_WAVES * TEX_SLOTS * 3 : 1];
     __shared__ int   s_lkeys[ADJ ? SHADE_WAVES * LIGHT_SLOTS : 1];
     __shared__ float s_lvals[ADJ ? SHADE_WAVES * LIGHT_SLOTS * 3 : 1];
-    // forward: the backlog of entries to shade (entry, radiance after the NEE)
-    __shared__ int64_t s_bl_s[ADJ ? 1 : 2 * TILE];
-    __shared__ float   s_bl_L[ADJ ? 1 : 2 * TILE * 3];
+    // forward: the backlog of entries to shade. Pass by pass: the entry and its
+    // radiance after the NEE (the rest is read again from the queue). Streaming
+    // (STASH): everything shading reads of the entry, so that the shading phase
+    // reads no queue field: (P0, L.x) (beta, L.y) (direction, L.z), the
+    // (scene, pixel, sample, bounce | emission flag << 30) word and the hit.
+    constexpr bool STASH = BATCH && !ADJ;
+    constexpr int BL = 2 * TILE;
+    __shared__ int64_t s_bl_s[(ADJ || STASH) ? 1 : BL];
+    __shared__ float   s_bl_L[(ADJ || STASH) ? 1 : BL * 3];
+    __shared__ lds_f4  s_bl_a[STASH ? BL : 1], s_bl_b[STASH ? BL : 1], s_bl_c[STASH ? BL : 1];
+    __shared__ lds_i4  s_bl_m[STASH ? BL : 1];
+    __shared__ lds_i2  s_bl_h[STASH ? BL : 1];
     __shared__ int32_t s_bl_n;
     WaveTable<TEX_SLOTS> ttab{s_tkeys + (ADJ ? wave * TEX_SLOTS : 0), s_tvals + (ADJ ? wave * TEX_SLOTS * 3 : 0)};
     WaveTable<LIGHT_SLOTS> ltab{s_lkeys + (ADJ ? wave * LIGHT_SLOTS : 0), s_lvals + (ADJ ? wave * LIGHT_SLOTS * 3 : 0)};
@@ -999,8 +1014,22 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             ShadeVtx v;
             if (static_cast<int>(threadIdx.x) < count) {
                 const int i = top - count + static_cast<int>(threadIdx.x);
-                shade_load<MODE>(v, a, g, bt, step, s_bl_s[i], false);
-                v.L = v3(s_bl_L[3 * i], s_bl_L[3 * i + 1], s_bl_L[3 * i + 2]);
+                if constexpr (STASH) {
+                    const lds_f4 A = s_bl_a[i], B = s_bl_b[i], C = s_bl_c[i];
+                    const lds_i4 M = s_bl_m[i];
+                    const lds_i2 H = s_bl_h[i];
+                    shade_view(v, bt, make_int4(M.x, M.y, M.z, M.w & 0x3FFFFFFF));
+                    v.count_emit = (M.w & 0x40000000) != 0;
+                    v.P0 = v3(A.x, A.y, A.z);
+                    v.org = make_float4(A.x, A.y, A.z, 0.f);
+                    v.beta = v3(B.x, B.y, B.z);
+                    v.cdir = make_float4(C.x, C.y, C.z, FLT_MAX);
+                    v.L = v3(A.w, B.w, C.w);
+                    v.ch = make_int2(H.x, H.y);
+                } else {
+                    shade_load<MODE>(v, a, g, bt, step, s_bl_s[i], false);
+                    v.L = v3(s_bl_L[3 * i], s_bl_L[3 * i + 1], s_bl_L[3 * i + 2]);
+                }
                 shade_hit<MODE>(v, a, mis_resolved, shadow_resolved);
                 if (!v.pending) shade_finish<MODE>(v, a, bt);
                 else if constexpr (!BATCH) shade_finish<MODE>(v, a, bt);   // pass by pass: every step
@@ -1028,10 +1057,18 @@ __global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs 
             for (int w = 0; w < wave; ++w) base += s_wcount[w];
             if (hit) {
                 const int i = base + static_cast<int>(__popcll(hm & ((1ull << lane) - 1ull)));
-                s_bl_s[i] = s;
-                s_bl_L[3 * i] = v.L.x;
-                s_bl_L[3 * i + 1] = v.L.y;
-                s_bl_L[3 * i + 2] = v.L.z;
+                if constexpr (STASH) {
+                    s_bl_a[i] = lds_f4{v.P0.x, v.P0.y, v.P0.z, v.L.x};
+                    s_bl_b[i] = lds_f4{v.beta.x, v.beta.y, v.beta.z, v.L.y};
+                    s_bl_c[i] = lds_f4{v.cdir.x, v.cdir.y, v.cdir.z, v.L.z};
+                    s_bl_m[i] = lds_i4{v.meta.x, v.meta.y, v.meta.z, v.meta.w | (v.count_emit ? 0x40000000 : 0)};
+                    s_bl_h[i] = lds_i2{v.ch.x, v.ch.y};
+                } else {
+                    s_bl_s[i] = s;
+                    s_bl_L[3 * i] = v.L.x;
+                    s_bl_L[3 * i + 1] = v.L.y;
+                    s_bl_L[3 * i + 2] = v.L.z;
+                }
             }
             __syncthreads();
             if (threadIdx.x == 0) {

@@ -168,6 +168,7 @@ __device__ __forceinline__ void cmpx(float& da, int32_t& ia, float& db, int32_t&
 // per lane.
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
 typedef int32_t lds_i2 __attribute__((ext_vector_type(2)));
+typedef int32_t lds_i4 __attribute__((ext_vector_type(4)));
 struct CoopLds {
     __attribute__((address_space(3))) lds_f4* ray;
     __attribute__((address_space(3))) unsigned long long* key;
