@@ -239,8 +239,8 @@ int merge_tables(mpt_batch* B) {
         npx += static_cast<int64_t>(h.width) * h.height;
         work += static_cast<int64_t>(h.width) * h.height * h.spp;
     }
-    if (np >= (int64_t{1} << 31) || nmat >= (int64_t{1} << 31) || ntexel >= (int64_t{1} << 30))
-        return fail(ctx, MPT_ERR_UNSUPPORTED, "batch tables too large");
+    if (np > MPT_MAX_PRIMITIVES || nmat >= (int64_t{1} << 31) || ntexel >= (int64_t{1} << 30))
+        return fail(ctx, MPT_ERR_UNSUPPORTED, "batch tables too large (at most MPT_MAX_PRIMITIVES primitives in all)");
     B->nprims = np;
     B->total_px = npx;
     HIP_TRY(ctx, B->tris.alloc(sizeof(float4) * 3 * static_cast<size_t>(np)));

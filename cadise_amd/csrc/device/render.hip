@@ -33,6 +33,8 @@ using namespace mpt;
 int build_and_upload(mpt_scene* S) {
     mpt_ctx* ctx = S->ctx;
     const HostScene& h = S->host;
+    if (h.nprims() > MPT_MAX_PRIMITIVES)
+        return fail(ctx, MPT_ERR_UNSUPPORTED, "more than MPT_MAX_PRIMITIVES primitives");
     const bool host_only = ctx->device < 0;
     if (!host_only) {
         HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -938,7 +940,8 @@ int mpt_scene_create(mpt_ctx* ctx, const mpt_scene_desc* d, const mpt_bvh_opts* 
     if (d->nlights <= 0 || !d->lights) return fail(ctx, MPT_ERR_INVALID, "no lights");
     if (d->width <= 0 || d->height <= 0 || d->tile_x <= 0 || d->tile_y <= 0 || d->spp <= 0 || d->max_depth < 0)
         return fail(ctx, MPT_ERR_INVALID, "bad film / renderer settings");
-    if (d->nprims >= (1ll << 31) - 1) return fail(ctx, MPT_ERR_INVALID, "too many primitives");
+    if (d->nprims > MPT_MAX_PRIMITIVES)
+        return fail(ctx, MPT_ERR_UNSUPPORTED, "more than MPT_MAX_PRIMITIVES primitives");
     auto S = std::make_unique<mpt_scene>();
     HostScene& h = S->host;
     const int64_t n = d->nprims;

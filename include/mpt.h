@@ -207,6 +207,12 @@ void        mpt_destroy(mpt_ctx* ctx);
 const char* mpt_last_error(const mpt_ctx* ctx);
 const char* mpt_version(void);
 
+/* At most MPT_MAX_PRIMITIVES primitives per scene, and per batch in all (the
+ * device addresses a primitive's three 16-byte records with 32-bit index
+ * arithmetic, 3 * index + 2 < 2^31); a larger scene returns MPT_ERR_UNSUPPORTED
+ * before any device work. */
+#define MPT_MAX_PRIMITIVES ((int64_t)715827882)
+
 /* Scene from CRSD (same grammar, keys and defaults as the reference reader). */
 int  mpt_scene_load_crsd(mpt_ctx* ctx, const char* path, const mpt_bvh_opts* opts, mpt_scene** out);
 int  mpt_scene_parse_crsd(mpt_ctx* ctx, const char* text, size_t len, const mpt_bvh_opts* opts, mpt_scene** out);

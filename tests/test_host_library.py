@@ -249,3 +249,38 @@ def test_trace_ray_count_cap(ctx):
         assert fn(*args(1 << 40)) == native.MPT_ERR_INVALID
         assert fn(*args(cap)) == native.MPT_ERR_DEVICE
     assert L.mpt_trace(sc.h, None, 5, 0, None) == native.MPT_ERR_INVALID
+
+
+def test_primitive_count_cap(ctx):
+    """Scenes of more than MPT_MAX_PRIMITIVES primitives are refused before any
+    array is read or device work is done: the device addresses a primitive's
+    three 16-byte records with 32-bit index arithmetic (3 * index + 2 < 2^31).
+    The description's arrays hold one primitive; only the count says more."""
+    import ctypes as C
+    header = open(os.path.join(ROOT, "include", "mpt.h")).read()
+    cap = int(re.search(r"#define MPT_MAX_PRIMITIVES \(\(int64_t\)(\d+)\)", header).group(1))
+    assert 3 * (cap - 1) + 2 < (1 << 31) <= 3 * cap + 2
+    t = np.zeros(1, np.int32)
+    v = np.zeros(9, np.float32)
+    m = np.zeros(1, np.int32)
+    mats = (native.MaterialDesc * 1)()
+    lights = (native.LightDesc * 1)()
+    lights[0].prim = 0
+    lights[0].color[:] = [1.0, 1.0, 1.0]
+    lights[0].watt = 1.0
+    d = native.SceneDesc()
+    d.prim_type, d.prim_v, d.prim_mat = t.ctypes.data, v.ctypes.data, m.ctypes.data
+    d.nmats, d.mats = 1, mats
+    d.nlights, d.lights = 1, lights
+    d.cam_dir[:] = [0.0, 0.0, -1.0]
+    d.cam_up[:] = [0.0, 1.0, 0.0]
+    d.fov_deg, d.sensor_width_mm = 40.0, 36.0
+    d.width = d.height = d.tile_x = d.tile_y = 8
+    d.spp, d.max_depth = 1, 4
+    d.output_filename = b"x.png"
+    L = native.load_library()
+    h = C.c_void_p()
+    for n in (cap + 1, 1 << 40):
+        d.nprims = n
+        assert L.mpt_scene_create(ctx.h, C.byref(d), None, C.byref(h)) == native.MPT_ERR_UNSUPPORTED
+        assert "MPT_MAX_PRIMITIVES" in L.mpt_last_error(ctx.h).decode()
