@@ -320,16 +320,18 @@ def bench_time_to_relmse(ctx, target=1e-3, cpu=True):
     sc.render(seed=3, spp_begin=0, spp_end=1, readback=False)
     sync()
     t0 = time.perf_counter()
-    done, chunk, reached, err = 0, 4, None, None
+    # progressive: after each chunk the film is read back and its relMSE checked;
+    # relMSE falls as 1/spp, so the next chunk ends where that law (with a 10%
+    # margin) puts the target, at least 4 spp further
+    done, nxt, reached, err = 0, 32, None, None
     while done < spp_max:
-        nxt = min(spp_max, done + chunk)
         film = sc.render(seed=0x1234, spp_begin=done, spp_end=nxt, accumulate=done > 0)
         done = nxt
         err = relmse(film_mean(film), ref)
         if err <= target:
             reached = done
             break
-        chunk = max(4, done // 4)
+        nxt = min(spp_max, max(done + 4, int(np.ceil(done * err / target * 1.1))))
     t = time.perf_counter() - t0
     log(f"cornell: GPU relMSE {err:.2e} at {done} spp in {t:.3f} s against {ref_desc}")
     out = {"workload": "cornell 256x256 depth 8 (36 triangles, box filter, stratified)", "target": target,
