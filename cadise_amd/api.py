@@ -245,6 +245,14 @@ class Scene:
     def is_intersecting(self, rays):
         return self.trace(rays, False)
 
+    def intersect_detail(self, rays: np.ndarray):
+        """Closest hits with their surface detail: n x 14 float32 (primitive
+        creation index, t, P, Ng, Ns, uvw); -1 and zeros for a miss."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(rays), 14), np.float32)
+        self.ctx.check(self.L.mpt_intersect_detail(self.h, rays.ctypes.data, len(rays), out.ctypes.data))
+        return out
+
     def is_occluded(self, rays):
         return self.trace(rays, True)[0].astype(bool)
 

@@ -31,6 +31,7 @@ __global__ void k_bregen(PassArgs a, BatchArgs bt, AdjArgs g, int32_t qc);
 __global__ void k_bcommit(PassArgs a, BatchArgs bt, int32_t qc);
 __global__ void k_film(PassArgs a);
 __global__ void k_srgb8(const float4* film, int64_t npix, uint8_t* out);
+__global__ void k_detail(DevScene sc, const float4* rays, const int2* hits, int64_t n, float* out);
 __global__ void k_zero_u64(unsigned long long* p, int n);
 __global__ void k_zero_f4(float4* p, int64_t n);
 __global__ void k_zero_i32(int32_t* p, int n);

@@ -1262,6 +1262,28 @@ __global__ void __launch_bounds__(256) k_srgb8(const float4* film, int64_t npix,
     }
 }
 
+// Scene::isIntersecting's surface intersection for rays whose closest hit is
+// known (leaf position, t): per ray (creation id, t, P, Ng, Ns, uvw) as
+// surface_detail computes it for shading (scene.cpp:30-48); -1 and zeros for a
+// miss.
+__global__ void __launch_bounds__(256) k_detail(DevScene sc, const float4* rays, const int2* hits, int64_t n,
+                                                float* out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* o = out + 14 * i;
+    const int2 h = hits[i];
+    if (h.x < 0) {
+        o[0] = -1.0f;
+        for (int k = 1; k < 14; ++k) o[k] = 0.0f;
+        return;
+    }
+    const float t = __int_as_float(h.y);
+    const SurfaceHit s = surface_detail(sc, h.x, v3f4(rays[2 * i]), v3f4(rays[2 * i + 1]), t);
+    const float v[14] = {static_cast<float>(s.prim), t, s.P.x, s.P.y, s.P.z, s.Ng.x, s.Ng.y, s.Ng.z,
+                         s.Ns.x, s.Ns.y, s.Ns.z, s.uvw.x, s.uvw.y, s.uvw.z};
+    for (int k = 0; k < 14; ++k) o[k] = v[k];
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;

@@ -348,7 +348,9 @@ int zero_stats(mpt_scene* S) {
     return MPT_OK;
 }
 
-int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev, hipStream_t stream) {
+// want_prim: hits carry the primitive's creation id (else its leaf position).
+int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev, hipStream_t stream,
+             int want_prim = 1) {
     if (n == 0) return MPT_OK;
     hipStream_t keep = S->stream;
     if (stream) S->stream = stream;
@@ -368,7 +370,7 @@ int do_trace(mpt_scene* S, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_h
         a.count = nullptr;
         a.n_host = static_cast<int64_t>(n);
         a.hit = reinterpret_cast<int2*>(hits_dev);
-        a.want_prim = 1;
+        a.want_prim = want_prim;
         a.work = work.as<int32_t>();
         a.stats = S->stats.as<unsigned long long>();
         a.stat_slot = any_hit ? ST_SHADOW_RAYS : ST_CLOSEST_RAYS;
@@ -1172,6 +1174,30 @@ int mpt_trace(mpt_scene* s, const mpt_ray* rays, size_t n, int any_hit, mpt_hit*
     const int rc = do_trace(s, r.as<mpt_ray>(), n, any_hit, h.as<mpt_hit>(), nullptr);
     if (rc != MPT_OK) return rc;
     HIP_TRY(s->ctx, hipMemcpy(hits, h.p, n * sizeof(mpt_hit), hipMemcpyDeviceToHost));
+    return MPT_OK;
+}
+
+int mpt_intersect_detail(mpt_scene* s, const mpt_ray* rays, size_t n, float* out) {
+    if (!s) return MPT_ERR_INVALID;
+    if (n > MPT_TRACE_MAX_RAYS) return fail(s->ctx, MPT_ERR_INVALID, "at most MPT_TRACE_MAX_RAYS rays per call");
+    if (n > 0 && (!rays || !out)) return fail(s->ctx, MPT_ERR_INVALID, "null ray / output buffer");
+    REQUIRE_DEVICE(s);
+    if (n == 0) return MPT_OK;
+    HIP_TRY(s->ctx, hipSetDevice(s->ctx->device));
+    DevBuf r, h, o;
+    HIP_TRY(s->ctx, r.alloc(n * sizeof(mpt_ray)));
+    HIP_TRY(s->ctx, h.alloc(n * sizeof(mpt_hit)));
+    HIP_TRY(s->ctx, o.alloc(n * 14 * sizeof(float)));
+    HIP_TRY(s->ctx, hipMemcpy(r.p, rays, n * sizeof(mpt_ray), hipMemcpyHostToDevice));
+    // closest hits by leaf position, then the detail of each
+    const int rc = do_trace(s, r.as<mpt_ray>(), n, 0, h.as<mpt_hit>(), nullptr, 0);
+    if (rc != MPT_OK) return rc;
+    const int64_t nn = static_cast<int64_t>(n);
+    hipLaunchKernelGGL(k_detail, dim3(static_cast<unsigned>((nn + 255) / 256)), dim3(256), 0, s->stream, s->sc,
+                       r.as<float4>(), h.as<int2>(), nn, o.as<float>());
+    HIP_TRY(s->ctx, hipGetLastError());
+    HIP_TRY(s->ctx, hipStreamSynchronize(s->stream));
+    HIP_TRY(s->ctx, hipMemcpy(out, o.p, n * 14 * sizeof(float), hipMemcpyDeviceToHost));
     return MPT_OK;
 }
 

@@ -104,7 +104,7 @@ class BatchTrial(C.Structure):
 
 EXPORTS = ["mpt_create", "mpt_destroy", "mpt_last_error", "mpt_version", "mpt_scene_load_crsd", "mpt_scene_parse_crsd",
            "mpt_scene_create", "mpt_scene_destroy", "mpt_scene_stats_get", "mpt_scene_binary_bvh", "mpt_scene_camera",
-           "mpt_scene_light_radiance", "mpt_scene_light_pick_pdf", "mpt_trace",
+           "mpt_scene_light_radiance", "mpt_scene_light_pick_pdf", "mpt_trace", "mpt_intersect_detail",
            "mpt_trace_device", "mpt_trace_device_noalloc", "mpt_render", "mpt_film_device", "mpt_film_srgb8", "mpt_film_srgb8_device", "mpt_film_save",
            "mpt_scene_output_filename", "mpt_image_write", "mpt_stats_get", "mpt_release_stream_buffers",
            "mpt_scene_texture_info", "mpt_scene_set_texture", "mpt_scene_get_texture", "mpt_scene_set_light_radiance",
@@ -141,6 +141,7 @@ def load_library():
     L.mpt_trace.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
     L.mpt_trace_device.argtypes = [vp, vp, C.c_size_t, C.c_int, vp, vp]
     L.mpt_trace_device_noalloc.argtypes = [vp, vp, C.c_size_t, C.c_int, vp, vp]
+    L.mpt_intersect_detail.argtypes = [vp, vp, C.c_size_t, vp]
     L.mpt_render.argtypes = [vp, C.POINTER(RenderParams), vp]
     L.mpt_film_device.argtypes = [vp, C.POINTER(vp)]
     L.mpt_film_srgb8.argtypes = [vp, vp]

@@ -251,6 +251,12 @@ int  mpt_scene_light_pick_pdf(const mpt_scene* scene, float* out);
 int  mpt_trace(mpt_scene* scene, const mpt_ray* rays, size_t n, int any_hit, mpt_hit* hits);
 int  mpt_trace_device(mpt_scene* scene, const mpt_ray* rays_dev, size_t n, int any_hit, mpt_hit* hits_dev,
                       void* stream);
+/* Closest hits with their surface detail, as Scene::isIntersecting fills a
+ * SurfaceIntersection (core/scene.cpp:30-48, triangle.cpp:125-160,
+ * rectangle.cpp:102-132, sphere.cpp:109-130): per ray 14 floats (primitive
+ * creation index, t, P, Ng, Ns, uvw); a miss gives -1 and zeros. Host buffers;
+ * n <= MPT_TRACE_MAX_RAYS. */
+int  mpt_intersect_detail(mpt_scene* scene, const mpt_ray* rays, size_t n, float* out);
 /* mpt_trace_device that allocates nothing. No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). The same traversal and

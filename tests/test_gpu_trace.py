@@ -180,3 +180,53 @@ def test_trace_degenerate_rays(ctx):
     for any_hit in (False, True):
         prim, _ = sc.trace(mix, any_hit)
         assert np.all((prim >= -1) & (prim < spec.num_prims())) if not any_hit else np.all((prim == 0) | (prim == 1))
+
+
+def _golden_names():
+    from tests.golden.make_golden import scene_set
+    return [n for n, _ in scene_set()]
+
+
+@pytest.mark.parametrize("name", _golden_names())
+def test_surface_detail_vs_reference_golden(ctx, name):
+    """Scene::isIntersecting's surface intersection (scene.cpp:30-48;
+    triangle.cpp:125-160, rectangle.cpp:102-132, sphere.cpp:109-130) through
+    mpt_intersect_detail, against the vectors the compiled reference wrote for
+    the golden rays (tests/golden/<name>.npz "detail": primitive, t, P, Ng, Ns,
+    uvw). Bit-identical for triangles and rectangles; a sphere's uvw comes from
+    acos / atan2 of its normal, whose last bits differ between the device's and
+    glibc's functions (a few ulps of an angle up to pi, which v = (pi - theta) /
+    pi turns into more ulps near 0), so it may differ by 5e-7 absolute (its P, Ng
+    and Ns are bit-identical). Rays with an exact-t tie compare t only; a ray that leaves the
+    scene is a miss on both sides (the reference reports its background sphere
+    as primitive -2)."""
+    import os
+    from cadise_amd.api import Scene
+    from cadise_amd.scenes import PRIM_SPHERE
+    from tests.golden.make_golden import scene_set
+    spec = dict(scene_set())[name]
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    rays, ref = g["rays"], g["detail"]
+    out = Scene.from_spec(ctx, spec).intersect_detail(rays)
+    ties = OracleScene(spec).ties(rays)
+    bits = lambda a: np.ascontiguousarray(a, np.float32).view(np.int32)
+    rp, gp = ref[:, 0].astype(np.int64), out[:, 0].astype(np.int64)
+    hit = rp >= 0
+    assert np.array_equal(gp >= 0, hit), f"{name}: {int(((gp >= 0) != hit).sum())} hit / miss mismatches"
+    assert np.all(out[~hit] == np.concatenate([[-1.0], np.zeros(13)]).astype(np.float32))
+    assert np.array_equal(bits(out[hit, 1]), bits(ref[hit, 1])), f"{name}: t not bit-identical"
+    uniq = hit & (ties <= 1)
+    assert np.array_equal(gp[uniq], rp[uniq]), f"{name}: primitive mismatches"
+    sphere = np.zeros(len(rays), bool)
+    sphere[uniq] = spec.prim_arrays()[0][rp[uniq]] == PRIM_SPHERE
+    flat, sph = uniq & ~sphere, uniq & sphere
+    assert np.array_equal(bits(out[flat, 2:14]), bits(ref[flat, 2:14])), \
+        f"{name}: {int((bits(out[flat, 2:14]) != bits(ref[flat, 2:14])).any(axis=1).sum())} triangle / rectangle rows differ"
+    assert np.array_equal(bits(out[sph, 2:11]), bits(ref[sph, 2:11])), f"{name}: sphere P / Ng / Ns differ"
+    ulps = np.abs(bits(out[sph, 11:14]).astype(np.int64) - bits(ref[sph, 11:14]).astype(np.int64))
+    dev = np.abs(out[sph, 11:14].astype(np.float64) - ref[sph, 11:14])
+    print(f"[detail {name}] rays {len(rays)}: hits {int(hit.sum())}, tied {int((hit & (ties > 1)).sum())}, "
+          f"triangle / rectangle {int(flat.sum())} bit-identical, sphere {int(sph.sum())} (uvw within "
+          f"{dev.max(initial=0.0):.2e}, {int(ulps.max(initial=0))} ulps; {int((ulps > 0).any(axis=1).sum())} rows "
+          f"not bit-identical)", flush=True)
+    assert dev.max(initial=0.0) <= 5e-7, f"{name}: sphere uvw {dev.max():.2e} apart"
