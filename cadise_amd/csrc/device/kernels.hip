@@ -844,7 +844,11 @@ __device__ __forceinline__ void shade_finish(const ShadeVtx& v, const PassArgs& 
 // shading code runs with full waves instead of waves half of whose lanes have
 // nothing to shade. The adjoint shades each tile in one pass.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step) {
+// At least 3 waves per SIMD: the adjoint modes fit 168 VGPRs without spilling
+// (left free they took 216, 2 waves; streaming adjoint 7.98 -> 7.84 ms on the
+// 512^2 textured workload); the forward modes need 119-121 either way.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_shade(PassArgs a, AdjArgs g, BatchArgs bt, int32_t step) {
     constexpr bool ADJ = (MODE & SHADE_ADJ) != 0;
     constexpr bool BATCH = (MODE & SHADE_BATCH) != 0;
     constexpr int TILE = 256;
